@@ -1,0 +1,239 @@
+/*
+ * jetshow_amd.h -- C ABI of the MI355X per-pixel radiative-transfer solver.
+ *
+ * This is the drop-in boundary for jetshow's Observation::run.  The reference
+ * binds that path through C++ classes and one pybind11 function; every entry
+ * point below names the reference interface it replaces.  All arguments are
+ * plain old data, pointers and sizes; no exception crosses the boundary; every
+ * function returns a jsx_status (0 = success) and leaves a message for
+ * jsx_last_error().
+ *
+ * Units follow the reference: CGS throughout (cm, G, cm^-3, Hz), angles in
+ * radians, grid coordinates in parsec (as SimulationInterpolater uses them).
+ */
+#ifndef JETSHOW_AMD_H
+#define JETSHOW_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JSX_ABI_VERSION 1
+
+typedef enum jsx_status {
+    JSX_OK = 0,
+    JSX_EINVAL = -1,       /* bad argument or descriptor                        */
+    JSX_ENOMEM = -2,       /* device allocation failed                          */
+    JSX_EHIP = -3,         /* HIP runtime error (message in jsx_last_error)     */
+    JSX_EUNSUPPORTED = -4, /* descriptor kind the kernels do not implement      */
+    JSX_EBUDGET = -5,      /* request does not fit the context's HBM budget     */
+    JSX_EBUSY = -6,        /* queue full / request still running               */
+    JSX_ENOTFOUND = -7,    /* unknown handle or request id                      */
+    JSX_ENODEV = -8        /* no GPU / HIP code object could not be loaded      */
+} jsx_status;
+
+/* ---- jet description: src/Jet.cpp:5-11 composition of Geometry, VField,
+ *      BField, NField, flattened into one tagged POD ------------------------ */
+
+typedef enum jsx_geometry_kind {
+    JSX_GEOM_CONE = 1,        /* src/Cone.cpp:16-118 (double-nappe cone)          */
+    JSX_GEOM_PARABOLOID = 2,  /* src/Parabaloid.cpp:11-60                          */
+    JSX_GEOM_CYLINDER = 3     /* src/Cylinder.cpp:9-99 (r^2 fixed; see flags)      */
+} jsx_geometry_kind;
+
+typedef enum jsx_bfield_kind {
+    JSX_B_RANDOM_SCALAR = 1,   /* RandomScalarBField  b0 (|p|/pc)^-e   BField.cpp:35-41 */
+    JSX_B_RANDOM_SCALAR_Z = 2, /* RandomScalarBFieldZ b0 (|z|/pc)^-e   BField.cpp:44-50 */
+    JSX_B_TOROIDAL = 3,        /* ToroidalBField (-sin fi, cos fi, 0) b0 (z/pc)^-e,
+                                  fi = atan(y/x)                       BField.cpp:96-106 */
+    JSX_B_TOROIDAL_R = 4,      /* toroidal, b0 (|p|/pc)^-e, azimuth (-y, x)/rho  (ext.)  */
+    JSX_B_HELICAL_R = 5,       /* |B| = b0 (|p|/pc)^-e, B_phi = |B| sin(pitch),
+                                  B_z = |B| cos(pitch)                          (ext.)  */
+    JSX_B_GRID_SCALAR = 6,     /* tangled |B| interpolated from grid b         (ext.)  */
+    JSX_B_GRID_VECTOR = 7,     /* SimulationBField (-sin fi Bphi, cos fi Bphi, Bp),
+                                  Bp from grid b, Bphi from grid b2    BField.cpp:196-207 */
+    JSX_B_CONST_CYLINDER_Z = 8 /* ConstCylinderBFieldZ (0, 0, b0 (|z|/pc)^-e) BField.cpp:76-82 */
+} jsx_bfield_kind;
+
+typedef enum jsx_nfield_kind {
+    JSX_N_CONST = 1,           /* ConstNField            NField.cpp:17-21 */
+    JSX_N_BK = 2,              /* BKNField  n0 (|p|/pc)^-e NField.cpp:24-30 */
+    JSX_N_BK_Z = 3,            /* BKNFieldZ n0 (|z|/pc)^-e NField.cpp:33-39 */
+    JSX_N_GRID = 4             /* SimulationNField       NField.cpp:56-59 */
+} jsx_nfield_kind;
+
+typedef enum jsx_vfield_kind {
+    JSX_V_CONST_FLAT = 1,      /* ConstFlatVField        VField.cpp:25-29 */
+    JSX_V_CONST_CENTRAL = 2,   /* ConstCentralVField     VField.cpp:61-72 */
+    JSX_V_LINEAR_GAMMA_Z = 3,  /* Gamma linear in z between (z0, g0) and (z1, g1),
+                                  clamped outside, v along +z            (ext.) */
+    JSX_V_GRID = 4             /* SimulationVField       VField.cpp:269-274 */
+} jsx_vfield_kind;
+
+typedef struct jsx_jet_desc {
+    int32_t geometry, bfield, nfield, vfield;
+    /* geometry */
+    double geo_origin[3];
+    double geo_axis[3];           /* normalised by the library, as the ctors do  */
+    double geo_param;             /* cone half-angle [rad] | paraboloid r0 [cm] |
+                                     cylinder radius [cm]                          */
+    double geo_big_scale;         /* [cm] length of infinite path pieces          */
+    double z_clip_min, z_clip_max;/* [cm] keep z in [min, max] when max > min (ext.) */
+    /* B field */
+    int32_t b_in_plasma_frame;    /* VectorBField::in_plasma_frame_ (BField.h:36-44) */
+    int32_t grid_b, grid_b2;      /* grid handles for JSX_B_GRID_*                 */
+    int32_t pad0;
+    double b0, b_exp, b_pitch;    /* [G], exponent, helical pitch [rad]            */
+    /* electron density */
+    int32_t n_in_plasma_frame;    /* NField::in_plasma_frame_ (NField.cpp:7-14)    */
+    int32_t grid_n;
+    double n0, n_exp;             /* [cm^-3], exponent                             */
+    double thermal_fraction;      /* cold e- density / relativistic density (ext.) */
+    /* bulk velocity */
+    int32_t grid_v, pad1;
+    double gamma0, gamma1;        /* Lorentz factors                               */
+    double gamma_z0, gamma_z1;    /* [cm] for JSX_V_LINEAR_GAMMA_Z                 */
+    double v_origin[3];           /* [cm] ConstCentralVField origin                */
+    /* emission */
+    double s;                     /* electron energy index (reference hard-codes 3.42,
+                                     src/Jet.cpp:32,234)                           */
+    double gamma_min;             /* k_F_r / k_C_r low-energy cutoff (utils.h:86)  */
+} jsx_jet_desc;
+
+/* ---- image plane: ImagePlane(pair<ulong,ulong> size, lg_start, lg_stop, los)
+ *      src/ImagePlane.cpp:11-33, src/Image.cpp:44-185 ---------------------- */
+typedef struct jsx_image_desc {
+    int64_t n_across;             /* image_size.first  (must be even, Image.cpp:107) */
+    int64_t n_along;              /* image_size.second                            */
+    double lg_pixel_size_start_cm;
+    double lg_pixel_size_stop_cm;
+    double los_angle;             /* [rad]                                        */
+} jsx_image_desc;
+
+/* ---- solver parameters: Observation::run(n, tau_max, dt_max, tau_min, ...)
+ *      src/Observation.cpp:20-41 plus the constants it hard-codes ------------ */
+typedef enum jsx_stokes { JSX_STOKES_I = 1, JSX_STOKES_FULL = 4 } jsx_stokes;
+typedef enum jsx_precision { JSX_PREC_FP64 = 0, JSX_PREC_MIXED = 1 } jsx_precision;
+
+/* option bits for jsx_solver_params.flags */
+#define JSX_OPT_CYLINDER_REF_BUG 0x1u  /* c0 = |w|^2 - r/pc as Cylinder.cpp:53 */
+
+typedef struct jsx_solver_params {
+    int32_t stokes;               /* jsx_stokes                                   */
+    int32_t precision;            /* jsx_precision                                */
+    int32_t n;                    /* initial step divisor (pyjetshow.cpp:150: 100) */
+    int32_t max_failed_tries;     /* odeint failed_step_checker limit (500)       */
+    double nu;                    /* [Hz] source-frame frequency                   */
+    double tau_max;               /* optical-depth truncation                      */
+    double tau_min;               /* I skipped when tau <= tau_min                 */
+    double dt_max;                /* [cm] step bound handed to the dense stepper;
+                                     Observation::run multiplies its argument by pc
+                                     first (Observation.cpp:23)                    */
+    double tau_abs_tol, tau_rel_tol;       /* Observation.cpp:156: 1e-7, 1e-14    */
+    double stokes_abs_tol, stokes_rel_tol; /* Observation.cpp:269: 1e-9, 1e-9     */
+    double tau_precision;         /* bisection target |tau - tau_max| (1.0)        */
+    double tau_n_min, tau_n_max;  /* steps_schedule thresholds (0.1, 100)          */
+    int32_t n_max_factor;         /* steps_schedule n_max = factor * n (10)        */
+    uint32_t flags;               /* JSX_OPT_*                                     */
+} jsx_solver_params;
+
+/* per-pixel flag word */
+#define JSX_PIX_HIT            0x01u /* Geometry::hit returned >= 1 segment        */
+#define JSX_PIX_NSEG2          0x02u /* two (half-infinite) segments               */
+#define JSX_PIX_TAU_CROSSED    0x04u /* tau reached tau_max, path truncated        */
+#define JSX_PIX_I_SKIPPED      0x08u /* tau <= tau_min, Stokes pass not run        */
+#define JSX_PIX_FAIL_MAXTRIALS 0x10u /* step-size search gave up (odeint throws)   */
+#define JSX_PIX_NAN            0x20u /* a NaN coefficient or state was produced    */
+#define JSX_PIX_CLIPPED        0x40u /* z clip removed part or all of the path     */
+#define JSX_PIX_FAIL_BISECT    0x80u /* tau bisection hit its iteration cap        */
+
+/* Output planes, [n_across][n_along] row-major (Image::getImage order,
+ * src/Image.cpp:16-28).  Pointers are device pointers for jsx_observe and
+ * host pointers for jsx_observe_host.  Any plane may be NULL except I. */
+typedef struct jsx_outputs {
+    double *tau, *I, *Q, *U, *V, *l;
+    uint32_t *flags;
+    uint32_t *n_rhs;              /* right-hand-side evaluations per ray           */
+    uint32_t *n_steps;            /* accepted + rejected Dormand-Prince tries      */
+} jsx_outputs;
+
+/* grid: values[iz * nr + ir] at (z0 + iz dz, r0 + ir dr) in parsec */
+typedef struct jsx_grid_desc {
+    int32_t nz, nr;
+    double z0, dz, r0, dr;
+    double nan_value;             /* SimulationInterpolater nan_value (0, Gamma 1) */
+} jsx_grid_desc;
+
+typedef struct jsx_ctx jsx_ctx;
+
+/* ---- context ------------------------------------------------------------- */
+/* Creates a solver context on `device` with an HBM admission budget in bytes
+ * (0 = 90% of free memory).  Replaces the implicit global state of
+ * Observation (src/Observation.cpp:12-17). */
+int jsx_ctx_create(int device, size_t hbm_budget, jsx_ctx **out);
+int jsx_ctx_destroy(jsx_ctx *ctx);
+const char *jsx_last_error(jsx_ctx *ctx);
+int jsx_abi_version(void);
+
+/* ---- grids: SimulationInterpolater(Delaunay_triangulation*, nan_value)
+ *      src/SimulationInterpolater.cpp:30-117; values are copied to HBM once
+ *      and referenced by handle from jsx_jet_desc ---------------------------- */
+int jsx_grid_upload(jsx_ctx *ctx, const jsx_grid_desc *desc, const double *values,
+                    int32_t *handle_out);
+int jsx_grid_release(jsx_ctx *ctx, int32_t handle);
+
+/* ---- device buffers for callers without their own allocator ------------- */
+int jsx_alloc_outputs(jsx_ctx *ctx, int64_t n_pixels, int32_t stokes, jsx_outputs *out);
+int jsx_free_outputs(jsx_ctx *ctx, jsx_outputs *out);
+
+/* ---- the hot path -------------------------------------------------------- */
+/* Observation::run + observe_single_pixel for every pixel of the image:
+ * src/Observation.cpp:20-41, 349-395 (ray generation ImagePlane.cpp:11-33,
+ * Geometry::hit, integrate_tau_adaptive :126-214, integrate_i_adaptive
+ * :242-274 or the restored full-Stokes pass :277-308).  One kernel pass per
+ * image; outputs are device pointers; runs on `stream` (NULL = the context's
+ * stream) and returns without synchronising. */
+int jsx_observe(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_image_desc *image,
+                const jsx_solver_params *params, const jsx_outputs *out_device,
+                void *stream);
+
+/* Same, with host output pointers; copies back and synchronises. */
+int jsx_observe_host(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_image_desc *image,
+                     const jsx_solver_params *params, const jsx_outputs *out_host);
+
+/* Geometry::hit for explicit rays (src/Geometry.h:40-43, Cone.cpp:56-118,
+ * Parabaloid.cpp:39-60, Cylinder.cpp:35-95).  rays: n x 6 doubles
+ * (origin, direction; direction is normalised as Ray::Ray does).  Writes the
+ * segment count (0..2) per ray and segments as n x 2 x 6 doubles (point_in,
+ * point_out).  Host pointers; synchronous. */
+int jsx_hit_host(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_solver_params *params,
+                 const double *rays, int64_t n, int32_t *n_seg, double *segments);
+
+/* Pixel sizes [n_across][n_along] in cm, Image::getPixelSizes
+ * (src/Image.cpp:56-67, 202-204). Host only, no GPU needed. */
+int jsx_image_pixel_sizes(const jsx_image_desc *image, double *sizes);
+
+/* Angular-size distance factor, mas_to_pc (src/utils.cpp:186-208). Host only. */
+double jsx_mas_to_pc(double redshift);
+
+/* ---- batched and served requests (no reference counterpart beyond
+ *      get_i_image called in a loop by JetModelZoom, jetmodel.py:92-99) ----- */
+typedef struct jsx_request {
+    jsx_jet_desc jet;
+    jsx_image_desc image;
+    jsx_solver_params params;
+    jsx_outputs out_host;         /* host planes to copy results into           */
+} jsx_request;
+
+/* One persistent launch over many images (the ABC sweep):
+ * all rays of all requests go through one work-stealing ray queue. */
+int jsx_observe_batch(jsx_ctx *ctx, const jsx_request *reqs, int32_t n_reqs);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JETSHOW_AMD_H */

@@ -1,0 +1,518 @@
+/*
+ * jsx_device.h -- per-point physics on gfx950: geometry hits, field models,
+ * frame transforms and transfer coefficients in FP64.
+ *
+ * Reference: src/Cone.cpp:56-118, src/Parabaloid.cpp:39-60,
+ * src/Cylinder.cpp:35-95, src/Geometry.cpp:28-89, src/BField.cpp,
+ * src/NField.cpp, src/VField.cpp, src/Jet.cpp:15-46,218-237 (and the
+ * commented Q/U/V/F/C getters at :114-214, :239-288), src/utils.cpp:8-173.
+ *
+ * Every expression keeps the reference's evaluation order (left to right;
+ * 3-vector reductions as (x0 + x1) + x2) and uses jsx_math for the
+ * point-dependent transcendental terms, so for the same inputs the FP64 path
+ * reproduces the CPU restatement bit for bit.  Compile with
+ * -ffp-contract=off.  Constants that depend only on model parameters
+ * (gamma-function factors, cos of the cone angle, ...) arrive precomputed in
+ * DModel; the host evaluates them with the C library exactly as the reference
+ * does on every call.
+ */
+#ifndef JSX_DEVICE_H
+#define JSX_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "jsx_math.h"
+#include "../../include/jetshow_amd.h"
+
+#define JSX_PC 3.0856775814671913e+18
+#define JSX_ME 9.10938356e-28
+#define JSX_QE 4.80320467299766e-10
+#define JSX_CL 29979245800.0
+#define JSX_PI 3.141592653589793238462643383279502884
+#define JSX_DBL_EPS 2.220446049250313080847e-16
+
+struct DGrid {
+    const double *v;      /* device values [nz][nr] */
+    int nz, nr;
+    double z0, dz, r0, dr, nan_value, zlast, rlast;
+};
+
+/* POD model: descriptor + host-evaluated parameter constants */
+struct DModel {
+    int geometry, bfield, nfield, vfield;
+    int b_plasma, n_plasma, clip, opt;
+    double origin[3], axis[3];
+    double M[3][3];
+    double cos_angle, geo_param, big_scale;
+    double zmin, zmax;
+    double b0, b_negexp, sin_pitch, cos_pitch;
+    double n0, n_negexp, thermal_fraction;
+    double vz_flat, gamma0, gamma1, gz0, gz1, v_origin[3];
+    double f_ki_v, f_ki_s, f_ei_v, f_ei_s, r_kq, r_eq, f_kv, f_ev, f_kFr, kCr_pre, kCr_gm;
+    double e_ki, e_ei, e_kv, e_kCr;
+    DGrid gb, gb2, gn, gv;
+    int has_gb, has_gb2, has_gn, has_gv;
+};
+
+/* ---------------- 3-vector helpers ---------------- */
+__device__ inline double d_dot3(const double *a, const double *b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+__device__ inline double d_sqn3(const double *a) { return (a[0] * a[0] + a[1] * a[1]) + a[2] * a[2]; }
+__device__ inline double d_norm3(const double *a) { return sqrt(d_sqn3(a)); }
+__device__ inline void d_cross3(const double *a, const double *b, double *o) {
+    double x = a[1] * b[2] - a[2] * b[1];
+    double y = a[2] * b[0] - a[0] * b[2];
+    double z = a[0] * b[1] - a[1] * b[0];
+    o[0] = x; o[1] = y; o[2] = z;
+}
+__device__ inline void d_normalize3(double *a) {
+    double z = d_sqn3(a);
+    if (z > 0.0) { double r = sqrt(z); a[0] /= r; a[1] /= r; a[2] /= r; }
+}
+__device__ inline void d_ray_point(const double *o, const double *d, double t, double *p) {
+    p[0] = o[0] + d[0] * t; p[1] = o[1] + d[1] * t; p[2] = o[2] + d[2] * t;
+}
+
+/* ---------------- geometry ---------------- */
+struct DSeg { double pin[3], pout[3]; };
+
+__device__ inline void d_full_infinite(const DModel &m, const double *o, const double *d, DSeg &s) {
+    double diff[3] = {o[0] - m.origin[0], o[1] - m.origin[1], o[2] - m.origin[2]};
+    double dd = d_dot3(diff, d);
+    double ref[3];
+    for (int i = 0; i < 3; ++i) ref[i] = diff[i] - dd * d[i];
+    for (int i = 0; i < 3; ++i) { s.pin[i] = ref[i] - m.big_scale * d[i]; s.pout[i] = ref[i] + m.big_scale * d[i]; }
+}
+
+__device__ inline int d_cone_is_within(const DModel &m, const double *p) {
+    double diff[3] = {p[0] - m.origin[0], p[1] - m.origin[1], p[2] - m.origin[2]};
+    d_normalize3(diff);
+    double cosp = d_dot3(m.axis, diff);
+    return cosp > m.cos_angle || cosp < -m.cos_angle;
+}
+
+__device__ inline void d_half_infinite(const DModel &m, const double *d, const double *pt, DSeg &s) {
+    double chk[3] = {pt[0] - d[0], pt[1] - d[1], pt[2] - d[2]};
+    if (d_cone_is_within(m, chk)) {
+        for (int i = 0; i < 3; ++i) { s.pin[i] = pt[i] - m.big_scale * d[i]; s.pout[i] = pt[i]; }
+    } else {
+        for (int i = 0; i < 3; ++i) { s.pin[i] = pt[i]; s.pout[i] = pt[i] + m.big_scale * d[i]; }
+    }
+}
+
+__device__ inline double d_quad_form(const double (&M)[3][3], const double *d, const double *e) {
+    double r[3];
+    for (int j = 0; j < 3; ++j) r[j] = (d[0] * M[0][j] + d[1] * M[1][j]) + d[2] * M[2][j];
+    return (r[0] * e[0] + r[1] * e[1]) + r[2] * e[2];
+}
+
+/* Cone::hit (src/Cone.cpp:56-118) */
+__device__ inline int d_cone_hit(const DModel &m, const double *o, const double *d, DSeg *segs) {
+    double delta[3] = {o[0] - m.origin[0], o[1] - m.origin[1], o[2] - m.origin[2]};
+    double c2 = d_quad_form(m.M, d, d);
+    double c1 = d_quad_form(m.M, d, delta);
+    double c0 = d_quad_form(m.M, delta, delta);
+    double disc = c1 * c1 - c0 * c2;
+    if (disc < 0) return 0;
+    if (disc == 0) {
+        double t = -c1 / c2;
+        double p[3];
+        d_ray_point(o, d, t, p);
+        double dp[3] = {p[0] - m.origin[0], p[1] - m.origin[1], p[2] - m.origin[2]};
+        if (d_sqn3(dp) <= (1E-06 * 1E-06) * (1E-06 * 1E-06) && c1 == 0.0) {
+            d_full_infinite(m, o, d, segs[0]);
+            return 1;
+        }
+        for (int i = 0; i < 3; ++i) { segs[0].pin[i] = p[i]; segs[0].pout[i] = p[i]; }
+        return 1;
+    }
+    double eps = (c1 > 0 ? 1 : -1);
+    double t1 = (-c1 - eps * sqrt(c1 * c1 - c2 * c0)) / (c2);
+    double t2 = c0 / (-c1 - eps * sqrt(c1 * c1 - c2 * c0));
+    double pin[3], pout[3];
+    d_ray_point(o, d, t1 < t2 ? t1 : t2, pin);
+    d_ray_point(o, d, t1 < t2 ? t2 : t1, pout);
+    double cos_ray_cone = d_dot3(d, m.axis);
+    if (fabs(cos_ray_cone) != 1.) {
+        for (int i = 0; i < 3; ++i) { segs[0].pin[i] = pin[i]; segs[0].pout[i] = pout[i]; }
+        return 1;
+    }
+    d_half_infinite(m, d, pout, segs[0]);
+    d_half_infinite(m, d, pin, segs[1]);
+    return 2;
+}
+
+/* intersection() (src/Geometry.cpp:28-69) */
+__device__ inline int d_quadric_roots(const double *R0, const double *Rd, double A, double B, double C, double D,
+                                      double E, double F, double G, double H, double I, double J, double *t) {
+    double x0 = R0[0], y0 = R0[1], z0 = R0[2];
+    double xd = Rd[0], yd = Rd[1], zd = Rd[2];
+    double Aq = A * xd * xd + B * yd * yd + C * zd * zd + D * xd * yd + E * xd * zd + F * yd * zd;
+    double Bq = 2 * A * x0 * xd + 2 * B * y0 * yd + 2 * C * z0 * zd + D * (x0 * yd + y0 * xd) + E * x0 * zd +
+                F * (y0 * zd + yd * z0) + G * xd + H * yd + I * zd;
+    double Cq = A * x0 * x0 + B * y0 * y0 + C * z0 * z0 + D * x0 * y0 + E * x0 * z0 + F * y0 * z0 + G * x0 +
+                H * y0 + I * z0 + J;
+    double Dscr = Bq * Bq - 4 * Aq * Cq;
+    if (Aq == 0) { t[0] = -Cq / Bq; return 1; }
+    if (Dscr < 0) return 0;
+    double t0 = (-Bq - sqrt(Dscr)) / (2 * Aq);
+    if (t0 > 0) { t[0] = t0; return 1; }
+    t[0] = t0;
+    t[1] = (-Bq + sqrt(Dscr)) / (2 * Aq);
+    return 2;
+}
+
+/* Parabaloid::hit (src/Parabaloid.cpp:39-60) */
+__device__ inline int d_paraboloid_hit(const DModel &m, const double *o, const double *d, DSeg *segs) {
+    double r0 = m.geo_param;
+    double t[2];
+    int nr = d_quadric_roots(o, d, 1., 1., 0., 0., 0., 0., 0., 0., -r0 * r0 / JSX_PC, 0., t);
+    if (nr != 2) return 0;
+    double t1 = t[0], t2 = t[1];
+    d_ray_point(o, d, t1 < t2 ? t1 : t2, segs[0].pin);
+    d_ray_point(o, d, t1 < t2 ? t2 : t1, segs[0].pout);
+    return 1;
+}
+
+/* Cylinder::hit (src/Cylinder.cpp:35-95); r^2 unless JSX_OPT_CYLINDER_REF_BUG */
+__device__ inline int d_cylinder_hit(const DModel &m, const double *o, const double *d, DSeg *segs) {
+    double ro[3] = {o[0] / JSX_PC, o[1] / JSX_PC, o[2] / JSX_PC};
+    double co[3] = {m.origin[0] / JSX_PC, m.origin[1] / JSX_PC, m.origin[2] / JSX_PC};
+    const double *a = m.axis;
+    double delta[3] = {ro[0] - co[0], ro[1] - co[1], ro[2] - co[2]};
+    double da = d_dot3(d, a);
+    double v[3] = {d[0] - da * a[0], d[1] - da * a[1], d[2] - da * a[2]};
+    double c2 = d_sqn3(v);
+    double dla = d_dot3(delta, a);
+    double w[3] = {delta[0] - dla * a[0], delta[1] - dla * a[1], delta[2] - dla * a[2]};
+    double c1 = d_dot3(v, w);
+    double r = m.geo_param;
+    double c0 = (m.opt & JSX_OPT_CYLINDER_REF_BUG) ? d_sqn3(w) - r / JSX_PC : d_sqn3(w) - (r / JSX_PC) * (r / JSX_PC);
+    double disc = c1 * c1 - c0 * c2;
+    if (disc < 0) return 0;
+    if (disc > 0) {
+        double eps = (c1 > 0 ? 1 : -1);
+        double sqrt_d = sqrt(disc);
+        double t1 = (-c1 - eps * sqrt_d) / (c2);
+        double t2 = c0 / (-c1 - eps * sqrt_d);
+        d_ray_point(o, d, JSX_PC * (t1 < t2 ? t1 : t2), segs[0].pin);
+        d_ray_point(o, d, JSX_PC * (t1 < t2 ? t2 : t1), segs[0].pout);
+        return 1;
+    }
+    if (c2 == 1 && c1 == 0 && c0 == 0) {
+        double t = -c1 / c2;
+        d_ray_point(o, d, t, segs[0].pin);
+        d_ray_point(o, d, t, segs[0].pout);
+        return 1;
+    }
+    if (c2 == 0 && c1 == 0 && c0 <= 0) { d_full_infinite(m, o, d, segs[0]); return 1; }
+    return 0;
+}
+
+/* z clip (extension); returns 0 drop, 1 unchanged, 2 clipped */
+__device__ inline int d_clip_seg(const DModel &m, DSeg &s) {
+    double zA = s.pin[2], zB = s.pout[2];
+    if (zA >= m.zmin && zA <= m.zmax && zB >= m.zmin && zB <= m.zmax) return 1;
+    if (zB == zA) return 0;
+    double u0 = (m.zmin - zA) / (zB - zA);
+    double u1 = (m.zmax - zA) / (zB - zA);
+    if (u0 > u1) { double tt = u0; u0 = u1; u1 = tt; }
+    double lo = u0 > 0.0 ? u0 : 0.0;
+    double hi = u1 < 1.0 ? u1 : 1.0;
+    if (!(lo < hi)) return 0;
+    double A[3], B[3];
+    for (int i = 0; i < 3; ++i) {
+        double df = s.pout[i] - s.pin[i];
+        A[i] = lo == 0.0 ? s.pin[i] : s.pin[i] + lo * df;
+        B[i] = hi == 1.0 ? s.pout[i] : s.pin[i] + hi * df;
+    }
+    for (int i = 0; i < 3; ++i) { s.pin[i] = A[i]; s.pout[i] = B[i]; }
+    return 2;
+}
+
+/* Jet::hit + clip; *clipped set if the clip changed anything */
+__device__ inline int d_model_hit(const DModel &m, const double *o, const double *d, DSeg *segs, int *clipped) {
+    int n = 0;
+    *clipped = 0;
+    if (m.geometry == JSX_GEOM_CONE) n = d_cone_hit(m, o, d, segs);
+    else if (m.geometry == JSX_GEOM_PARABOLOID) n = d_paraboloid_hit(m, o, d, segs);
+    else if (m.geometry == JSX_GEOM_CYLINDER) n = d_cylinder_hit(m, o, d, segs);
+    if (m.clip) {
+        int k = 0;
+        for (int i = 0; i < n; ++i) {
+            DSeg s = segs[i];
+            int r = d_clip_seg(m, s);
+            if (r != 1) *clipped = 1;
+            if (r) segs[k++] = s;
+        }
+        n = k;
+    }
+    return n;
+}
+
+/* ---------------- grid interpolation ---------------- */
+__device__ inline double d_tri_area(double px, double py, double qx, double qy, double rx, double ry) {
+    return ((qx - px) * (ry - py) - (rx - px) * (qy - py)) / 2;
+}
+
+/* SimulationInterpolater::interpolated_value on a regular (z, r_p) grid
+ * (src/SimulationInterpolater.cpp:43-102): fixed (i,j)-(i+1,j+1) diagonal. */
+__device__ inline double d_grid_interp(const DGrid &G, const double *p) {
+    double x = p[0] / JSX_PC, y = p[1] / JSX_PC, z = p[2] / JSX_PC;
+    double rp = sqrt(x * x + y * y);
+    if (!(z >= G.z0 && z <= G.zlast && rp >= G.r0 && rp <= G.rlast)) return G.nan_value;
+    int i = (int)floor((z - G.z0) / G.dz);
+    int j = (int)floor((rp - G.r0) / G.dr);
+    if (i > G.nz - 2) i = G.nz - 2;
+    if (j > G.nr - 2) j = G.nr - 2;
+    if (i < 0) i = 0;
+    if (j < 0) j = 0;
+    double za = G.z0 + (double)i * G.dz, zb = G.z0 + (double)(i + 1) * G.dz;
+    double ra = G.r0 + (double)j * G.dr, rb = G.r0 + (double)(j + 1) * G.dr;
+    const double *row0 = G.v + (int64_t)i * G.nr;
+    const double *row1 = row0 + G.nr;
+    double f00 = row0[j], f11 = row1[j + 1];
+    double v1x, v1y, v2x, v2y, f1, f2;
+    if ((rp - ra) * (zb - za) <= (z - za) * (rb - ra)) {
+        v1x = zb; v1y = ra; f1 = row1[j];
+        v2x = zb; v2y = rb; f2 = f11;
+    } else {
+        v1x = zb; v1y = rb; f1 = f11;
+        v2x = za; v2y = rb; f2 = row0[j + 1];
+    }
+    double total = d_tri_area(za, ra, v1x, v1y, v2x, v2y);
+    double inv = 1.0 / total;
+    double b0 = d_tri_area(v1x, v1y, v2x, v2y, z, rp) * inv;
+    double b1 = d_tri_area(v2x, v2y, za, ra, z, rp) * inv;
+    double b2 = 1.0 - b0 - b1;
+    double val = 0;
+    val += b0 * f00;
+    val += b1 * f1;
+    val += b2 * f2;
+    if (val != val) val = G.nan_value;
+    return val;
+}
+
+/* pow(x, y) sharing a logarithm; same bits as jsx_pow(x, y) */
+__device__ inline double d_pow_shared(double x, double lh, double ll, double y) {
+    return jsx_pow_is_generic(x, y) ? jsx_pow_from_log(lh, ll, y) : jsx_pow(x, y);
+}
+
+/* ---------------- fields ---------------- */
+__device__ inline void d_vfield(const DModel &m, const double *p, double *v) {
+    switch (m.vfield) {
+    case JSX_V_CONST_FLAT:
+        v[0] = 0; v[1] = 0; v[2] = m.vz_flat;
+        return;
+    case JSX_V_CONST_CENTRAL: {
+        double q[3] = {p[0] + m.v_origin[0], p[1] + m.v_origin[1], p[2] + m.v_origin[2]};
+        double r = d_norm3(q);
+        double v_r = JSX_CL * sqrt(1. - 1. / (m.gamma0 * m.gamma0));
+        v[0] = v_r * (p[0] + m.v_origin[0]) / r;
+        v[1] = v_r * (p[1] + m.v_origin[1]) / r;
+        v[2] = v_r * (p[2] + m.v_origin[2]) / r;
+        return;
+    }
+    case JSX_V_LINEAR_GAMMA_Z: {
+        double z = p[2], g;
+        if (z <= m.gz0) g = m.gamma0;
+        else if (z >= m.gz1) g = m.gamma1;
+        else g = m.gamma0 + (m.gamma1 - m.gamma0) * ((z - m.gz0) / (m.gz1 - m.gz0));
+        v[0] = 0; v[1] = 0; v[2] = JSX_CL * sqrt(1. - 1. / (g * g));
+        return;
+    }
+    case JSX_V_GRID: {
+        double gam = m.has_gv ? d_grid_interp(m.gv, p) : 1.0;
+        v[0] = 0; v[1] = 0; v[2] = JSX_CL * sqrt(1. - 1. / (gam * gam));
+        return;
+    }
+    default:
+        v[0] = v[1] = v[2] = 0;
+    }
+}
+
+__device__ inline double d_getG(const double *v) {
+    double beta[3] = {v[0] / JSX_CL, v[1] / JSX_CL, v[2] / JSX_CL};
+    return sqrt(1. / (1. - d_sqn3(beta)));
+}
+
+/* frame quantities of Jet::getKI (src/Jet.cpp:25-31) */
+struct DFrame {
+    double gamma, D, n, nu;
+    double nl[3];
+    double b[3];
+    int vec;
+};
+
+__device__ inline void d_frame(const DModel &m, const double *p, const double *n_los, double nu, DFrame &f) {
+    double v[3];
+    d_vfield(m, p, v);
+    double gamma = d_getG(v);
+    f.gamma = gamma;
+    /* fields with a power law in |p| / pc share log(|p| / pc) */
+    double rr = 0.0, lr_h = 0.0, lr_l = 0.0;
+    int need_r = (m.bfield == JSX_B_RANDOM_SCALAR || m.bfield == JSX_B_TOROIDAL_R || m.bfield == JSX_B_HELICAL_R ||
+                  m.nfield == JSX_N_BK);
+    if (need_r) {
+        rr = d_norm3(p) / JSX_PC;
+        if (rr > 0.0 && rr < 1.79769313486231570815e+308) lr_h = jsx_log_dd(rr, &lr_l);
+    }
+    /* B field, plasma frame (BField.cpp:35-146,196-224) */
+    double bl[3];
+    f.vec = 1;
+    switch (m.bfield) {
+    case JSX_B_RANDOM_SCALAR:
+        f.b[0] = m.b0 * d_pow_shared(rr, lr_h, lr_l, m.b_negexp);
+        f.vec = 0;
+        break;
+    case JSX_B_RANDOM_SCALAR_Z: {
+        double r = fabs(p[2]);
+        f.b[0] = m.b0 * jsx_pow(r / JSX_PC, m.b_negexp);
+        f.vec = 0;
+        break;
+    }
+    case JSX_B_GRID_SCALAR:
+        f.b[0] = m.has_gb ? d_grid_interp(m.gb, p) : 0.0;
+        f.vec = 0;
+        break;
+    case JSX_B_TOROIDAL: {
+        double sf, cf;
+        jsx_sincos_atan(p[1] / p[0], &sf, &cf);
+        double bb = m.b0 * jsx_pow(p[2] / JSX_PC, m.b_negexp);
+        bl[0] = -sf * bb; bl[1] = cf * bb; bl[2] = 0;
+        break;
+    }
+    case JSX_B_TOROIDAL_R:
+    case JSX_B_HELICAL_R: {
+        double bb = m.b0 * d_pow_shared(rr, lr_h, lr_l, m.b_negexp);
+        double rho = sqrt(p[0] * p[0] + p[1] * p[1]);
+        double ex = -p[1] / rho, ey = p[0] / rho;
+        if (m.bfield == JSX_B_TOROIDAL_R) {
+            bl[0] = bb * ex; bl[1] = bb * ey; bl[2] = 0;
+        } else {
+            double bphi = bb * m.sin_pitch;
+            bl[0] = bphi * ex; bl[1] = bphi * ey; bl[2] = bb * m.cos_pitch;
+        }
+        break;
+    }
+    case JSX_B_CONST_CYLINDER_Z:
+        bl[0] = 0.0; bl[1] = 0.0; bl[2] = m.b0 * jsx_pow(fabs(p[2]) / JSX_PC, m.b_negexp);
+        break;
+    case JSX_B_GRID_VECTOR: {
+        double sf, cf;
+        jsx_sincos_atan(p[1] / p[0], &sf, &cf);
+        double vp = m.has_gb ? d_grid_interp(m.gb, p) : 0.0;
+        double vf = m.has_gb2 ? d_grid_interp(m.gb2, p) : 0.0;
+        bl[0] = -sf * vf; bl[1] = cf * vf; bl[2] = vp;
+        break;
+    }
+    default:
+        f.b[0] = 0;
+        f.vec = 0;
+    }
+    if (f.vec) {
+        if (m.b_plasma) {
+            f.b[0] = bl[0]; f.b[1] = bl[1]; f.b[2] = bl[2];
+        } else { /* getBprime (utils.cpp:164-168) */
+            double s1 = gamma / ((1. + gamma) * (JSX_CL * JSX_CL));
+            double vb = d_dot3(v, bl);
+            for (int i = 0; i < 3; ++i) f.b[i] = bl[i] / gamma + (s1 * v[i]) * vb;
+        }
+    }
+    /* getD (utils.cpp:152-155) */
+    double beta[3] = {v[0] / JSX_CL, v[1] / JSX_CL, v[2] / JSX_CL};
+    double D = 1. / (gamma * (1. - d_dot3(beta, n_los)));
+    f.D = D;
+    /* NField (NField.cpp:7-59) */
+    double n;
+    switch (m.nfield) {
+    case JSX_N_CONST: n = m.n0; break;
+    case JSX_N_BK: n = m.n0 * d_pow_shared(rr, lr_h, lr_l, m.n_negexp); break;
+    case JSX_N_BK_Z: n = m.n0 * jsx_pow(fabs(p[2]) / JSX_PC, m.n_negexp); break;
+    case JSX_N_GRID: n = m.has_gn ? d_grid_interp(m.gn, p) : 0.0; break;
+    default: n = 0;
+    }
+    f.n = m.n_plasma ? n : n / gamma;
+    /* get_n_los_prime (utils.cpp:157-162) */
+    double fct = (D + 1.) * (gamma / (gamma + 1.));
+    for (int i = 0; i < 3; ++i) f.nl[i] = D * n_los[i] - fct * beta[i];
+    f.nu = nu / D;
+}
+
+/* lab-frame transfer coefficients (sky Stokes frame for Q/U) */
+struct DCoef { double kI, kQ, kU, kV, kF, kC, hQ, eI, eQ, eU, eV; };
+
+template <int FULL>
+__device__ inline void d_coefficients(const DModel &m, const double *p, const double *n_los, double nu, DCoef &o) {
+    DFrame f;
+    d_frame(m, p, n_los, nu, f);
+    double D = f.D, n = f.n, fnu = f.nu;
+    double nup = sqrt(n * JSX_QE * JSX_QE / (JSX_PI * JSX_ME));
+    if (!f.vec) {
+        double bs = f.b[0];
+        double nub = JSX_QE * bs / (2. * JSX_PI * JSX_ME * JSX_CL);
+        double x = nub / fnu;
+        double lh = 0.0, ll = 0.0;
+        if (x > 0.0 && x < 1.79769313486231570815e+308) lh = jsx_log_dd(x, &ll);
+        double k0 = JSX_PI * nup * nup * nub / (JSX_CL * fnu * fnu);
+        double ki = k0 * d_pow_shared(x, lh, ll, m.e_ki) * m.f_ki_s;
+        double e0 = JSX_PI * nup * nup * nub * JSX_ME / JSX_CL;
+        double ei = e0 * d_pow_shared(x, lh, ll, m.e_ei) * m.f_ei_s;
+        o.kI = ki / D;
+        o.eI = ei * D * D;
+        o.kQ = o.kU = o.kV = o.kF = o.kC = o.hQ = o.eQ = o.eU = o.eV = 0.0;
+        return;
+    }
+    const double *b = f.b, *nl = f.nl;
+    double cr[3];
+    d_cross3(nl, b, cr);
+    double nub = JSX_QE * d_norm3(cr) / (2. * JSX_PI * JSX_ME * JSX_CL);
+    double x = nub / fnu;
+    double lh = 0.0, ll = 0.0;
+    if (x > 0.0 && x < 1.79769313486231570815e+308) lh = jsx_log_dd(x, &ll);
+    double k0 = JSX_PI * nup * nup * nub / (JSX_CL * fnu * fnu);
+    double ki = k0 * d_pow_shared(x, lh, ll, m.e_ki) * m.f_ki_v;
+    double e0 = JSX_PI * nup * nup * nub * JSX_ME / JSX_CL;
+    double ei = e0 * d_pow_shared(x, lh, ll, m.e_ei) * m.f_ei_v;
+    o.kI = ki / D;
+    o.eI = ei * D * D;
+    if (!FULL) return;
+    double kq = m.r_kq * ki;
+    double eq = m.r_eq * ei;
+    double bnorm = d_norm3(b);
+    double cos_theta = d_dot3(b, nl) / bnorm;
+    double nubv = JSX_QE * bnorm / (2. * JSX_PI * JSX_ME * JSX_CL);
+    double k0v = JSX_PI * nup * nup * nubv / (JSX_CL * fnu * fnu);
+    double kv = -k0v * cos_theta * d_pow_shared(x, lh, ll, m.e_kv) * m.f_kv;
+    double e0v = JSX_PI * nup * nup * nubv * JSX_ME / JSX_CL;
+    double ev = -e0v * cos_theta * d_pow_shared(x, lh, ll, m.e_ki) * m.f_ev;
+    double kFc = 2. * JSX_PI * nup * nup * nubv * cos_theta / (JSX_CL * fnu * fnu);
+    double kCc = -JSX_PI * nup * nup * nub * nub / (JSX_CL * fnu * fnu * fnu);
+    double kF = m.f_kFr * kFc;
+    double kC = m.kCr_pre * (m.kCr_gm - d_pow_shared(x, lh, ll, m.e_kCr)) * kCc;
+    if (m.thermal_fraction != 0.0) {
+        double nth = m.thermal_fraction * n;
+        double nupt = sqrt(nth * JSX_QE * JSX_QE / (JSX_PI * JSX_ME));
+        kF = kF + 2. * JSX_PI * nupt * nupt * nubv * cos_theta / (JSX_CL * fnu * fnu);
+        kC = kC + -JSX_PI * nupt * nupt * nub * nub / (JSX_CL * fnu * fnu * fnu);
+    }
+    const double *a = m.axis;
+    double an = d_dot3(a, nl);
+    double e1[3] = {a[0] - an * nl[0], a[1] - an * nl[1], a[2] - an * nl[2]};
+    double e2[3];
+    d_cross3(nl, a, e2);
+    double bx = d_dot3(b, e1), by = d_dot3(b, e2);
+    double q2 = bx * bx + by * by;
+    double c2 = 1.0, s2 = 0.0;
+    if (q2 > 0.0) { c2 = -(bx * bx - by * by) / q2; s2 = -(2.0 * bx * by) / q2; }
+    o.kQ = (kq * c2) / D;
+    o.kU = (kq * s2) / D;
+    o.kV = kv / D;
+    o.kF = kF / D;
+    o.kC = (kC * c2) / D;
+    o.hQ = -(kC * s2) / D;
+    o.eQ = (eq * c2) * D * D;
+    o.eU = (eq * s2) * D * D;
+    o.eV = ev * D * D;
+}
+
+#endif /* JSX_DEVICE_H */

@@ -1,0 +1,536 @@
+/*
+ * jsx_host.hip -- the C ABI (include/jetshow_amd.h): contexts, descriptor
+ * lowering, grids in HBM, image tables, kernel launches.
+ *
+ * Lowering evaluates every parameter-only constant with the C library in the
+ * same expression form the reference evaluates it on each call
+ * (src/utils.cpp:40-144, src/Cone.cpp:22,69-70, src/VField.cpp:28), so the
+ * kernels see exactly the values the reference would compute.
+ */
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "jsx_kernels.h"
+#include "jsx_mixed.h"
+
+#define JSX_OPT_PERSISTENT 0x100u   /* internal: persistent queue kernel        */
+
+struct GridRec {
+    jsx_grid_desc desc;
+    double *dev;
+    size_t bytes;
+};
+
+struct ImageRec {
+    double *along = nullptr, *across_half = nullptr;
+    float *along_f = nullptr, *across_half_f = nullptr;
+    int64_t na = 0, nl = 0;
+    size_t bytes = 0;
+};
+
+struct jsx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t budget = 0, used = 0;
+    std::string err;
+    std::mutex mu;
+    std::map<int32_t, GridRec> grids;
+    int32_t next_grid = 1;
+    std::map<std::vector<double>, ImageRec> images;
+    unsigned long long *d_head = nullptr;
+    int n_cu = 256;
+    int persist_blocks_per_cu = 4;
+    int persist_block = 256;
+};
+
+static int fail(jsx_ctx *ctx, int code, const char *fmt, ...) {
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return code;
+}
+
+#define HIPCHECK(ctx, call)                                                                       \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail((ctx), JSX_EHIP, "%s failed: %s", #call, hipGetErrorString(e_));          \
+    } while (0)
+
+extern "C" int jsx_abi_version(void) { return JSX_ABI_VERSION; }
+
+extern "C" const char *jsx_last_error(jsx_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+extern "C" int jsx_ctx_create(int device, size_t hbm_budget, jsx_ctx **out) {
+    if (!out) return JSX_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return JSX_ENODEV;
+    if (device < 0 || device >= ndev) return JSX_EINVAL;
+    jsx_ctx *ctx = new jsx_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess) { delete ctx; return JSX_ENODEV; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->n_cu = prop.multiProcessorCount;
+    size_t free_b = 0, total_b = 0;
+    (void)hipMemGetInfo(&free_b, &total_b);
+    ctx->budget = hbm_budget ? hbm_budget : (size_t)(0.9 * (double)free_b);
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return JSX_EHIP; }
+    if (hipMalloc(&ctx->d_head, 64 * sizeof(unsigned long long)) != hipSuccess) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return JSX_ENOMEM;
+    }
+    *out = ctx;
+    return JSX_OK;
+}
+
+extern "C" int jsx_ctx_destroy(jsx_ctx *ctx) {
+    if (!ctx) return JSX_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &kv : ctx->grids) (void)hipFree(kv.second.dev);
+    for (auto &kv : ctx->images) {
+        (void)hipFree(kv.second.along); (void)hipFree(kv.second.across_half);
+        (void)hipFree(kv.second.along_f); (void)hipFree(kv.second.across_half_f);
+    }
+    (void)hipFree(ctx->d_head);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return JSX_OK;
+}
+
+/* ---------------- grids ---------------- */
+extern "C" int jsx_grid_upload(jsx_ctx *ctx, const jsx_grid_desc *desc, const double *values, int32_t *handle_out) {
+    if (!ctx || !desc || !values || !handle_out) return fail(ctx, JSX_EINVAL, "null argument");
+    if (desc->nz < 2 || desc->nr < 2 || !(desc->dz > 0) || !(desc->dr > 0))
+        return fail(ctx, JSX_EINVAL, "grid needs nz, nr >= 2 and positive spacing");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    size_t bytes = sizeof(double) * (size_t)desc->nz * (size_t)desc->nr;
+    if (ctx->used + bytes > ctx->budget) return fail(ctx, JSX_EBUDGET, "grid of %zu bytes exceeds HBM budget", bytes);
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    GridRec r;
+    r.desc = *desc;
+    r.bytes = bytes;
+    HIPCHECK(ctx, hipMalloc(&r.dev, bytes));
+    HIPCHECK(ctx, hipMemcpy(r.dev, values, bytes, hipMemcpyHostToDevice));
+    ctx->used += bytes;
+    int32_t h = ctx->next_grid++;
+    ctx->grids[h] = r;
+    *handle_out = h;
+    return JSX_OK;
+}
+
+extern "C" int jsx_grid_release(jsx_ctx *ctx, int32_t handle) {
+    if (!ctx) return JSX_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->grids.find(handle);
+    if (it == ctx->grids.end()) return fail(ctx, JSX_ENOTFOUND, "unknown grid handle %d", handle);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(it->second.dev);
+    ctx->used -= it->second.bytes;
+    ctx->grids.erase(it);
+    return JSX_OK;
+}
+
+/* ---------------- lowering ---------------- */
+static void h_normalize3(double *a) {
+    double z = (a[0] * a[0] + a[1] * a[1]) + a[2] * a[2];
+    if (z > 0.0) { double r = sqrt(z); a[0] /= r; a[1] /= r; a[2] /= r; }
+}
+
+static int lower_grid(jsx_ctx *ctx, int32_t h, DGrid &g, int &has) {
+    has = 0;
+    if (h == 0) return JSX_OK;
+    auto it = ctx->grids.find(h);
+    if (it == ctx->grids.end()) return fail(ctx, JSX_ENOTFOUND, "unknown grid handle %d", h);
+    const jsx_grid_desc &d = it->second.desc;
+    g.v = it->second.dev;
+    g.nz = d.nz; g.nr = d.nr;
+    g.z0 = d.z0; g.dz = d.dz; g.r0 = d.r0; g.dr = d.dr; g.nan_value = d.nan_value;
+    g.zlast = d.z0 + (double)(d.nz - 1) * d.dz;
+    g.rlast = d.r0 + (double)(d.nr - 1) * d.dr;
+    has = 1;
+    return JSX_OK;
+}
+
+static int lower_model(jsx_ctx *ctx, const jsx_jet_desc &j, unsigned opt, DModel &m) {
+    std::memset(&m, 0, sizeof m);
+    if (j.geometry < JSX_GEOM_CONE || j.geometry > JSX_GEOM_CYLINDER)
+        return fail(ctx, JSX_EUNSUPPORTED, "geometry kind %d", j.geometry);
+    if (j.bfield < JSX_B_RANDOM_SCALAR || j.bfield > JSX_B_CONST_CYLINDER_Z)
+        return fail(ctx, JSX_EUNSUPPORTED, "bfield kind %d", j.bfield);
+    if (j.nfield < JSX_N_CONST || j.nfield > JSX_N_GRID) return fail(ctx, JSX_EUNSUPPORTED, "nfield kind %d", j.nfield);
+    if (j.vfield < JSX_V_CONST_FLAT || j.vfield > JSX_V_GRID) return fail(ctx, JSX_EUNSUPPORTED, "vfield kind %d", j.vfield);
+    m.geometry = j.geometry; m.bfield = j.bfield; m.nfield = j.nfield; m.vfield = j.vfield;
+    m.b_plasma = j.b_in_plasma_frame; m.n_plasma = j.n_in_plasma_frame;
+    m.opt = (int)opt;
+    for (int i = 0; i < 3; ++i) { m.origin[i] = j.geo_origin[i]; m.axis[i] = j.geo_axis[i]; m.v_origin[i] = j.v_origin[i]; }
+    h_normalize3(m.axis);
+    m.cos_angle = cos(j.geo_param);
+    double cc = cos(j.geo_param) * cos(j.geo_param);
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) m.M[r][c] = m.axis[r] * m.axis[c] - cc * (r == c ? 1.0 : 0.0);
+    m.geo_param = j.geo_param;
+    m.big_scale = j.geometry == JSX_GEOM_CYLINDER ? 100. * j.geo_param : j.geo_big_scale;
+    m.clip = j.z_clip_max > j.z_clip_min;
+    m.zmin = j.z_clip_min; m.zmax = j.z_clip_max;
+    m.b0 = j.b0; m.b_negexp = -j.b_exp;
+    m.sin_pitch = sin(j.b_pitch); m.cos_pitch = cos(j.b_pitch);
+    m.n0 = j.n0; m.n_negexp = -j.n_exp;
+    m.thermal_fraction = j.thermal_fraction;
+    m.gamma0 = j.gamma0; m.gamma1 = j.gamma1; m.gz0 = j.gamma_z0; m.gz1 = j.gamma_z1;
+    m.vz_flat = 29979245800.0 * sqrt(1. - 1. / (j.gamma0 * j.gamma0));
+    const double s = j.s, PI = 3.141592653589793238462643383279502884;
+    m.f_ki_v = (pow(3., (s + 1.) / 2.) / 4.) * tgamma(s / 4. + 11. / 6.) * tgamma(s / 4. + 1. / 6.);
+    m.f_ki_s = m.f_ki_v * (sqrt(PI / 4.) * tgamma((6. + s) / 4.) / tgamma((8. + s) / 4.));
+    m.f_ei_v = pow(3., s / 2.) / (2. * (s + 1)) * tgamma(s / 4. + 19. / 12.) * tgamma(s / 4. - 1. / 12.);
+    m.f_ei_s = m.f_ei_v * (sqrt(PI / 4.) * tgamma((5. + s) / 4.) / tgamma((7. + s) / 4.));
+    m.r_kq = (s + 2.) / (s + 10. / 3);
+    m.r_eq = (s + 1.0) / (s + 7. / 3.);
+    m.f_kv = pow(3., s / 2.) * (s + 3.) * (s + 2.) / (4. * (s + 1.)) * tgamma(s / 4. + 11. / 12.) * tgamma(s / 4. + 7. / 12.);
+    m.f_ev = pow(3., (s - 1.) / 2.) * (s + 2.) / (2. * s) * tgamma(s / 4. + 2. / 3.) * tgamma(s / 4. + 1. / 3.);
+    const double gm = j.gamma_min;
+    m.f_kFr = (s + 2.) * log(gm) / ((s + 1.) * pow(gm, s + 1.));
+    m.kCr_pre = 2. / (s - 2.);
+    m.kCr_gm = pow(gm, 2. - s);
+    m.e_ki = s / 2.;
+    m.e_ei = (s - 1.) / 2.;
+    m.e_kv = (s + 1.) / 2.;
+    m.e_kCr = (s - 2.) / 2.;
+    int rc;
+    if (j.bfield == JSX_B_GRID_SCALAR || j.bfield == JSX_B_GRID_VECTOR) {
+        if ((rc = lower_grid(ctx, j.grid_b, m.gb, m.has_gb))) return rc;
+        if (j.bfield == JSX_B_GRID_VECTOR && (rc = lower_grid(ctx, j.grid_b2, m.gb2, m.has_gb2))) return rc;
+    }
+    if (j.nfield == JSX_N_GRID && (rc = lower_grid(ctx, j.grid_n, m.gn, m.has_gn))) return rc;
+    if (j.vfield == JSX_V_GRID && (rc = lower_grid(ctx, j.grid_v, m.gv, m.has_gv))) return rc;
+    return JSX_OK;
+}
+
+static int lower_solver(jsx_ctx *ctx, const jsx_solver_params &p, DSolver &S) {
+    if (!(p.tau_max > 0)) return fail(ctx, JSX_EINVAL, "tau_max must be > 0");
+    if (p.n < 1) return fail(ctx, JSX_EINVAL, "n must be >= 1");
+    if (!(p.nu > 0)) return fail(ctx, JSX_EINVAL, "nu must be > 0");
+    if (p.stokes != JSX_STOKES_I && p.stokes != JSX_STOKES_FULL) return fail(ctx, JSX_EINVAL, "stokes must be 1 or 4");
+    if (p.precision != JSX_PREC_FP64 && p.precision != JSX_PREC_MIXED)
+        return fail(ctx, JSX_EINVAL, "precision must be 0 or 1");
+    if (!(p.tau_n_min > 0) || !(p.tau_n_max > p.tau_n_min)) return fail(ctx, JSX_EINVAL, "bad steps_schedule thresholds");
+    std::memset(&S, 0, sizeof S);
+    S.nu = p.nu; S.tau_max = p.tau_max; S.tau_min = p.tau_min; S.dt_max = p.dt_max;
+    S.tau_abs = p.tau_abs_tol; S.tau_rel = p.tau_rel_tol; S.st_abs = p.stokes_abs_tol; S.st_rel = p.stokes_rel_tol;
+    S.tau_prec = p.tau_precision;
+    S.tau_n_min = p.tau_n_min; S.tau_n_max = p.tau_n_max;
+    S.n = p.n;
+    S.n_max = p.n_max_factor * p.n;
+    S.max_fail = p.max_failed_tries;
+    S.stokes = p.stokes;
+    S.sched_b = -log(p.tau_n_min);
+    S.sched_k = S.n_max / pow((log(p.tau_n_max) + S.sched_b), 2.0);
+    S.emin = pow(5.0, -5.0);
+    return JSX_OK;
+}
+
+/* image plane tables: include/logspace.h:25-33, src/Image.cpp:56-126 */
+static int image_tables(const jsx_image_desc &im, std::vector<double> &size_along, std::vector<double> &along,
+                        std::vector<double> &across_half) {
+    if (im.n_across <= 0 || im.n_along <= 0 || (im.n_across % 2) != 0) return JSX_EINVAL;
+    int64_t na = im.n_across, nl = im.n_along, h = na / 2;
+    size_along.resize(nl);
+    along.resize(nl);
+    across_half.resize((size_t)(h * nl));
+    double cur = pow(10, im.lg_pixel_size_start_cm);
+    double base = pow(10, (im.lg_pixel_size_stop_cm - im.lg_pixel_size_start_cm) / (int)(nl - 1));
+    for (int64_t j = 0; j < nl; ++j) { size_along[j] = cur; cur *= base; }
+    double acc = 0.0;
+    for (int64_t j = 0; j < nl; ++j) {
+        acc = j == 0 ? size_along[0] : acc + size_along[j];
+        along[j] = acc - 0.5 * size_along[j];
+    }
+    for (int64_t col = 0; col < nl; ++col) {
+        double sz = size_along[col] * 1.0;
+        double c = 0.0;
+        for (int64_t k = 0; k < h; ++k) {
+            c = k == 0 ? sz : c + sz;
+            across_half[k * nl + col] = c - 0.5 * sz;
+        }
+    }
+    return JSX_OK;
+}
+
+extern "C" int jsx_image_pixel_sizes(const jsx_image_desc *image, double *sizes) {
+    if (!image || !sizes) return JSX_EINVAL;
+    std::vector<double> sa, al, ah;
+    int rc = image_tables(*image, sa, al, ah);
+    if (rc) return rc;
+    for (int64_t i = 0; i < image->n_across; ++i)
+        for (int64_t j = 0; j < image->n_along; ++j) sizes[i * image->n_along + j] = sa[j];
+    return JSX_OK;
+}
+
+static int lower_image(jsx_ctx *ctx, const jsx_image_desc &im, DImage &D, ImageRec **rec_out) {
+    std::vector<double> key = {(double)im.n_across, (double)im.n_along, im.lg_pixel_size_start_cm,
+                               im.lg_pixel_size_stop_cm};
+    auto it = ctx->images.find(key);
+    ImageRec *rec;
+    if (it == ctx->images.end()) {
+        std::vector<double> sa, al, ah;
+        if (image_tables(im, sa, al, ah)) return fail(ctx, JSX_EINVAL, "image needs n_along > 0 and even n_across > 0");
+        ImageRec r;
+        r.na = im.n_across; r.nl = im.n_along;
+        r.bytes = sizeof(double) * (al.size() + ah.size());
+        HIPCHECK(ctx, hipMalloc(&r.along, sizeof(double) * al.size()));
+        HIPCHECK(ctx, hipMalloc(&r.across_half, sizeof(double) * (ah.size() ? ah.size() : 1)));
+        HIPCHECK(ctx, hipMemcpy(r.along, al.data(), sizeof(double) * al.size(), hipMemcpyHostToDevice));
+        if (ah.size()) HIPCHECK(ctx, hipMemcpy(r.across_half, ah.data(), sizeof(double) * ah.size(), hipMemcpyHostToDevice));
+        std::vector<float> alf(al.begin(), al.end()), ahf(ah.begin(), ah.end());
+        HIPCHECK(ctx, hipMalloc(&r.along_f, sizeof(float) * alf.size()));
+        HIPCHECK(ctx, hipMalloc(&r.across_half_f, sizeof(float) * (ahf.size() ? ahf.size() : 1)));
+        HIPCHECK(ctx, hipMemcpy(r.along_f, alf.data(), sizeof(float) * alf.size(), hipMemcpyHostToDevice));
+        if (ahf.size()) HIPCHECK(ctx, hipMemcpy(r.across_half_f, ahf.data(), sizeof(float) * ahf.size(), hipMemcpyHostToDevice));
+        rec = &(ctx->images[key] = r);
+    } else {
+        rec = &it->second;
+    }
+    D.along = rec->along;
+    D.across_half = rec->across_half;
+    D.na = im.n_across; D.nl = im.n_along;
+    double raw[3] = {-sin(im.los_angle), 0, -cos(im.los_angle)};
+    h_normalize3(raw);
+    for (int i = 0; i < 3; ++i) { D.dir[i] = raw[i]; D.d2[i] = raw[i]; }
+    h_normalize3(D.d2);
+    D.zscale = 1. / sin(im.los_angle);
+    if (rec_out) *rec_out = rec;
+    return JSX_OK;
+}
+
+static DOut lower_out(const jsx_outputs &o) {
+    DOut d;
+    d.tau = o.tau; d.I = o.I; d.Q = o.Q; d.U = o.U; d.V = o.V; d.l = o.l;
+    d.flags = o.flags; d.n_rhs = o.n_rhs; d.n_steps = o.n_steps;
+    return d;
+}
+
+/* ---------------- observe ---------------- */
+static int observe_locked(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_image_desc *image,
+                          const jsx_solver_params *params, const jsx_outputs *out, hipStream_t st) {
+    if (!jet || !image || !params || !out) return fail(ctx, JSX_EINVAL, "null argument");
+    if (!out->I) return fail(ctx, JSX_EINVAL, "outputs.I must be set");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    DModel m;
+    DSolver S;
+    DImage D;
+    ImageRec *rec = nullptr;
+    int rc;
+    if ((rc = lower_model(ctx, *jet, params->flags, m))) return rc;
+    if ((rc = lower_solver(ctx, *params, S))) return rc;
+    if ((rc = lower_image(ctx, *image, D, &rec))) return rc;
+    DOut O = lower_out(*out);
+    int64_t npix = image->n_across * image->n_along;
+    if (npix == 0) return JSX_OK;
+    if (params->precision == JSX_PREC_MIXED) {
+        return jsx_launch_mixed(ctx->n_cu, ctx->d_head, m, D, rec->along_f, rec->across_half_f, S, O, npix, st,
+                                params->flags) == hipSuccess
+                   ? JSX_OK
+                   : fail(ctx, JSX_EHIP, "mixed-precision launch failed: %s", hipGetErrorString(hipGetLastError()));
+    }
+    const bool full = params->stokes == JSX_STOKES_FULL;
+    if (params->flags & JSX_OPT_PERSISTENT) {
+        HIPCHECK(ctx, hipMemsetAsync(ctx->d_head, 0, sizeof(unsigned long long), st));
+        int blocks = ctx->n_cu * ctx->persist_blocks_per_cu;
+        int order = (params->flags & 0x200u) ? 1 : 0;
+        if (full)
+            hipLaunchKernelGGL(jsx_k_persist<4>, dim3(blocks), dim3(ctx->persist_block), 0, st, m, D, S, O, npix,
+                               ctx->d_head, order);
+        else
+            hipLaunchKernelGGL(jsx_k_persist<1>, dim3(blocks), dim3(ctx->persist_block), 0, st, m, D, S, O, npix,
+                               ctx->d_head, order);
+    } else {
+        int threads = 256;
+        int64_t blocks = (npix + threads - 1) / threads;
+        if (full)
+            hipLaunchKernelGGL(jsx_k_plain<4>, dim3((unsigned)blocks), dim3(threads), 0, st, m, D, S, O, npix);
+        else
+            hipLaunchKernelGGL(jsx_k_plain<1>, dim3((unsigned)blocks), dim3(threads), 0, st, m, D, S, O, npix);
+    }
+    HIPCHECK(ctx, hipGetLastError());
+    return JSX_OK;
+}
+
+extern "C" int jsx_observe(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_image_desc *image,
+                           const jsx_solver_params *params, const jsx_outputs *out_device, void *stream) {
+    if (!ctx) return JSX_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    return observe_locked(ctx, jet, image, params, out_device, st);
+}
+
+extern "C" int jsx_alloc_outputs(jsx_ctx *ctx, int64_t n, int32_t stokes, jsx_outputs *o) {
+    if (!ctx || !o || n <= 0) return fail(ctx, JSX_EINVAL, "bad arguments");
+    std::memset(o, 0, sizeof *o);
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    size_t nd = (size_t)n * sizeof(double), nu = (size_t)n * sizeof(uint32_t);
+    if (hipMalloc(&o->tau, nd) != hipSuccess || hipMalloc(&o->I, nd) != hipSuccess || hipMalloc(&o->l, nd) != hipSuccess ||
+        hipMalloc(&o->flags, nu) != hipSuccess || hipMalloc(&o->n_rhs, nu) != hipSuccess ||
+        hipMalloc(&o->n_steps, nu) != hipSuccess) {
+        jsx_free_outputs(ctx, o);
+        return fail(ctx, JSX_ENOMEM, "output allocation of %lld pixels failed", (long long)n);
+    }
+    if (stokes == JSX_STOKES_FULL) {
+        if (hipMalloc(&o->Q, nd) != hipSuccess || hipMalloc(&o->U, nd) != hipSuccess || hipMalloc(&o->V, nd) != hipSuccess) {
+            jsx_free_outputs(ctx, o);
+            return fail(ctx, JSX_ENOMEM, "output allocation failed");
+        }
+    }
+    return JSX_OK;
+}
+
+extern "C" int jsx_free_outputs(jsx_ctx *ctx, jsx_outputs *o) {
+    if (!ctx || !o) return JSX_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    void *ps[] = {o->tau, o->I, o->Q, o->U, o->V, o->l, o->flags, o->n_rhs, o->n_steps};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    std::memset(o, 0, sizeof *o);
+    return JSX_OK;
+}
+
+extern "C" int jsx_observe_host(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_image_desc *image,
+                                const jsx_solver_params *params, const jsx_outputs *out_host) {
+    if (!ctx || !image || !params || !out_host) return JSX_EINVAL;
+    int64_t n = image->n_across * image->n_along;
+    if (n <= 0 || (image->n_across % 2)) return fail(ctx, JSX_EINVAL, "image needs n_along > 0 and even n_across > 0");
+    jsx_outputs d;
+    int rc = jsx_alloc_outputs(ctx, n, JSX_STOKES_FULL, &d);
+    if (rc) return rc;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        rc = observe_locked(ctx, jet, image, params, &d, ctx->stream);
+        if (rc == JSX_OK) {
+            hipError_t e = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess) rc = fail(ctx, JSX_EHIP, "kernel failed: %s", hipGetErrorString(e));
+        }
+    }
+    if (rc == JSX_OK) {
+        size_t nd = (size_t)n * sizeof(double), nu = (size_t)n * sizeof(uint32_t);
+        struct { void *h; void *dv; size_t b; } cp[] = {
+            {out_host->tau, d.tau, nd}, {out_host->I, d.I, nd}, {out_host->Q, d.Q, nd}, {out_host->U, d.U, nd},
+            {out_host->V, d.V, nd}, {out_host->l, d.l, nd}, {out_host->flags, d.flags, nu},
+            {out_host->n_rhs, d.n_rhs, nu}, {out_host->n_steps, d.n_steps, nu}};
+        for (auto &c : cp)
+            if (c.h && hipMemcpy(c.h, c.dv, c.b, hipMemcpyDeviceToHost) != hipSuccess) {
+                rc = fail(ctx, JSX_EHIP, "copy back failed");
+                break;
+            }
+    }
+    jsx_free_outputs(ctx, &d);
+    return rc;
+}
+
+extern "C" int jsx_hit_host(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_solver_params *params, const double *rays,
+                            int64_t n, int32_t *n_seg, double *segments) {
+    if (!ctx || !jet || !rays || !n_seg || !segments || n <= 0) return fail(ctx, JSX_EINVAL, "bad arguments");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    DModel m;
+    int rc = lower_model(ctx, *jet, params ? params->flags : 0, m);
+    if (rc) return rc;
+    double *d_rays = nullptr, *d_segs = nullptr;
+    int32_t *d_ns = nullptr;
+    HIPCHECK(ctx, hipMalloc(&d_rays, sizeof(double) * 6 * n));
+    HIPCHECK(ctx, hipMalloc(&d_segs, sizeof(double) * 12 * n));
+    HIPCHECK(ctx, hipMalloc(&d_ns, sizeof(int32_t) * n));
+    HIPCHECK(ctx, hipMemcpy(d_rays, rays, sizeof(double) * 6 * n, hipMemcpyHostToDevice));
+    int threads = 256;
+    hipLaunchKernelGGL(jsx_k_hit, dim3((unsigned)((n + threads - 1) / threads)), dim3(threads), 0, ctx->stream, m,
+                       d_rays, n, d_ns, d_segs);
+    HIPCHECK(ctx, hipGetLastError());
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHECK(ctx, hipMemcpy(n_seg, d_ns, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    HIPCHECK(ctx, hipMemcpy(segments, d_segs, sizeof(double) * 12 * n, hipMemcpyDeviceToHost));
+    (void)hipFree(d_rays); (void)hipFree(d_segs); (void)hipFree(d_ns);
+    return JSX_OK;
+}
+
+/* ---------------- cosmology (host, once per image) ----------------
+ * comoving_transfer_distance2 + mas_to_pc (src/utils.cpp:186-208, 373-378):
+ * controlled Dormand-Prince (abs = rel = 1.49e-8, dt0 = 1e-9) over the Ctd
+ * integrand, with the same step controller the kernels use. */
+static double ctd(double t) {
+    const double omega_M = 0.2865, omega_V = 0.7134130719051658, gamma_nu = 8.69280948342326e-05;
+    return pow((omega_M + (1. + t) * gamma_nu) * (1. + t) * (1. + t) * (1. + t) + omega_V, -0.5);
+}
+
+extern "C" double jsx_mas_to_pc(double z) {
+    const double eps_abs = 1.49e-8, eps_rel = 1.49e-8, EPS = 2.220446049250313080847e-16;
+    double x = 0.0, t = 0.0, dt = 1E-09, k1 = 0.0;
+    int first = 1;
+    while (z - t > EPS) {
+        if ((t + dt) - z > EPS) dt = z - t;
+        if (first) { k1 = ctd(t); first = 0; }
+        for (int tries = 0; tries < 500; ++tries) {
+            double k3 = ctd(t + dt * DP_A3);
+            double k4 = ctd(t + dt * DP_A4);
+            double k5 = ctd(t + dt * DP_A5);
+            double k6 = ctd(t + dt);
+            double xn = x + dt * DP_C1 * k1 + dt * DP_C3 * k3 + dt * DP_C4 * k4 + dt * DP_C5 * k5 + dt * DP_C6 * k6;
+            double k7 = ctd(t + dt);
+            double xe = dt * DP_DC1 * k1 + dt * DP_DC3 * k3 + dt * DP_DC4 * k4 + dt * DP_DC5 * k5 + dt * DP_DC6 * k6 +
+                        dt * DP_DC7 * k7;
+            double err = fabs(xe) / (eps_abs + eps_rel * (fabs(x) + dt * fabs(k1)));
+            if (err > 1.0) {
+                double fac = 0.9 * pow(err, -1.0 / 3);
+                dt = dt * (fac > 0.2 ? fac : 0.2);
+                continue;
+            }
+            t = t + dt;
+            if (err < 0.5) {
+                double e2 = pow(5.0, -5.0) > err ? pow(5.0, -5.0) : err;
+                dt = dt * (0.9 * pow(e2, -1.0 / 5.0));
+            }
+            x = xn;
+            k1 = k7;
+            break;
+        }
+    }
+    const double H0 = 69.32;
+    double result = (299792.458 / H0) * pow(10., 6.) * x;
+    double d_a = result / (1. + z);
+    return 4.84813681109536e-09 * d_a;
+}
+
+/* ---------------- batch ---------------- */
+extern "C" int jsx_observe_batch(jsx_ctx *ctx, const jsx_request *reqs, int32_t n_reqs) {
+    if (!ctx || (!reqs && n_reqs)) return JSX_EINVAL;
+    for (int32_t i = 0; i < n_reqs; ++i) {
+        int rc = jsx_observe_host(ctx, &reqs[i].jet, &reqs[i].image, &reqs[i].params, &reqs[i].out_host);
+        if (rc) return rc;
+    }
+    return JSX_OK;
+}
+
+extern "C" size_t jsx_sizeof(int which) {
+    switch (which) {
+    case 0: return sizeof(jsx_jet_desc);
+    case 1: return sizeof(jsx_image_desc);
+    case 2: return sizeof(jsx_solver_params);
+    case 3: return sizeof(jsx_outputs);
+    case 4: return sizeof(jsx_grid_desc);
+    case 5: return sizeof(jsx_request);
+    default: return 0;
+    }
+}

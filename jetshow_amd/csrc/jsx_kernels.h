@@ -1,0 +1,104 @@
+/*
+ * jsx_kernels.h -- kernels for the FP64 path.
+ *
+ *   jsx_k_plain<NS>   : one thread per pixel, one pass per image (the plain
+ *                       form of Observation::run, src/Observation.cpp:20-41).
+ *   jsx_k_persist<NS> : persistent kernel; each lane that finishes (or misses)
+ *                       takes the next ray from a device ray queue, so the
+ *                       wavefront stays full while rays of very different
+ *                       step counts drain.  Ray ids are handed out per
+ *                       wavefront with one atomic per refill round.
+ */
+#ifndef JSX_KERNELS_H
+#define JSX_KERNELS_H
+
+#include "jsx_solver.h"
+
+/* ray id -> pixel: 0 row-major, 1 column-major (apex columns first) */
+__device__ inline int64_t d_ray_to_pixel(const DImage &im, int64_t id, int order) {
+    if (order == 1) {
+        int64_t col = id / im.na, row = id % im.na;
+        return row * im.nl + col;
+    }
+    return id;
+}
+
+template <int NS>
+__global__ __launch_bounds__(256) void jsx_k_plain(DModel m, DImage im, DSolver S, DOut out, int64_t npix) {
+    int64_t pix = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= npix) return;
+    Lane<NS> L;
+    lane_begin(L, m, im, S, pix);
+    while (L.phase != PH_DONE) {
+        double r[NS];
+        lane_rhs(L, m, im, S, r);
+        lane_consume(L, S, im, r);
+    }
+    lane_write(L, out, pix);
+}
+
+__device__ inline int d_lane_id() { return __lane_id(); }
+
+template <int NS>
+__global__ __launch_bounds__(256) void jsx_k_persist(DModel m, DImage im, DSolver S, DOut out, int64_t nrays,
+                                                     unsigned long long *head, int order) {
+    Lane<NS> L;
+    int have = 0, exhausted = 0;
+    const int lane = d_lane_id();
+    for (;;) {
+        int want = !have && !exhausted;
+        unsigned long long mask = __ballot(want);
+        if (mask) {
+            int cnt = __popcll(mask);
+            int leader = __ffsll((long long)mask) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(head, (unsigned long long)cnt);
+            unsigned lo = __shfl((unsigned)(base & 0xffffffffull), leader);
+            unsigned hi = __shfl((unsigned)(base >> 32), leader);
+            base = ((unsigned long long)hi << 32) | lo;
+            if (want) {
+                unsigned long long below = lane ? (mask & ((1ull << lane) - 1ull)) : 0ull;
+                int64_t id = (int64_t)(base + (unsigned long long)__popcll(below));
+                if (id < nrays) {
+                    int64_t pix = d_ray_to_pixel(im, id, order);
+                    lane_begin(L, m, im, S, pix);
+                    if (L.phase == PH_DONE) lane_write(L, out, pix);
+                    else have = 1;
+                } else {
+                    exhausted = 1;
+                }
+            }
+        }
+        if (!__any(have) && __all(exhausted)) break;
+        if (have) {
+            double r[NS];
+            lane_rhs(L, m, im, S, r);
+            lane_consume(L, S, im, r);
+            if (L.phase == PH_DONE) {
+                lane_write(L, out, L.pix);
+                have = 0;
+            }
+        }
+    }
+}
+
+/* Geometry::hit for explicit rays */
+__global__ void jsx_k_hit(DModel m, const double *rays, int64_t n, int32_t *n_seg, double *segs) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    double o[3] = {rays[6 * q], rays[6 * q + 1], rays[6 * q + 2]};
+    double d[3] = {rays[6 * q + 3], rays[6 * q + 4], rays[6 * q + 5]};
+    d_normalize3(d);
+    DSeg s[2];
+    int clipped;
+    int ns = d_model_hit(m, o, d, s, &clipped);
+    n_seg[q] = ns;
+    for (int k = 0; k < 3; ++k) {
+        segs[12 * q + k] = ns > 0 ? s[0].pin[k] : 0.0;
+        segs[12 * q + 3 + k] = ns > 0 ? s[0].pout[k] : 0.0;
+        segs[12 * q + 6 + k] = ns > 1 ? s[1].pin[k] : 0.0;
+        segs[12 * q + 9 + k] = ns > 1 ? s[1].pout[k] : 0.0;
+    }
+}
+
+#endif /* JSX_KERNELS_H */

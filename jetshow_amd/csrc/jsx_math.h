@@ -1,0 +1,196 @@
+/*
+ * jsx_math.h -- deterministic FP64 elementary functions shared by the HIP
+ * kernels and by the host.
+ *
+ * Why this exists: the per-pixel solver decides discrete things from
+ * transcendental results (accept/reject of a Dormand-Prince step, whether tau
+ * crossed tau_max, whether I is integrated at all).  glibc's libm and the
+ * GPU's device library give results that differ in the last ulp, so a CPU
+ * restatement and a GPU kernel built on different libraries can take
+ * different discrete paths on marginal rays.  Every function here uses only
+ * IEEE-754 operations that are correctly rounded on both x86-64 and gfx950
+ * (+, -, *, /, sqrt, fma, floor) in a fixed order, so the same inputs give the
+ * same bits on both sides.  Compile every user with -ffp-contract=off.
+ *
+ * Methods (no vendor library code is reproduced):
+ *   exp : Cody & Waite argument reduction x = k ln2 + r with ln2 split into a
+ *         32-bit head and a tail (Cody & Waite, "Software Manual for the
+ *         Elementary Functions", 1980), Taylor polynomial of degree 13 on
+ *         |r| <= ln2/2 evaluated by Horner's rule with fma, exact scaling by 2^k.
+ *   log : x = 2^k m, m in [sqrt(1/2), sqrt(2)), f = m - 1 (exact),
+ *         s = f/(2+f), log(1+f) = f - f^2/2 + s (f^2/2 + R(s^2)) with
+ *         R(z) = sum_{j>=1} 2 z^j/(2j+1) (the atanh series), result kept as a
+ *         double-double head/tail using Knuth's two-sum and an fma two-product.
+ *   pow : exp(y * log(x)) with the product formed in double-double, which
+ *         keeps the result within ~1 ulp over the ranges the solver uses.
+ *   sin(atan(u)), cos(atan(u)) : the algebraic identities u/sqrt(1+u^2),
+ *         1/sqrt(1+u^2) (scaled for |u| > 1), which is what the reference's
+ *         ToroidalBField / SimulationBField compute through atan/sin/cos.
+ */
+#ifndef JSX_MATH_H
+#define JSX_MATH_H
+
+#include <stdint.h>
+#include <math.h>
+
+#if defined(__HIPCC__)
+#define JSX_HD __host__ __device__ inline
+#else
+#define JSX_HD static inline
+#endif
+
+#define JSX_LN2_HI 6.93147180369123816490e-01   /* ln 2 head, 32 significant bits */
+#define JSX_LN2_LO 1.90821492927058770002e-10   /* ln 2 - JSX_LN2_HI */
+#define JSX_INV_LN2 1.44269504088896338700e+00
+#define JSX_SQRT2 1.41421356237309514547e+00
+#define JSX_EXP_OVF 7.09782712893383973096e+02  /* ln(DBL_MAX) */
+#define JSX_EXP_UNF -7.45133219101941108420e+02 /* ln(smallest subnormal / 2) */
+
+typedef union { double d; uint64_t u; } jsx_bits;
+
+JSX_HD double jsx_from_bits(uint64_t u) { jsx_bits b; b.u = u; return b.d; }
+JSX_HD uint64_t jsx_to_bits(double d) { jsx_bits b; b.d = d; return b.u; }
+
+/* 2^k for -1022 <= k <= 1023, exact. */
+JSX_HD double jsx_pow2i(int k) { return jsx_from_bits((uint64_t)(k + 1023) << 52); }
+
+/* exp(hi + lo) for |lo| <= ulp(hi). */
+JSX_HD double jsx_exp_dd(double hi, double lo) {
+    if (hi != hi) return hi;
+    if (hi > JSX_EXP_OVF) return jsx_from_bits(0x7ff0000000000000ULL);
+    if (hi < JSX_EXP_UNF) return 0.0;
+    double kd = floor(hi * JSX_INV_LN2 + 0.5);
+    double r = (hi - kd * JSX_LN2_HI) - kd * JSX_LN2_LO;
+    r = r + lo;
+    /* Taylor coefficients 1/j!, folded by the compiler with correct rounding */
+    double p = 1.0 / 6227020800.0;                 /* 1/13! */
+    p = fma(p, r, 1.0 / 479001600.0);               /* 1/12! */
+    p = fma(p, r, 1.0 / 39916800.0);
+    p = fma(p, r, 1.0 / 3628800.0);
+    p = fma(p, r, 1.0 / 362880.0);
+    p = fma(p, r, 1.0 / 40320.0);
+    p = fma(p, r, 1.0 / 5040.0);
+    p = fma(p, r, 1.0 / 720.0);
+    p = fma(p, r, 1.0 / 120.0);
+    p = fma(p, r, 1.0 / 24.0);
+    p = fma(p, r, 1.0 / 6.0);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    int k = (int)kd;
+    if (k >= -1021 && k <= 1023) return p * jsx_pow2i(k);
+    if (k > 1023) return (p * jsx_pow2i(1023)) * jsx_pow2i(k - 1023);
+    /* result is subnormal or close to it: scale in two exact-or-once-rounded steps */
+    return (p * jsx_pow2i(k + 1000)) * jsx_pow2i(-1000);
+}
+
+JSX_HD double jsx_exp(double x) { return jsx_exp_dd(x, 0.0); }
+
+/* log(x) as head + tail; *lo receives the tail.  x must be finite and > 0. */
+JSX_HD double jsx_log_dd(double x, double *lo) {
+    int kadj = 0;
+    if (x < 2.2250738585072014e-308) { x = x * 18014398509481984.0; kadj = -54; } /* 2^54 */
+    uint64_t u = jsx_to_bits(x);
+    int e = (int)((u >> 52) & 0x7ff) - 1023 + kadj;
+    double m = jsx_from_bits((u & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);
+    if (m > JSX_SQRT2) { m = m * 0.5; e = e + 1; }
+    double f = m - 1.0;                       /* exact (Sterbenz) */
+    double s = f / (2.0 + f);
+    double z = s * s;
+    double R = 2.0 / 21.0;
+    R = fma(R, z, 2.0 / 19.0);
+    R = fma(R, z, 2.0 / 17.0);
+    R = fma(R, z, 2.0 / 15.0);
+    R = fma(R, z, 2.0 / 13.0);
+    R = fma(R, z, 2.0 / 11.0);
+    R = fma(R, z, 2.0 / 9.0);
+    R = fma(R, z, 2.0 / 7.0);
+    R = fma(R, z, 2.0 / 5.0);
+    R = fma(R, z, 2.0 / 3.0);
+    R = R * z;
+    double fsq = f * f;
+    double fsq_lo = fma(f, f, -fsq);
+    double hfsq = 0.5 * fsq;
+    double hfsq_lo = 0.5 * fsq_lo;
+    double t = s * (hfsq + R);
+    double kd = (double)e;
+    double a = kd * JSX_LN2_HI;               /* exact: 32-bit head times |k| < 2^11 */
+    /* two-sum(a, f) */
+    double h1 = a + f;
+    double bb = h1 - a;
+    double l1 = (a - (h1 - bb)) + (f - bb);
+    /* two-sum(h1, -hfsq) */
+    double h2 = h1 - hfsq;
+    double bb2 = h2 - h1;
+    double l2 = (h1 - (h2 - bb2)) + (-hfsq - bb2);
+    double tail = ((l1 + l2) - hfsq_lo) + (t + kd * JSX_LN2_LO);
+    double hi = h2 + tail;
+    *lo = tail - (hi - h2);
+    return hi;
+}
+
+JSX_HD double jsx_log(double x) {
+    if (x != x) return x;
+    if (x < 0.0) return jsx_from_bits(0x7ff8000000000000ULL);
+    if (x == 0.0) return -jsx_from_bits(0x7ff0000000000000ULL);
+    if (x == jsx_from_bits(0x7ff0000000000000ULL)) return x;
+    double lo;
+    double hi = jsx_log_dd(x, &lo);
+    return hi + lo;
+}
+
+/* exp(y * (lh + ll)) where (lh, ll) = jsx_log_dd(x): lets callers that raise one
+ * base to several exponents share the logarithm and still get exactly the
+ * bits jsx_pow(x, y) returns. */
+JSX_HD double jsx_pow_from_log(double lh, double ll, double y) {
+    double ph = y * lh;
+    double pl = fma(y, lh, -ph) + y * ll;
+    return jsx_exp_dd(ph, pl);
+}
+
+/* True when jsx_pow(x, y) takes the generic exp(y log x) route, i.e. when
+ * jsx_pow_from_log gives the same bits. */
+JSX_HD int jsx_pow_is_generic(double x, double y) {
+    return x > 0.0 && x < 1.79769313486231570815e+308 && x == x && y == y &&
+           y != 0.0 && y != 1.0 && y != 2.0 && y != -1.0 && x != 1.0;
+}
+
+JSX_HD double jsx_pow(double x, double y) {
+    if (y == 0.0) return 1.0;
+    if (x == 1.0) return 1.0;
+    if (x != x || y != y) return x + y;
+    if (y == 1.0) return x;
+    if (y == 2.0) return x * x;
+    if (y == -1.0) return 1.0 / x;
+    const double inf = jsx_from_bits(0x7ff0000000000000ULL);
+    if (x == 0.0) return y < 0.0 ? inf : 0.0;
+    if (x < 0.0) {
+        if (floor(y) != y) return jsx_from_bits(0x7ff8000000000000ULL);
+        double h = 0.5 * y;
+        int odd = (floor(h) != h) && (y < 9007199254740992.0) && (y > -9007199254740992.0);
+        double r = jsx_pow(-x, y);
+        return odd ? -r : r;
+    }
+    if (x == inf) return y > 0.0 ? inf : 0.0;
+    double ll;
+    double lh = jsx_log_dd(x, &ll);
+    return jsx_pow_from_log(lh, ll, y);
+}
+
+/* sin(atan(u)) and cos(atan(u)) without the trigonometric detour. */
+JSX_HD void jsx_sincos_atan(double u, double *s, double *c) {
+    if (u != u) { *s = u; *c = u; return; }
+    double au = u < 0.0 ? -u : u;
+    if (au <= 1.0) {
+        double q = sqrt(1.0 + u * u);
+        *s = u / q;
+        *c = 1.0 / q;
+    } else {
+        double v = 1.0 / au;
+        double q = sqrt(1.0 + v * v);
+        *s = (u < 0.0 ? -1.0 : 1.0) / q;
+        *c = v / q;
+    }
+}
+
+#endif /* JSX_MATH_H */

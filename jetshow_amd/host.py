@@ -1,0 +1,133 @@
+"""Python host side of the solver: loads libjetshow_amd.so (the HIP build) and
+wraps its C ABI with numpy buffers.
+
+There is no CPU fallback: if the shared library is missing or no GPU is
+visible, every compute call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libjetshow_amd.so")
+_lib = None
+
+# exported symbols of include/jetshow_amd.h
+ABI_SYMBOLS = [
+    "jsx_ctx_create", "jsx_ctx_destroy", "jsx_last_error", "jsx_abi_version",
+    "jsx_grid_upload", "jsx_grid_release", "jsx_alloc_outputs", "jsx_free_outputs",
+    "jsx_observe", "jsx_observe_host", "jsx_hit_host", "jsx_image_pixel_sizes",
+    "jsx_mas_to_pc", "jsx_observe_batch",
+]
+
+
+class JsxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (abi.STATUS_NAMES.get(code, str(code)), msg))
+        self.code = code
+
+
+def lib():
+    """The loaded HIP library; raises when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("%s is missing: build it with __graft_entry__.build()" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        L.jsx_last_error.restype = C.c_char_p
+        L.jsx_mas_to_pc.restype = C.c_double
+        L.jsx_mas_to_pc.argtypes = [C.c_double]
+        L.jsx_sizeof.restype = C.c_size_t
+        L.jsx_ctx_create.argtypes = [C.c_int, C.c_size_t, C.POINTER(C.c_void_p)]
+        for name in ("jsx_ctx_destroy",):
+            getattr(L, name).argtypes = [C.c_void_p]
+        L.jsx_last_error.argtypes = [C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def mas_to_pc(redshift):
+    """Parsecs per milliarcsecond at `redshift` (src/utils.cpp:205-208)."""
+    return lib().jsx_mas_to_pc(float(redshift))
+
+
+def pixel_sizes(image):
+    """Pixel sizes [n_across, n_along] in cm (Image::getPixelSizes)."""
+    out = np.zeros((image.n_across, image.n_along))
+    rc = lib().jsx_image_pixel_sizes(C.byref(image), out.ctypes.data_as(C.c_void_p))
+    if rc:
+        raise JsxError(rc, "bad image description")
+    return out
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class Context:
+    """A solver context on one GPU (jsx_ctx)."""
+
+    def __init__(self, device=0, hbm_budget=0):
+        self._h = C.c_void_p()
+        rc = lib().jsx_ctx_create(int(device), C.c_size_t(hbm_budget), C.byref(self._h))
+        if rc:
+            raise JsxError(rc, "jsx_ctx_create(device=%d) failed (no GPU visible?)" % device)
+
+    def _check(self, rc):
+        if rc:
+            raise JsxError(rc, lib().jsx_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            lib().jsx_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def upload_grid(self, values, z0, dz, r0, dr, nan_value=0.0):
+        """Copy a [nz, nr] node grid (parsec coordinates) to HBM; returns its handle."""
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        g = abi.GridDesc()
+        g.nz, g.nr = v.shape
+        g.z0, g.dz, g.r0, g.dr, g.nan_value = z0, dz, r0, dr, nan_value
+        h = C.c_int32()
+        self._check(lib().jsx_grid_upload(self._h, C.byref(g), _ptr(v), C.byref(h)))
+        return h.value
+
+    def release_grid(self, handle):
+        self._check(lib().jsx_grid_release(self._h, C.c_int32(handle)))
+
+    def observe(self, jet, image, params):
+        """Run the hot path for one image; returns numpy planes [n_across, n_along]."""
+        shape = (image.n_across, image.n_along)
+        res = {k: np.zeros(shape) for k in ("tau", "I", "Q", "U", "V", "l")}
+        for k in ("flags", "n_rhs", "n_steps"):
+            res[k] = np.zeros(shape, dtype=np.uint32)
+        o = abi.Outputs()
+        for k in ("tau", "I", "Q", "U", "V", "l"):
+            setattr(o, k, res[k].ctypes.data_as(C.POINTER(C.c_double)))
+        for k in ("flags", "n_rhs", "n_steps"):
+            setattr(o, k, res[k].ctypes.data_as(C.POINTER(C.c_uint32)))
+        self._check(lib().jsx_observe_host(self._h, C.byref(jet), C.byref(image), C.byref(params), C.byref(o)))
+        return res
+
+    def hit(self, jet, rays, params=None):
+        """Geometry::hit for rays [n, 6]; returns (n_seg [n], segments [n, 2, 6])."""
+        r = np.ascontiguousarray(rays, dtype=np.float64).reshape(-1, 6)
+        n = r.shape[0]
+        ns = np.zeros(n, dtype=np.int32)
+        seg = np.zeros((n, 2, 6))
+        self._check(lib().jsx_hit_host(self._h, C.byref(jet), C.byref(params) if params is not None else None,
+                                       _ptr(r), C.c_int64(n), _ptr(ns), _ptr(seg)))
+        return ns, seg

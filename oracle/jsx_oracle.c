@@ -1,0 +1,1293 @@
+/*
+ * jsx_oracle.c -- CPU restatement of jetshow's per-pixel radiative transfer.
+ *
+ * TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+ * CPU-baseline leg of bench.py as the checker.  The product (jetshow_amd/)
+ * never links or calls it.
+ *
+ * Each function cites the reference file:line it follows.  Evaluation order
+ * of every floating-point expression follows the reference's C++ (left to
+ * right, Eigen 3-vector reductions as (x0 + x1) + x2), because the HIP
+ * kernels are required to reproduce these bits.  Boost.odeint is not vendored
+ * in the reference; the Dormand-Prince stepper, controller, dense output and
+ * adaptive range below restate odeint's published algorithm (see the
+ * "Dormand-Prince" section).
+ *
+ * Parity pins (see tests/test_oracle_*.py): Cone known answers from
+ * tests/simple_test.cpp, closed-form quadric roots, analytic tau for a
+ * power-law along a straight ray, the uniform slab I = S (1 - e^{-kL}), the
+ * matrix exponential of a constant-coefficient Stokes slab, Dormand-Prince
+ * order/exactness checks, golden coefficient vectors evaluated with mpmath.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "jsx_oracle.h"
+#include "../jetshow_amd/csrc/jsx_math.h"
+
+/* constants: include/utils.h:17-36 */
+static const double K_MAS_TO_RAD = 4.84813681109536e-09;
+static const double K_PC = 3.0856775814671913e+18;
+static const double K_ME = 9.10938356e-28;
+static const double K_QE = 4.80320467299766e-10;
+static const double K_C = 29979245800.0;
+static const double K_PI = 3.141592653589793238462643383279502884;
+
+#define BISECT_CAP 200
+
+/* ------------------------------------------------------------------------ */
+/* math-mode dispatch for point-dependent transcendental terms               */
+
+typedef struct {
+    int mode;
+} orc_math;
+
+static double M_pow(const orc_math *m, double x, double y) {
+    return m->mode == ORC_MATH_JSX ? jsx_pow(x, y) : pow(x, y);
+}
+static double M_log(const orc_math *m, double x) {
+    return m->mode == ORC_MATH_JSX ? jsx_log(x) : log(x);
+}
+/* sin(atan(u)), cos(atan(u)) as BField.cpp:103-105 evaluates them */
+static void M_sincos_atan(const orc_math *m, double u, double *s, double *c) {
+    if (m->mode == ORC_MATH_JSX) {
+        jsx_sincos_atan(u, s, c);
+    } else {
+        double fi = atan(u);
+        *s = sin(fi);
+        *c = cos(fi);
+    }
+}
+static double M_hypot(const orc_math *m, double x, double y) {
+    return m->mode == ORC_MATH_JSX ? sqrt(x * x + y * y) : hypot(x, y);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Eigen-style 3-vector helpers (reduction order (x0 + x1) + x2)             */
+
+static double dot3(const double *a, const double *b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+static double sqn3(const double *a) { return (a[0] * a[0] + a[1] * a[1]) + a[2] * a[2]; }
+static double norm3(const double *a) { return sqrt(sqn3(a)); }
+static void cross3(const double *a, const double *b, double *o) {
+    double x = a[1] * b[2] - a[2] * b[1];
+    double y = a[2] * b[0] - a[0] * b[2];
+    double z = a[0] * b[1] - a[1] * b[0];
+    o[0] = x; o[1] = y; o[2] = z;
+}
+static void normalize3(double *a) { /* Eigen normalize(): divide by sqrt(squaredNorm) if > 0 */
+    double z = sqn3(a);
+    if (z > 0.0) { double r = sqrt(z); a[0] /= r; a[1] /= r; a[2] /= r; }
+}
+static void sub3(const double *a, const double *b, double *o) { o[0] = a[0] - b[0]; o[1] = a[1] - b[1]; o[2] = a[2] - b[2]; }
+/* Ray::point (src/Ray.cpp:17-19): origin_ + direction_ * t */
+static void ray_point(const double *o, const double *d, double t, double *p) {
+    p[0] = o[0] + d[0] * t; p[1] = o[1] + d[1] * t; p[2] = o[2] + d[2] * t;
+}
+
+/* ------------------------------------------------------------------------ */
+/* model: descriptor + parameter-only constants (evaluated with libm once;   */
+/* the reference evaluates the same expressions on every call)              */
+
+typedef struct {
+    const jsx_jet_desc *j;
+    const orc_grid *grids;
+    int n_grids;
+    orc_math mth;
+    unsigned opt;
+    /* geometry */
+    double axis[3];
+    double cos_angle, cc;   /* Cone: cos(angle), cos(angle)*cos(angle) (Cone.cpp:22,69-70) */
+    double M[3][3];
+    double big_scale;
+    /* fields */
+    double vz_flat;         /* ConstFlatVField c*sqrt(1 - 1/G^2) (VField.cpp:28) */
+    double sin_pitch, cos_pitch;
+    /* emission constants (src/utils.cpp) */
+    double s;
+    double f_ki_v, f_ki_s, f_ei_v, f_ei_s, r_kq, r_eq, f_kv, f_ev, f_kFr, kCr_pre, kCr_gm;
+    double e_ki, e_ei, e_kv, e_kCr;
+} orc_model;
+
+static const orc_grid *grid_of(const orc_model *m, int h) {
+    if (h < 1 || h > m->n_grids) return NULL;
+    return &m->grids[h - 1];
+}
+
+static void model_init(orc_model *m, const jsx_jet_desc *j, const orc_grid *grids, int n_grids,
+                       int math_mode, unsigned opt) {
+    memset(m, 0, sizeof(*m));
+    m->j = j; m->grids = grids; m->n_grids = n_grids; m->mth.mode = math_mode; m->opt = opt;
+    m->axis[0] = j->geo_axis[0]; m->axis[1] = j->geo_axis[1]; m->axis[2] = j->geo_axis[2];
+    normalize3(m->axis);   /* Cone.cpp:20, Parabaloid.cpp:14, Cylinder.cpp:12 */
+    m->cos_angle = cos(j->geo_param);
+    m->cc = cos(j->geo_param) * cos(j->geo_param);
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            m->M[r][c] = m->axis[r] * m->axis[c] - m->cc * (r == c ? 1.0 : 0.0);
+    m->big_scale = j->geo_big_scale;
+    if (j->geometry == JSX_GEOM_CYLINDER) m->big_scale = 100. * j->geo_param; /* Cylinder.cpp:97-99 */
+    m->vz_flat = K_C * sqrt(1. - 1. / (j->gamma0 * j->gamma0));
+    m->sin_pitch = sin(j->b_pitch);
+    m->cos_pitch = cos(j->b_pitch);
+    double s = j->s;
+    m->s = s;
+    /* k_i factor (utils.cpp:40) and random-B factor (utils.cpp:47-49) */
+    m->f_ki_v = (pow(3., (s + 1.) / 2.) / 4.) * tgamma(s / 4. + 11. / 6.) * tgamma(s / 4. + 1. / 6.);
+    m->f_ki_s = m->f_ki_v * (sqrt(K_PI / 4.) * tgamma((6. + s) / 4.) / tgamma((8. + s) / 4.));
+    /* eta_i factors (utils.cpp:120, 126-128) */
+    m->f_ei_v = pow(3., s / 2.) / (2. * (s + 1)) * tgamma(s / 4. + 19. / 12.) * tgamma(s / 4. - 1. / 12.);
+    m->f_ei_s = m->f_ei_v * (sqrt(K_PI / 4.) * tgamma((5. + s) / 4.) / tgamma((7. + s) / 4.));
+    m->r_kq = (s + 2.) / (s + 10. / 3);               /* utils.cpp:64 */
+    m->r_eq = (s + 1.0) / (s + 7. / 3.);              /* utils.cpp:133 */
+    m->f_kv = pow(3., s / 2.) * (s + 3.) * (s + 2.) / (4. * (s + 1.)) * tgamma(s / 4. + 11. / 12.) *
+              tgamma(s / 4. + 7. / 12.);              /* utils.cpp:73 */
+    m->f_ev = pow(3., (s - 1.) / 2.) * (s + 2.) / (2. * s) * tgamma(s / 4. + 2. / 3.) *
+              tgamma(s / 4. + 1. / 3.);               /* utils.cpp:142 */
+    double gm = j->gamma_min;
+    m->f_kFr = (s + 2.) * log(gm) / ((s + 1.) * pow(gm, s + 1.)); /* utils.cpp:89 */
+    m->kCr_pre = 2. / (s - 2.);                       /* utils.cpp:96 */
+    m->kCr_gm = pow(gm, 2. - s);
+    m->e_ki = s / 2.;
+    m->e_ei = (s - 1.) / 2.;
+    m->e_kv = (s + 1.) / 2.;
+    m->e_kCr = (s - 2.) / 2.;
+}
+
+/* ------------------------------------------------------------------------ */
+/* geometry: src/Geometry.cpp, Cone.cpp, Parabaloid.cpp, Cylinder.cpp,        */
+/* Intersection.cpp                                                         */
+
+typedef struct { double pin[3], pout[3]; } orc_seg;
+
+/* Intersection(ray, geo) full infinite path (Intersection.cpp:41-48) */
+static void full_infinite(const orc_model *m, const double *o, const double *d, orc_seg *s) {
+    double diff[3], ref[3];
+    sub3(o, m->j->geo_origin, diff);
+    double dd = dot3(diff, d);
+    for (int i = 0; i < 3; ++i) ref[i] = diff[i] - dd * d[i];
+    for (int i = 0; i < 3; ++i) {
+        s->pin[i] = ref[i] - m->big_scale * d[i];
+        s->pout[i] = ref[i] + m->big_scale * d[i];
+    }
+}
+
+/* Cone::is_within (Cone.cpp:48-53) */
+static int cone_is_within(const orc_model *m, const double *p) {
+    double diff[3];
+    sub3(p, m->j->geo_origin, diff);
+    normalize3(diff);
+    double cosp = dot3(m->axis, diff);
+    return cosp > m->cos_angle || cosp < -m->cos_angle;
+}
+
+/* Intersection(ray, point, geo) half infinite path (Intersection.cpp:51-62) */
+static void half_infinite(const orc_model *m, const double *d, const double *pt, orc_seg *s) {
+    double chk[3];
+    sub3(pt, d, chk);
+    if (cone_is_within(m, chk)) {
+        for (int i = 0; i < 3; ++i) { s->pin[i] = pt[i] - m->big_scale * d[i]; s->pout[i] = pt[i]; }
+    } else {
+        for (int i = 0; i < 3; ++i) { s->pin[i] = pt[i]; s->pout[i] = pt[i] + m->big_scale * d[i]; }
+    }
+}
+
+/* (d^T M) e as the Eigen row-vector product evaluates it */
+static double quad_form(const double M[3][3], const double *d, const double *e) {
+    double r[3];
+    for (int j = 0; j < 3; ++j) r[j] = (d[0] * M[0][j] + d[1] * M[1][j]) + d[2] * M[2][j];
+    return (r[0] * e[0] + r[1] * e[1]) + r[2] * e[2];
+}
+
+/* Cone::hit (Cone.cpp:56-118) */
+static int cone_hit(const orc_model *m, const double *o, const double *d, orc_seg *segs) {
+    double delta[3];
+    sub3(o, m->j->geo_origin, delta);
+    double c2 = quad_form(m->M, d, d);
+    double c1 = quad_form(m->M, d, delta);
+    double c0 = quad_form(m->M, delta, delta);
+    double disc = c1 * c1 - c0 * c2;
+    if (disc < 0) return 0;
+    if (disc == 0) {
+        double t = -c1 / c2;
+        double p[3], dp[3];
+        ray_point(o, d, t, p);
+        sub3(p, m->j->geo_origin, dp);
+        /* isMuchSmallerThan(1e-6, 1e-6): squaredNorm <= (1e-6 * 1e-6)^2 */
+        if (sqn3(dp) <= (1E-06 * 1E-06) * (1E-06 * 1E-06) && c1 == 0.0) {
+            full_infinite(m, o, d, &segs[0]);
+            return 1;
+        }
+        for (int i = 0; i < 3; ++i) { segs[0].pin[i] = p[i]; segs[0].pout[i] = p[i]; }
+        return 1;
+    }
+    double eps = (c1 > 0 ? 1 : -1);
+    double t1 = (-c1 - eps * sqrt(c1 * c1 - c2 * c0)) / (c2);
+    double t2 = c0 / (-c1 - eps * sqrt(c1 * c1 - c2 * c0));
+    double pin[3], pout[3];
+    ray_point(o, d, t1 < t2 ? t1 : t2, pin);
+    ray_point(o, d, t1 < t2 ? t2 : t1, pout);
+    double cos_ray_cone = dot3(d, m->axis);
+    if (fabs(cos_ray_cone) != 1.) {
+        memcpy(segs[0].pin, pin, sizeof pin);
+        memcpy(segs[0].pout, pout, sizeof pout);
+        return 1;
+    }
+    /* two half-infinite pieces, point_out first (Cone.cpp:114-115) */
+    half_infinite(m, d, pout, &segs[0]);
+    half_infinite(m, d, pin, &segs[1]);
+    return 2;
+}
+
+/* intersection() generic quadric (Geometry.cpp:28-69); returns number of roots */
+static int quadric_roots(const double *R0, const double *Rd, double A, double B, double C, double D,
+                         double E, double F, double G, double H, double I, double J, double *t) {
+    double x0 = R0[0], y0 = R0[1], z0 = R0[2];
+    double xd = Rd[0], yd = Rd[1], zd = Rd[2];
+    double Aq = A * xd * xd + B * yd * yd + C * zd * zd + D * xd * yd + E * xd * zd + F * yd * zd;
+    double Bq = 2 * A * x0 * xd + 2 * B * y0 * yd + 2 * C * z0 * zd + D * (x0 * yd + y0 * xd) + E * x0 * zd +
+                F * (y0 * zd + yd * z0) + G * xd + H * yd + I * zd;
+    double Cq = A * x0 * x0 + B * y0 * y0 + C * z0 * z0 + D * x0 * y0 + E * x0 * z0 + F * y0 * z0 + G * x0 +
+                H * y0 + I * z0 + J;
+    double Dscr = Bq * Bq - 4 * Aq * Cq;
+    if (Aq == 0) { t[0] = -Cq / Bq; return 1; }
+    if (Dscr < 0) return 0;
+    double t0 = (-Bq - sqrt(Dscr)) / (2 * Aq);
+    if (t0 > 0) { t[0] = t0; return 1; }
+    t[0] = t0;
+    t[1] = (-Bq + sqrt(Dscr)) / (2 * Aq);
+    return 2;
+}
+
+/* Parabaloid::hit (Parabaloid.cpp:39-60): only the two-root case is a hit */
+static int paraboloid_hit(const orc_model *m, const double *o, const double *d, orc_seg *segs) {
+    double r0 = m->j->geo_param;
+    double t[2];
+    int nr = quadric_roots(o, d, 1., 1., 0., 0., 0., 0., 0., 0., -r0 * r0 / K_PC, 0., t);
+    if (nr != 2) return 0;
+    double t1 = t[0], t2 = t[1];
+    ray_point(o, d, t1 < t2 ? t1 : t2, segs[0].pin);
+    ray_point(o, d, t1 < t2 ? t2 : t1, segs[0].pout);
+    return 1;
+}
+
+/* Cylinder::hit (Cylinder.cpp:35-95), working in parsec as the reference
+ * does.  The reference subtracts r/pc where r^2 is meant (:53); the fixed
+ * form is the default, JSX_OPT_CYLINDER_REF_BUG selects the reference's. */
+static int cylinder_hit(const orc_model *m, const double *o, const double *d, orc_seg *segs) {
+    double ro[3] = {o[0] / K_PC, o[1] / K_PC, o[2] / K_PC};
+    const double *org = m->j->geo_origin;
+    double co[3] = {org[0] / K_PC, org[1] / K_PC, org[2] / K_PC};
+    const double *a = m->axis;
+    double delta[3];
+    sub3(ro, co, delta);
+    double da = dot3(d, a);
+    double v[3] = {d[0] - da * a[0], d[1] - da * a[1], d[2] - da * a[2]};
+    double c2 = sqn3(v);
+    double dla = dot3(delta, a);
+    double w[3] = {delta[0] - dla * a[0], delta[1] - dla * a[1], delta[2] - dla * a[2]};
+    double c1 = dot3(v, w);
+    double r = m->j->geo_param;
+    double c0 = (m->opt & JSX_OPT_CYLINDER_REF_BUG) ? sqn3(w) - r / K_PC : sqn3(w) - (r / K_PC) * (r / K_PC);
+    double disc = c1 * c1 - c0 * c2;
+    if (disc < 0) return 0;
+    if (disc > 0) {
+        double eps = (c1 > 0 ? 1 : -1);
+        double sqrt_d = sqrt(disc);
+        double t1 = (-c1 - eps * sqrt_d) / (c2);
+        double t2 = c0 / (-c1 - eps * sqrt_d);
+        ray_point(o, d, K_PC * (t1 < t2 ? t1 : t2), segs[0].pin);
+        ray_point(o, d, K_PC * (t1 < t2 ? t2 : t1), segs[0].pout);
+        return 1;
+    }
+    if (c2 == 1 && c1 == 0 && c0 == 0) {
+        double t = -c1 / c2;
+        ray_point(o, d, t, segs[0].pin);
+        ray_point(o, d, t, segs[0].pout);
+        return 1;
+    }
+    if (c2 == 0 && c1 == 0 && c0 <= 0) { full_infinite(m, o, d, &segs[0]); return 1; }
+    return 0; /* c0 > 0 along the border, or the reference's unreturned branches */
+}
+
+/* z clip of a segment to [zmin, zmax] (extension for finite-length jets) */
+static int clip_seg(const orc_model *m, orc_seg *s) {
+    double zmin = m->j->z_clip_min, zmax = m->j->z_clip_max;
+    double zA = s->pin[2], zB = s->pout[2];
+    if (zA >= zmin && zA <= zmax && zB >= zmin && zB <= zmax) return 1;
+    if (zB == zA) return 0;
+    double u0 = (zmin - zA) / (zB - zA);
+    double u1 = (zmax - zA) / (zB - zA);
+    if (u0 > u1) { double tt = u0; u0 = u1; u1 = tt; }
+    double lo = u0 > 0.0 ? u0 : 0.0;
+    double hi = u1 < 1.0 ? u1 : 1.0;
+    if (!(lo < hi)) return 0;
+    double A[3], B[3];
+    for (int i = 0; i < 3; ++i) {
+        double df = s->pout[i] - s->pin[i];
+        A[i] = lo == 0.0 ? s->pin[i] : s->pin[i] + lo * df;
+        B[i] = hi == 1.0 ? s->pout[i] : s->pin[i] + hi * df;
+    }
+    memcpy(s->pin, A, sizeof A);
+    memcpy(s->pout, B, sizeof B);
+    return 2;
+}
+
+/* Jet::hit -> Geometry::hit (Jet.cpp:290-292), then the optional z clip.
+ * Returns segment count; *clipped set when the clip changed anything. */
+static int model_hit(const orc_model *m, const double *o, const double *d, orc_seg *segs, int *clipped) {
+    int n = 0;
+    *clipped = 0;
+    switch (m->j->geometry) {
+    case JSX_GEOM_CONE: n = cone_hit(m, o, d, segs); break;
+    case JSX_GEOM_PARABOLOID: n = paraboloid_hit(m, o, d, segs); break;
+    case JSX_GEOM_CYLINDER: n = cylinder_hit(m, o, d, segs); break;
+    default: return 0;
+    }
+    if (m->j->z_clip_max > m->j->z_clip_min) {
+        int k = 0;
+        for (int i = 0; i < n; ++i) {
+            orc_seg s = segs[i];
+            int r = clip_seg(m, &s);
+            if (r != 1) *clipped = 1;
+            if (r) segs[k++] = s;
+        }
+        n = k;
+    }
+    return n;
+}
+
+/* ------------------------------------------------------------------------ */
+/* grid interpolation: SimulationInterpolater::interpolated_value            */
+/* (SimulationInterpolater.cpp:43-102) on a regular (z, r_p) node grid split */
+/* into triangles along the (i,j)-(i+1,j+1) diagonal; barycentric weights as  */
+/* CGAL Triangle_coordinates_2 forms them.                                   */
+
+static double tri_area(double px, double py, double qx, double qy, double rx, double ry) {
+    return ((qx - px) * (ry - py) - (rx - px) * (qy - py)) / 2;
+}
+
+static double grid_interp(const orc_grid *g, const double *p, const orc_math *mth) {
+    const jsx_grid_desc *G = &g->desc;
+    double x = p[0] / K_PC, y = p[1] / K_PC, z = p[2] / K_PC;
+    double rp = M_hypot(mth, x, y);
+    double zlast = G->z0 + (double)(G->nz - 1) * G->dz;
+    double rlast = G->r0 + (double)(G->nr - 1) * G->dr;
+    if (!(z >= G->z0 && z <= zlast && rp >= G->r0 && rp <= rlast)) return G->nan_value;
+    int i = (int)floor((z - G->z0) / G->dz);
+    int j = (int)floor((rp - G->r0) / G->dr);
+    if (i > G->nz - 2) i = G->nz - 2;
+    if (j > G->nr - 2) j = G->nr - 2;
+    if (i < 0) i = 0;
+    if (j < 0) j = 0;
+    double za = G->z0 + (double)i * G->dz, zb = G->z0 + (double)(i + 1) * G->dz;
+    double ra = G->r0 + (double)j * G->dr, rb = G->r0 + (double)(j + 1) * G->dr;
+    const double *V = g->values;
+    double f00 = V[(int64_t)i * G->nr + j], f11 = V[(int64_t)(i + 1) * G->nr + (j + 1)];
+    double v0x = za, v0y = ra, v1x, v1y, v2x, v2y, f1, f2;
+    /* lower triangle (za,ra),(zb,ra),(zb,rb) when (r-ra)/dr <= (z-za)/dz */
+    if ((rp - ra) * (zb - za) <= (z - za) * (rb - ra)) {
+        v1x = zb; v1y = ra; f1 = V[(int64_t)(i + 1) * G->nr + j];
+        v2x = zb; v2y = rb; f2 = f11;
+    } else {
+        v1x = zb; v1y = rb; f1 = f11;
+        v2x = za; v2y = rb; f2 = V[(int64_t)i * G->nr + (j + 1)];
+    }
+    double total = tri_area(v0x, v0y, v1x, v1y, v2x, v2y);
+    double inv = 1.0 / total;
+    double b0 = tri_area(v1x, v1y, v2x, v2y, z, rp) * inv;
+    double b1 = tri_area(v2x, v2y, v0x, v0y, z, rp) * inv;
+    double b2 = 1.0 - b0 - b1;
+    double val = 0;
+    val += b0 * f00;
+    val += b1 * f1;
+    val += b2 * f2;
+    if (val != val) val = G->nan_value;
+    return val;
+}
+
+double orc_grid_interp(const orc_grid *grid, const double *point_cm, int32_t math_mode) {
+    orc_math mth = {math_mode};
+    return grid_interp(grid, point_cm, &mth);
+}
+
+/* ------------------------------------------------------------------------ */
+/* fields: src/BField.cpp, NField.cpp, VField.cpp                            */
+
+/* VField::vf */
+static void vfield(const orc_model *m, const double *p, double *v) {
+    const jsx_jet_desc *j = m->j;
+    switch (j->vfield) {
+    case JSX_V_CONST_FLAT: /* VField.cpp:27-29 */
+        v[0] = 0; v[1] = 0; v[2] = m->vz_flat;
+        return;
+    case JSX_V_CONST_CENTRAL: { /* VField.cpp:64-72 */
+        double q[3] = {p[0] + j->v_origin[0], p[1] + j->v_origin[1], p[2] + j->v_origin[2]};
+        double r = norm3(q);
+        double v_r = K_C * sqrt(1. - 1. / (j->gamma0 * j->gamma0));
+        v[0] = v_r * (p[0] + j->v_origin[0]) / r;
+        v[1] = v_r * (p[1] + j->v_origin[1]) / r;
+        v[2] = v_r * (p[2] + j->v_origin[2]) / r;
+        return;
+    }
+    case JSX_V_LINEAR_GAMMA_Z: { /* extension: Gamma linear in z, clamped */
+        double z = p[2];
+        double g;
+        if (z <= j->gamma_z0) g = j->gamma0;
+        else if (z >= j->gamma_z1) g = j->gamma1;
+        else g = j->gamma0 + (j->gamma1 - j->gamma0) * ((z - j->gamma_z0) / (j->gamma_z1 - j->gamma_z0));
+        v[0] = 0; v[1] = 0; v[2] = K_C * sqrt(1. - 1. / (g * g));
+        return;
+    }
+    case JSX_V_GRID: { /* SimulationVField (VField.cpp:269-274) */
+        const orc_grid *g = grid_of(m, j->grid_v);
+        double gam = g ? grid_interp(g, p, &m->mth) : 1.0;
+        v[0] = 0; v[1] = 0; v[2] = K_C * sqrt(1. - 1. / (gam * gam));
+        return;
+    }
+    default:
+        v[0] = v[1] = v[2] = 0;
+    }
+}
+
+/* getG (utils.cpp:147-150) */
+static double getG(const double *v) {
+    double beta[3] = {v[0] / K_C, v[1] / K_C, v[2] / K_C};
+    return sqrt(1. / (1. - sqn3(beta)));
+}
+/* getD (utils.cpp:152-155) */
+static double getD(const double *n_los, const double *v) {
+    double beta[3] = {v[0] / K_C, v[1] / K_C, v[2] / K_C};
+    return 1. / (getG(v) * (1. - dot3(beta, n_los)));
+}
+/* get_n_los_prime (utils.cpp:157-162) */
+static void get_n_los_prime(const double *n_los, const double *v, double *out) {
+    double df = getD(n_los, v);
+    double gamma = getG(v);
+    double beta[3] = {v[0] / K_C, v[1] / K_C, v[2] / K_C};
+    double f = (df + 1.) * (gamma / (gamma + 1.));
+    for (int i = 0; i < 3; ++i) out[i] = df * n_los[i] - f * beta[i];
+}
+/* getBprime (utils.cpp:164-168) */
+static void getBprime(const double *b, const double *v, double *out) {
+    double gamma = getG(v);
+    double s1 = gamma / ((1. + gamma) * (K_C * K_C));
+    double vb = dot3(v, b);
+    for (int i = 0; i < 3; ++i) out[i] = b[i] / gamma + (s1 * v[i]) * vb;
+}
+
+/* B field in the plasma frame. Returns 1 for a vector field (b[3]), 0 for a
+ * scalar (tangled) field (b[0]).  BField.cpp:35-146,196-224 */
+static int bfield_plasma(const orc_model *m, const double *p, const double *v, double *b) {
+    const jsx_jet_desc *j = m->j;
+    const orc_math *mth = &m->mth;
+    double bl[3];
+    switch (j->bfield) {
+    case JSX_B_RANDOM_SCALAR: { /* BField.cpp:38-41 */
+        double r = norm3(p);
+        b[0] = j->b0 * M_pow(mth, r / K_PC, -j->b_exp);
+        return 0;
+    }
+    case JSX_B_RANDOM_SCALAR_Z: { /* BField.cpp:47-50 */
+        double r = fabs(p[2]);
+        b[0] = j->b0 * M_pow(mth, r / K_PC, -j->b_exp);
+        return 0;
+    }
+    case JSX_B_GRID_SCALAR: {
+        const orc_grid *g = grid_of(m, j->grid_b);
+        b[0] = g ? grid_interp(g, p, mth) : 0.0;
+        return 0;
+    }
+    case JSX_B_TOROIDAL: { /* BField.cpp:99-106 */
+        double x = p[0], y = p[1], z = p[2];
+        double sf, cf;
+        M_sincos_atan(mth, y / x, &sf, &cf);
+        double bb = j->b0 * M_pow(mth, z / K_PC, -j->b_exp);
+        bl[0] = -sf * bb; bl[1] = cf * bb; bl[2] = 0;
+        break;
+    }
+    case JSX_B_TOROIDAL_R:
+    case JSX_B_HELICAL_R: {
+        double r = norm3(p);
+        double bb = j->b0 * M_pow(mth, r / K_PC, -j->b_exp);
+        double rho = sqrt(p[0] * p[0] + p[1] * p[1]);
+        double ex = -p[1] / rho, ey = p[0] / rho;
+        if (j->bfield == JSX_B_TOROIDAL_R) {
+            bl[0] = bb * ex; bl[1] = bb * ey; bl[2] = 0;
+        } else {
+            double bphi = bb * m->sin_pitch;
+            bl[0] = bphi * ex; bl[1] = bphi * ey; bl[2] = bb * m->cos_pitch;
+        }
+        break;
+    }
+    case JSX_B_CONST_CYLINDER_Z: { /* ConstCylinderBFieldZ (BField.cpp:79-82) */
+        double r = fabs(p[2]);
+        bl[0] = 0.0; bl[1] = 0.0; bl[2] = j->b0 * M_pow(mth, r / K_PC, -j->b_exp);
+        break;
+    }
+    case JSX_B_GRID_VECTOR: { /* SimulationBField (BField.cpp:199-207) */
+        double sf, cf;
+        M_sincos_atan(mth, p[1] / p[0], &sf, &cf);
+        const orc_grid *gp = grid_of(m, j->grid_b);
+        const orc_grid *gf = grid_of(m, j->grid_b2);
+        double vp = gp ? grid_interp(gp, p, mth) : 0.0;
+        double vf = gf ? grid_interp(gf, p, mth) : 0.0;
+        bl[0] = -sf * vf; bl[1] = cf * vf; bl[2] = vp;
+        break;
+    }
+    default:
+        b[0] = 0;
+        return 0;
+    }
+    /* VectorBField::bf_plasma_frame (BField.cpp:212-219) */
+    if (j->b_in_plasma_frame) { b[0] = bl[0]; b[1] = bl[1]; b[2] = bl[2]; }
+    else getBprime(bl, v, b);
+    return 1;
+}
+
+/* NField::nf (NField.cpp:17-59) */
+static double nfield(const orc_model *m, const double *p) {
+    const jsx_jet_desc *j = m->j;
+    switch (j->nfield) {
+    case JSX_N_CONST: return j->n0;
+    case JSX_N_BK: { double r = norm3(p); return j->n0 * M_pow(&m->mth, r / K_PC, -j->n_exp); }
+    case JSX_N_BK_Z: { double z = fabs(p[2]); return j->n0 * M_pow(&m->mth, z / K_PC, -j->n_exp); }
+    case JSX_N_GRID: { const orc_grid *g = grid_of(m, j->grid_n); return g ? grid_interp(g, p, &m->mth) : 0.0; }
+    default: return 0;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* emission / absorption coefficients: src/utils.cpp:8-144                   */
+
+static double nu_p(double n) { return sqrt(n * K_QE * K_QE / (K_PI * K_ME)); }
+static double nu_b_vec(const double *b, const double *n_los) {
+    double cr[3];
+    cross3(n_los, b, cr);
+    return K_QE * norm3(cr) / (2. * K_PI * K_ME * K_C);
+}
+static double nu_b_scalar(double b) { return K_QE * b / (2. * K_PI * K_ME * K_C); }
+static double nu_b_value(const double *b) { return K_QE * norm3(b) / (2. * K_PI * K_ME * K_C); }
+
+/* frame quantities Jet::getKI builds (Jet.cpp:25-31) */
+typedef struct {
+    double v[3], gamma, D, n, nl[3], nu;
+    int vec;
+    double b[3];
+} orc_frame;
+
+static void jet_frame(const orc_model *m, const double *p, const double *n_los, double nu, orc_frame *f) {
+    vfield(m, p, f->v);
+    f->gamma = getG(f->v);
+    f->vec = bfield_plasma(m, p, f->v, f->b);
+    f->D = getD(n_los, f->v);
+    double n = nfield(m, p);
+    f->n = m->j->n_in_plasma_frame ? n : n / f->gamma;   /* NField.cpp:7-14 */
+    get_n_los_prime(n_los, f->v, f->nl);
+    f->nu = nu / f->D;
+}
+
+/* all eleven lab-frame coefficients (Jet.cpp:15-46, 114-288 restored) */
+typedef struct { double kI, kQ, kU, kV, kF, kC, hQ, eI, eQ, eU, eV; } orc_coef;
+
+static void jet_coefficients(const orc_model *m, const double *p, const double *n_los, double nu,
+                             int full, orc_coef *o) {
+    orc_frame f;
+    jet_frame(m, p, n_los, nu, &f);
+    const orc_math *mth = &m->mth;
+    double D = f.D, n = f.n, fnu = f.nu;
+    double nup = nu_p(n);
+    memset(o, 0, sizeof(*o));
+    if (!f.vec) {
+        /* random B: k_i, eta_i scalar overloads (utils.cpp:31-33,46-51,111-113,125-130) */
+        double bs = f.b[0];
+        double nub = nu_b_scalar(bs);
+        double k0 = K_PI * nup * nup * nu_b_scalar(bs) / (K_C * fnu * fnu);
+        double ki = k0 * M_pow(mth, nub / fnu, m->e_ki) * m->f_ki_s;
+        double e0 = K_PI * nup * nup * nu_b_scalar(bs) * K_ME / K_C;
+        double ei = e0 * M_pow(mth, nub / fnu, m->e_ei) * m->f_ei_s;
+        o->kI = ki / D;
+        o->eI = ei * D * D;
+        return; /* tangled field: no net polarisation or Faraday terms */
+    }
+    const double *b = f.b, *nl = f.nl;
+    double nub = nu_b_vec(b, nl);
+    double k0 = K_PI * nup * nup * nu_b_vec(b, nl) / (K_C * fnu * fnu);
+    double ki = k0 * M_pow(mth, nub / fnu, m->e_ki) * m->f_ki_v;
+    double e0 = K_PI * nup * nup * nu_b_vec(b, nl) * K_ME / K_C;
+    double ei = e0 * M_pow(mth, nub / fnu, m->e_ei) * m->f_ei_v;
+    o->kI = ki / D;
+    o->eI = ei * D * D;
+    if (!full) return;
+    double kq = m->r_kq * ki;                                   /* utils.cpp:63-65 */
+    double eq = m->r_eq * ei;                                   /* utils.cpp:132-134 */
+    double cos_theta = dot3(b, nl) / norm3(b);                  /* utils.cpp:72 */
+    double nubv = nu_b_value(b);
+    double k0v = K_PI * nup * nup * nubv / (K_C * fnu * fnu);
+    double kv = -k0v * cos_theta * M_pow(mth, nub / fnu, m->e_kv) * m->f_kv;     /* :71-75 */
+    double e0v = K_PI * nup * nup * nubv * K_ME / K_C;
+    double ev = -e0v * cos_theta * M_pow(mth, nub / fnu, m->e_ki) * m->f_ev;     /* :140-144 */
+    double kFc = 2. * K_PI * nup * nup * nubv * cos_theta / (K_C * fnu * fnu);  /* :77-80 */
+    double kCc = -K_PI * nup * nup * nub * nub / (K_C * fnu * fnu * fnu);       /* :82-84 */
+    double kF = m->f_kFr * kFc;                                                  /* :86-91 */
+    double kC = m->kCr_pre * (m->kCr_gm - M_pow(mth, nub / fnu, m->e_kCr)) * kCc; /* :93-98 */
+    if (m->j->thermal_fraction != 0.0) {
+        /* cold thermal electrons (extension): k_F_c, k_C_c at n_th */
+        double nth = m->j->thermal_fraction * n;
+        double nupt = nu_p(nth);
+        kF = kF + 2. * K_PI * nupt * nupt * nubv * cos_theta / (K_C * fnu * fnu);
+        kC = kC + -K_PI * nupt * nupt * nub * nub / (K_C * fnu * fnu * fnu);
+    }
+    /* Stokes frame (extension): the reference's k_u = eta_u = h_Q = 0 hold in
+     * a frame whose Q axis is perpendicular to the projected B'.  Rotate into
+     * the sky frame built from n' and the jet axis. */
+    double a[3] = {m->axis[0], m->axis[1], m->axis[2]};
+    double an = dot3(a, nl);
+    double e1[3] = {a[0] - an * nl[0], a[1] - an * nl[1], a[2] - an * nl[2]};
+    double e2[3];
+    cross3(nl, a, e2);
+    double bx = dot3(b, e1), by = dot3(b, e2);
+    double q2 = bx * bx + by * by;
+    double c2 = 1.0, s2 = 0.0;
+    if (q2 > 0.0) { c2 = -(bx * bx - by * by) / q2; s2 = -(2.0 * bx * by) / q2; }
+    o->kQ = (kq * c2) / D;
+    o->kU = (kq * s2) / D;
+    o->kV = kv / D;
+    o->kF = kF / D;
+    o->kC = (kC * c2) / D;
+    o->hQ = -(kC * s2) / D;
+    o->eQ = (eq * c2) * D * D;
+    o->eU = (eq * s2) * D * D;
+    o->eV = ev * D * D;
+}
+
+/* ------------------------------------------------------------------------ */
+/* systems: src/System.cpp:15-69                                             */
+
+typedef struct {
+    const orc_model *m;
+    double p0[3];     /* point the system starts from                          */
+    double dir[3];    /* +d for Tau (point = p_in + t d), -d for I/FullStokes   */
+    double n_los[3];  /* -d                                                    */
+    double nu;
+    int kind;         /* 0 tau, 1 I, 4 full                                    */
+    uint32_t *n_rhs;
+    int nan_seen;
+} orc_sys;
+
+static void sys_rhs(orc_sys *S, const double *x, double *dxdt, double t) {
+    double p[3];
+    if (S->kind == 0) {
+        /* Tau: point_in - t * ray_direction with ray_direction = -d */
+        for (int i = 0; i < 3; ++i) p[i] = S->p0[i] + t * S->dir[i];
+    } else {
+        /* I: point_in + t * ray_direction with ray_direction = -d */
+        for (int i = 0; i < 3; ++i) p[i] = S->p0[i] - t * S->dir[i];
+    }
+    orc_coef c;
+    jet_coefficients(S->m, p, S->n_los, S->nu, S->kind == 4, &c);
+    (*S->n_rhs)++;
+    if (S->kind == 0) {
+        dxdt[0] = c.kI;
+    } else if (S->kind == 1) {
+        dxdt[0] = c.eI - c.kI * x[0];
+    } else {
+        dxdt[0] = c.eI - c.kI * x[0] - c.kQ * x[1] - c.kU * x[2] - c.kV * x[3];
+        dxdt[1] = c.eQ - c.kI * x[1] - c.kQ * x[0] - c.kF * x[2] - c.hQ * x[3];
+        dxdt[2] = c.eU - c.kI * x[2] - c.kU * x[0] + c.kF * x[1] - c.kC * x[3];
+        dxdt[3] = c.eV - c.kI * x[3] - c.kV * x[0] + c.hQ * x[1] + c.kC * x[2];
+    }
+    int ns = S->kind == 4 ? 4 : 1;
+    for (int i = 0; i < ns; ++i)
+        if (dxdt[i] != dxdt[i]) S->nan_seen = 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Dormand-Prince 5(4) -- restatement of Boost.odeint's runge_kutta_dopri5,  */
+/* controlled_runge_kutta (default_error_checker, default_step_adjuster),    */
+/* dense_output_runge_kutta, integrate_adaptive and the dense adaptive range */
+/* (call sites src/Observation.cpp:154-159,166-187,262-271)                 */
+
+typedef void (*orc_rhs_fn)(void *ctx, const double *x, double *dxdt, double t);
+
+typedef struct {
+    orc_rhs_fn f;
+    void *ctx;
+    int ns;
+    double k2[4], k3[4], k4[4], k5[4], k6[4];
+    /* controller */
+    double eps_abs, eps_rel, max_dt;
+    uint32_t *n_steps;
+    const orc_math *mth;
+} orc_dp;
+
+/* Butcher tableau of Dormand & Prince (1980), as odeint spells it */
+#define DP_A2 (1.0 / 5.0)
+#define DP_A3 (3.0 / 10.0)
+#define DP_A4 (4.0 / 5.0)
+#define DP_A5 (8.0 / 9.0)
+#define DP_B21 (1.0 / 5.0)
+#define DP_B31 (3.0 / 40.0)
+#define DP_B32 (9.0 / 40.0)
+#define DP_B41 (44.0 / 45.0)
+#define DP_B42 (-56.0 / 15.0)
+#define DP_B43 (32.0 / 9.0)
+#define DP_B51 (19372.0 / 6561.0)
+#define DP_B52 (-25360.0 / 2187.0)
+#define DP_B53 (64448.0 / 6561.0)
+#define DP_B54 (-212.0 / 729.0)
+#define DP_B61 (9017.0 / 3168.0)
+#define DP_B62 (-355.0 / 33.0)
+#define DP_B63 (46732.0 / 5247.0)
+#define DP_B64 (49.0 / 176.0)
+#define DP_B65 (-5103.0 / 18656.0)
+#define DP_C1 (35.0 / 384.0)
+#define DP_C3 (500.0 / 1113.0)
+#define DP_C4 (125.0 / 192.0)
+#define DP_C5 (-2187.0 / 6784.0)
+#define DP_C6 (11.0 / 84.0)
+#define DP_DC1 (DP_C1 - 5179.0 / 57600.0)
+#define DP_DC3 (DP_C3 - 7571.0 / 16695.0)
+#define DP_DC4 (DP_C4 - 393.0 / 640.0)
+#define DP_DC5 (DP_C5 - -92097.0 / 339200.0)
+#define DP_DC6 (DP_C6 - 187.0 / 2100.0)
+#define DP_DC7 (-1.0 / 40.0)
+
+/* one Dormand-Prince step with error estimate (FSAL: k1 in, k7 out) */
+static void dp_do_step(orc_dp *S, const double *x, const double *k1, double t, double dt, double *xo,
+                       double *k7, double *xerr) {
+    int ns = S->ns;
+    double xt[4] = {0, 0, 0, 0};
+    for (int i = 0; i < ns; ++i) xt[i] = x[i] + dt * DP_B21 * k1[i];
+    S->f(S->ctx, xt, S->k2, t + dt * DP_A2);
+    for (int i = 0; i < ns; ++i) xt[i] = x[i] + dt * DP_B31 * k1[i] + dt * DP_B32 * S->k2[i];
+    S->f(S->ctx, xt, S->k3, t + dt * DP_A3);
+    for (int i = 0; i < ns; ++i)
+        xt[i] = x[i] + dt * DP_B41 * k1[i] + dt * DP_B42 * S->k2[i] + dt * DP_B43 * S->k3[i];
+    S->f(S->ctx, xt, S->k4, t + dt * DP_A4);
+    for (int i = 0; i < ns; ++i)
+        xt[i] = x[i] + dt * DP_B51 * k1[i] + dt * DP_B52 * S->k2[i] + dt * DP_B53 * S->k3[i] +
+                dt * DP_B54 * S->k4[i];
+    S->f(S->ctx, xt, S->k5, t + dt * DP_A5);
+    for (int i = 0; i < ns; ++i)
+        xt[i] = x[i] + dt * DP_B61 * k1[i] + dt * DP_B62 * S->k2[i] + dt * DP_B63 * S->k3[i] +
+                dt * DP_B64 * S->k4[i] + dt * DP_B65 * S->k5[i];
+    S->f(S->ctx, xt, S->k6, t + dt);
+    for (int i = 0; i < ns; ++i)
+        xo[i] = x[i] + dt * DP_C1 * k1[i] + dt * DP_C3 * S->k3[i] + dt * DP_C4 * S->k4[i] +
+                dt * DP_C5 * S->k5[i] + dt * DP_C6 * S->k6[i];
+    S->f(S->ctx, xo, k7, t + dt);
+    for (int i = 0; i < ns; ++i)
+        xerr[i] = dt * DP_DC1 * k1[i] + dt * DP_DC3 * S->k3[i] + dt * DP_DC4 * S->k4[i] +
+                  dt * DP_DC5 * S->k5[i] + dt * DP_DC6 * S->k6[i] + dt * DP_DC7 * k7[i];
+}
+
+/* controlled try_step (odeint controlled_runge_kutta, FSAL version).
+ * Returns 1 on success (t advanced, dt possibly grown), 0 on failure. */
+static int dp_try_step(orc_dp *S, const double *x, const double *k1, double *t, double *xo, double *k7,
+                       double *dt) {
+    (*S->n_steps)++;
+    if (S->max_dt != 0.0 && !(*dt - S->max_dt <= DBL_EPSILON)) { *dt = S->max_dt; return 0; }
+    double xerr[4];
+    dp_do_step(S, x, k1, *t, *dt, xo, k7, xerr);
+    /* default_error_checker: |err| / (eps_abs + eps_rel (|x| + dt |dxdt|)), max-norm */
+    double err;
+    if (S->ns == 1) {
+        err = fabs(xerr[0]) / (S->eps_abs + S->eps_rel * (fabs(x[0]) + *dt * fabs(k1[0])));
+    } else {
+        err = 0.0;
+        for (int i = 0; i < S->ns; ++i) {
+            double e = fabs(xerr[i]) / (S->eps_abs + S->eps_rel * (fabs(x[i]) + *dt * fabs(k1[i])));
+            err = (err < e) ? e : err;
+        }
+    }
+    if (err > 1.0) {
+        /* decrease: dt *= max(0.9 err^(-1/(q-1)), 1/5) with error order q = 4 */
+        double fac = 0.9 * M_pow(S->mth, err, -1.0 / 3);
+        *dt = *dt * (fac > 0.2 ? fac : 0.2);
+        if (S->max_dt != 0.0) *dt = *dt > 0 ? (*dt < S->max_dt ? *dt : S->max_dt) : (*dt > -S->max_dt ? *dt : -S->max_dt);
+        return 0;
+    }
+    *t = *t + *dt;
+    if (err < 0.5) {
+        /* increase: err = max(5^-p, err); dt *= 0.9 err^(-1/p) with order p = 5 */
+        double emin = pow(5.0, -5.0);
+        err = emin > err ? emin : err;
+        *dt = *dt * (0.9 * M_pow(S->mth, err, -1.0 / 5.0));
+        if (S->max_dt != 0.0) *dt = *dt > 0 ? (*dt < S->max_dt ? *dt : S->max_dt) : (*dt > -S->max_dt ? *dt : -S->max_dt);
+    }
+    return 1;
+}
+
+static int less_with_sign(double t1, double t2, double dt) {
+    if (dt > 0) return t2 - t1 > DBL_EPSILON;
+    return t1 - t2 > DBL_EPSILON;
+}
+
+/* integrate_adaptive(make_controlled(abs, rel, dopri5), sys, x, 0, t_end, dt).
+ * Returns 0 or JSX_PIX_FAIL_MAXTRIALS. */
+static uint32_t dp_integrate_adaptive(orc_dp *S, double *x, double t_end, double dt, int max_fail) {
+    double t = 0.0, k1[4], xo[4], k7[4];
+    int first = 1;
+    while (less_with_sign(t, t_end, dt)) {
+        if (less_with_sign(t_end, t + dt, dt)) dt = t_end - t;
+        if (first) { S->f(S->ctx, x, k1, t); first = 0; }
+        int tries = 0;
+        for (;;) {
+            int ok = dp_try_step(S, x, k1, &t, xo, k7, &dt);
+            if (tries++ >= max_fail) return JSX_PIX_FAIL_MAXTRIALS;
+            if (ok) break;
+        }
+        for (int i = 0; i < S->ns; ++i) { x[i] = xo[i]; k1[i] = k7[i]; }
+    }
+    return 0;
+}
+
+/* dense-output stepper state (odeint dense_output_runge_kutta, FSAL) */
+typedef struct {
+    double x, k, x_old, k_old, t, t_old, dt;
+    int deriv_ok;
+} orc_dense;
+
+static void dense_init(orc_dense *D, double x0, double t0, double dt0) {
+    D->x = x0; D->t = t0; D->dt = dt0; D->deriv_ok = 0;
+}
+
+static uint32_t dense_do_step(orc_dp *S, orc_dense *D, int max_fail) {
+    if (!D->deriv_ok) { S->f(S->ctx, &D->x, &D->k, D->t); D->deriv_ok = 1; }
+    D->t_old = D->t;
+    double xo, ko;
+    int tries = 0;
+    for (;;) {
+        int ok = dp_try_step(S, &D->x, &D->k, &D->t, &xo, &ko, &D->dt);
+        if (tries++ >= max_fail) return JSX_PIX_FAIL_MAXTRIALS;
+        if (ok) break;
+    }
+    D->x_old = D->x; D->k_old = D->k;
+    D->x = xo; D->k = ko;
+    return 0;
+}
+
+/* Hairer's continuous extension of Dormand-Prince 5(4) as odeint's
+ * runge_kutta_dopri5::calc_state forms it */
+static double dense_calc_state(const orc_dp *S, const orc_dense *D, double t) {
+    double dt = (D->t - D->t_old);
+    double theta = (t - D->t_old) / dt;
+    double X1 = 5.0 * (2558722523.0 - 31403016.0 * theta) / 11282082432.0;
+    double X3 = 100.0 * (882725551.0 - 15701508.0 * theta) / 32700410799.0;
+    double X4 = 25.0 * (443332067.0 - 31403016.0 * theta) / 1880347072.0;
+    double X5 = 32805.0 * (23143187.0 - 3489224.0 * theta) / 199316789632.0;
+    double X6 = 55.0 * (29972135.0 - 7076736.0 * theta) / 822651844.0;
+    double X7 = 10.0 * (7414447.0 - 829305.0 * theta) / 29380423.0;
+    double theta_m_1 = theta - 1.0;
+    double theta_sq = theta * theta;
+    double A = theta_sq * (3.0 - 2.0 * theta);
+    double B = theta_sq * theta_m_1;
+    double C = theta_sq * theta_m_1 * theta_m_1;
+    double Dd = theta * theta_m_1 * theta_m_1;
+    double b1 = A * DP_C1 - C * X1 + Dd;
+    double b3 = A * DP_C3 + C * X3;
+    double b4 = A * DP_C4 - C * X4;
+    double b5 = A * DP_C5 + C * X5;
+    double b6 = A * DP_C6 - C * X6;
+    double b7 = B + C * X7;
+    return D->x_old + dt * b1 * D->k_old + dt * b3 * S->k3[0] + dt * b4 * S->k4[0] + dt * b5 * S->k5[0] +
+           dt * b6 * S->k6[0] + dt * b7 * D->k;
+}
+
+/* ------------------------------------------------------------------------ */
+/* steps_schedule (src/utils.cpp:354-366)                                    */
+
+static int steps_schedule_m(const orc_math *m, double tau, int n_min, int n_max, double tau_min, double tau_max) {
+    if (tau <= tau_min) return n_min;
+    if (tau >= tau_max) return n_max;
+    double b = -log(tau_min);
+    double k = n_max / pow((log(tau_max) + b), 2.0);
+    double n = k * M_pow(m, (M_log(m, tau) + b), 2.0) + n_min;
+    return (int)ceil(n);
+}
+
+int32_t orc_steps_schedule(double tau, int32_t n_min, int32_t n_max, double tau_min, double tau_max) {
+    orc_math m = {ORC_MATH_LIBM};
+    return steps_schedule_m(&m, tau, n_min, n_max, tau_min, tau_max);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Observation: observe_single_pixel and the two integration passes          */
+
+typedef struct {
+    double tau, iquv[4], l;
+    uint32_t flags, n_rhs, n_steps;
+} orc_pix;
+
+static void rhs_adapter(void *ctx, const double *x, double *dxdt, double t) {
+    sys_rhs((orc_sys *)ctx, x, dxdt, t);
+}
+
+/* Observation::observe_single_pixel (Observation.cpp:349-395) */
+static void observe_pixel(const orc_model *m, const jsx_solver_params *P, const double *o, const double *d,
+                          orc_pix *out) {
+    memset(out, 0, sizeof(*out));
+    orc_seg segs[2];
+    int clipped = 0;
+    int nseg = model_hit(m, o, d, segs, &clipped);
+    if (clipped) out->flags |= JSX_PIX_CLIPPED;
+    if (nseg == 0) return;
+    out->flags |= JSX_PIX_HIT;
+    if (nseg == 2) out->flags |= JSX_PIX_NSEG2;
+    uint32_t n_rhs = 0, n_steps = 0;
+    double n_los[3] = {-1. * d[0], -1. * d[1], -1. * d[2]};
+    /* Ray(point_in, ray_direction) normalises again (Ray.cpp:12) */
+    double d2[3] = {d[0], d[1], d[2]};
+    normalize3(d2);
+
+    /* ---- integrate_tau_adaptive (Observation.cpp:126-214) ---- */
+    double background_tau = 0., thickness = 0.;
+    int nseg_live = nseg;
+    for (int si = 0; si < nseg_live; ++si) {
+        double *pin = segs[si].pin, *pout = segs[si].pout;
+        double dv[3];
+        sub3(pout, pin, dv);
+        double length = norm3(dv);
+        double dt = length / P->n;
+        orc_sys S = {m, {pin[0], pin[1], pin[2]}, {d[0], d[1], d[2]},
+                     {n_los[0], n_los[1], n_los[2]}, P->nu, 0, &n_rhs, 0};
+        orc_dp dp;
+        memset(&dp, 0, sizeof dp);
+        dp.f = rhs_adapter; dp.ctx = &S; dp.ns = 1;
+        dp.eps_abs = P->tau_abs_tol; dp.eps_rel = P->tau_rel_tol; dp.max_dt = P->dt_max;
+        dp.n_steps = &n_steps;
+        dp.mth = &m->mth;
+        orc_dense D;
+        dense_init(&D, 0.0, 0.0, dt);
+        int found = 0;
+        uint32_t fail = 0;
+        if (!(D.x >= P->tau_max)) {
+            for (;;) {
+                if (!less_with_sign(D.t, length, D.dt)) break;
+                if (less_with_sign(length, D.t + D.dt, D.dt)) dense_init(&D, D.x, D.t, length - D.t);
+                fail = dense_do_step(&dp, &D, P->max_failed_tries);
+                if (fail) break;
+                if (D.x >= P->tau_max) { found = 1; break; }
+            }
+        } else {
+            found = 1; /* degenerate tau_max <= 0: the range's first state already satisfies it */
+        }
+        if (S.nan_seen) out->flags |= JSX_PIX_NAN;
+        if (fail) { out->flags |= fail; out->n_rhs = n_rhs; out->n_steps = n_steps;
+                    out->tau = background_tau + D.x; out->l = thickness + length; return; }
+        if (found) {
+            double t0 = D.t_old, t1 = D.t, t_m;
+            double x_m = D.x;
+            int it = 0;
+            while (fabs(x_m - P->tau_max) > P->tau_precision) {
+                if (++it > BISECT_CAP) { out->flags |= JSX_PIX_FAIL_BISECT; break; }
+                t_m = 0.5 * (t0 + t1);
+                x_m = dense_calc_state(&dp, &D, t_m);
+                if (x_m > P->tau_max) t1 = t_m;
+                else t0 = t_m;
+            }
+            t_m = 0.5 * (t0 + t1);
+            x_m = dense_calc_state(&dp, &D, t_m);
+            ray_point(pin, d2, t_m, segs[si].pout);
+            out->flags |= JSX_PIX_TAU_CROSSED;
+            nseg_live = si + 1; /* erase further segments */
+        }
+        background_tau += D.x;
+        thickness += length;
+        if (found) break;
+    }
+
+    /* ---- integrate_i_adaptive (:242-274) / full Stokes (:277-308) ---- */
+    double x[4] = {0., 0., 0., 0.};
+    int full = P->stokes == JSX_STOKES_FULL;
+    if (background_tau > P->tau_min) {
+        for (int si = nseg_live - 1; si >= 0; --si) {
+            double *pin = segs[si].pin, *pout = segs[si].pout;
+            double dv[3];
+            sub3(pout, pin, dv);
+            double length = norm3(dv);
+            double auto_n = P->n;
+            if (background_tau > P->tau_n_min)
+                auto_n = steps_schedule_m(&m->mth, background_tau, P->n, P->n_max_factor * P->n, P->tau_n_min,
+                                          P->tau_n_max);
+            double dt = length / auto_n;
+            orc_sys S = {m, {pout[0], pout[1], pout[2]}, {d[0], d[1], d[2]},
+                         {n_los[0], n_los[1], n_los[2]}, P->nu, full ? 4 : 1, &n_rhs, 0};
+            orc_dp dp;
+            memset(&dp, 0, sizeof dp);
+            dp.f = rhs_adapter; dp.ctx = &S; dp.ns = full ? 4 : 1;
+            dp.eps_abs = P->stokes_abs_tol; dp.eps_rel = P->stokes_rel_tol; dp.max_dt = 0.0;
+            dp.n_steps = &n_steps;
+            dp.mth = &m->mth;
+            uint32_t fail = dp_integrate_adaptive(&dp, x, length, dt, P->max_failed_tries);
+            if (S.nan_seen) out->flags |= JSX_PIX_NAN;
+            if (fail) { out->flags |= fail; break; }
+        }
+    } else {
+        out->flags |= JSX_PIX_I_SKIPPED;
+    }
+    for (int i = 0; i < 4; ++i) out->iquv[i] = x[i];
+    if (x[0] != x[0]) out->flags |= JSX_PIX_NAN;
+    out->tau = background_tau;
+    out->l = thickness;
+    out->n_rhs = n_rhs;
+    out->n_steps = n_steps;
+}
+
+/* ------------------------------------------------------------------------ */
+/* image plane: include/logspace.h:25-33, src/Image.cpp:44-185,              */
+/* src/ImagePlane.cpp:11-33                                                  */
+
+typedef struct {
+    int64_t na, nl;           /* n_across, n_along */
+    double *size_along;       /* [nl]               */
+    double *along;            /* [nl] centre        */
+    double *across_half;      /* [na/2][nl]: cumsum_across[col][k] at k*nl+col */
+    double dir[3];
+    double zscale;            /* 1/sin(los)         */
+} orc_plane;
+
+static int plane_build(const jsx_image_desc *im, orc_plane *pl) {
+    if (im->n_across <= 0 || im->n_along <= 0 || (im->n_across % 2) != 0) return JSX_EINVAL;
+    int64_t na = im->n_across, nl = im->n_along, h = na / 2;
+    pl->na = na; pl->nl = nl;
+    pl->size_along = (double *)malloc(sizeof(double) * nl);
+    pl->along = (double *)malloc(sizeof(double) * nl);
+    pl->across_half = (double *)malloc(sizeof(double) * (size_t)(h * nl));
+    if (!pl->size_along || !pl->along || !pl->across_half) return JSX_ENOMEM;
+    /* pyLogspace: first = 10^start, each next *= 10^((stop-start)/(num-1)) */
+    double cur = pow(10, im->lg_pixel_size_start_cm);
+    double base = pow(10, (im->lg_pixel_size_stop_cm - im->lg_pixel_size_start_cm) / (int)(nl - 1));
+    for (int64_t j = 0; j < nl; ++j) { pl->size_along[j] = cur; cur *= base; }
+    /* along centres: partial_sum - size/2 (Image.cpp:70-75) */
+    double acc = 0.0;
+    for (int64_t j = 0; j < nl; ++j) {
+        acc = j == 0 ? pl->size_along[0] : acc + pl->size_along[j];
+        pl->along[j] = acc - 0.5 * pl->size_along[j];
+    }
+    /* across: per column, partial sums of n_across/2 copies of the size (:104-126) */
+    for (int64_t col = 0; col < nl; ++col) {
+        double sz = pl->size_along[col] * 1.0;
+        double c = 0.0;
+        for (int64_t k = 0; k < h; ++k) {
+            c = k == 0 ? sz : c + sz;
+            pl->across_half[k * nl + col] = c - 0.5 * sz;
+        }
+    }
+    double raw[3] = {-sin(im->los_angle), 0, -cos(im->los_angle)};
+    normalize3(raw);  /* Ray::Ray (Ray.cpp:12) */
+    pl->dir[0] = raw[0]; pl->dir[1] = raw[1]; pl->dir[2] = raw[2];
+    pl->zscale = 1. / sin(im->los_angle);
+    return 0;
+}
+
+static void plane_free(orc_plane *pl) {
+    free(pl->size_along); free(pl->along); free(pl->across_half);
+}
+
+/* pixel centre (0, across, along) and ray origin for row i (across), column j */
+static void plane_pixel(const orc_plane *pl, int64_t i, int64_t j, double *coord, double *origin) {
+    int64_t h = pl->na / 2;
+    double across = i < h ? pl->across_half[(h - 1 - i) * pl->nl + j] : -pl->across_half[(i - h) * pl->nl + j];
+    coord[0] = 0; coord[1] = across; coord[2] = pl->along[j];
+    origin[0] = 1. * coord[0]; origin[1] = 1. * coord[1]; origin[2] = pl->zscale * coord[2];
+}
+
+int orc_image(const jsx_image_desc *image, double *sizes, double *coords, double *rays) {
+    orc_plane pl;
+    memset(&pl, 0, sizeof pl);
+    int rc = plane_build(image, &pl);
+    if (rc) { plane_free(&pl); return rc; }
+    for (int64_t i = 0; i < pl.na; ++i)
+        for (int64_t j = 0; j < pl.nl; ++j) {
+            int64_t q = i * pl.nl + j;
+            double c[3], o[3];
+            plane_pixel(&pl, i, j, c, o);
+            if (sizes) sizes[q] = pl.size_along[j];
+            if (coords) { coords[3 * q] = c[0]; coords[3 * q + 1] = c[1]; coords[3 * q + 2] = c[2]; }
+            if (rays) {
+                for (int k = 0; k < 3; ++k) { rays[6 * q + k] = o[k]; rays[6 * q + 3 + k] = pl.dir[k]; }
+            }
+        }
+    plane_free(&pl);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* public entry points                                                       */
+
+int orc_observe(const jsx_jet_desc *jet, const jsx_image_desc *image, const jsx_solver_params *params,
+                const orc_grid *grids, int32_t n_grids, const int64_t *pixel_idx, int64_t n_pix, double *tau,
+                double *I, double *Q, double *U, double *V, double *l, uint32_t *flags, uint32_t *n_rhs,
+                uint32_t *n_steps, int32_t math_mode, int32_t n_threads) {
+    orc_plane pl;
+    memset(&pl, 0, sizeof pl);
+    int rc = plane_build(image, &pl);
+    if (rc) { plane_free(&pl); return rc; }
+    orc_model m;
+    model_init(&m, jet, grids, n_grids, math_mode, params->flags);
+    int64_t total = pl.na * pl.nl;
+    if (!pixel_idx) n_pix = total;
+#ifdef _OPENMP
+    int nt = n_threads > 0 ? n_threads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic) num_threads(nt)
+#endif
+    for (int64_t q = 0; q < n_pix; ++q) {
+        int64_t idx = pixel_idx ? pixel_idx[q] : q;
+        if (idx < 0 || idx >= total) continue;
+        int64_t i = idx / pl.nl, j = idx % pl.nl;
+        double c[3], o[3];
+        plane_pixel(&pl, i, j, c, o);
+        orc_pix px;
+        observe_pixel(&m, params, o, pl.dir, &px);
+        if (tau) tau[q] = px.tau;
+        if (I) I[q] = px.iquv[0];
+        if (Q) Q[q] = px.iquv[1];
+        if (U) U[q] = px.iquv[2];
+        if (V) V[q] = px.iquv[3];
+        if (l) l[q] = px.l;
+        if (flags) flags[q] = px.flags;
+        if (n_rhs) n_rhs[q] = px.n_rhs;
+        if (n_steps) n_steps[q] = px.n_steps;
+    }
+    (void)n_threads;
+    plane_free(&pl);
+    return 0;
+}
+
+int orc_hit(const jsx_jet_desc *jet, const jsx_solver_params *params, const double *rays, int64_t n,
+            int32_t *n_seg, double *segments) {
+    orc_model m;
+    model_init(&m, jet, NULL, 0, ORC_MATH_LIBM, params ? params->flags : 0);
+    for (int64_t q = 0; q < n; ++q) {
+        double o[3] = {rays[6 * q], rays[6 * q + 1], rays[6 * q + 2]};
+        double d[3] = {rays[6 * q + 3], rays[6 * q + 4], rays[6 * q + 5]};
+        normalize3(d); /* Ray::Ray */
+        orc_seg segs[2];
+        int clipped;
+        int ns = model_hit(&m, o, d, segs, &clipped);
+        n_seg[q] = ns;
+        for (int s = 0; s < 2; ++s)
+            for (int k = 0; k < 3; ++k) {
+                segments[12 * q + 6 * s + k] = s < ns ? segs[s].pin[k] : 0.0;
+                segments[12 * q + 6 * s + 3 + k] = s < ns ? segs[s].pout[k] : 0.0;
+            }
+    }
+    return 0;
+}
+
+int orc_coefficients(const jsx_jet_desc *jet, const orc_grid *grids, int32_t n_grids, double nu,
+                     const double *point, const double *n_los, int32_t math_mode, double *out) {
+    orc_model m;
+    model_init(&m, jet, grids, n_grids, math_mode, 0);
+    orc_coef c;
+    jet_coefficients(&m, point, n_los, nu, 1, &c);
+    double v[11] = {c.kI, c.kQ, c.kU, c.kV, c.kF, c.kC, c.hQ, c.eI, c.eQ, c.eU, c.eV};
+    memcpy(out, v, sizeof v);
+    return 0;
+}
+
+int orc_fields(const jsx_jet_desc *jet, const orc_grid *grids, int32_t n_grids, double nu,
+               const double *point, const double *n_los, int32_t math_mode, double *out) {
+    orc_model m;
+    model_init(&m, jet, grids, n_grids, math_mode, 0);
+    orc_frame f;
+    jet_frame(&m, point, n_los, nu, &f);
+    double o[14] = {f.v[0], f.v[1], f.v[2], f.gamma, f.b[0], f.vec ? f.b[1] : 0, f.vec ? f.b[2] : 0,
+                    f.n, f.D, f.nl[0], f.nl[1], f.nl[2], f.nu, (double)f.vec};
+    memcpy(out, o, sizeof o);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* cosmology: Ctd and comoving_transfer_distance2 (utils.cpp:186-208,        */
+/* 373-378), through the same controlled Dormand-Prince                     */
+
+typedef struct { double omega_M, omega_V, gamma_nu; uint32_t n; } orc_ctd;
+
+static void ctd_rhs(void *ctx, const double *x, double *dxdt, double t) {
+    orc_ctd *c = (orc_ctd *)ctx;
+    (void)x;
+    c->n++;
+    dxdt[0] = pow((c->omega_M + (1. + t) * c->gamma_nu) * (1. + t) * (1. + t) * (1. + t) + c->omega_V, -0.5);
+}
+
+double orc_mas_to_pc(double z) {
+    const double H0 = 69.32;
+    orc_ctd c = {0.2865, 0.7134130719051658, 8.69280948342326e-05, 0};
+    uint32_t steps = 0;
+    orc_dp dp;
+    memset(&dp, 0, sizeof dp);
+    orc_math lm = {ORC_MATH_LIBM};
+    dp.f = ctd_rhs; dp.ctx = &c; dp.ns = 1; dp.eps_abs = 1.49e-8; dp.eps_rel = 1.49e-8; dp.n_steps = &steps;
+    dp.mth = &lm;
+    double x = 0.0;
+    dp_integrate_adaptive(&dp, &x, z, 1E-09, 500);
+    double result = (299792.458 / H0) * pow(10., 6.) * x;
+    double d_a = result / (1. + z);
+    return K_MAS_TO_RAD * d_a;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Dormand-Prince checks                                                     */
+
+typedef struct { int problem; uint32_t n; } orc_test_sys;
+
+static void test_rhs(void *ctx, const double *x, double *dxdt, double t) {
+    orc_test_sys *s = (orc_test_sys *)ctx;
+    s->n++;
+    switch (s->problem) {
+    case 0: dxdt[0] = -x[0]; break;
+    case 1: dxdt[0] = 5.0 * t * t * t * t; break;
+    case 2: dxdt[0] = x[0]; break;
+    case 3: dxdt[0] = 4.0 * t * t * t; break;
+    default: dxdt[0] = 0;
+    }
+}
+
+int64_t orc_dopri5_check(int32_t problem, double t_end, double dt, double abs_tol, double rel_tol, double *out) {
+    orc_test_sys ts = {problem, 0};
+    uint32_t steps = 0;
+    orc_dp dp;
+    memset(&dp, 0, sizeof dp);
+    orc_math lm = {ORC_MATH_LIBM};
+    dp.f = test_rhs; dp.ctx = &ts; dp.ns = 1; dp.eps_abs = abs_tol; dp.eps_rel = rel_tol; dp.n_steps = &steps;
+    dp.mth = &lm;
+    if (problem == 0) {
+        double x = 1.0;
+        uint32_t f = dp_integrate_adaptive(&dp, &x, t_end, dt, 500);
+        out[0] = x; out[1] = (double)steps; out[2] = (double)f;
+    } else if (problem == 1 || problem == 3) {
+        double x = 0.0, k1, xo, k7, xerr;
+        test_rhs(&ts, &x, &k1, 0.0);
+        dp_do_step(&dp, &x, &k1, 0.0, dt, &xo, &k7, &xerr);
+        out[0] = xo; out[1] = xerr;
+    } else if (problem == 2) {
+        orc_dense D;
+        dense_init(&D, 1.0, 0.0, dt);
+        dp.eps_abs = 1e30; dp.eps_rel = 0.0; /* accept the first try */
+        dense_do_step(&dp, &D, 500);
+        double th = out[1];
+        out[0] = dense_calc_state(&dp, &D, D.t_old + th * (D.t - D.t_old));
+        out[2] = D.t;
+        out[3] = D.x;
+    }
+    return (int64_t)ts.n;
+}
+
+void orc_jsx_math(int32_t fn, const double *x, const double *y, double *out, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) {
+        switch (fn) {
+        case 0: out[i] = jsx_exp(x[i]); break;
+        case 1: out[i] = jsx_log(x[i]); break;
+        case 2: out[i] = jsx_pow(x[i], y[i]); break;
+        case 3: { double s, c; jsx_sincos_atan(x[i], &s, &c); out[2 * i] = s; out[2 * i + 1] = c; break; }
+        default: out[i] = 0;
+        }
+    }
+}

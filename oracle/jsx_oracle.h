@@ -1,0 +1,91 @@
+/*
+ * jsx_oracle.h -- CPU oracle for the per-pixel radiative-transfer path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library restates jetshow's algorithm on the
+ * CPU so that tests, the smoke check and the benchmark's CPU baseline can
+ * check the HIP kernels against it.  Nothing in jetshow_amd/ may link, load or
+ * call it.
+ *
+ * It consumes the same POD descriptors as the product's C ABI
+ * (include/jetshow_amd.h) so both sides see identical inputs.
+ */
+#ifndef JSX_ORACLE_H
+#define JSX_ORACLE_H
+
+#include <stdint.h>
+#include "../include/jetshow_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* math modes: which implementation of pow / sin(atan) / log the
+ * point-dependent terms use.  Constants that depend only on model parameters
+ * always use the C library, as the product's host code does. */
+#define ORC_MATH_LIBM 0   /* glibc pow, atan, sin, cos, hypot, log          */
+#define ORC_MATH_JSX  1   /* jetshow_amd/csrc/jsx_math.h (bit-exact target) */
+
+typedef struct orc_grid {
+    jsx_grid_desc desc;
+    const double *values;
+} orc_grid;
+
+/* Observation::run restricted to the listed pixel indices (row-major
+ * [n_across][n_along]); pixel_idx == NULL means all pixels.  Output arrays
+ * are indexed like pixel_idx (length n_pix).  n_threads <= 0 uses OpenMP's
+ * default.  Returns 0 or a jsx_status. */
+int orc_observe(const jsx_jet_desc *jet, const jsx_image_desc *image,
+                const jsx_solver_params *params, const orc_grid *grids, int32_t n_grids,
+                const int64_t *pixel_idx, int64_t n_pix,
+                double *tau, double *I, double *Q, double *U, double *V, double *l,
+                uint32_t *flags, uint32_t *n_rhs, uint32_t *n_steps,
+                int32_t math_mode, int32_t n_threads);
+
+/* Geometry::hit for explicit rays (n x 6: origin, direction). */
+int orc_hit(const jsx_jet_desc *jet, const jsx_solver_params *params,
+            const double *rays, int64_t n, int32_t *n_seg, double *segments);
+
+/* Image plane: pixel sizes, pixel centre coordinates (across, along) and ray
+ * origins/directions, each [n_across][n_along] (coordinates x3, rays x6). */
+int orc_image(const jsx_image_desc *image, double *sizes, double *coords, double *rays);
+
+/* Transfer coefficients at one point, lab frame, as Jet::get* return them:
+ * out[0..10] = kI, kQ, kU, kV, kF, kC, hQ, etaI, etaQ, etaU, etaV (sky frame). */
+int orc_coefficients(const jsx_jet_desc *jet, const orc_grid *grids, int32_t n_grids,
+                     double nu, const double *point, const double *n_los,
+                     int32_t math_mode, double *out);
+
+/* Field values at a point: out = v[3], gamma, b'[3] (scalar fields fill b'[0]),
+ * n' (plasma frame), D, n_los'[3], nu'. */
+int orc_fields(const jsx_jet_desc *jet, const orc_grid *grids, int32_t n_grids,
+               double nu, const double *point, const double *n_los,
+               int32_t math_mode, double *out);
+
+/* SimulationInterpolater::interpolated_value on a regular grid. */
+double orc_grid_interp(const orc_grid *grid, const double *point_cm, int32_t math_mode);
+
+/* steps_schedule (src/utils.cpp:354-366). */
+int32_t orc_steps_schedule(double tau, int32_t n_min, int32_t n_max, double tau_min,
+                           double tau_max);
+
+/* mas_to_pc (src/utils.cpp:186-208) through the oracle's Dormand-Prince. */
+double orc_mas_to_pc(double z);
+
+/* Dormand-Prince specification checks on built-in problems:
+ *   problem 0: y' = -y, y(0) = 1, controlled integrate_adaptive to t_end
+ *   problem 1: y' = 5 t^4 (quadrature), one do_step of size dt from 0
+ *   problem 2: y' = y, dense-output stepper: one step of dt, then calc_state at
+ *              theta = out[1] (in) -> out[0] value
+ *   problem 3: y' = 4 t^3, one do_step, returns error estimate
+ * out: problem-specific (see tests). Returns number of RHS evaluations. */
+int64_t orc_dopri5_check(int32_t problem, double t_end, double dt, double abs_tol,
+                         double rel_tol, double *out);
+
+/* jsx_math functions compiled into the oracle (for ulp tests vs glibc). */
+void orc_jsx_math(int32_t fn, const double *x, const double *y, double *out, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

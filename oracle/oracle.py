@@ -1,0 +1,156 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py as the checker.  The product never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+MATH_LIBM, MATH_JSX = 0, 1
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.orc_mas_to_pc.restype = C.c_double
+        L.orc_mas_to_pc.argtypes = [C.c_double]
+        L.orc_grid_interp.restype = C.c_double
+        L.orc_dopri5_check.restype = C.c_int64
+        L.orc_dopri5_check.argtypes = [C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_void_p]
+        L.orc_steps_schedule.restype = C.c_int32
+        L.orc_steps_schedule.argtypes = [C.c_double, C.c_int32, C.c_int32, C.c_double, C.c_double]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def _grid_type():
+    from jetshow_amd import abi
+
+    class OrcGrid(C.Structure):
+        _fields_ = [("desc", abi.GridDesc), ("values", C.c_void_p)]
+    return OrcGrid
+
+
+_ORC_GRID = None
+
+
+def make_grids(grids):
+    """grids: list of (abi.GridDesc, values ndarray) -> (ctypes array, keepalive)."""
+    global _ORC_GRID
+    if _ORC_GRID is None:
+        _ORC_GRID = _grid_type()
+    if not grids:
+        return None, 0, []
+    arr = (_ORC_GRID * len(grids))()
+    keep = []
+    for i, (d, v) in enumerate(grids):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        keep.append(v)
+        arr[i].desc = d
+        arr[i].values = v.ctypes.data
+    return arr, len(grids), keep
+
+
+def observe(jet, image, params, pixel_idx=None, math=MATH_JSX, threads=0, grids=None):
+    """Observation::run on the CPU for the listed pixel indices (all if None).
+    Returns dict of 1-D arrays indexed like pixel_idx."""
+    n_all = image.n_across * image.n_along
+    if pixel_idx is None:
+        idx = None
+        n = n_all
+    else:
+        idx = np.ascontiguousarray(pixel_idx, dtype=np.int64)
+        n = len(idx)
+    res = {k: np.zeros(n) for k in ("tau", "I", "Q", "U", "V", "l")}
+    for k in ("flags", "n_rhs", "n_steps"):
+        res[k] = np.zeros(n, dtype=np.uint32)
+    garr, ng, keep = make_grids(grids or [])
+    rc = lib().orc_observe(C.byref(jet), C.byref(image), C.byref(params), garr, ng, _p(idx), C.c_int64(n),
+                           _p(res["tau"]), _p(res["I"]), _p(res["Q"]), _p(res["U"]), _p(res["V"]), _p(res["l"]),
+                           _p(res["flags"]), _p(res["n_rhs"]), _p(res["n_steps"]), int(math), int(threads))
+    del keep
+    if rc:
+        raise RuntimeError("orc_observe failed: %d" % rc)
+    return res
+
+
+def hit(jet, rays, params=None):
+    r = np.ascontiguousarray(rays, dtype=np.float64).reshape(-1, 6)
+    n = r.shape[0]
+    ns = np.zeros(n, dtype=np.int32)
+    seg = np.zeros((n, 2, 6))
+    lib().orc_hit(C.byref(jet), C.byref(params) if params is not None else None, _p(r), C.c_int64(n), _p(ns), _p(seg))
+    return ns, seg
+
+
+def image(img):
+    n = img.n_across * img.n_along
+    sizes = np.zeros(n)
+    coords = np.zeros((n, 3))
+    rays = np.zeros((n, 6))
+    rc = lib().orc_image(C.byref(img), _p(sizes), _p(coords), _p(rays))
+    if rc:
+        raise RuntimeError("orc_image failed: %d" % rc)
+    return sizes, coords, rays
+
+
+def coefficients(jet, nu, point, n_los, math=MATH_JSX, grids=None):
+    out = np.zeros(11)
+    garr, ng, keep = make_grids(grids or [])
+    lib().orc_coefficients(C.byref(jet), garr, ng, C.c_double(nu), _p(np.asarray(point, float)),
+                           _p(np.asarray(n_los, float)), int(math), _p(out))
+    return out
+
+
+def fields(jet, nu, point, n_los, math=MATH_JSX, grids=None):
+    out = np.zeros(14)
+    garr, ng, keep = make_grids(grids or [])
+    lib().orc_fields(C.byref(jet), garr, ng, C.c_double(nu), _p(np.asarray(point, float)),
+                     _p(np.asarray(n_los, float)), int(math), _p(out))
+    return out
+
+
+def grid_interp(desc, values, point_cm, math=MATH_JSX):
+    garr, ng, keep = make_grids([(desc, values)])
+    return lib().orc_grid_interp(C.byref(garr[0]), _p(np.asarray(point_cm, float)), int(math))
+
+
+def mas_to_pc(z):
+    return lib().orc_mas_to_pc(float(z))
+
+
+def steps_schedule(tau, n_min, n_max, tau_min=0.1, tau_max=100.0):
+    return lib().orc_steps_schedule(float(tau), n_min, n_max, tau_min, tau_max)
+
+
+def dopri5_check(problem, t_end=1.0, dt=0.1, abs_tol=1e-10, rel_tol=1e-10, theta=0.5):
+    out = np.zeros(4)
+    out[1] = theta
+    nrhs = lib().orc_dopri5_check(problem, t_end, dt, abs_tol, rel_tol, _p(out))
+    return out, nrhs
+
+
+def jsx_math(fn, x, y=None):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y if y is not None else np.zeros_like(x), dtype=np.float64)
+    out = np.zeros(len(x) * (2 if fn == 3 else 1))
+    lib().orc_jsx_math(int(fn), _p(x), _p(y), _p(out), C.c_int64(len(x)))
+    return out.reshape(-1, 2) if fn == 3 else out

@@ -1,0 +1,66 @@
+"""jsx_math (shared deterministic exp/log/pow) against the C library: the
+shared implementation must stay within a few ulp of glibc over the argument
+ranges the solver uses, and handle the special cases the same way."""
+import numpy as np
+
+from oracle import oracle
+
+
+def ulp_err(a, b):
+    a = np.asarray(a, float)
+    b = np.asarray(b, float)
+    sp = np.spacing(np.abs(b))
+    return np.abs(a - b) / sp
+
+
+def test_exp():
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.uniform(-700, 700, 200000), rng.uniform(-1, 1, 100000), rng.uniform(-745, -708, 2000)])
+    got = oracle.jsx_math(0, x)
+    ref = np.exp(x)
+    norm = ref > 2.3e-308
+    assert ulp_err(got[norm], ref[norm]).max() <= 1.0
+    assert np.abs(got[~norm] - ref[~norm]).max() <= 2 * 4.9e-324
+    assert oracle.jsx_math(0, [800.0])[0] == np.inf and oracle.jsx_math(0, [-800.0])[0] == 0.0
+
+
+def test_log():
+    rng = np.random.default_rng(1)
+    x = np.concatenate([10 ** rng.uniform(-300, 300, 200000), rng.uniform(0.5, 2, 100000), [1e-310, 5e-324]])
+    got = oracle.jsx_math(1, x)
+    ref = np.log(x)
+    assert ulp_err(got, ref).max() <= 1.0
+    assert oracle.jsx_math(1, [1.0])[0] == 0.0
+
+
+def test_pow():
+    rng = np.random.default_rng(2)
+    x = 10 ** rng.uniform(-12, 12, 200000)
+    y = rng.uniform(-6, 6, 200000)
+    got = oracle.jsx_math(2, x, y)
+    ref = np.power(x, y)
+    ok = (ref > 1e-300) & (ref < 1e300)
+    assert ulp_err(got[ok], ref[ok]).max() <= 2.0
+    # special cases used by the solver: y in {0, 1, 2, -1}, x == 1, negative x with integer y
+    xs = np.array([3.0, 3.0, 3.0, 3.0, 1.0, -2.0, -2.0, 0.0])
+    ys = np.array([0.0, 1.0, 2.0, -1.0, 7.3, 3.0, 2.0, -1.5])
+    with np.errstate(divide="ignore"):
+        assert np.array_equal(oracle.jsx_math(2, xs, ys), np.power(xs, ys))
+
+
+def test_sincos_atan():
+    rng = np.random.default_rng(3)
+    u = np.concatenate([rng.standard_normal(100000) * 10 ** rng.uniform(-3, 5, 100000), [0.0, -0.0, 1e300, -1e300]])
+    sc = oracle.jsx_math(3, u)
+    fi = np.arctan(u)
+    # absolute agreement with the C library's atan/sin/cos composition ...
+    assert np.abs(sc[:, 0] - np.sin(fi)).max() <= 4e-16
+    assert np.abs(sc[:, 1] - np.cos(fi)).max() <= 4e-16
+    # ... and relative accuracy against the exact values (extended precision)
+    ul = u.astype(np.longdouble)
+    q = np.sqrt(1 + ul * ul)
+    big = np.abs(ul) > 1e150
+    c_exact = np.where(big, 1 / np.abs(ul), 1 / q)
+    s_exact = np.where(big, np.sign(ul), ul / q)
+    assert (np.abs(sc[:, 1] - c_exact) <= 5e-16 * c_exact).all()
+    assert (np.abs(sc[:, 0] - s_exact) <= 5e-16 * np.abs(s_exact)).all()
