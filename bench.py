@@ -1,0 +1,237 @@
+"""Benchmark: integrated pixels/s of the per-pixel radiative-transfer solver.
+
+Workload (one step = one full image through the hot path, inputs resident in
+HBM): full-Stokes IQUV on the paraboloid jet (r0 = 0.1 pc at z = 1 pc, helical
+B 0.2 G (r/pc)^-1 pitch 45 deg, N = 500 (r/pc)^-2 with s = 2.5 plus cold
+thermal electrons at 0.1 x, Gamma 2 -> 15 over z in [0, 5] pc, LOS 0.05 rad,
+512 x 512 pixels of 0.01 pc, 15.4 GHz observed at z = 0.1).
+
+Timed: the batched mixed-precision solver (persistent work-queue kernel, FP32
+coefficients and stages, FP64 roots / step control / tau placement).  Beside
+it: the plain FP64 kernel (one thread per pixel), the error of the mixed
+result against FP64, per-image p50/p99 latency, the roofline of the dominant
+kernel and the CPU oracle on a bounded pixel sample.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+OPT_PERSISTENT = 0x100
+OPT_APEX_FIRST = 0x200
+
+# FP64 / FP32 VALU operations per right-hand-side evaluation of each kernel,
+# measured with rocprofv3 SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_{F64,F32} counters
+# on single-ray launches (FMA counted as 2); see profiles/.
+OPS_PER_RHS = {
+    "mixed_full": {"f64": 74.0, "f32": 190.0},
+    "fp64_full": {"f64": 760.0, "f32": 0.0},
+}
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n-side", type=int, default=512)
+    ap.add_argument("--rel-tol", type=float, default=1e-6, help="mixed-precision Stokes rel. tolerance")
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--json-out", default=None)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="single-wavefront launches (2 rays) of the FP64 and mixed full-Stokes kernels for "
+                         "rocprofv3 instruction counters; prints the RHS evaluations per launch")
+    return ap.parse_args()
+
+
+def calibrate():
+    """One wavefront per launch: every loop trip of that wavefront is one RHS
+    evaluation for each active lane, so counters / max(n_rhs) = VALU operations
+    per RHS evaluation per lane."""
+    from jetshow_amd import abi, host, models
+    ctx = host.Context(0)
+    j, im, p = models.paraboloid_helical_model(n_side=2, pixel_pc=0.3)
+    im.n_along = 1
+    for name, q in (("fp64_full", p), ("mixed_full", models.with_precision(p, abi.PREC_MIXED, rel_tol=1e-6))):
+        r = ctx.observe(j, im, q)
+        print(json.dumps({"calibration": name, "max_n_rhs": int(r["n_rhs"].max()),
+                          "n_rhs": r["n_rhs"].reshape(-1).tolist()}), flush=True)
+    ctx.close()
+
+
+def main():
+    a = parse()
+    if a.calibrate:
+        calibrate()
+        return
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (every ray is independent; "
+              "see DESIGN.md for how images and models would shard). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    import ctypes as C
+
+    import torch
+
+    from jetshow_amd import abi, host, models
+
+    lib = host.lib()
+    dev = torch.device("cuda:0")
+    torch.cuda.init()
+    ctx = host.Context(0)
+    j, im, p64 = models.paraboloid_helical_model(n_side=a.n_side)
+    npix = im.n_across * im.n_along
+    pm = models.with_precision(p64, abi.PREC_MIXED, rel_tol=a.rel_tol)
+    pm.flags |= OPT_PERSISTENT | OPT_APEX_FIRST
+
+    def planes():
+        t = {k: torch.zeros(npix, dtype=torch.float64, device=dev) for k in ("tau", "I", "Q", "U", "V", "l")}
+        for k in ("flags", "n_rhs", "n_steps"):
+            t[k] = torch.zeros(npix, dtype=torch.int32, device=dev)
+        o = abi.Outputs()
+        for k in ("tau", "I", "Q", "U", "V", "l"):
+            setattr(o, k, C.cast(C.c_void_p(t[k].data_ptr()), C.POINTER(C.c_double)))
+        for k in ("flags", "n_rhs", "n_steps"):
+            setattr(o, k, C.cast(C.c_void_p(t[k].data_ptr()), C.POINTER(C.c_uint32)))
+        return t, o
+
+    stream = torch.cuda.Stream()
+    sh = C.c_void_p(stream.cuda_stream)
+
+    def run(params, o):
+        rc = lib.jsx_observe(ctx.handle, C.byref(j), C.byref(im), C.byref(params), C.byref(o), sh)
+        if rc:
+            raise RuntimeError(lib.jsx_last_error(ctx.handle).decode())
+
+    def timed(params, steps, warmup):
+        t, o = planes()
+        for _ in range(warmup):
+            run(params, o)
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        t0 = time.perf_counter()
+        with torch.cuda.stream(stream):
+            for e0, e1 in evs:
+                e0.record(stream)
+                run(params, o)
+                e1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        lat = np.array([e0.elapsed_time(e1) for e0, e1 in evs])
+        return t, lat, wall
+
+    # headline: batched mixed-precision full-Stokes solver
+    tm, lat_m, wall_m = timed(pm, a.steps, a.warmup)
+    ms_step = float(lat_m.sum() / a.steps)
+    value = npix * a.steps / (lat_m.sum() / 1e3)
+    res = {k: v.cpu().numpy() for k, v in tm.items()}
+    n_rhs_tot = float(res["n_rhs"].astype(np.float64).sum())
+
+    # plain FP64 kernel and accuracy of the mixed result against it
+    plain = {}
+    if not a.no_plain:
+        t64, lat64, _ = timed(p64, max(1, a.plain_steps), 0)
+        r64 = {k: v.cpu().numpy() for k, v in t64.items()}
+        hit = (r64["flags"] & abi.PIX_HIT) != 0
+        assert np.array_equal(hit, (res["flags"] & abi.PIX_HIT) != 0), "hit masks differ between FP64 and mixed"
+        I64, Im = r64["I"][hit], res["I"][hit]
+        pos = I64 > 0
+        rel = np.abs(Im[pos] - I64[pos]) / I64[pos]
+        peak = I64.max()
+        qerr = max(np.abs(res[k][hit] - r64[k][hit]).max() for k in ("Q", "U", "V")) / peak
+        evpa = 0.5 * np.degrees(np.arctan2(res["U"][hit], res["Q"][hit]) - np.arctan2(r64["U"][hit], r64["Q"][hit]))
+        evpa = (evpa + 90.0) % 180.0 - 90.0
+        pol = np.hypot(r64["Q"][hit], r64["U"][hit]) > 1e-3 * peak
+        plain = {
+            "fp64_plain_pixels_per_s": npix / (lat64.mean() / 1e3),
+            "fp64_plain_ms_per_image": float(lat64.mean()),
+            "speedup_vs_fp64_plain": float(lat64.mean() / (lat_m.mean())),
+            "max_rel_err_I": float(rel.max()) if rel.size else 0.0,
+            "p99_rel_err_I": float(np.percentile(rel, 99)) if rel.size else 0.0,
+            "max_err_QUV_over_peak_I": float(qerr),
+            "max_evpa_err_deg": float(np.abs(evpa[pol]).max()) if pol.any() else 0.0,
+            "hit_pixels": int(hit.sum()),
+            "hit_mask_identical": True,
+        }
+
+    # roofline of the dominant kernel (the mixed persistent kernel): operations
+    # per RHS (per lane) x RHS evaluations / kernel time
+    ops = OPS_PER_RHS["mixed_full"]
+    f64_ops, f32_ops = ops["f64"] * n_rhs_tot, ops["f32"] * n_rhs_tot
+    kern_s = lat_m.mean() / 1e3
+    achieved = (f64_ops + f32_ops) / kern_s / 1e12
+    peak = (f64_ops + f32_ops) / (f64_ops / PEAK_TFLOPS["f64"] + f32_ops / PEAK_TFLOPS["f32"])
+    roofline = {"bound": "mfma", "pipe": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                "frac": achieved / peak, "traffic": None,
+                "note": "compute-bound VALU code (no MFMA); peak = FP64/FP32 vector peaks weighted by op mix"}
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(j, im, p64, a.cpu_seconds)
+
+    out = {
+        "metric": "integrated pixels/s at <=1e-5 rel Stokes-I err vs FP64; p50/p99 solve latency",
+        "value": value,
+        "unit": "pixels/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (analytic jet model, seeded)",
+        "config": {"workload": "paraboloid helical-B full-Stokes IQUV, 512x512 x 1 frequency, FP32+compensated "
+                               "persistent solver vs FP64",
+                   "n_pixels": npix, "rel_tol_mixed": max(a.rel_tol, 1e-6), "rel_tol_fp64": p64.stokes_rel_tol},
+        "latency_ms": {"p50": float(np.percentile(lat_m, 50)), "p99": float(np.percentile(lat_m, 99))},
+        "rhs_evaluations": n_rhs_tot,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    out.update(plain)
+    line = json.dumps(out)
+    print(line, flush=True)
+    if a.json_out:
+        with open(a.json_out, "w") as f:
+            f.write(line + "\n")
+    ctx.close()
+
+
+def cpu_baseline(j, im, p64, seconds):
+    """The CPU oracle (C-library math) timed on a strided pixel sample of the
+    same image with all host cores; extrapolated to pixels/s."""
+    from oracle import oracle
+    ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    ncores = max(1, min(ncores, 16))
+    npix = im.n_across * im.n_along
+    stride = 64
+    done, t_used, count = 0, 0.0, 0
+    rng_off = 0
+    while t_used < seconds and rng_off < stride:
+        idx = np.arange(rng_off, npix, stride, dtype=np.int64)
+        t0 = time.perf_counter()
+        oracle.observe(j, im, p64, idx, math=oracle.MATH_LIBM, threads=ncores)
+        t_used += time.perf_counter() - t0
+        count += len(idx)
+        rng_off += 7
+        done += 1
+    return {"value": count / t_used, "unit": "pixels/s", "cores": ncores, "kind": "port",
+            "sample": "%d strided pixel sets (every %dth pixel, offsets 0,7,...) = %d pixels of the same image, "
+                      "%.1f s" % (done, stride, count, t_used)}
+
+
+if __name__ == "__main__":
+    main()

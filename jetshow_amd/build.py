@@ -29,6 +29,30 @@ def sources():
     return [os.path.join(CSRC, "jsx_host.hip")]
 
 
+CXX_DIR = os.path.join(HERE, "cxx")
+
+
+def build_cxx(verbose=False):
+    """The reference's class API (cxx/jetshow.cpp) and the pyjetshow module:
+    host C++ only, linked against libjetshow_amd.so (rpath $ORIGIN)."""
+    import sysconfig
+
+    import pybind11
+    ext = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
+    out = os.path.join(HERE, "pyjetshow" + ext)
+    inc = [pybind11.get_include(), sysconfig.get_paths()["include"], CXX_DIR]
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall"] + ["-I" + i for i in inc] + [
+        os.path.join(CXX_DIR, "jetshow.cpp"), os.path.join(CXX_DIR, "pyjetshow.cpp"),
+        "-L" + HERE, "-ljetshow_amd", "-Wl,-rpath,$ORIGIN", "-o", out]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stderr)
+        raise RuntimeError("g++ failed building %s" % out)
+    return out
+
+
 def build(verbose=False, extra=()):
     cmd = [HIPCC] + FLAGS + list(extra) + ["-o", OUT] + sources()
     if verbose:
@@ -40,6 +64,7 @@ def build(verbose=False, extra=()):
         raise RuntimeError("hipcc failed building %s" % OUT)
     if verbose and msgs:
         print("\n".join(msgs))
+    build_cxx(verbose)
     return OUT
 
 

@@ -229,22 +229,32 @@ __device__ inline int d_clip_seg(const DModel &m, DSeg &s) {
     return 2;
 }
 
-/* Jet::hit + clip; *clipped set if the clip changed anything */
+/* Jet::hit + clip; *clipped set if the clip changed anything.  The two
+ * possible segments are handled with constant indices so they stay in
+ * registers. */
 __device__ inline int d_model_hit(const DModel &m, const double *o, const double *d, DSeg *segs, int *clipped) {
     int n = 0;
     *clipped = 0;
     if (m.geometry == JSX_GEOM_CONE) n = d_cone_hit(m, o, d, segs);
     else if (m.geometry == JSX_GEOM_PARABOLOID) n = d_paraboloid_hit(m, o, d, segs);
     else if (m.geometry == JSX_GEOM_CYLINDER) n = d_cylinder_hit(m, o, d, segs);
-    if (m.clip) {
-        int k = 0;
-        for (int i = 0; i < n; ++i) {
-            DSeg s = segs[i];
-            int r = d_clip_seg(m, s);
-            if (r != 1) *clipped = 1;
-            if (r) segs[k++] = s;
+    if (m.clip && n > 0) {
+        DSeg a = segs[0], b = segs[1];
+        int ra = d_clip_seg(m, a);
+        int rb = n > 1 ? d_clip_seg(m, b) : 1;
+        if (ra != 1 || rb != 1) *clipped = 1;
+        if (n == 1) {
+            segs[0] = a;
+            n = ra ? 1 : 0;
+        } else if (ra && rb) {
+            segs[0] = a; segs[1] = b;
+        } else if (ra) {
+            segs[0] = a; n = 1;
+        } else if (rb) {
+            segs[0] = b; n = 1;
+        } else {
+            n = 0;
         }
-        n = k;
     }
     return n;
 }

@@ -21,6 +21,7 @@
 #include "jsx_kernels.h"
 #include "jsx_mixed.h"
 
+
 #define JSX_OPT_PERSISTENT 0x100u   /* internal: persistent queue kernel        */
 
 struct GridRec {
@@ -46,6 +47,11 @@ struct jsx_ctx {
     int32_t next_grid = 1;
     std::map<std::vector<double>, ImageRec> images;
     unsigned long long *d_head = nullptr;
+    /* ring of launch-constant slots: pinned host mirror -> device, one event per slot */
+    static const int kSlots = 64;
+    DParamsM *h_params = nullptr, *d_params = nullptr;   /* slots sized for the larger struct */
+    hipEvent_t slot_ev[kSlots];
+    int slot_next = 0;
     int n_cu = 256;
     int persist_blocks_per_cu = 4;
     int persist_block = 256;
@@ -89,11 +95,14 @@ extern "C" int jsx_ctx_create(int device, size_t hbm_budget, jsx_ctx **out) {
     (void)hipMemGetInfo(&free_b, &total_b);
     ctx->budget = hbm_budget ? hbm_budget : (size_t)(0.9 * (double)free_b);
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return JSX_EHIP; }
-    if (hipMalloc(&ctx->d_head, 64 * sizeof(unsigned long long)) != hipSuccess) {
+    if (hipMalloc(&ctx->d_head, 64 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&ctx->d_params, sizeof(DParamsM) * jsx_ctx::kSlots) != hipSuccess ||
+        hipHostMalloc(&ctx->h_params, sizeof(DParamsM) * jsx_ctx::kSlots, hipHostMallocDefault) != hipSuccess) {
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return JSX_ENOMEM;
     }
+    for (int i = 0; i < jsx_ctx::kSlots; ++i) (void)hipEventCreateWithFlags(&ctx->slot_ev[i], hipEventDisableTiming);
     *out = ctx;
     return JSX_OK;
 }
@@ -108,6 +117,9 @@ extern "C" int jsx_ctx_destroy(jsx_ctx *ctx) {
         (void)hipFree(kv.second.along_f); (void)hipFree(kv.second.across_half_f);
     }
     (void)hipFree(ctx->d_head);
+    (void)hipFree(ctx->d_params);
+    (void)hipHostFree(ctx->h_params);
+    for (int i = 0; i < jsx_ctx::kSlots; ++i) (void)hipEventDestroy(ctx->slot_ev[i]);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return JSX_OK;
@@ -325,6 +337,65 @@ static DOut lower_out(const jsx_outputs &o) {
     return d;
 }
 
+/* stage launch constants into a device slot on stream st */
+static int stage_bytes(jsx_ctx *ctx, const void *src, size_t n, hipStream_t st, void **dev) {
+    int k = ctx->slot_next;
+    ctx->slot_next = (k + 1) % jsx_ctx::kSlots;
+    HIPCHECK(ctx, hipEventSynchronize(ctx->slot_ev[k]));
+    DParamsM *h = ctx->h_params + k;
+    std::memcpy((void *)h, src, n);
+    HIPCHECK(ctx, hipMemcpyAsync(ctx->d_params + k, h, n, hipMemcpyHostToDevice, st));
+    HIPCHECK(ctx, hipEventRecord(ctx->slot_ev[k], st));
+    *dev = (void *)(ctx->d_params + k);
+    return JSX_OK;
+}
+
+static int stage_params(jsx_ctx *ctx, const DModel &m, const DImage &D, const DSolver &S, hipStream_t st,
+                        DParams **dev) {
+    DParams h;
+    h.m = m; h.im = D; h.S = S;
+    return stage_bytes(ctx, &h, sizeof h, st, (void **)dev);
+}
+
+/* FP32 constants of the mixed path (see jsx_mixed.h for the formulas) */
+static void lower_mixed(const jsx_jet_desc &j, const jsx_solver_params &p, const DModel &m, const DImage &D,
+                        MConsts &c) {
+    std::memset(&c, 0, sizeof c);
+    const double PI = 3.141592653589793238462643383279502884, PC = 3.0856775814671913e+18;
+    const double ME = 9.10938356e-28, QE = 4.80320467299766e-10, CL = 29979245800.0;
+    const double CB = QE / (2. * PI * ME * CL), CP = QE * QE / (PI * ME), nu = p.nu, s = j.s;
+    const bool vec = !(j.bfield == JSX_B_RANDOM_SCALAR || j.bfield == JSX_B_RANDOM_SCALAR_Z ||
+                       j.bfield == JSX_B_GRID_SCALAR);
+    const double fk = vec ? m.f_ki_v : m.f_ki_s, fe = vec ? m.f_ei_v : m.f_ei_s;
+    const double g = CB / nu;
+    c.A_k = (float)(PI * CP * CB * fk * pow(g, s / 2.) / (CL * nu * nu) * PC);
+    c.A_e = (float)(PI * CP * CB * ME * fe * pow(g, (s - 1.) / 2.) / CL * PC);
+    c.A_kv = (float)(PI * CP * CB * m.f_kv * pow(g, (s + 1.) / 2.) / (CL * nu * nu) * PC);
+    c.A_ev = (float)(PI * CP * CB * ME * m.f_ev * pow(g, s / 2.) / CL * PC);
+    c.A_F = (float)(2. * PI * CP * CB / (CL * nu * nu) * PC);
+    c.A_C = (float)(PI * CP * CB * CB / (CL * nu * nu * nu) * PC);
+    c.r_kq = (float)m.r_kq; c.r_eq = (float)m.r_eq; c.f_kFr = (float)m.f_kFr;
+    c.kCr_pre = (float)m.kCr_pre; c.kCr_gm = (float)m.kCr_gm; c.kC_xfac = (float)pow(g, (s - 2.) / 2.);
+    c.e_ki = (float)m.e_ki; c.e_ei = (float)m.e_ei; c.e_kv = (float)m.e_kv; c.e_kCr = (float)m.e_kCr;
+    c.n0 = (float)j.n0; c.b0 = (float)j.b0; c.n_negexp = (float)m.n_negexp; c.b_negexp = (float)m.b_negexp;
+    double gam = j.gamma0 > 1.0 ? j.gamma0 : 1.0;
+    double beta = sqrt(1. - 1. / (gam * gam));
+    c.gam_flat = (float)gam; c.beta_flat = (float)beta; c.omb_flat = (float)(1. / (gam * gam * (1. + beta)));
+    double nz = -D.dir[2];
+    /* 1 - cos(theta) without cancellation: |n - z|^2 / 2 */
+    c.one_minus_mu = (float)(0.5 * (D.dir[0] * D.dir[0] + D.dir[1] * D.dir[1] + (nz - 1.0) * (nz - 1.0)));
+    for (int i = 0; i < 3; ++i) { c.n_los[i] = (float)(-D.dir[i]); c.axis[i] = (float)m.axis[i]; }
+    c.sin_pitch = (float)m.sin_pitch; c.cos_pitch = (float)m.cos_pitch;
+    c.thermal_fraction = (float)j.thermal_fraction;
+    c.inv_pc = 1.0 / PC;
+    /* FP32 stage values put a floor under the usable tolerances */
+    c.tau_abs = p.tau_abs_tol > 1e-6 ? p.tau_abs_tol : 1e-6;
+    c.tau_rel = p.tau_rel_tol > 1e-6 ? p.tau_rel_tol : 1e-6;
+    c.st_abs = p.stokes_abs_tol;
+    c.st_rel = p.stokes_rel_tol > 1e-6 ? p.stokes_rel_tol : 1e-6;
+    c.dt_max_pc = p.dt_max / PC;
+}
+
 /* ---------------- observe ---------------- */
 static int observe_locked(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_image_desc *image,
                           const jsx_solver_params *params, const jsx_outputs *out, hipStream_t st) {
@@ -342,30 +413,53 @@ static int observe_locked(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_image
     DOut O = lower_out(*out);
     int64_t npix = image->n_across * image->n_along;
     if (npix == 0) return JSX_OK;
-    if (params->precision == JSX_PREC_MIXED) {
-        return jsx_launch_mixed(ctx->n_cu, ctx->d_head, m, D, rec->along_f, rec->across_half_f, S, O, npix, st,
-                                params->flags) == hipSuccess
-                   ? JSX_OK
-                   : fail(ctx, JSX_EHIP, "mixed-precision launch failed: %s", hipGetErrorString(hipGetLastError()));
-    }
     const bool full = params->stokes == JSX_STOKES_FULL;
+    if (params->precision == JSX_PREC_MIXED) {
+        DParamsM hm;
+        hm.m = m; hm.im = D; hm.S = S;
+        lower_mixed(*jet, *params, m, D, hm.c);
+        DParamsM *dpm = nullptr;
+        if ((rc = stage_bytes(ctx, &hm, sizeof hm, st, (void **)&dpm))) return rc;
+        if (params->flags & JSX_OPT_PERSISTENT) {
+            HIPCHECK(ctx, hipMemsetAsync(ctx->d_head, 0, sizeof(unsigned long long), st));
+            int blocks = ctx->n_cu * ctx->persist_blocks_per_cu;
+            int order = (params->flags & 0x200u) ? 1 : 0;
+            if (full)
+                hipLaunchKernelGGL(jsx_km_persist<4>, dim3(blocks), dim3(ctx->persist_block), 0, st, dpm, O, npix,
+                                   ctx->d_head, order);
+            else
+                hipLaunchKernelGGL(jsx_km_persist<1>, dim3(blocks), dim3(ctx->persist_block), 0, st, dpm, O, npix,
+                                   ctx->d_head, order);
+        } else {
+            int threads = 256;
+            int64_t blocks = (npix + threads - 1) / threads;
+            if (full)
+                hipLaunchKernelGGL(jsx_km_plain<4>, dim3((unsigned)blocks), dim3(threads), 0, st, dpm, O, npix);
+            else
+                hipLaunchKernelGGL(jsx_km_plain<1>, dim3((unsigned)blocks), dim3(threads), 0, st, dpm, O, npix);
+        }
+        HIPCHECK(ctx, hipGetLastError());
+        return JSX_OK;
+    }
+    DParams *dp = nullptr;
+    if ((rc = stage_params(ctx, m, D, S, st, &dp))) return rc;
     if (params->flags & JSX_OPT_PERSISTENT) {
         HIPCHECK(ctx, hipMemsetAsync(ctx->d_head, 0, sizeof(unsigned long long), st));
         int blocks = ctx->n_cu * ctx->persist_blocks_per_cu;
         int order = (params->flags & 0x200u) ? 1 : 0;
         if (full)
-            hipLaunchKernelGGL(jsx_k_persist<4>, dim3(blocks), dim3(ctx->persist_block), 0, st, m, D, S, O, npix,
+            hipLaunchKernelGGL(jsx_k_persist<4>, dim3(blocks), dim3(ctx->persist_block), 0, st, dp, O, npix,
                                ctx->d_head, order);
         else
-            hipLaunchKernelGGL(jsx_k_persist<1>, dim3(blocks), dim3(ctx->persist_block), 0, st, m, D, S, O, npix,
+            hipLaunchKernelGGL(jsx_k_persist<1>, dim3(blocks), dim3(ctx->persist_block), 0, st, dp, O, npix,
                                ctx->d_head, order);
     } else {
         int threads = 256;
         int64_t blocks = (npix + threads - 1) / threads;
         if (full)
-            hipLaunchKernelGGL(jsx_k_plain<4>, dim3((unsigned)blocks), dim3(threads), 0, st, m, D, S, O, npix);
+            hipLaunchKernelGGL(jsx_k_plain<4>, dim3((unsigned)blocks), dim3(threads), 0, st, dp, O, npix);
         else
-            hipLaunchKernelGGL(jsx_k_plain<1>, dim3((unsigned)blocks), dim3(threads), 0, st, m, D, S, O, npix);
+            hipLaunchKernelGGL(jsx_k_plain<1>, dim3((unsigned)blocks), dim3(threads), 0, st, dp, O, npix);
     }
     HIPCHECK(ctx, hipGetLastError());
     return JSX_OK;

@@ -14,6 +14,31 @@
 
 #include "jsx_solver.h"
 
+/* All per-launch constants live in device memory and are read through a
+ * pointer that is re-laundered every loop trip: the compiler must then
+ * re-fetch them with scalar loads instead of pinning ~150 dwords of model
+ * constants in registers across the loop (which spills them into VGPR lanes
+ * and caps occupancy at one wavefront per SIMD). */
+struct DParams {
+    DModel m;
+    DImage im;
+    DSolver S;
+};
+
+/* the launch-constant pointer, viewed in the constant address space so that
+ * reads are scalar (s_load) and re-issued after every launder point */
+#if defined(__HIP_DEVICE_COMPILE__)
+#define JSX_CONST __attribute__((address_space(4)))
+#else
+#define JSX_CONST
+#endif
+
+__device__ __forceinline__ const JSX_CONST DParams *d_launder(const DParams *p) {
+    const JSX_CONST DParams *q = (const JSX_CONST DParams *)p;
+    asm volatile("" : "+s"(q));
+    return q;
+}
+
 /* ray id -> pixel: 0 row-major, 1 column-major (apex columns first) */
 __device__ inline int64_t d_ray_to_pixel(const DImage &im, int64_t id, int order) {
     if (order == 1) {
@@ -24,15 +49,18 @@ __device__ inline int64_t d_ray_to_pixel(const DImage &im, int64_t id, int order
 }
 
 template <int NS>
-__global__ __launch_bounds__(256) void jsx_k_plain(DModel m, DImage im, DSolver S, DOut out, int64_t npix) {
+__global__ __launch_bounds__(256) void jsx_k_plain(const DParams *__restrict__ P, DOut out, int64_t npix) {
     int64_t pix = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (pix >= npix) return;
     Lane<NS> L;
-    lane_begin(L, m, im, S, pix);
+    const JSX_CONST DParams *q = d_launder(P);
+    lane_begin(L, q->m, q->im, q->S, pix);
     while (L.phase != PH_DONE) {
+        q = d_launder(P);
         double r[NS];
-        lane_rhs(L, m, im, S, r);
-        lane_consume(L, S, im, r);
+        lane_rhs(L, q->m, q->im, q->S, r);
+        q = d_launder(P);
+        lane_consume(L, q->S, q->im, r);
     }
     lane_write(L, out, pix);
 }
@@ -40,7 +68,7 @@ __global__ __launch_bounds__(256) void jsx_k_plain(DModel m, DImage im, DSolver 
 __device__ inline int d_lane_id() { return __lane_id(); }
 
 template <int NS>
-__global__ __launch_bounds__(256) void jsx_k_persist(DModel m, DImage im, DSolver S, DOut out, int64_t nrays,
+__global__ __launch_bounds__(256) void jsx_k_persist(const DParams *__restrict__ P, DOut out, int64_t nrays,
                                                      unsigned long long *head, int order) {
     Lane<NS> L;
     int have = 0, exhausted = 0;
@@ -60,8 +88,9 @@ __global__ __launch_bounds__(256) void jsx_k_persist(DModel m, DImage im, DSolve
                 unsigned long long below = lane ? (mask & ((1ull << lane) - 1ull)) : 0ull;
                 int64_t id = (int64_t)(base + (unsigned long long)__popcll(below));
                 if (id < nrays) {
-                    int64_t pix = d_ray_to_pixel(im, id, order);
-                    lane_begin(L, m, im, S, pix);
+                    const JSX_CONST DParams *q = d_launder(P);
+                    int64_t pix = d_ray_to_pixel(q->im, id, order);
+                    lane_begin(L, q->m, q->im, q->S, pix);
                     if (L.phase == PH_DONE) lane_write(L, out, pix);
                     else have = 1;
                 } else {
@@ -72,8 +101,10 @@ __global__ __launch_bounds__(256) void jsx_k_persist(DModel m, DImage im, DSolve
         if (!__any(have) && __all(exhausted)) break;
         if (have) {
             double r[NS];
-            lane_rhs(L, m, im, S, r);
-            lane_consume(L, S, im, r);
+            const JSX_CONST DParams *q = d_launder(P);
+            lane_rhs(L, q->m, q->im, q->S, r);
+            q = d_launder(P);
+            lane_consume(L, q->S, q->im, r);
             if (L.phase == PH_DONE) {
                 lane_write(L, out, L.pix);
                 have = 0;

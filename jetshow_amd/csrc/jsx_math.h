@@ -164,17 +164,20 @@ JSX_HD double jsx_pow(double x, double y) {
     if (y == -1.0) return 1.0 / x;
     const double inf = jsx_from_bits(0x7ff0000000000000ULL);
     if (x == 0.0) return y < 0.0 ? inf : 0.0;
+    double sign = 1.0;
     if (x < 0.0) {
+        /* negative base: defined for integer exponents, sign from parity */
         if (floor(y) != y) return jsx_from_bits(0x7ff8000000000000ULL);
         double h = 0.5 * y;
         int odd = (floor(h) != h) && (y < 9007199254740992.0) && (y > -9007199254740992.0);
-        double r = jsx_pow(-x, y);
-        return odd ? -r : r;
+        sign = odd ? -1.0 : 1.0;
+        x = -x;
+        if (x == 1.0) return sign;
     }
-    if (x == inf) return y > 0.0 ? inf : 0.0;
+    if (x == inf) return sign * (y > 0.0 ? inf : 0.0);
     double ll;
     double lh = jsx_log_dd(x, &ll);
-    return jsx_pow_from_log(lh, ll, y);
+    return sign * jsx_pow_from_log(lh, ll, y);
 }
 
 /* sin(atan(u)) and cos(atan(u)) without the trigonometric detour. */

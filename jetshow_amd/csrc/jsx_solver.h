@@ -82,12 +82,10 @@ enum { SUB_INIT = 0 };
 template <int NS>
 struct Lane {
     int64_t pix;
-    double o[3];
     double s0in[3], s0out[3], s1in[3], s1out[3];
-    double cin[3], cout[3];
     int nseg, seg, phase, sub, tries, deriv_ok, first;
     double L, t, dt, t_old, tq;
-    double x[NS], xt[NS], xn[NS];
+    double x[NS], xt[NS];    /* xt: stage input; after stage 7 it holds the new state */
     double k1[NS], k2[NS], k3[NS], k4[NS], k5[NS], k6[NS], k7[NS];
     double x_old, k_old;
     double tau_bg, thick;
@@ -117,17 +115,18 @@ __device__ inline void d_pixel_origin(const DImage &im, int64_t pix, double *o) 
     o[2] = im.zscale * along;
 }
 
+/* endpoints of the current segment */
 template <int NS>
-__device__ inline void lane_select_seg(Lane<NS> &L, int s) {
-    for (int i = 0; i < 3; ++i) {
-        L.cin[i] = s ? L.s1in[i] : L.s0in[i];
-        L.cout[i] = s ? L.s1out[i] : L.s0out[i];
-    }
-}
+__device__ inline double lane_cin(const Lane<NS> &L, int i) { return L.seg ? L.s1in[i] : L.s0in[i]; }
+template <int NS>
+__device__ inline double lane_cout(const Lane<NS> &L, int i) { return L.seg ? L.s1out[i] : L.s0out[i]; }
+
+template <int NS>
+__device__ inline void lane_select_seg(Lane<NS> &L, int s) { L.seg = s; }
 
 template <int NS>
 __device__ inline double lane_seg_length(const Lane<NS> &L) {
-    double dv[3] = {L.cout[0] - L.cin[0], L.cout[1] - L.cin[1], L.cout[2] - L.cin[2]};
+    double dv[3] = {lane_cout(L, 0) - lane_cin(L, 0), lane_cout(L, 1) - lane_cin(L, 1), lane_cout(L, 2) - lane_cin(L, 2)};
     return d_norm3(dv);
 }
 
@@ -163,9 +162,8 @@ __device__ inline void lane_prepare_stage(Lane<NS> &L, int s) {
         break;
     default: /* 7: the new state, derivative evaluated there */
         for (int i = 0; i < NS; ++i)
-            L.xn[i] = L.x[i] + dt * DP_C1 * L.k1[i] + dt * DP_C3 * L.k3[i] + dt * DP_C4 * L.k4[i] +
+            L.xt[i] = L.x[i] + dt * DP_C1 * L.k1[i] + dt * DP_C3 * L.k3[i] + dt * DP_C4 * L.k4[i] +
                       dt * DP_C5 * L.k5[i] + dt * DP_C6 * L.k6[i];
-        for (int i = 0; i < NS; ++i) L.xt[i] = L.xn[i];
         L.tq = t + dt;
         break;
     }
@@ -317,11 +315,13 @@ __device__ inline void lane_begin(Lane<NS> &L, const DModel &m, const DImage &im
     L.pix = pix;
     L.flags = 0; L.n_rhs = 0; L.n_steps = 0;
     L.tau_bg = 0.0; L.thick = 0.0;
+    L.seg = 0;
     for (int i = 0; i < NS; ++i) L.x[i] = 0.0;
-    d_pixel_origin(im, pix, L.o);
+    double o[3];
+    d_pixel_origin(im, pix, o);
     DSeg segs[2];
     int clipped = 0;
-    int ns = d_model_hit(m, L.o, im.dir, segs, &clipped);
+    int ns = d_model_hit(m, o, im.dir, segs, &clipped);
     if (clipped) L.flags |= JSX_PIX_CLIPPED;
     L.nseg = ns;
     if (ns == 0) { L.phase = PH_DONE; return; }
@@ -349,9 +349,9 @@ __device__ inline void lane_begin(Lane<NS> &L, const DModel &m, const DImage &im
 template <int NS>
 __device__ inline void lane_point(const Lane<NS> &L, const DImage &im, double *p) {
     if (L.phase == PH_TAU) {
-        for (int i = 0; i < 3; ++i) p[i] = L.cin[i] + L.tq * im.dir[i];
+        for (int i = 0; i < 3; ++i) p[i] = lane_cin(L, i) + L.tq * im.dir[i];
     } else {
-        for (int i = 0; i < 3; ++i) p[i] = L.cout[i] - L.tq * im.dir[i];
+        for (int i = 0; i < 3; ++i) p[i] = lane_cout(L, i) - L.tq * im.dir[i];
     }
 }
 
@@ -426,7 +426,7 @@ __device__ inline void lane_consume(Lane<NS> &L, const DSolver &S, const DImage 
     if (L.phase == PH_TAU) {
         L.x_old = L.x[0];
         L.k_old = L.k1[0];
-        L.x[0] = L.xn[0];
+        L.x[0] = L.xt[0];
         L.k1[0] = L.k7[0];
         if (L.x[0] >= S.tau_max) {
             /* bisection on the dense output (Observation.cpp:166-187) */
@@ -442,8 +442,8 @@ __device__ inline void lane_consume(Lane<NS> &L, const DSolver &S, const DImage 
             }
             t_m = 0.5 * (t0 + t1);
             /* Ray(point_in, direction).point(t_m), direction normalised again */
-            double po[3];
-            d_ray_point(L.cin, im.d2, t_m, po);
+            double po[3], ci[3] = {lane_cin(L, 0), lane_cin(L, 1), lane_cin(L, 2)};
+            d_ray_point(ci, im.d2, t_m, po);
             for (int i = 0; i < 3; ++i) {
                 if (L.seg) L.s1out[i] = po[i];
                 else L.s0out[i] = po[i];
@@ -455,7 +455,7 @@ __device__ inline void lane_consume(Lane<NS> &L, const DSolver &S, const DImage 
         lane_tau_range_next(L, S);
         return;
     }
-    for (int i = 0; i < NS; ++i) { L.x[i] = L.xn[i]; L.k1[i] = L.k7[i]; }
+    for (int i = 0; i < NS; ++i) { L.x[i] = L.xt[i]; L.k1[i] = L.k7[i]; }
     lane_stokes_loop(L, S);
 }
 

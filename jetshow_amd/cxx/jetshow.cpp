@@ -1,0 +1,237 @@
+/*
+ * jetshow.cpp -- the reference's class API over the C ABI.  Every compute
+ * call goes to the GPU through include/jetshow_amd.h; nothing here computes
+ * radiative transfer on the CPU.
+ */
+#include "jetshow.h"
+
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+namespace jetshow {
+
+static void set3(double *dst, const Vector3d &v) { dst[0] = v[0]; dst[1] = v[1]; dst[2] = v[2]; }
+
+/* ---- Ray (src/Ray.cpp:9-19) ---- */
+Ray::Ray(const Vector3d &origin, const Vector3d &direction) : origin_(origin), direction_(direction) {
+    double z = (direction_[0] * direction_[0] + direction_[1] * direction_[1]) + direction_[2] * direction_[2];
+    if (z > 0.0) {
+        double r = std::sqrt(z);
+        for (int i = 0; i < 3; ++i) direction_[i] /= r;
+    }
+}
+
+Vector3d Ray::point(double t) const {
+    return Vector3d(origin_[0] + direction_[0] * t, origin_[1] + direction_[1] * t, origin_[2] + direction_[2] * t);
+}
+
+/* ---- Context ---- */
+Context::Context(int device, size_t hbm_budget) {
+    int rc = jsx_ctx_create(device, hbm_budget, &ctx_);
+    if (rc) throw Error(rc, "jsx_ctx_create failed: no usable GPU");
+}
+Context::~Context() {
+    if (ctx_) jsx_ctx_destroy(ctx_);
+}
+void Context::check(int rc) const {
+    if (rc) throw Error(rc, jsx_last_error(ctx_));
+}
+Context &Context::instance() {
+    static Context ctx(0, 0);
+    return ctx;
+}
+int32_t Context::upload_grid(const std::vector<double> &values, int nz, int nr, double z0, double dz, double r0,
+                             double dr, double nan_value) {
+    if ((size_t)nz * (size_t)nr != values.size()) throw Error(JSX_EINVAL, "grid size mismatch");
+    jsx_grid_desc g;
+    g.nz = nz; g.nr = nr; g.z0 = z0; g.dz = dz; g.r0 = r0; g.dr = dr; g.nan_value = nan_value;
+    int32_t h = 0;
+    check(jsx_grid_upload(ctx_, &g, values.data(), &h));
+    return h;
+}
+
+/* ---- geometry ---- */
+std::list<Intersection> BaseGeometry::hit(Ray &ray) const {
+    jsx_jet_desc d;
+    std::memset(&d, 0, sizeof d);
+    d.bfield = JSX_B_RANDOM_SCALAR; d.nfield = JSX_N_CONST; d.vfield = JSX_V_CONST_FLAT;
+    d.gamma0 = 2.0; d.s = 2.5; d.gamma_min = 100.0;
+    lower(d);
+    double r[6] = {ray.origin()[0], ray.origin()[1], ray.origin()[2],
+                   ray.direction()[0], ray.direction()[1], ray.direction()[2]};
+    int32_t ns = 0;
+    double seg[12];
+    Context &C = Context::instance();
+    C.check(jsx_hit_host(C.get(), &d, nullptr, r, 1, &ns, seg));
+    std::list<Intersection> out;
+    for (int s = 0; s < ns; ++s)
+        out.emplace_back(ray.direction(), Vector3d(seg[6 * s], seg[6 * s + 1], seg[6 * s + 2]),
+                         Vector3d(seg[6 * s + 3], seg[6 * s + 4], seg[6 * s + 5]));
+    return out;
+}
+
+Cone::Cone(const Vector3d &origin, const Vector3d &direction, const double &angle, const double &scale)
+    : origin_(origin), direction_(direction), angle_(angle), big_scale_(scale) {}
+void Cone::lower(jsx_jet_desc &d) const {
+    d.geometry = JSX_GEOM_CONE;
+    set3(d.geo_origin, origin_); set3(d.geo_axis, direction_);
+    d.geo_param = angle_; d.geo_big_scale = big_scale_;
+    d.z_clip_min = zmin_; d.z_clip_max = zmax_;
+}
+Parabaloid::Parabaloid(const Vector3d &origin, const Vector3d &direction, const double &r0, const double &big_scale)
+    : origin_(origin), direction_(direction), r0_(r0), big_scale_(big_scale) {}
+void Parabaloid::lower(jsx_jet_desc &d) const {
+    d.geometry = JSX_GEOM_PARABOLOID;
+    set3(d.geo_origin, origin_); set3(d.geo_axis, direction_);
+    d.geo_param = r0_; d.geo_big_scale = big_scale_;
+}
+Cylinder::Cylinder(const Vector3d &origin, const Vector3d &direction, const double &r)
+    : origin_(origin), direction_(direction), r_(r) {}
+void Cylinder::lower(jsx_jet_desc &d) const {
+    d.geometry = JSX_GEOM_CYLINDER;
+    set3(d.geo_origin, origin_); set3(d.geo_axis, direction_);
+    d.geo_param = r_; d.geo_big_scale = 100. * r_;
+    d.z_clip_min = zmin_; d.z_clip_max = zmax_;
+}
+
+/* ---- fields ---- */
+void RandomScalarBField::lower(jsx_jet_desc &d) const { d.bfield = JSX_B_RANDOM_SCALAR; d.b0 = b_0_; d.b_exp = m_b_; d.b_in_plasma_frame = 1; }
+void RandomScalarBFieldZ::lower(jsx_jet_desc &d) const { d.bfield = JSX_B_RANDOM_SCALAR_Z; d.b0 = b_0_; d.b_exp = m_b_; d.b_in_plasma_frame = 1; }
+void ToroidalBField::lower(jsx_jet_desc &d) const { d.bfield = JSX_B_TOROIDAL; d.b0 = b_0_; d.b_exp = n_b_; d.b_in_plasma_frame = in_plasma_frame_; }
+void ConstCylinderBFieldZ::lower(jsx_jet_desc &d) const { d.bfield = JSX_B_CONST_CYLINDER_Z; d.b0 = b_0_; d.b_exp = n_b_; d.b_in_plasma_frame = in_plasma_frame_; }
+void ToroidalRBField::lower(jsx_jet_desc &d) const { d.bfield = JSX_B_TOROIDAL_R; d.b0 = b_0_; d.b_exp = n_b_; d.b_in_plasma_frame = in_plasma_frame_; }
+void HelicalRBField::lower(jsx_jet_desc &d) const {
+    d.bfield = JSX_B_HELICAL_R; d.b0 = b_0_; d.b_exp = n_b_; d.b_pitch = pitch_; d.b_in_plasma_frame = in_plasma_frame_;
+}
+void SimulationBField::lower(jsx_jet_desc &d) const { d.bfield = JSX_B_GRID_VECTOR; d.grid_b = gp_; d.grid_b2 = gf_; d.b_in_plasma_frame = in_plasma_frame_; }
+void SimulationScalarBField::lower(jsx_jet_desc &d) const { d.bfield = JSX_B_GRID_SCALAR; d.grid_b = g_; d.b_in_plasma_frame = 1; }
+void ConstNField::lower(jsx_jet_desc &d) const { d.nfield = JSX_N_CONST; d.n0 = n_; d.n_in_plasma_frame = in_plasma_frame_; }
+void BKNField::lower(jsx_jet_desc &d) const { d.nfield = JSX_N_BK; d.n0 = n_0_; d.n_exp = n_n_; d.n_in_plasma_frame = in_plasma_frame_; }
+void BKNFieldZ::lower(jsx_jet_desc &d) const { d.nfield = JSX_N_BK_Z; d.n0 = n_0_; d.n_exp = n_n_; d.n_in_plasma_frame = in_plasma_frame_; }
+void SimulationNField::lower(jsx_jet_desc &d) const { d.nfield = JSX_N_GRID; d.grid_n = g_; d.n_in_plasma_frame = in_plasma_frame_; }
+void ConstFlatVField::lower(jsx_jet_desc &d) const { d.vfield = JSX_V_CONST_FLAT; d.gamma0 = gamma_; }
+void ConstCentralVField::lower(jsx_jet_desc &d) const { d.vfield = JSX_V_CONST_CENTRAL; d.gamma0 = gamma_; set3(d.v_origin, origin_); }
+void LinearGammaZVField::lower(jsx_jet_desc &d) const {
+    d.vfield = JSX_V_LINEAR_GAMMA_Z; d.gamma0 = g0_; d.gamma1 = g1_; d.gamma_z0 = z0_; d.gamma_z1 = z1_;
+}
+void SimulationVField::lower(jsx_jet_desc &d) const { d.vfield = JSX_V_GRID; d.grid_v = g_; }
+
+/* ---- Jet ---- */
+Jet::Jet(BaseGeometry *geo, VField *vfield, ScalarBField *bField, NField *nField)
+    : geometry_(geo), vfield_(vfield), sbfield_(bField), nfield_(nField) {}
+Jet::Jet(BaseGeometry *geo, VField *vfield, VectorBField *bField, NField *nField)
+    : geometry_(geo), vfield_(vfield), vbfield_(bField), nfield_(nField) {}
+
+std::list<Intersection> Jet::hit(Ray &ray) { return geometry_->hit(ray); }
+
+jsx_jet_desc Jet::descriptor() const {
+    jsx_jet_desc d;
+    std::memset(&d, 0, sizeof d);
+    geometry_->lower(d);
+    if (sbfield_) sbfield_->lower(d);
+    if (vbfield_) vbfield_->lower(d);
+    nfield_->lower(d);
+    vfield_->lower(d);
+    d.s = s_;
+    d.gamma_min = gamma_min_;
+    d.thermal_fraction = thermal_;
+    return d;
+}
+
+/* ---- ImagePlane ---- */
+ImagePlane::ImagePlane(std::pair<unsigned long int, unsigned long int> image_size_, double pixel_size_start,
+                       double pixel_size_stop, double los_angle)
+    : image_size(image_size_) {
+    desc_.n_across = (int64_t)image_size_.first;
+    desc_.n_along = (int64_t)image_size_.second;
+    desc_.lg_pixel_size_start_cm = pixel_size_start;
+    desc_.lg_pixel_size_stop_cm = pixel_size_stop;
+    desc_.los_angle = los_angle;
+}
+
+std::vector<std::vector<double>> ImagePlane::getPixelSizes() {
+    std::vector<double> flat((size_t)(desc_.n_across * desc_.n_along));
+    int rc = jsx_image_pixel_sizes(&desc_, flat.data());
+    if (rc) throw Error(rc, "bad image description");
+    std::vector<std::vector<double>> out(image_size.first, std::vector<double>(image_size.second));
+    for (size_t i = 0; i < image_size.first; ++i)
+        for (size_t j = 0; j < image_size.second; ++j) out[i][j] = flat[i * image_size.second + j];
+    return out;
+}
+
+/* Image::getImage (src/Image.cpp:16-28); pixels never set read as 0 */
+std::vector<std::vector<double>> ImagePlane::getImage(std::string value) {
+    std::vector<std::vector<double>> out(image_size.first, std::vector<double>(image_size.second, 0.0));
+    auto it = planes.find(value);
+    if (it == planes.end()) return out;
+    for (size_t i = 0; i < image_size.first; ++i)
+        for (size_t j = 0; j < image_size.second; ++j) out[i][j] = it->second[i * image_size.second + j];
+    return out;
+}
+
+/* Image::getStripe (src/Image.cpp:31-41) */
+std::vector<double> ImagePlane::getStripe(std::string value) {
+    std::vector<double> stripe;
+    auto it = planes.find(value);
+    for (size_t i = 0; i < image_size.first; ++i)
+        for (size_t j = image_size.second / 2; j < image_size.second; ++j)
+            if (i == image_size.first / 2)
+                stripe.push_back(it == planes.end() ? 0.0 : it->second[i * image_size.second + j]);
+    return stripe;
+}
+
+/* ---- Observation ---- */
+Observation::Observation(Jet *newjet, ImagePlane *newimagePlane, double newnu)
+    : nu(newnu), jet(newjet), imagePlane(newimagePlane) {
+    std::memset(&params, 0, sizeof params);
+    params.stokes = JSX_STOKES_I;
+    params.precision = JSX_PREC_FP64;
+    params.max_failed_tries = 500;
+    params.tau_abs_tol = 1e-7;       /* Observation.cpp:156 */
+    params.tau_rel_tol = 1e-14;
+    params.stokes_abs_tol = 1e-9;    /* Observation.cpp:269 */
+    params.stokes_rel_tol = 1e-9;
+    params.tau_precision = 1.0;      /* Observation.cpp:132 */
+    params.tau_n_min = 0.1;          /* steps_schedule defaults (utils.h:209-210) */
+    params.tau_n_max = 100.0;
+    params.n_max_factor = 10;
+}
+
+void Observation::run(int n, double tau_max, double dt_max, double tau_min, std::string integration_type,
+                      std::string output_type, int n_max, double tau_n_min, double tau_n_max) {
+    (void)n_max; (void)tau_n_min; (void)tau_n_max;
+    if (integration_type != "adaptive")
+        throw Unsupported("integration_type '" + integration_type + "' (only the adaptive path is live)");
+    if (output_type != "I" && output_type != "full") throw Unsupported("output_type '" + output_type + "'");
+    Context &C = context ? *context : Context::instance();
+    jsx_solver_params p = params;
+    p.n = n;
+    p.tau_max = tau_max;
+    p.dt_max = dt_max * pc;           /* Observation.cpp:23 */
+    p.tau_min = tau_min;
+    p.nu = nu;
+    p.stokes = output_type == "full" ? JSX_STOKES_FULL : JSX_STOKES_I;
+    jsx_jet_desc jd = jet->descriptor();
+    jsx_image_desc id = imagePlane->descriptor();
+    size_t N = (size_t)(id.n_across * id.n_along);
+    auto &P = imagePlane->planes;
+    const char *names[] = {"tau", "I", "Q", "U", "V", "l"};
+    for (auto nm : names) P[nm].assign(N, 0.0);
+    imagePlane->flags.assign(N, 0u);
+    jsx_outputs o;
+    std::memset(&o, 0, sizeof o);
+    o.tau = P["tau"].data(); o.I = P["I"].data(); o.Q = P["Q"].data(); o.U = P["U"].data(); o.V = P["V"].data();
+    o.l = P["l"].data();
+    o.flags = imagePlane->flags.data();
+    C.check(jsx_observe_host(C.get(), &jd, &id, &p, &o));
+    if (output_type == "I") { P.erase("Q"); P.erase("U"); P.erase("V"); }
+}
+
+std::vector<std::vector<double>> Observation::getImage(std::string value) { return imagePlane->getImage(value); }
+std::vector<double> Observation::getStripe(std::string value) { return imagePlane->getStripe(value); }
+std::pair<unsigned long int, unsigned long int> Observation::getImageSize() { return imagePlane->image_size; }
+
+double mas_to_pc(double z) { return jsx_mas_to_pc(z); }
+
+}  // namespace jetshow

@@ -1,0 +1,381 @@
+/*
+ * jetshow.h -- the reference's C++ class API for the per-pixel path, backed by
+ * the MI355X C ABI (include/jetshow_amd.h).
+ *
+ * Class names, constructor arguments and method names follow the reference
+ * (include/Geometry.h, Cone.h, Parabaloid.h, Cylinder.h, BField.h, NField.h,
+ * VField.h, Jet.h, ImagePlane.h, Observation.h, Ray.h, Intersection.h) so code
+ * written against jetshow compiles against this header with the include
+ * changed.  Differences:
+ *   - Vector3d is a small value type here (Eigen is not a dependency);
+ *   - every class can describe itself as a POD descriptor (lower()); kinds the
+ *     kernels do not implement throw jetshow::Unsupported instead of running
+ *     anywhere else;
+ *   - errors from the GPU surface as jetshow::Error exceptions carrying the
+ *     jsx_status code (the reference has no error channel).
+ */
+#ifndef JETSHOW_CXX_H
+#define JETSHOW_CXX_H
+
+#include <cstdint>
+#include <list>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/jetshow_amd.h"
+
+namespace jetshow {
+
+/* include/utils.h:17-36 */
+const double mas_to_rad = 4.84813681109536e-09;
+const double rad_to_mas = 206264806.24709633;
+const double pc = 3.0856775814671913e+18;
+const double m_e = 9.10938356e-28;
+const double q_e = 4.80320467299766e-10;
+const double c = 29979245800.0;
+const double to_jy = 1E+23;
+const double pi = 3.141592653589793238462643383279502884;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int code_, const std::string &msg) : std::runtime_error(msg), code(code_) {}
+};
+struct Unsupported : Error {
+    explicit Unsupported(const std::string &msg) : Error(JSX_EUNSUPPORTED, msg) {}
+};
+
+struct Vector3d {
+    double v[3];
+    Vector3d() : v{0, 0, 0} {}
+    Vector3d(double x, double y, double z) : v{x, y, z} {}
+    double &operator[](int i) { return v[i]; }
+    double operator[](int i) const { return v[i]; }
+};
+
+/* ---- Ray / Intersection (src/Ray.cpp, src/Intersection.cpp) ---- */
+class Ray {
+public:
+    Ray(const Vector3d &origin, const Vector3d &direction);  /* normalises direction */
+    Vector3d point(double t) const;
+    const Vector3d &origin() const { return origin_; }
+    const Vector3d &direction() const { return direction_; }
+
+private:
+    Vector3d origin_, direction_;
+};
+
+class Intersection {
+public:
+    Intersection(const Vector3d &dir, const Vector3d &pin, const Vector3d &pout)
+        : direction_(dir), borders_(pin, pout) {}
+    const std::pair<Vector3d, Vector3d> get_path() const { return borders_; }
+    const Vector3d direction() const { return direction_; }
+    void set_point_out(const Vector3d &p) { borders_.second = p; }
+    void set_point_in(const Vector3d &p) { borders_.first = p; }
+
+private:
+    Vector3d direction_;
+    std::pair<Vector3d, Vector3d> borders_;
+};
+
+class Context;
+
+/* ---- geometry (include/Geometry.h:40-55) ---- */
+class BaseGeometry {
+public:
+    virtual ~BaseGeometry() = default;
+    /* Geometry::hit on the GPU (jsx_hit_host) */
+    std::list<Intersection> hit(Ray &ray) const;
+    virtual void lower(jsx_jet_desc &d) const = 0;
+};
+
+class Geometry : public BaseGeometry {
+public:
+    virtual const double big_scale() const = 0;
+    virtual const Vector3d &origin() const = 0;
+};
+
+class Cone : public Geometry {
+public:
+    Cone(const Vector3d &origin, const Vector3d &direction, const double &angle, const double &scale);
+    const Vector3d &origin() const override { return origin_; }
+    const Vector3d &direction() const { return direction_; }
+    const double angle() const { return angle_; }
+    const double big_scale() const override { return big_scale_; }
+    void lower(jsx_jet_desc &d) const override;
+    /* finite-length extension: keep z in [zmin, zmax] (cm) */
+    void set_z_clip(double zmin, double zmax) { zmin_ = zmin; zmax_ = zmax; }
+
+private:
+    Vector3d origin_, direction_;
+    double angle_, big_scale_, zmin_ = 0, zmax_ = 0;
+};
+
+class Parabaloid : public Geometry {
+public:
+    Parabaloid(const Vector3d &origin, const Vector3d &direction, const double &r0, const double &big_scale);
+    const Vector3d &origin() const override { return origin_; }
+    const Vector3d &direction() const { return direction_; }
+    double r0() const { return r0_; }
+    const double big_scale() const override { return big_scale_; }
+    void lower(jsx_jet_desc &d) const override;
+
+private:
+    Vector3d origin_, direction_;
+    double r0_, big_scale_;
+};
+
+class Cylinder : public Geometry {
+public:
+    Cylinder(const Vector3d &origin, const Vector3d &direction, const double &r);
+    const Vector3d &origin() const override { return origin_; }
+    const Vector3d &direction() const { return direction_; }
+    double r() const { return r_; }
+    const double big_scale() const override { return 100. * r_; }
+    void lower(jsx_jet_desc &d) const override;
+    void set_z_clip(double zmin, double zmax) { zmin_ = zmin; zmax_ = zmax; }
+
+private:
+    Vector3d origin_, direction_;
+    double r_, zmin_ = 0, zmax_ = 0;
+};
+
+/* ---- fields (include/BField.h, NField.h, VField.h) ---- */
+class ScalarBField {
+public:
+    virtual ~ScalarBField() = default;
+    virtual void lower(jsx_jet_desc &d) const = 0;
+};
+class VectorBField {
+public:
+    explicit VectorBField(bool in_plasma_frame) : in_plasma_frame_(in_plasma_frame) {}
+    virtual ~VectorBField() = default;
+    virtual void lower(jsx_jet_desc &d) const = 0;
+
+protected:
+    bool in_plasma_frame_;
+};
+
+class RandomScalarBField : public ScalarBField {
+public:
+    RandomScalarBField(double b_0, double m_b) : b_0_(b_0), m_b_(m_b) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    double b_0_, m_b_;
+};
+class RandomScalarBFieldZ : public ScalarBField {
+public:
+    RandomScalarBFieldZ(double b_0, double m_b) : b_0_(b_0), m_b_(m_b) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    double b_0_, m_b_;
+};
+class ToroidalBField : public VectorBField {
+public:
+    ToroidalBField(double b_0, double n_b, bool in_plasma_frame) : VectorBField(in_plasma_frame), b_0_(b_0), n_b_(n_b) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    double b_0_, n_b_;
+};
+class ConstCylinderBFieldZ : public VectorBField {
+public:
+    ConstCylinderBFieldZ(double b_0, double n_b, bool in_plasma_frame)
+        : VectorBField(in_plasma_frame), b_0_(b_0), n_b_(n_b) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    double b_0_, n_b_;
+};
+/* extensions: power law in |p| with a true azimuthal / helical direction */
+class ToroidalRBField : public VectorBField {
+public:
+    ToroidalRBField(double b_0, double n_b, bool in_plasma_frame) : VectorBField(in_plasma_frame), b_0_(b_0), n_b_(n_b) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    double b_0_, n_b_;
+};
+class HelicalRBField : public VectorBField {
+public:
+    HelicalRBField(double b_0, double n_b, double pitch, bool in_plasma_frame)
+        : VectorBField(in_plasma_frame), b_0_(b_0), n_b_(n_b), pitch_(pitch) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    double b_0_, n_b_, pitch_;
+};
+/* SimulationBField: grid handles from Context::upload_grid */
+class SimulationBField : public VectorBField {
+public:
+    SimulationBField(int32_t grid_p, int32_t grid_fi, bool in_plasma_frame)
+        : VectorBField(in_plasma_frame), gp_(grid_p), gf_(grid_fi) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    int32_t gp_, gf_;
+};
+class SimulationScalarBField : public ScalarBField {
+public:
+    explicit SimulationScalarBField(int32_t grid) : g_(grid) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    int32_t g_;
+};
+
+class NField {
+public:
+    explicit NField(bool in_plasma_frame) : in_plasma_frame_(in_plasma_frame) {}
+    virtual ~NField() = default;
+    virtual void lower(jsx_jet_desc &d) const = 0;
+protected:
+    bool in_plasma_frame_;
+};
+class ConstNField : public NField {
+public:
+    ConstNField(double n, bool in_plasma_frame) : NField(in_plasma_frame), n_(n) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    double n_;
+};
+class BKNField : public NField {
+public:
+    BKNField(double n_0, double n_n, bool in_plasma_frame) : NField(in_plasma_frame), n_0_(n_0), n_n_(n_n) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    double n_0_, n_n_;
+};
+class BKNFieldZ : public NField {
+public:
+    BKNFieldZ(double n_0, double n_n, bool in_plasma_frame) : NField(in_plasma_frame), n_0_(n_0), n_n_(n_n) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    double n_0_, n_n_;
+};
+class SimulationNField : public NField {
+public:
+    SimulationNField(int32_t grid, bool in_plasma_frame) : NField(in_plasma_frame), g_(grid) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    int32_t g_;
+};
+
+class VField {
+public:
+    virtual ~VField() = default;
+    virtual void lower(jsx_jet_desc &d) const = 0;
+};
+class ConstFlatVField : public VField {
+public:
+    explicit ConstFlatVField(double gamma) : gamma_(gamma) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    double gamma_;
+};
+class ConstCentralVField : public VField {
+public:
+    explicit ConstCentralVField(double gamma, Vector3d origin = Vector3d(0, 0, 0)) : gamma_(gamma), origin_(origin) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    double gamma_;
+    Vector3d origin_;
+};
+class LinearGammaZVField : public VField {
+public:
+    LinearGammaZVField(double gamma0, double gamma1, double z0, double z1) : g0_(gamma0), g1_(gamma1), z0_(z0), z1_(z1) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    double g0_, g1_, z0_, z1_;
+};
+class SimulationVField : public VField {
+public:
+    explicit SimulationVField(int32_t grid) : g_(grid) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    int32_t g_;
+};
+
+/* ---- Jet (include/Jet.h:11-61) ---- */
+class Jet {
+public:
+    Jet(BaseGeometry *geo, VField *vfield, ScalarBField *bField, NField *nField);
+    Jet(BaseGeometry *geo, VField *vfield, VectorBField *bField, NField *nField);
+    std::list<Intersection> hit(Ray &ray);
+    /* emission parameters the reference hard-codes (s = 3.42, Jet.cpp:32) */
+    void set_s(double s) { s_ = s; }
+    void set_gamma_min(double g) { gamma_min_ = g; }
+    void set_thermal_fraction(double f) { thermal_ = f; }
+    jsx_jet_desc descriptor() const;
+
+private:
+    BaseGeometry *geometry_;
+    VField *vfield_;
+    ScalarBField *sbfield_ = nullptr;
+    VectorBField *vbfield_ = nullptr;
+    NField *nfield_;
+    double s_ = 3.42, gamma_min_ = 100.0, thermal_ = 0.0;
+};
+
+/* ---- ImagePlane (include/ImagePlane.h:22-35) ---- */
+class ImagePlane {
+public:
+    ImagePlane(std::pair<unsigned long int, unsigned long int> image_size, double pixel_size_start,
+               double pixel_size_stop, double los_angle);
+    std::vector<std::vector<double>> getImage(std::string value);
+    std::vector<std::vector<double>> getPixelSizes();
+    std::vector<double> getStripe(std::string value);
+    const std::pair<unsigned long int, unsigned long int> image_size;
+    jsx_image_desc descriptor() const { return desc_; }
+    /* result planes written by Observation::run, row-major [n_across * n_along] */
+    std::map<std::string, std::vector<double>> planes;
+    std::vector<uint32_t> flags;
+
+private:
+    jsx_image_desc desc_;
+};
+
+/* ---- GPU context shared by the classes ---- */
+class Context {
+public:
+    explicit Context(int device = 0, size_t hbm_budget = 0);
+    ~Context();
+    Context(const Context &) = delete;
+    Context &operator=(const Context &) = delete;
+    jsx_ctx *get() const { return ctx_; }
+    int32_t upload_grid(const std::vector<double> &values, int nz, int nr, double z0, double dz, double r0,
+                        double dr, double nan_value);
+    static Context &instance();   /* process-wide default context on device 0 */
+    void check(int rc) const;
+
+private:
+    jsx_ctx *ctx_ = nullptr;
+};
+
+/* ---- Observation (include/Observation.h:12-52) ---- */
+class Observation {
+public:
+    Observation(Jet *newjet, ImagePlane *imagePlane, double nu);
+    /* src/Observation.cpp:20-41: dt_max is multiplied by pc here, as the
+     * reference does; n_max, tau_n_min, tau_n_max are accepted and, as in the
+     * reference's adaptive path, not used. integration_type must be
+     * "adaptive"; output_type "I" or "full". */
+    void run(int n, double tau_max, double dt_max, double tau_min, std::string integration_type,
+             std::string output_type, int n_max, double tau_n_min, double tau_n_max);
+    std::vector<std::vector<double>> getImage(std::string value);
+    std::vector<double> getStripe(std::string value);
+    std::pair<unsigned long int, unsigned long int> getImageSize();
+    const double nu;
+    /* solver parameters beyond the reference signature */
+    jsx_solver_params params;
+    Context *context = nullptr;
+
+private:
+    Jet *jet;
+    ImagePlane *imagePlane;
+};
+
+/* mas_to_pc (src/utils.cpp:205-208) */
+double mas_to_pc(double z);
+
+}  // namespace jetshow
+
+#endif
