@@ -1,0 +1,184 @@
+/*
+ * jsx_batch.h -- one persistent launch over the rays of many images.
+ *
+ * Ray ids 0..total-1 enumerate every pixel of every request (request r owns
+ * [ray_off[r], ray_off[r+1])).  The id space is dealt round-robin in blocks of
+ * `block` ids to 8 queues, one per XCD; a wavefront drains the queue of the
+ * XCD it runs on and, when that queue is empty, steals from the others in
+ * turn.  Each lane that finishes (or misses) takes a new ray, so wavefronts
+ * stay full while rays of very different step counts drain; one atomic per
+ * wavefront per refill round hands out ids.  A lane's request index selects
+ * its launch constants; when every active lane of the wavefront serves the
+ * same request the constants are read with scalar loads, otherwise with
+ * vector loads.
+ *
+ * Within a request, ids map to pixels either row-major or apex columns first
+ * (longest rays first).
+ */
+#ifndef JSX_BATCH_H
+#define JSX_BATCH_H
+
+#include "jsx_mixed.h"
+
+#define JSX_NQ 8
+
+struct BatchQueues {
+    unsigned long long head[JSX_NQ * 8];   /* one counter per queue, 64 B apart */
+};
+
+/* policy: FP64 machine */
+template <int NS>
+struct PolFP64 {
+    using LaneT = Lane<NS>;
+    using Params = DParams;
+    template <class Q>
+    __device__ static void begin(LaneT &L, const Q &q, int64_t pix) { lane_begin(L, q.m, q.im, q.S, pix); }
+    template <class Q>
+    __device__ static void rhs_consume(LaneT &L, const Q *q0) {
+        double r[NS];
+        const Q *q = q0;
+        asm volatile("" : "+v"(q));
+        lane_rhs(L, q->m, q->im, q->S, r);
+        q = q0;
+        asm volatile("" : "+v"(q));
+        lane_consume(L, q->S, q->im, r);
+    }
+    template <class Q>
+    __device__ static void rhs_consume_uniform(LaneT &L, const Q *q0) {
+        double r[NS];
+        const Q *q = q0;
+        asm volatile("" : "+s"(q));
+        lane_rhs(L, q->m, q->im, q->S, r);
+        q = q0;
+        asm volatile("" : "+s"(q));
+        lane_consume(L, q->S, q->im, r);
+    }
+    __device__ static void write(const LaneT &L, const DOut &o, int64_t pix) { lane_write(L, o, pix); }
+    __device__ static bool done(const LaneT &L) { return L.phase == PH_DONE; }
+    __device__ static int64_t pix(const LaneT &L) { return L.pix; }
+};
+
+/* policy: mixed-precision machine */
+template <int NS>
+struct PolMixed {
+    using LaneT = MLane<NS>;
+    using Params = DParamsM;
+    template <class Q>
+    __device__ static void begin(LaneT &L, const Q &q, int64_t pix) { ml_begin(L, q.m, q.im, q.S, q.c, pix); }
+    template <class Q>
+    __device__ static void rhs_consume(LaneT &L, const Q *q0) {
+        float r[NS];
+        const Q *q = q0;
+        asm volatile("" : "+v"(q));
+        ml_rhs(L, q->m, q->im, q->c, r);
+        q = q0;
+        asm volatile("" : "+v"(q));
+        ml_consume(L, q->S, q->im, q->c, r);
+    }
+    template <class Q>
+    __device__ static void rhs_consume_uniform(LaneT &L, const Q *q0) {
+        float r[NS];
+        const Q *q = q0;
+        asm volatile("" : "+s"(q));
+        ml_rhs(L, q->m, q->im, q->c, r);
+        q = q0;
+        asm volatile("" : "+s"(q));
+        ml_consume(L, q->S, q->im, q->c, r);
+    }
+    __device__ static void write(const LaneT &L, const DOut &o, int64_t pix) { ml_write(L, o, pix); }
+    __device__ static bool done(const LaneT &L) { return L.phase == PH_DONE; }
+    __device__ static int64_t pix(const LaneT &L) { return L.pix; }
+};
+
+/* workgroups are dealt to the 8 XCDs round-robin by workgroup id */
+__device__ inline int d_xcd_of_block() { return (int)(blockIdx.x % JSX_NQ); }
+
+/* id -> (request, pixel) */
+__device__ inline int d_find_request(const int64_t *__restrict__ off, int n_req, int64_t id) {
+    int lo = 0, hi = n_req - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= id) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+/* queue q, local counter h -> global id with blocks of `block` ids dealt round-robin */
+__device__ inline int64_t d_queue_id(int q, unsigned long long h, int64_t block) {
+    unsigned long long b = h / (unsigned long long)block, r = h % (unsigned long long)block;
+    return (int64_t)(b * JSX_NQ * (unsigned long long)block + (unsigned long long)q * block + r);
+}
+
+template <class P>
+__global__ __launch_bounds__(256) void jsx_k_batch(const typename P::Params *__restrict__ reqs,
+                                                   const int64_t *__restrict__ ray_off, const DOut *__restrict__ outs,
+                                                   int n_req, int64_t total, BatchQueues *qs, int64_t block,
+                                                   int order, int steal) {
+    using LaneT = typename P::LaneT;
+    using Params = typename P::Params;
+    LaneT L;
+    int have = 0, exhausted = 0, req = 0;
+    const int lane = __lane_id();
+    int myq = steal ? d_xcd_of_block() : 0;
+    int qtry = 0;   /* queues tried since the last successful grab */
+    const int nq = steal ? JSX_NQ : 1;
+    for (;;) {
+        int want = !have && !exhausted;
+        unsigned long long mask = __ballot(want);
+        if (mask) {
+            int cnt = __popcll(mask);
+            int leader = __ffsll((long long)mask) - 1;
+            unsigned long long base = 0;
+            int q = (myq + qtry) % nq;
+            if (lane == leader) base = atomicAdd(&qs->head[q * 8], (unsigned long long)cnt);
+            unsigned lo = __shfl((unsigned)(base & 0xffffffffull), leader);
+            unsigned hi = __shfl((unsigned)(base >> 32), leader);
+            base = ((unsigned long long)hi << 32) | lo;
+            unsigned long long below = lane ? (mask & ((1ull << lane) - 1ull)) : 0ull;
+            int64_t id = steal ? d_queue_id(q, base + (unsigned long long)__popcll(below), block)
+                               : (int64_t)(base + (unsigned long long)__popcll(below));
+            /* the smallest id handed out this round decides whether the queue ran dry */
+            int64_t first = steal ? d_queue_id(q, base, block) : (int64_t)base;
+            if (first >= total) {
+                qtry++;
+                if (qtry >= nq) exhausted = want ? 1 : exhausted;
+                if (qtry >= nq) qtry = nq;
+            }
+            if (want && !exhausted) {
+                if (id < total) {
+                    req = d_find_request(ray_off, n_req, id);
+                    const Params *qp = reqs + req;
+                    int64_t local = id - ray_off[req];
+                    int64_t pix = d_ray_to_pixel(qp->im, local, order);
+                    P::begin(L, *qp, pix);
+                    if (P::done(L)) P::write(L, outs[req], pix);
+                    else have = 1;
+                }
+            }
+        }
+        if (!__any(have) && __all(exhausted || have)) {
+            if (__all(exhausted)) break;
+        }
+        if (have) {
+            int r0 = __builtin_amdgcn_readfirstlane(req);
+            if (__all(req == r0)) {
+                P::rhs_consume_uniform(L, reqs + r0);
+            } else {
+                P::rhs_consume(L, reqs + req);
+            }
+            if (P::done(L)) {
+                P::write(L, outs[req], P::pix(L));
+                have = 0;
+            }
+        }
+    }
+}
+
+/* zero the queue heads (a kernel, so the sequence can be captured in a graph) */
+__global__ void jsx_k_reset_queues(BatchQueues *qs) {
+    int i = threadIdx.x;
+    if (i < JSX_NQ * 8) qs->head[i] = 0ull;
+}
+
+#endif /* JSX_BATCH_H */

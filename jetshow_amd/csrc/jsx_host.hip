@@ -20,6 +20,7 @@
 
 #include "jsx_kernels.h"
 #include "jsx_mixed.h"
+#include "jsx_batch.h"
 
 
 #define JSX_OPT_PERSISTENT 0x100u   /* internal: persistent queue kernel        */
@@ -53,8 +54,18 @@ struct jsx_ctx {
     hipEvent_t slot_ev[kSlots];
     int slot_next = 0;
     int n_cu = 256;
+    /* persistent batch launch shape (autotuned) */
     int persist_blocks_per_cu = 4;
-    int persist_block = 256;
+    int64_t queue_block = 64;
+    int steal = 1;
+    /* batch buffers (grow-only) */
+    BatchQueues *d_queues = nullptr;
+    void *d_reqs = nullptr;
+    size_t reqs_cap = 0;
+    int64_t *d_off = nullptr;
+    DOut *d_outs = nullptr;
+    size_t n_cap = 0;
+    std::vector<unsigned char> h_reqs;
 };
 
 static int fail(jsx_ctx *ctx, int code, const char *fmt, ...) {
@@ -120,6 +131,7 @@ extern "C" int jsx_ctx_destroy(jsx_ctx *ctx) {
     (void)hipFree(ctx->d_params);
     (void)hipHostFree(ctx->h_params);
     for (int i = 0; i < jsx_ctx::kSlots; ++i) (void)hipEventDestroy(ctx->slot_ev[i]);
+    (void)hipFree(ctx->d_queues); (void)hipFree(ctx->d_reqs); (void)hipFree(ctx->d_off); (void)hipFree(ctx->d_outs);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return JSX_OK;
@@ -396,6 +408,82 @@ static void lower_mixed(const jsx_jet_desc &j, const jsx_solver_params &p, const
     c.dt_max_pc = p.dt_max / PC;
 }
 
+/* ---------------- persistent batch launch ---------------- */
+struct BatchItem {
+    DModel m;
+    DImage im;
+    DSolver S;
+    MConsts c;
+    DOut out;
+    int64_t npix;
+};
+
+/* Upload per-request constants and launch one persistent kernel over all
+ * rays.  All items share precision and Stokes mode.  Synchronises the stream
+ * before reusing the context's batch buffers. */
+static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes, const std::vector<BatchItem> &items,
+                        int order) {
+    size_t n = items.size();
+    if (!n) return JSX_OK;
+    size_t esz = precision == JSX_PREC_MIXED ? sizeof(DParamsM) : sizeof(DParams);
+    HIPCHECK(ctx, hipStreamSynchronize(st));
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (!ctx->d_queues) HIPCHECK(ctx, hipMalloc(&ctx->d_queues, sizeof(BatchQueues)));
+    if (ctx->reqs_cap < n * esz) {
+        (void)hipFree(ctx->d_reqs);
+        ctx->reqs_cap = n * esz;
+        HIPCHECK(ctx, hipMalloc(&ctx->d_reqs, ctx->reqs_cap));
+    }
+    if (ctx->n_cap < n) {
+        (void)hipFree(ctx->d_off); (void)hipFree(ctx->d_outs);
+        ctx->n_cap = n;
+        HIPCHECK(ctx, hipMalloc(&ctx->d_off, sizeof(int64_t) * (n + 1)));
+        HIPCHECK(ctx, hipMalloc(&ctx->d_outs, sizeof(DOut) * n));
+    }
+    ctx->h_reqs.resize(n * esz);
+    std::vector<int64_t> off(n + 1, 0);
+    std::vector<DOut> outs(n);
+    for (size_t i = 0; i < n; ++i) {
+        const BatchItem &it = items[i];
+        if (precision == JSX_PREC_MIXED) {
+            DParamsM q; q.m = it.m; q.im = it.im; q.S = it.S; q.c = it.c;
+            std::memcpy(&ctx->h_reqs[i * esz], &q, esz);
+        } else {
+            DParams q; q.m = it.m; q.im = it.im; q.S = it.S;
+            std::memcpy(&ctx->h_reqs[i * esz], &q, esz);
+        }
+        off[i + 1] = off[i] + it.npix;
+        outs[i] = it.out;
+    }
+    HIPCHECK(ctx, hipMemcpyAsync(ctx->d_reqs, ctx->h_reqs.data(), n * esz, hipMemcpyHostToDevice, st));
+    HIPCHECK(ctx, hipMemcpyAsync(ctx->d_off, off.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, st));
+    HIPCHECK(ctx, hipMemcpyAsync(ctx->d_outs, outs.data(), sizeof(DOut) * n, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(jsx_k_reset_queues, dim3(1), dim3(64), 0, st, ctx->d_queues);
+    int blocks = ctx->n_cu * ctx->persist_blocks_per_cu;
+    int64_t total = off[n];
+    int nreq = (int)n;
+    const bool full = stokes == JSX_STOKES_FULL;
+    if (precision == JSX_PREC_MIXED) {
+        const DParamsM *R = (const DParamsM *)ctx->d_reqs;
+        if (full)
+            hipLaunchKernelGGL(jsx_k_batch<PolMixed<4>>, dim3(blocks), dim3(256), 0, st, R, ctx->d_off, ctx->d_outs, nreq,
+                               total, ctx->d_queues, ctx->queue_block, order, ctx->steal);
+        else
+            hipLaunchKernelGGL(jsx_k_batch<PolMixed<1>>, dim3(blocks), dim3(256), 0, st, R, ctx->d_off, ctx->d_outs, nreq,
+                               total, ctx->d_queues, ctx->queue_block, order, ctx->steal);
+    } else {
+        const DParams *R = (const DParams *)ctx->d_reqs;
+        if (full)
+            hipLaunchKernelGGL(jsx_k_batch<PolFP64<4>>, dim3(blocks), dim3(256), 0, st, R, ctx->d_off, ctx->d_outs, nreq,
+                               total, ctx->d_queues, ctx->queue_block, order, ctx->steal);
+        else
+            hipLaunchKernelGGL(jsx_k_batch<PolFP64<1>>, dim3(blocks), dim3(256), 0, st, R, ctx->d_off, ctx->d_outs, nreq,
+                               total, ctx->d_queues, ctx->queue_block, order, ctx->steal);
+    }
+    HIPCHECK(ctx, hipGetLastError());
+    return JSX_OK;
+}
+
 /* ---------------- observe ---------------- */
 static int observe_locked(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_image_desc *image,
                           const jsx_solver_params *params, const jsx_outputs *out, hipStream_t st) {
@@ -418,19 +506,14 @@ static int observe_locked(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_image
         DParamsM hm;
         hm.m = m; hm.im = D; hm.S = S;
         lower_mixed(*jet, *params, m, D, hm.c);
+        if (params->flags & JSX_OPT_PERSISTENT) {
+            std::vector<BatchItem> one(1);
+            one[0].m = m; one[0].im = D; one[0].S = S; one[0].c = hm.c; one[0].out = O; one[0].npix = npix;
+            return launch_batch(ctx, st, JSX_PREC_MIXED, params->stokes, one, (params->flags & 0x200u) ? 1 : 0);
+        }
         DParamsM *dpm = nullptr;
         if ((rc = stage_bytes(ctx, &hm, sizeof hm, st, (void **)&dpm))) return rc;
-        if (params->flags & JSX_OPT_PERSISTENT) {
-            HIPCHECK(ctx, hipMemsetAsync(ctx->d_head, 0, sizeof(unsigned long long), st));
-            int blocks = ctx->n_cu * ctx->persist_blocks_per_cu;
-            int order = (params->flags & 0x200u) ? 1 : 0;
-            if (full)
-                hipLaunchKernelGGL(jsx_km_persist<4>, dim3(blocks), dim3(ctx->persist_block), 0, st, dpm, O, npix,
-                                   ctx->d_head, order);
-            else
-                hipLaunchKernelGGL(jsx_km_persist<1>, dim3(blocks), dim3(ctx->persist_block), 0, st, dpm, O, npix,
-                                   ctx->d_head, order);
-        } else {
+        {
             int threads = 256;
             int64_t blocks = (npix + threads - 1) / threads;
             if (full)
@@ -441,19 +524,15 @@ static int observe_locked(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_image
         HIPCHECK(ctx, hipGetLastError());
         return JSX_OK;
     }
+    if (params->flags & JSX_OPT_PERSISTENT) {
+        std::vector<BatchItem> one(1);
+        std::memset(&one[0].c, 0, sizeof(MConsts));
+        one[0].m = m; one[0].im = D; one[0].S = S; one[0].out = O; one[0].npix = npix;
+        return launch_batch(ctx, st, JSX_PREC_FP64, params->stokes, one, (params->flags & 0x200u) ? 1 : 0);
+    }
     DParams *dp = nullptr;
     if ((rc = stage_params(ctx, m, D, S, st, &dp))) return rc;
-    if (params->flags & JSX_OPT_PERSISTENT) {
-        HIPCHECK(ctx, hipMemsetAsync(ctx->d_head, 0, sizeof(unsigned long long), st));
-        int blocks = ctx->n_cu * ctx->persist_blocks_per_cu;
-        int order = (params->flags & 0x200u) ? 1 : 0;
-        if (full)
-            hipLaunchKernelGGL(jsx_k_persist<4>, dim3(blocks), dim3(ctx->persist_block), 0, st, dp, O, npix,
-                               ctx->d_head, order);
-        else
-            hipLaunchKernelGGL(jsx_k_persist<1>, dim3(blocks), dim3(ctx->persist_block), 0, st, dp, O, npix,
-                               ctx->d_head, order);
-    } else {
+    {
         int threads = 256;
         int64_t blocks = (npix + threads - 1) / threads;
         if (full)

@@ -65,54 +65,6 @@ __global__ __launch_bounds__(256) void jsx_k_plain(const DParams *__restrict__ P
     lane_write(L, out, pix);
 }
 
-__device__ inline int d_lane_id() { return __lane_id(); }
-
-template <int NS>
-__global__ __launch_bounds__(256) void jsx_k_persist(const DParams *__restrict__ P, DOut out, int64_t nrays,
-                                                     unsigned long long *head, int order) {
-    Lane<NS> L;
-    int have = 0, exhausted = 0;
-    const int lane = d_lane_id();
-    for (;;) {
-        int want = !have && !exhausted;
-        unsigned long long mask = __ballot(want);
-        if (mask) {
-            int cnt = __popcll(mask);
-            int leader = __ffsll((long long)mask) - 1;
-            unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(head, (unsigned long long)cnt);
-            unsigned lo = __shfl((unsigned)(base & 0xffffffffull), leader);
-            unsigned hi = __shfl((unsigned)(base >> 32), leader);
-            base = ((unsigned long long)hi << 32) | lo;
-            if (want) {
-                unsigned long long below = lane ? (mask & ((1ull << lane) - 1ull)) : 0ull;
-                int64_t id = (int64_t)(base + (unsigned long long)__popcll(below));
-                if (id < nrays) {
-                    const JSX_CONST DParams *q = d_launder(P);
-                    int64_t pix = d_ray_to_pixel(q->im, id, order);
-                    lane_begin(L, q->m, q->im, q->S, pix);
-                    if (L.phase == PH_DONE) lane_write(L, out, pix);
-                    else have = 1;
-                } else {
-                    exhausted = 1;
-                }
-            }
-        }
-        if (!__any(have) && __all(exhausted)) break;
-        if (have) {
-            double r[NS];
-            const JSX_CONST DParams *q = d_launder(P);
-            lane_rhs(L, q->m, q->im, q->S, r);
-            q = d_launder(P);
-            lane_consume(L, q->S, q->im, r);
-            if (L.phase == PH_DONE) {
-                lane_write(L, out, L.pix);
-                have = 0;
-            }
-        }
-    }
-}
-
 /* Geometry::hit for explicit rays */
 __global__ void jsx_k_hit(DModel m, const double *rays, int64_t n, int32_t *n_seg, double *segs) {
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -635,50 +635,4 @@ __global__ __launch_bounds__(256) void jsx_km_plain(const DParamsM *__restrict__
     ml_write(L, out, pix);
 }
 
-template <int NS>
-__global__ __launch_bounds__(256) void jsx_km_persist(const DParamsM *__restrict__ P, DOut out, int64_t nrays,
-                                                      unsigned long long *head, int order) {
-    MLane<NS> L;
-    int have = 0, exhausted = 0;
-    const int lane = __lane_id();
-    for (;;) {
-        int want = !have && !exhausted;
-        unsigned long long mask = __ballot(want);
-        if (mask) {
-            int cnt = __popcll(mask);
-            int leader = __ffsll((long long)mask) - 1;
-            unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(head, (unsigned long long)cnt);
-            unsigned lo = __shfl((unsigned)(base & 0xffffffffull), leader);
-            unsigned hi = __shfl((unsigned)(base >> 32), leader);
-            base = ((unsigned long long)hi << 32) | lo;
-            if (want) {
-                unsigned long long below = lane ? (mask & ((1ull << lane) - 1ull)) : 0ull;
-                int64_t id = (int64_t)(base + (unsigned long long)__popcll(below));
-                if (id < nrays) {
-                    const JSX_CONST DParamsM *q = d_launder_m(P);
-                    int64_t pix = d_ray_to_pixel(q->im, id, order);
-                    ml_begin(L, q->m, q->im, q->S, q->c, pix);
-                    if (L.phase == PH_DONE) ml_write(L, out, pix);
-                    else have = 1;
-                } else {
-                    exhausted = 1;
-                }
-            }
-        }
-        if (!__any(have) && __all(exhausted)) break;
-        if (have) {
-            float r[NS];
-            const JSX_CONST DParamsM *q = d_launder_m(P);
-            ml_rhs(L, q->m, q->im, q->c, r);
-            q = d_launder_m(P);
-            ml_consume(L, q->S, q->im, q->c, r);
-            if (L.phase == PH_DONE) {
-                ml_write(L, out, L.pix);
-                have = 0;
-            }
-        }
-    }
-}
-
 #endif /* JSX_MIXED_H */
