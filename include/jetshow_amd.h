@@ -178,6 +178,20 @@ int jsx_ctx_destroy(jsx_ctx *ctx);
 const char *jsx_last_error(jsx_ctx *ctx);
 int jsx_abi_version(void);
 
+/* Launch-shape and scheduling knobs of the batched solver (set by the
+ * autotuner, jetshow_amd/tune.py), and counters of the last batched launch.
+ * No reference counterpart (the reference has no GPU launch shape).
+ *   "blocks_per_cu"   persistent workgroups per CU (1..16)
+ *   "queue_block"     ray ids per queue block (>= 1)
+ *   "steal"           1: per-XCD queues with stealing, 0: one global queue
+ *   "defer_rhs"       right-hand sides after which a mixed full-Stokes ray is
+ *                     split into time-parallel pieces (0 disables)
+ *   "split_w_min"     smallest stiffness per piece
+ * Read-only: "stat_parked" rays split by the last batched launch (waits for it),
+ *            "stat_pieces" pieces it integrated. */
+int jsx_ctx_set_option(jsx_ctx *ctx, const char *name, double value);
+int jsx_ctx_get_option(jsx_ctx *ctx, const char *name, double *value);
+
 /* ---- grids: SimulationInterpolater(Delaunay_triangulation*, nan_value)
  *      src/SimulationInterpolater.cpp:30-117; values are copied to HBM once
  *      and referenced by handle from jsx_jet_desc ---------------------------- */
@@ -229,8 +243,54 @@ typedef struct jsx_request {
 } jsx_request;
 
 /* One persistent launch over many images (the ABC sweep):
- * all rays of all requests go through one work-stealing ray queue. */
+ * all rays of all requests go through one work-stealing ray queue.  Requests
+ * are grouped by (precision, stokes) and by the context's HBM budget; host
+ * planes are filled on return. */
 int jsx_observe_batch(jsx_ctx *ctx, const jsx_request *reqs, int32_t n_reqs);
+
+/* ---- request server: continuous batching of images from many callers.
+ * Replaces JetModelZoom's loop of blocking get_i_image calls
+ * (jetmodel.py:92-99, called per ABC proposal from abc_sampler.py / the
+ * emcee samplers) with asynchronous submission.  A server owns one kernel
+ * variant (precision x stokes); requests must match it. --------------------- */
+typedef struct jsx_server jsx_server;
+
+/* field slot a grid payload fills in the request's jet descriptor */
+typedef enum jsx_field_slot { JSX_FIELD_B = 1, JSX_FIELD_B2 = 2, JSX_FIELD_N = 3, JSX_FIELD_V = 4 } jsx_field_slot;
+
+/* a grid that lives in HBM only while its request is admitted */
+typedef struct jsx_grid_payload {
+    int32_t field;                /* jsx_field_slot                               */
+    int32_t pad;
+    jsx_grid_desc desc;
+    const double *values;         /* host; must stay valid until the request is done */
+} jsx_grid_payload;
+
+typedef struct jsx_request_timing {
+    double t_submit, t_admit, t_done;  /* seconds, steady clock                 */
+    int64_t n_pixels;
+    int32_t status;                    /* jsx_status of the request             */
+    int32_t pad;
+} jsx_request_timing;
+
+/* Starts the scheduler thread.  hbm_budget bounds the HBM held by admitted
+ * requests (planes + grids; 0 = half the context budget); max_active bounds
+ * requests in flight; epoch_budget is the number of integrator loop trips a
+ * lane runs per epoch before its ray state is parked (0 = default). */
+int jsx_server_create(jsx_ctx *ctx, int32_t precision, int32_t stokes, int32_t max_active, size_t hbm_budget,
+                      int32_t epoch_budget, jsx_server **out);
+/* Queues one image; req->out_host planes (host) are filled when it is done.
+ * Grid payloads, if any, are uploaded on admission and released when done;
+ * with n_grids = 0 the jet descriptor's grid handles are used as they are. */
+int jsx_server_submit(jsx_server *srv, const jsx_request *req, int32_t n_grids, const jsx_grid_payload *grids,
+                      uint64_t *id_out);
+/* *done = 1 once the request's planes are in host memory. */
+int jsx_server_poll(jsx_server *srv, uint64_t id, int32_t *done, jsx_request_timing *timing);
+/* Blocks until done, fills timing, forgets the id; returns the request's status. */
+int jsx_server_wait(jsx_server *srv, uint64_t id, jsx_request_timing *timing);
+int jsx_server_stats(jsx_server *srv, int64_t *epochs, size_t *hbm_used);
+/* Finishes every queued request, then stops the scheduler. */
+int jsx_server_destroy(jsx_server *srv);
 
 #ifdef __cplusplus
 }

@@ -94,6 +94,18 @@ class Request(C.Structure):
     ]
 
 
+FIELD_B, FIELD_B2, FIELD_N, FIELD_V = 1, 2, 3, 4
+
+
+class GridPayload(C.Structure):
+    _fields_ = [("field", C.c_int32), ("pad", C.c_int32), ("desc", GridDesc), ("values", C.POINTER(C.c_double))]
+
+
+class RequestTiming(C.Structure):
+    _fields_ = [("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
+                ("n_pixels", C.c_int64), ("status", C.c_int32), ("pad", C.c_int32)]
+
+
 def default_params(nu_hz, stokes=STOKES_I, precision=PREC_FP64, tau_max=1e4):
     """Solver constants Observation::run uses when called from pyjetshow
     (src/pyjetshow.cpp:144-156, src/Observation.cpp:23,132,156,269)."""

@@ -18,7 +18,7 @@
 #ifndef JSX_BATCH_H
 #define JSX_BATCH_H
 
-#include "jsx_mixed.h"
+#include "jsx_split.h"
 
 #define JSX_NQ 8
 
@@ -37,25 +37,27 @@ struct PolFP64 {
     __device__ static void rhs_consume(LaneT &L, const Q *q0) {
         double r[NS];
         const Q *q = q0;
-        asm volatile("" : "+v"(q));
+        JSX_OPAQUE_V(q);
         lane_rhs(L, q->m, q->im, q->S, r);
         q = q0;
-        asm volatile("" : "+v"(q));
+        JSX_OPAQUE_V(q);
         lane_consume(L, q->S, q->im, r);
     }
     template <class Q>
     __device__ static void rhs_consume_uniform(LaneT &L, const Q *q0) {
         double r[NS];
         const Q *q = q0;
-        asm volatile("" : "+s"(q));
+        JSX_OPAQUE_S(q);
         lane_rhs(L, q->m, q->im, q->S, r);
         q = q0;
-        asm volatile("" : "+s"(q));
+        JSX_OPAQUE_S(q);
         lane_consume(L, q->S, q->im, r);
     }
     __device__ static void write(const LaneT &L, const DOut &o, int64_t pix) { lane_write(L, o, pix); }
     __device__ static bool done(const LaneT &L) { return L.phase == PH_DONE; }
     __device__ static int64_t pix(const LaneT &L) { return L.pix; }
+    /* FP64 rays are never split: results stay identical to the sequential oracle */
+    __device__ static bool park(const LaneT &, int, const SplitWS &, unsigned) { return false; }
 };
 
 /* policy: mixed-precision machine */
@@ -69,25 +71,32 @@ struct PolMixed {
     __device__ static void rhs_consume(LaneT &L, const Q *q0) {
         float r[NS];
         const Q *q = q0;
-        asm volatile("" : "+v"(q));
+        JSX_OPAQUE_V(q);
         ml_rhs(L, q->m, q->im, q->c, r);
         q = q0;
-        asm volatile("" : "+v"(q));
+        JSX_OPAQUE_V(q);
         ml_consume(L, q->S, q->im, q->c, r);
     }
     template <class Q>
     __device__ static void rhs_consume_uniform(LaneT &L, const Q *q0) {
         float r[NS];
         const Q *q = q0;
-        asm volatile("" : "+s"(q));
+        JSX_OPAQUE_S(q);
         ml_rhs(L, q->m, q->im, q->c, r);
         q = q0;
-        asm volatile("" : "+s"(q));
+        JSX_OPAQUE_S(q);
         ml_consume(L, q->S, q->im, q->c, r);
     }
     __device__ static void write(const LaneT &L, const DOut &o, int64_t pix) { ml_write(L, o, pix); }
     __device__ static bool done(const LaneT &L) { return L.phase == PH_DONE; }
     __device__ static int64_t pix(const LaneT &L) { return L.pix; }
+    /* full-Stokes rays past the budget go to the time-parallel path */
+    __device__ static bool park(const LaneT &L, int slot, const SplitWS &ws, unsigned defer_rhs) {
+        if constexpr (NS == 4) {
+            if (ws.rays && split_should_park(L, defer_rhs)) return split_park(L, slot, ws);
+        }
+        return false;
+    }
 };
 
 /* workgroups are dealt to the 8 XCDs round-robin by workgroup id */
@@ -114,7 +123,7 @@ template <class P>
 __global__ __launch_bounds__(256) void jsx_k_batch(const typename P::Params *__restrict__ reqs,
                                                    const int64_t *__restrict__ ray_off, const DOut *__restrict__ outs,
                                                    int n_req, int64_t total, BatchQueues *qs, int64_t block,
-                                                   int order, int steal) {
+                                                   int order, int steal, SplitWS ws, unsigned defer_rhs) {
     using LaneT = typename P::LaneT;
     using Params = typename P::Params;
     LaneT L;
@@ -169,6 +178,8 @@ __global__ __launch_bounds__(256) void jsx_k_batch(const typename P::Params *__r
             }
             if (P::done(L)) {
                 P::write(L, outs[req], P::pix(L));
+                have = 0;
+            } else if (P::park(L, req, ws, defer_rhs)) {
                 have = 0;
             }
         }

@@ -66,6 +66,11 @@ struct jsx_ctx {
     DOut *d_outs = nullptr;
     size_t n_cap = 0;
     std::vector<unsigned char> h_reqs;
+    /* time-parallel path for long full-Stokes rays (mixed precision) */
+    unsigned defer_rhs = 8192;          /* 0 disables parking                  */
+    float split_w_min = 256.0f;
+    SplitWS split{};
+    void *split_mem = nullptr;
 };
 
 static int fail(jsx_ctx *ctx, int code, const char *fmt, ...) {
@@ -132,8 +137,62 @@ extern "C" int jsx_ctx_destroy(jsx_ctx *ctx) {
     (void)hipHostFree(ctx->h_params);
     for (int i = 0; i < jsx_ctx::kSlots; ++i) (void)hipEventDestroy(ctx->slot_ev[i]);
     (void)hipFree(ctx->d_queues); (void)hipFree(ctx->d_reqs); (void)hipFree(ctx->d_off); (void)hipFree(ctx->d_outs);
+    (void)hipFree(ctx->split_mem);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+    return JSX_OK;
+}
+
+/* ---------------- options ---------------- */
+extern "C" int jsx_ctx_set_option(jsx_ctx *ctx, const char *name, double v) {
+    if (!ctx || !name) return JSX_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    std::string k(name);
+    if (k == "blocks_per_cu") {
+        if (!(v >= 1 && v <= 16)) return fail(ctx, JSX_EINVAL, "blocks_per_cu must be in [1, 16]");
+        ctx->persist_blocks_per_cu = (int)v;
+        if (ctx->split_mem) {
+            int t = ctx->n_cu * ctx->persist_blocks_per_cu * 4 * (64 / JSX_SPLIT_GROUP) * 2;
+            ctx->split.target_pieces = t < ctx->split.piece_cap / 2 ? t : ctx->split.piece_cap / 2;
+        }
+    } else if (k == "queue_block") {
+        if (!(v >= 1 && v <= (double)(1 << 20))) return fail(ctx, JSX_EINVAL, "queue_block must be in [1, 2^20]");
+        ctx->queue_block = (int64_t)v;
+    } else if (k == "steal") {
+        ctx->steal = v != 0.0;
+    } else if (k == "defer_rhs") {
+        if (!(v >= 0 && v < 4e9)) return fail(ctx, JSX_EINVAL, "defer_rhs out of range");
+        ctx->defer_rhs = (unsigned)v;
+    } else if (k == "split_w_min") {
+        if (!(v > 0)) return fail(ctx, JSX_EINVAL, "split_w_min must be positive");
+        ctx->split_w_min = (float)v;
+        if (ctx->split_mem) ctx->split.w_min = (float)v;
+    } else {
+        return fail(ctx, JSX_EINVAL, "unknown option '%s'", name);
+    }
+    return JSX_OK;
+}
+
+extern "C" int jsx_ctx_get_option(jsx_ctx *ctx, const char *name, double *v) {
+    if (!ctx || !name || !v) return JSX_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    std::string k(name);
+    if (k == "blocks_per_cu") *v = ctx->persist_blocks_per_cu;
+    else if (k == "queue_block") *v = (double)ctx->queue_block;
+    else if (k == "steal") *v = ctx->steal;
+    else if (k == "defer_rhs") *v = ctx->defer_rhs;
+    else if (k == "split_w_min") *v = ctx->split_w_min;
+    else if (k == "stat_parked" || k == "stat_pieces") {
+        unsigned c[2] = {0, 0};
+        if (ctx->split_mem) {
+            HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+            HIPCHECK(ctx, hipMemcpy(c, ctx->split.n_long, 2 * sizeof(unsigned), hipMemcpyDeviceToHost));
+        }
+        unsigned parked = c[0] < (unsigned)ctx->split.cap ? c[0] : (unsigned)ctx->split.cap;
+        *v = k == "stat_parked" ? (double)parked : (double)c[1];
+    } else {
+        return fail(ctx, JSX_EINVAL, "unknown option '%s'", name);
+    }
     return JSX_OK;
 }
 
@@ -369,6 +428,16 @@ static int stage_params(jsx_ctx *ctx, const DModel &m, const DImage &D, const DS
     return stage_bytes(ctx, &h, sizeof h, st, (void **)dev);
 }
 
+#ifndef JSX_MIXED_TAU_FLOOR
+#define JSX_MIXED_TAU_FLOOR 1e-7
+#endif
+#ifndef JSX_MIXED_STOKES_FLOOR_FULL
+#define JSX_MIXED_STOKES_FLOOR_FULL 1e-6
+#endif
+#ifndef JSX_MIXED_STOKES_FLOOR_I
+#define JSX_MIXED_STOKES_FLOOR_I 1e-7
+#endif
+
 /* FP32 constants of the mixed path (see jsx_mixed.h for the formulas) */
 static void lower_mixed(const jsx_jet_desc &j, const jsx_solver_params &p, const DModel &m, const DImage &D,
                         MConsts &c) {
@@ -400,11 +469,19 @@ static void lower_mixed(const jsx_jet_desc &j, const jsx_solver_params &p, const
     c.sin_pitch = (float)m.sin_pitch; c.cos_pitch = (float)m.cos_pitch;
     c.thermal_fraction = (float)j.thermal_fraction;
     c.inv_pc = 1.0 / PC;
-    /* FP32 stage values put a floor under the usable tolerances */
-    c.tau_abs = p.tau_abs_tol > 1e-6 ? p.tau_abs_tol : 1e-6;
-    c.tau_rel = p.tau_rel_tol > 1e-6 ? p.tau_rel_tol : 1e-6;
+    /* FP32 stage values put a floor under the usable relative tolerance: the
+       embedded error estimate dt * sum(dc_j k_j) carries the stages' rounding,
+       ~1e-7 |k| x |dc| ~ 8e-9 dt |k|, i.e. an error-norm noise of ~8e-9 / rel,
+       harmless (< 0.1) down to rel = 1e-7.  Full Stokes keeps 1e-6: there the
+       per-component norm also weighs the small Q, U, V, whose noise relative
+       to their own scale is larger, and 1e-7 would triple the rejected steps
+       of Faraday-rotating rays for no gain in I */
+    const double ft = JSX_MIXED_TAU_FLOOR;
+    const double fs = p.stokes == JSX_STOKES_FULL ? JSX_MIXED_STOKES_FLOOR_FULL : JSX_MIXED_STOKES_FLOOR_I;
+    c.tau_abs = p.tau_abs_tol > ft ? p.tau_abs_tol : ft;
+    c.tau_rel = p.tau_rel_tol > ft ? p.tau_rel_tol : ft;
     c.st_abs = p.stokes_abs_tol;
-    c.st_rel = p.stokes_rel_tol > 1e-6 ? p.stokes_rel_tol : 1e-6;
+    c.st_rel = p.stokes_rel_tol > fs ? p.stokes_rel_tol : fs;
     c.dt_max_pc = p.dt_max / PC;
 }
 
@@ -417,6 +494,36 @@ struct BatchItem {
     DOut out;
     int64_t npix;
 };
+
+/* one allocation for the split path's tables (kept for the context's life) */
+static int split_workspace(jsx_ctx *ctx) {
+    if (ctx->split_mem) return JSX_OK;
+    const int cap = 16384, piece_cap = 1 << 17;
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
+    size_t o_cnt = take(4 * sizeof(unsigned));
+    size_t o_rays = take(sizeof(LongRay) * cap);
+    size_t o_prof = take(sizeof(float) * (size_t)cap * JSX_SPLIT_NSAMP);
+    size_t o_pcs = take(sizeof(Piece) * piece_cap);
+    size_t o_pout = take(sizeof(PieceOut) * piece_cap);
+    HIPCHECK(ctx, hipMalloc(&ctx->split_mem, off));
+    char *b = (char *)ctx->split_mem;
+    SplitWS &w = ctx->split;
+    w.n_long = (unsigned *)(b + o_cnt);
+    w.n_pieces = w.n_long + 1;
+    w.piece_head = w.n_long + 2;
+    w.rays = (LongRay *)(b + o_rays);
+    w.cap = cap;
+    w.prof = (float *)(b + o_prof);
+    w.pieces = (Piece *)(b + o_pcs);
+    w.pout = (PieceOut *)(b + o_pout);
+    w.piece_cap = piece_cap;
+    /* about two pieces per 5-lane group the piece kernel keeps resident */
+    w.target_pieces = ctx->n_cu * ctx->persist_blocks_per_cu * 4 * (64 / JSX_SPLIT_GROUP) * 2;
+    if (w.target_pieces > piece_cap / 2) w.target_pieces = piece_cap / 2;
+    w.w_min = ctx->split_w_min;
+    return JSX_OK;
+}
 
 /* Upload per-request constants and launch one persistent kernel over all
  * rays.  All items share precision and Stokes mode.  Synchronises the stream
@@ -463,22 +570,38 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
     int64_t total = off[n];
     int nreq = (int)n;
     const bool full = stokes == JSX_STOKES_FULL;
+    SplitWS ws{};
+    const bool split = precision == JSX_PREC_MIXED && full && ctx->defer_rhs > 0;
+    if (split) {
+        int rc = split_workspace(ctx);
+        if (rc) return rc;
+        ws = ctx->split;
+        hipLaunchKernelGGL(jsx_k_split_reset, dim3(1), dim3(64), 0, st, ws);
+    }
     if (precision == JSX_PREC_MIXED) {
         const DParamsM *R = (const DParamsM *)ctx->d_reqs;
         if (full)
             hipLaunchKernelGGL(jsx_k_batch<PolMixed<4>>, dim3(blocks), dim3(256), 0, st, R, ctx->d_off, ctx->d_outs, nreq,
-                               total, ctx->d_queues, ctx->queue_block, order, ctx->steal);
+                               total, ctx->d_queues, ctx->queue_block, order, ctx->steal, ws, ctx->defer_rhs);
         else
             hipLaunchKernelGGL(jsx_k_batch<PolMixed<1>>, dim3(blocks), dim3(256), 0, st, R, ctx->d_off, ctx->d_outs, nreq,
-                               total, ctx->d_queues, ctx->queue_block, order, ctx->steal);
+                               total, ctx->d_queues, ctx->queue_block, order, ctx->steal, ws, 0u);
+        if (split) {
+            const int cap = ws.cap;
+            hipLaunchKernelGGL(jsx_k_split_sample, dim3((cap * 64 + 255) / 256), dim3(256), 0, st, R, ws);
+            hipLaunchKernelGGL(jsx_k_split_alloc, dim3(1), dim3(1024), 0, st, ws);
+            hipLaunchKernelGGL(jsx_k_split_cut, dim3(cap), dim3(256), 0, st, ws);
+            hipLaunchKernelGGL(jsx_k_split_piece, dim3(blocks), dim3(256), 0, st, R, ws);
+            hipLaunchKernelGGL(jsx_k_split_combine, dim3((cap + 255) / 256), dim3(256), 0, st, ctx->d_outs, ws);
+        }
     } else {
         const DParams *R = (const DParams *)ctx->d_reqs;
         if (full)
             hipLaunchKernelGGL(jsx_k_batch<PolFP64<4>>, dim3(blocks), dim3(256), 0, st, R, ctx->d_off, ctx->d_outs, nreq,
-                               total, ctx->d_queues, ctx->queue_block, order, ctx->steal);
+                               total, ctx->d_queues, ctx->queue_block, order, ctx->steal, ws, 0u);
         else
             hipLaunchKernelGGL(jsx_k_batch<PolFP64<1>>, dim3(blocks), dim3(256), 0, st, R, ctx->d_off, ctx->d_outs, nreq,
-                               total, ctx->d_queues, ctx->queue_block, order, ctx->steal);
+                               total, ctx->d_queues, ctx->queue_block, order, ctx->steal, ws, 0u);
     }
     HIPCHECK(ctx, hipGetLastError());
     return JSX_OK;
@@ -686,12 +809,79 @@ extern "C" double jsx_mas_to_pc(double z) {
     return 4.84813681109536e-09 * d_a;
 }
 
-/* ---------------- batch ---------------- */
+/* ---------------- batch ----------------
+ * Requests are taken in order into groups that share (precision, stokes) and
+ * whose output planes fit the HBM budget; each group is one persistent launch
+ * (apex columns first), then its planes are copied to the callers' buffers. */
+static int batch_group(jsx_ctx *ctx, const jsx_request *reqs, int32_t i0, int32_t i1) {
+    std::vector<BatchItem> items(i1 - i0);
+    std::vector<jsx_outputs> dev(i1 - i0);
+    int rc = JSX_OK;
+    int32_t n_alloc = 0;
+    for (int32_t i = i0; i < i1 && rc == JSX_OK; ++i) {
+        const jsx_request &r = reqs[i];
+        BatchItem &it = items[i - i0];
+        it.npix = r.image.n_across * r.image.n_along;
+        if ((rc = jsx_alloc_outputs(ctx, it.npix, JSX_STOKES_FULL, &dev[i - i0]))) break;
+        n_alloc++;
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if ((rc = lower_model(ctx, r.jet, r.params.flags, it.m))) break;
+        if ((rc = lower_solver(ctx, r.params, it.S))) break;
+        if ((rc = lower_image(ctx, r.image, it.im, nullptr))) break;
+        if (r.params.precision == JSX_PREC_MIXED) lower_mixed(r.jet, r.params, it.m, it.im, it.c);
+        else std::memset(&it.c, 0, sizeof it.c);
+        it.out = lower_out(dev[i - i0]);
+    }
+    if (rc == JSX_OK) {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        rc = launch_batch(ctx, ctx->stream, reqs[i0].params.precision, reqs[i0].params.stokes, items, 1);
+        if (rc == JSX_OK) {
+            hipError_t e = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess) rc = fail(ctx, JSX_EHIP, "batch kernel failed: %s", hipGetErrorString(e));
+        }
+    }
+    for (int32_t i = i0; i < i0 + n_alloc; ++i) {
+        const jsx_outputs &h = reqs[i].out_host, &d = dev[i - i0];
+        size_t n = (size_t)items[i - i0].npix, nd = n * sizeof(double), nu = n * sizeof(uint32_t);
+        struct { void *h; void *dv; size_t b; } cp[] = {
+            {h.tau, d.tau, nd}, {h.I, d.I, nd}, {h.Q, d.Q, nd}, {h.U, d.U, nd}, {h.V, d.V, nd},
+            {h.l, d.l, nd}, {h.flags, d.flags, nu}, {h.n_rhs, d.n_rhs, nu}, {h.n_steps, d.n_steps, nu}};
+        if (rc == JSX_OK)
+            for (auto &c : cp)
+                if (c.h && hipMemcpy(c.h, c.dv, c.b, hipMemcpyDeviceToHost) != hipSuccess) {
+                    rc = fail(ctx, JSX_EHIP, "copy back failed");
+                    break;
+                }
+        jsx_free_outputs(ctx, &dev[i - i0]);
+    }
+    return rc;
+}
+
 extern "C" int jsx_observe_batch(jsx_ctx *ctx, const jsx_request *reqs, int32_t n_reqs) {
-    if (!ctx || (!reqs && n_reqs)) return JSX_EINVAL;
+    if (!ctx || (!reqs && n_reqs) || n_reqs < 0) return JSX_EINVAL;
+    const size_t per_pix = 6 * sizeof(double) + 3 * sizeof(uint32_t);
     for (int32_t i = 0; i < n_reqs; ++i) {
-        int rc = jsx_observe_host(ctx, &reqs[i].jet, &reqs[i].image, &reqs[i].params, &reqs[i].out_host);
+        const jsx_image_desc &im = reqs[i].image;
+        if (im.n_across <= 0 || im.n_along <= 0 || (im.n_across % 2))
+            return fail(ctx, JSX_EINVAL, "request %d: image needs n_along > 0 and even n_across > 0", i);
+        if (!reqs[i].out_host.I) return fail(ctx, JSX_EINVAL, "request %d: out_host.I must be set", i);
+        if ((size_t)(im.n_across * im.n_along) * per_pix > ctx->budget - ctx->used)
+            return fail(ctx, JSX_EBUDGET, "request %d does not fit the HBM budget", i);
+    }
+    int32_t i0 = 0;
+    while (i0 < n_reqs) {
+        size_t bytes = 0;
+        int32_t i1 = i0;
+        while (i1 < n_reqs && reqs[i1].params.precision == reqs[i0].params.precision &&
+               reqs[i1].params.stokes == reqs[i0].params.stokes) {
+            size_t b = (size_t)(reqs[i1].image.n_across * reqs[i1].image.n_along) * per_pix;
+            if (i1 > i0 && bytes + b > ctx->budget - ctx->used) break;
+            bytes += b;
+            ++i1;
+        }
+        int rc = batch_group(ctx, reqs, i0, i1);
         if (rc) return rc;
+        i0 = i1;
     }
     return JSX_OK;
 }
@@ -704,6 +894,10 @@ extern "C" size_t jsx_sizeof(int which) {
     case 3: return sizeof(jsx_outputs);
     case 4: return sizeof(jsx_grid_desc);
     case 5: return sizeof(jsx_request);
+    case 6: return sizeof(jsx_grid_payload);
+    case 7: return sizeof(jsx_request_timing);
     default: return 0;
     }
 }
+
+#include "jsx_serve.h"

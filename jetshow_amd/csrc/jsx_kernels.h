@@ -29,13 +29,17 @@ struct DParams {
  * reads are scalar (s_load) and re-issued after every launder point */
 #if defined(__HIP_DEVICE_COMPILE__)
 #define JSX_CONST __attribute__((address_space(4)))
+#define JSX_OPAQUE_S(x) asm volatile("" : "+s"(x))   /* value now unknown, kept in SGPRs */
+#define JSX_OPAQUE_V(x) asm volatile("" : "+v"(x))   /* value now unknown, kept in VGPRs */
 #else
 #define JSX_CONST
+#define JSX_OPAQUE_S(x) asm volatile("" : "+r"(x))
+#define JSX_OPAQUE_V(x) asm volatile("" : "+r"(x))
 #endif
 
 __device__ __forceinline__ const JSX_CONST DParams *d_launder(const DParams *p) {
     const JSX_CONST DParams *q = (const JSX_CONST DParams *)p;
-    asm volatile("" : "+s"(q));
+    JSX_OPAQUE_S(q);
     return q;
 }
 

@@ -286,11 +286,11 @@ __device__ inline double ml_seg_length(const MLane<NS> &L) {
     return d_norm3(dv);
 }
 
-/* dt * sum_j c_j k_j, the 5th-order step increment */
+/* h * sum_j c_j k_j, the 5th-order increment of a step of size h */
 template <int NS>
-__device__ inline float ml_increment(const MLane<NS> &L, int i) {
-    return (float)L.dt * ((float)DP_C1 * L.k1[i] + (float)DP_C3 * L.k3[i] + (float)DP_C4 * L.k4[i] +
-                          (float)DP_C5 * L.k5[i] + (float)DP_C6 * L.k6[i]);
+__device__ inline float ml_increment(const MLane<NS> &L, int i, double h) {
+    return (float)h * ((float)DP_C1 * L.k1[i] + (float)DP_C3 * L.k3[i] + (float)DP_C4 * L.k4[i] +
+                       (float)DP_C5 * L.k5[i] + (float)DP_C6 * L.k6[i]);
 }
 
 template <int NS>
@@ -319,7 +319,7 @@ __device__ inline void ml_prepare_stage(MLane<NS> &L, int s) {
                                      (float)DP_B64 * L.k4[i] + (float)DP_B65 * L.k5[i]);
         L.tq = t + dtd; break;
     default:
-        for (int i = 0; i < NS; ++i) L.xt[i] = L.x[i] + ml_increment(L, i);
+        for (int i = 0; i < NS; ++i) L.xt[i] = L.x[i] + ml_increment(L, i, L.dt);
         L.tq = t + dtd; break;
     }
     L.sub = s;
@@ -500,8 +500,50 @@ __device__ inline void ml_rhs(const MLane<NS> &L, const DModel &m, const DImage 
     }
 }
 
+/* FP64 error norm of the finished step from the FP32 stage derivatives.
+ * The scale |x_i| + dt |k1_i| of each of Q, U, V is floored at the largest of
+ * the three: a component passing through zero is then controlled relative to
+ * the polarized intensity, not to itself, where the FP32 rounding of the large
+ * terms that drive it (k_F U, k_C V, ...) would dominate and reject sound
+ * steps (a third of all steps on Faraday-thick rays).  `floor` (0 for a
+ * whole ray) bounds every scale from below; the pieces of a split ray use it
+ * (jsx_split.h): a propagator column starts at a unit vector and multiplies
+ * the Stokes vector accumulated before it, so its entries' errors matter in
+ * absolute terms (floor 1); a piece's particular solution starts at zero but
+ * its error matters relative to the ray's intensity (floor |I| where the ray
+ * was split).  Without the floors both would keep resolving decays and
+ * build-ups to 1e-6 of their own small size, at a fraction of the step the
+ * whole ray takes there. */
 template <int NS>
-__device__ inline void ml_consume(MLane<NS> &L, const DSolver &S, const DImage &im, const MConsts &c, const float *r) {
+__device__ inline double ml_step_error(const MLane<NS> &L, const MConsts &c, double floor = 0.0) {
+    double dt = L.dt, err = 0.0;
+    int nl = L.phase == PH_TAU ? 1 : NS;
+    double ea = L.phase == PH_TAU ? c.tau_abs : c.st_abs, er = L.phase == PH_TAU ? c.tau_rel : c.st_rel;
+    double xe[NS], sc[NS], pmax = 0.0;
+    for (int i = 0; i < nl; ++i) {
+        xe[i] = dt * (DP_DC1 * (double)L.k1[i] + DP_DC3 * (double)L.k3[i] + DP_DC4 * (double)L.k4[i] +
+                      DP_DC5 * (double)L.k5[i] + DP_DC6 * (double)L.k6[i] + DP_DC7 * (double)L.k7[i]);
+        sc[i] = fabs(ml_xval(L, i)) + dt * fabs((double)L.k1[i]);
+        if (i > 0) pmax = sc[i] > pmax ? sc[i] : pmax;
+    }
+    for (int i = 0; i < nl; ++i) {
+        double s = i > 0 ? pmax : sc[i];
+        s = s > floor ? s : floor;
+        double e = fabs(xe[i]) / (ea + er * s);
+        err = (err < e) ? e : err;
+    }
+    return err;
+}
+
+/* error of the finished step: one lane per ray */
+struct MlNoReduce {
+    template <int NS>
+    __device__ double operator()(const MLane<NS> &L, const MConsts &c) const { return ml_step_error(L, c); }
+};
+
+template <int NS, class Red = MlNoReduce>
+__device__ inline void ml_consume(MLane<NS> &L, const DSolver &S, const DImage &im, const MConsts &c, const float *r,
+                                  Red red = Red()) {
     L.n_rhs++;
     int ns_live = L.phase == PH_TAU ? 1 : NS;
     for (int i = 0; i < ns_live; ++i)
@@ -521,16 +563,8 @@ __device__ inline void ml_consume(MLane<NS> &L, const DSolver &S, const DImage &
     default: break;
     }
     for (int i = 0; i < NS; ++i) L.k7[i] = r[i];
-    /* FP64 error norm from the FP32 stage derivatives */
-    double dt = L.dt, err = 0.0;
-    int nl = L.phase == PH_TAU ? 1 : NS;
-    double ea = L.phase == PH_TAU ? c.tau_abs : c.st_abs, er = L.phase == PH_TAU ? c.tau_rel : c.st_rel;
-    for (int i = 0; i < nl; ++i) {
-        double xe = dt * (DP_DC1 * (double)L.k1[i] + DP_DC3 * (double)L.k3[i] + DP_DC4 * (double)L.k4[i] +
-                          DP_DC5 * (double)L.k5[i] + DP_DC6 * (double)L.k6[i] + DP_DC7 * (double)L.k7[i]);
-        double e = fabs(xe) / (ea + er * (fabs(ml_xval(L, i)) + dt * fabs((double)L.k1[i])));
-        err = (err < e) ? e : err;
-    }
+    double err = red(L, c);
+    const double h = L.dt;   /* the step just taken; the controller rescales L.dt */
     int ok;
     double maxdt = L.phase == PH_TAU ? c.dt_max_pc : 0.0;
     if (err > 1.0) {
@@ -553,7 +587,7 @@ __device__ inline void ml_consume(MLane<NS> &L, const DSolver &S, const DImage &
     if (L.phase == PH_TAU) {
         L.x_old = ml_xval(L, 0);
         L.k_old = (double)L.k1[0];
-        m_kahan_add(L.x[0], L.xlo[0], ml_increment(L, 0));
+        m_kahan_add(L.x[0], L.xlo[0], ml_increment(L, 0, h));
         L.k1[0] = L.k7[0];
         double xv = ml_xval(L, 0);
         if (xv >= S.tau_max) {
@@ -582,7 +616,7 @@ __device__ inline void ml_consume(MLane<NS> &L, const DSolver &S, const DImage &
         return;
     }
     for (int i = 0; i < NS; ++i) {
-        m_kahan_add(L.x[i], L.xlo[i], ml_increment(L, i));
+        m_kahan_add(L.x[i], L.xlo[i], ml_increment(L, i, h));
         L.k1[i] = L.k7[i];
     }
     ml_stokes_loop(L, S, c);
@@ -614,7 +648,7 @@ struct DParamsM {
 
 __device__ __forceinline__ const JSX_CONST DParamsM *d_launder_m(const DParamsM *p) {
     const JSX_CONST DParamsM *q = (const JSX_CONST DParamsM *)p;
-    asm volatile("" : "+s"(q));
+    JSX_OPAQUE_S(q);
     return q;
 }
 

@@ -1,0 +1,527 @@
+/*
+ * jsx_split.h -- time-parallel integration of Faraday-thick rays (mixed
+ * full-Stokes path).
+ *
+ * The Stokes pass integrates the linear system dS/dt = eps(t) - K(t) S.  Over
+ * [a, b] its solution is S(b) = Phi(b, a) S(a) + P(b, a), where P is the
+ * solution started from S(a) = 0 and the columns of the 4x4 propagator Phi are
+ * the solutions of dS/dt = -K S started from the unit vectors.  Integrating
+ * pieces of one ray independently and composing
+ *     S_k = Phi_k S_{k-1} + P_k
+ * in order gives the same S(end) up to the integration tolerance, so one long
+ * ray becomes many short, concurrent ones.  In the image this matters for the
+ * rays that graze the apex of a Faraday-thick jet: the explicit Dormand-Prince
+ * step is bounded by the rotation rate |k_F| there, and a single ray can need
+ * 10^7 right-hand sides -- one lane for minutes while the rest of the GPU idles.
+ *
+ * Pipeline (all on one stream, no host round trip; counts live in HBM):
+ *  1. the batch kernel parks a ray whose right-hand-side count passes
+ *     `defer_rhs` at the start of a step of its Stokes pass: state, step
+ *     size, position and the remaining path go to a LongRay record;
+ *  2. jsx_k_split_sample: one wavefront per parked ray evaluates the transfer
+ *     coefficients at NSAMP points of the remaining path and stores the
+ *     cumulative stiffness  w(u) = int (k_I + |k_F| + |k_C| + |h_Q| + |k_P|) du
+ *     plus a floor, the estimate of the step count;
+ *  3. jsx_k_split_alloc (one workgroup): pieces per ray proportional to w
+ *     (so every piece costs about the same), prefix offsets;
+ *  4. jsx_k_split_cut: piece boundaries at equal-w quantiles of each of the
+ *     ray's (one or two) remaining path intervals;
+ *  5. jsx_k_split_piece: groups of 5 lanes (particular solution and the four
+ *     propagator columns) integrate one piece each, with one shared step
+ *     sequence (the group's largest error norm decides);
+ *  6. jsx_k_split_combine: one lane per parked ray composes its pieces in
+ *     order and writes the pixel.
+ */
+#ifndef JSX_SPLIT_H
+#define JSX_SPLIT_H
+
+#include "jsx_mixed.h"
+
+#define JSX_SPLIT_NSAMP 512     /* coefficient samples per parked ray (8 per lane) */
+#define JSX_SPLIT_GROUP 5       /* lanes per piece                                 */
+
+struct LongRay {
+    int slot, seg;                /* request index; geometric segment being integrated */
+    int64_t pix;
+    uint32_t flags, n_rhs, n_steps, pad;
+    double tau_bg, thick;
+    double t, L, dt;              /* parsec: position, segment length, proposed step */
+    double cout[3];               /* origin of the current segment's Stokes pass [cm] */
+    double s0out[3], L0;          /* segment 0 (still to do when seg == 1)           */
+    double x[4];                  /* Stokes vector at t                              */
+    int piece0, npieces;          /* filled by the planner                           */
+    float w_total, w_split;       /* cumulative stiffness: total, at the interval end */
+};
+
+struct Piece {
+    int ray, interval;            /* interval 0: rest of the current segment; 1: segment 0 */
+    double ta, tb, dt0;
+};
+
+struct PieceOut {
+    double P[4];
+    double Phi[16];               /* column-major: Phi[4 * j + i] = row i of column j */
+    uint32_t flags, n_rhs, n_steps, pad;
+};
+
+struct SplitWS {
+    unsigned *n_long;             /* parked rays (may exceed cap: then clamp)        */
+    LongRay *rays;
+    int cap;
+    float *prof;                  /* cap x NSAMP cumulative stiffness               */
+    unsigned *n_pieces;           /* pieces planned                                 */
+    unsigned *piece_head;         /* next piece to integrate                        */
+    Piece *pieces;
+    PieceOut *pout;
+    int piece_cap;
+    int target_pieces;            /* concurrency the planner aims at                */
+    float w_min;                  /* smallest stiffness per piece worth a split     */
+};
+
+/* ---- 1. parking (called by the batch kernel for PolMixed<4>) ---- */
+__device__ inline bool split_should_park(const MLane<4> &L, unsigned defer_rhs) {
+    return L.phase == PH_STOKES && L.sub == 2 && L.tries == 0 && L.n_rhs > defer_rhs;
+}
+
+/* returns false (ray continues in place) when the record table is full */
+__device__ inline bool split_park(const MLane<4> &L, int slot, const SplitWS &ws) {
+    unsigned k = atomicAdd(ws.n_long, 1u);
+    if (k >= (unsigned)ws.cap) return false;
+    LongRay &R = ws.rays[k];
+    R.slot = slot;
+    R.seg = L.seg;
+    R.pix = L.pix;
+    R.flags = L.flags;
+    R.n_rhs = L.n_rhs;
+    R.n_steps = L.n_steps - 1;    /* the step just begun is not taken here */
+    R.tau_bg = L.tau_bg;
+    R.thick = L.thick;
+    R.t = L.t;
+    R.L = L.L;
+    R.dt = L.dt;
+    for (int i = 0; i < 3; ++i) {
+        R.cout[i] = L.cout[i];
+        R.s0out[i] = L.s0out[i];
+    }
+    double d[3] = {L.s0out[0] - L.s0in[0], L.s0out[1] - L.s0in[1], L.s0out[2] - L.s0in[2]};
+    R.L0 = L.seg ? d_norm3(d) * (1.0 / JSX_PC) : 0.0;
+    for (int i = 0; i < 4; ++i) R.x[i] = ml_xval(L, i);
+    return true;
+}
+
+/* u in [0, total) of the concatenated remaining path -> point [pc] */
+__device__ inline void split_point(const LongRay &R, const DImage &im, double inv_pc, double u, double *p) {
+    double restA = R.L - R.t;
+    if (u < restA || !R.seg) {
+        double t = R.t + u;
+        for (int i = 0; i < 3; ++i) p[i] = R.cout[i] * inv_pc - t * im.dir[i];
+    } else {
+        double t = u - restA;
+        for (int i = 0; i < 3; ++i) p[i] = R.s0out[i] * inv_pc - t * im.dir[i];
+    }
+}
+
+__device__ inline float split_stiffness(const MCoef &k) {
+    float pol = __builtin_sqrtf(k.kQ * k.kQ + k.kU * k.kU + k.kV * k.kV);
+    return fabsf(k.kI) + fabsf(k.kF) + fabsf(k.kC) + fabsf(k.hQ) + pol;
+}
+
+/* ---- 2. sampling: one wavefront per parked ray ---- */
+__global__ __launch_bounds__(256) void jsx_k_split_sample(const DParamsM *__restrict__ reqs, SplitWS ws) {
+    const int lane = __lane_id();
+    const unsigned r = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    unsigned n = *ws.n_long;
+    if (n > (unsigned)ws.cap) n = ws.cap;
+    if (r >= n) return;
+    LongRay &R = ws.rays[r];
+    const DParamsM *q = reqs + R.slot;
+    const double total = (R.L - R.t) + (R.seg ? R.L0 : 0.0);
+    const int per = JSX_SPLIT_NSAMP / 64;
+    const double du = total / JSX_SPLIT_NSAMP;
+    float w[JSX_SPLIT_NSAMP / 64];
+    float acc = 0.f;
+    /* floor: a piece's step count never drops below what its length needs when
+       the coefficients vanish (~16 steps over the whole remaining path) */
+    const float floor_w = 16.0f / JSX_SPLIT_NSAMP;
+#pragma unroll
+    for (int j = 0; j < per; ++j) {
+        int kk = lane * per + j;
+        double p[3];
+        split_point(R, q->im, q->c.inv_pc, (kk + 0.5) * du, p);
+        MCoef k;
+        m_coefficients<1>(q->m, q->c, p, k);
+        float c = split_stiffness(k) * (float)du;
+        if (!(c == c) || c > 1e30f) c = 1e30f;
+        acc += c + floor_w;
+        w[j] = acc;
+    }
+    /* inclusive scan of the lane totals across the wavefront */
+    float tot = acc;
+    for (int off = 1; off < 64; off <<= 1) {
+        float v = __shfl_up(tot, off);
+        if (lane >= off) tot += v;
+    }
+    float before = tot - acc;
+    float *prof = ws.prof + (size_t)r * JSX_SPLIT_NSAMP;
+#pragma unroll
+    for (int j = 0; j < per; ++j) prof[lane * per + j] = before + w[j];
+    float wt = __shfl(tot, 63);
+    if (lane == 0) R.w_total = wt;
+    /* stiffness up to the end of the current segment (interval 0): the
+       cumulative value of the last sample inside it */
+    if (R.seg) {
+        int kb = (int)floor((R.L - R.t) / du);
+        if (kb > JSX_SPLIT_NSAMP) kb = JSX_SPLIT_NSAMP;
+        if (kb <= 0) {
+            if (lane == 0) R.w_split = 0.f;
+        } else {
+#pragma unroll
+            for (int j = 0; j < per; ++j)
+                if (lane * per + j == kb - 1) R.w_split = before + w[j];
+        }
+    } else if (lane == 0) {
+        R.w_split = wt;
+    }
+}
+
+/* ---- 3. piece counts (one workgroup) ---- */
+__global__ __launch_bounds__(1024) void jsx_k_split_alloc(SplitWS ws) {
+    __shared__ float s_sum[1024];
+    __shared__ float s_wp;
+    __shared__ int s_tot[1024];
+    const int tid = threadIdx.x;
+    unsigned n = *ws.n_long;
+    if (n > (unsigned)ws.cap) n = ws.cap;
+    float local = 0.f;
+    for (unsigned r = tid; r < n; r += blockDim.x) local += ws.rays[r].w_total;
+    s_sum[tid] = local;
+    __syncthreads();
+    for (int h = 512; h > 0; h >>= 1) {
+        if (tid < h) s_sum[tid] += s_sum[tid + h];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        float wp = s_sum[0] / (float)ws.target_pieces;
+        s_wp = wp > ws.w_min ? wp : ws.w_min;
+    }
+    __syncthreads();
+    /* grow the per-piece stiffness until the pieces fit the table */
+    for (int iter = 0; iter < 32; ++iter) {
+        float wp = s_wp;
+        int cnt = 0;
+        for (unsigned r = tid; r < n; r += blockDim.x) {
+            const LongRay &R = ws.rays[r];
+            float np = ceilf(R.w_total / wp);
+            int k = np < 1.f ? 1 : (np > 65536.f ? 65536 : (int)np);
+            if (R.seg) k = k < 2 ? 2 : k;
+            cnt += k;
+        }
+        s_tot[tid] = cnt;
+        __syncthreads();
+        for (int h = 512; h > 0; h >>= 1) {
+            if (tid < h) s_tot[tid] += s_tot[tid + h];
+            __syncthreads();
+        }
+        int total = s_tot[0];
+        __syncthreads();
+        if (total <= ws.piece_cap) break;
+        if (tid == 0) s_wp = wp * 2.0f;
+        __syncthreads();
+    }
+    /* exclusive prefix of the piece counts in ray order (serial over chunks) */
+    const float wp = s_wp;
+    __shared__ int s_base;
+    if (tid == 0) s_base = 0;
+    __syncthreads();
+    for (unsigned r0 = 0; r0 < n; r0 += blockDim.x) {
+        unsigned r = r0 + tid;
+        int k = 0;
+        if (r < n) {
+            const LongRay &R = ws.rays[r];
+            float np = ceilf(R.w_total / wp);
+            k = np < 1.f ? 1 : (np > 65536.f ? 65536 : (int)np);
+            if (R.seg) k = k < 2 ? 2 : k;
+        }
+        s_tot[tid] = k;
+        __syncthreads();
+        for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+            int v = tid >= off ? s_tot[tid - off] : 0;
+            __syncthreads();
+            s_tot[tid] += v;
+            __syncthreads();
+        }
+        if (r < n) {
+            ws.rays[r].piece0 = s_base + s_tot[tid] - k;
+            ws.rays[r].npieces = k;
+        }
+        __syncthreads();
+        if (tid == blockDim.x - 1) s_base += s_tot[tid];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        int tot = s_base;
+        *ws.n_pieces = (unsigned)(tot < ws.piece_cap ? tot : ws.piece_cap);
+    }
+}
+
+/* u of the cumulative stiffness level w (linear between samples) */
+__device__ inline double split_u_of_w(const float *prof, double du, float w) {
+    int lo = 0, hi = JSX_SPLIT_NSAMP - 1;
+    if (w <= prof[0]) return du * (double)(w / (prof[0] > 0.f ? prof[0] : 1.f));
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (prof[mid] < w) lo = mid + 1;
+        else hi = mid;
+    }
+    float w0 = prof[lo - 1], w1 = prof[lo];
+    double f = w1 > w0 ? (double)((w - w0) / (w1 - w0)) : 0.0;
+    f = f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
+    return du * ((double)lo + f);
+}
+
+/* ---- 4. piece boundaries: one thread per piece slot of a ray ---- */
+__global__ __launch_bounds__(256) void jsx_k_split_cut(SplitWS ws) {
+    const unsigned r = blockIdx.x;
+    unsigned n = *ws.n_long;
+    if (n > (unsigned)ws.cap) n = ws.cap;
+    if (r >= n) return;
+    const LongRay &R = ws.rays[r];
+    const int np = R.npieces, p0 = R.piece0;
+    if (p0 + np > ws.piece_cap) return;
+    const double restA = R.L - R.t;
+    const double total = restA + (R.seg ? R.L0 : 0.0);
+    const double du = total / JSX_SPLIT_NSAMP;
+    const float *prof = ws.prof + (size_t)r * JSX_SPLIT_NSAMP;
+    /* pieces of interval 0 in proportion to its stiffness (>= 1 each when seg == 1) */
+    int nA = np;
+    if (R.seg) {
+        float fa = R.w_total > 0.f ? R.w_split / R.w_total : 0.5f;
+        nA = (int)rintf(fa * (float)np);
+        nA = nA < 1 ? 1 : (nA > np - 1 ? np - 1 : nA);
+    }
+    for (int k = threadIdx.x; k < np; k += blockDim.x) {
+        Piece pc;
+        pc.ray = (int)r;
+        double ua, ub;
+        if (k < nA) {
+            float wa0 = 0.f, wa1 = R.seg ? R.w_split : R.w_total;
+            float wl = wa0 + (wa1 - wa0) * (float)k / (float)nA;
+            float wh = wa0 + (wa1 - wa0) * (float)(k + 1) / (float)nA;
+            ua = k == 0 ? 0.0 : split_u_of_w(prof, du, wl);
+            ub = k + 1 == nA ? restA : split_u_of_w(prof, du, wh);
+            if (ua > restA) ua = restA;
+            if (ub > restA) ub = restA;
+            pc.interval = 0;
+            pc.ta = R.t + ua;
+            pc.tb = k + 1 == nA ? R.L : R.t + ub;
+        } else {
+            int kb = k - nA, nB = np - nA;
+            float wb0 = R.w_split, wb1 = R.w_total;
+            float wl = wb0 + (wb1 - wb0) * (float)kb / (float)nB;
+            float wh = wb0 + (wb1 - wb0) * (float)(kb + 1) / (float)nB;
+            ua = kb == 0 ? restA : split_u_of_w(prof, du, wl);
+            ub = kb + 1 == nB ? total : split_u_of_w(prof, du, wh);
+            if (ua < restA) ua = restA;
+            if (ub < restA) ub = restA;
+            pc.interval = 1;
+            pc.ta = kb == 0 ? 0.0 : ua - restA;
+            pc.tb = kb + 1 == nB ? R.L0 : ub - restA;
+        }
+        if (pc.tb < pc.ta) pc.tb = pc.ta;
+        /* first piece keeps the parked step size; the others start from a
+           fraction of their length and adapt */
+        pc.dt0 = (k == 0) ? R.dt : (pc.tb - pc.ta) / 100.0;
+        if (!(pc.dt0 > 0.0)) pc.dt0 = (pc.tb - pc.ta) > 0.0 ? (pc.tb - pc.ta) : 1e-30;
+        ws.pieces[p0 + k] = pc;
+        ws.pout[p0 + k].flags = 0u;
+    }
+}
+
+/* ---- 5. pieces: groups of 5 lanes share one step sequence ---- */
+struct SplitGroupMax {
+    int lead;
+    double floor;
+    __device__ double operator()(const MLane<4> &L, const MConsts &c) const {
+        double e = ml_step_error(L, c, floor);
+        double v = (e == e) ? e : __builtin_inf();
+        double m = v;
+#pragma unroll
+        for (int i = 0; i < JSX_SPLIT_GROUP; ++i) {
+            double o = __shfl(v, lead + i);
+            m = o > m ? o : m;
+        }
+        return m;
+    }
+};
+
+/* Error-scale floor of a piece lane (see ml_step_error).  Relative to the
+ * ray's intensity, an error dPhi in propagator column j reaches the result
+ * multiplied by the Stokes component S_j entering the piece, so column j may
+ * carry |I| / |S_j| times the absolute error of the I column (floor 1).  S is
+ * estimated by the state where the ray was parked; the ratio is clamped to
+ * [1, 1e3] so that a polarization that builds up later along the ray is still
+ * held to 1e-3 of the tolerance.  The particular solution is held relative
+ * to that intensity. */
+__device__ inline double split_scale_floor(const LongRay &R, int role) {
+    const double I = fabs(R.x[0]);
+    if (role == 0) return I;
+    if (role == 1) return 1.0;
+    const double sj = fabs(R.x[role - 1]);
+    double f = sj > 0.0 ? I / sj : 1e3;
+    return f < 1.0 ? 1.0 : (f > 1e3 ? 1e3 : f);
+}
+
+__device__ inline void split_piece_rhs(const MLane<4> &L, const DModel &m, const DImage &im, const MConsts &c,
+                                       float emit, float *r) {
+    double p[3];
+    ml_point_pc(L, im, c, p);
+    MCoef k;
+    m_coefficients<1>(m, c, p, k);
+    const float *x = L.xt;
+    r[0] = emit * k.eI - k.kI * x[0] - k.kQ * x[1] - k.kU * x[2] - k.kV * x[3];
+    r[1] = emit * k.eQ - k.kI * x[1] - k.kQ * x[0] - k.kF * x[2] - k.hQ * x[3];
+    r[2] = emit * k.eU - k.kI * x[2] - k.kU * x[0] + k.kF * x[1] - k.kC * x[3];
+    r[3] = emit * k.eV - k.kI * x[3] - k.kV * x[0] + k.hQ * x[1] + k.kC * x[2];
+}
+
+__global__ __launch_bounds__(256) void jsx_k_split_piece(const DParamsM *__restrict__ reqs, SplitWS ws) {
+    const int lane = __lane_id();
+    const int grp = lane / JSX_SPLIT_GROUP;
+    const int role = lane - grp * JSX_SPLIT_GROUP;
+    const int lead = grp * JSX_SPLIT_GROUP;
+    const bool in_group = lane < (64 / JSX_SPLIT_GROUP) * JSX_SPLIT_GROUP;
+    const unsigned npieces = *ws.n_pieces;
+    unsigned nrays = *ws.n_long;
+    if (nrays > (unsigned)ws.cap) nrays = ws.cap;
+    MLane<4> L;
+    int have = 0, exhausted = in_group ? 0 : 1, slot = 0;
+    int pid = 0;
+    double sfloor = 0.0;   /* error-scale floor of this lane's role (see ml_step_error) */
+    for (;;) {
+        /* group leaders without a piece take the next ones, one atomic per wavefront */
+        int want = in_group && role == 0 && !have && !exhausted;
+        unsigned long long mask = __ballot(want);
+        if (mask) {
+            int cnt = __popcll(mask);
+            int leader = __ffsll((long long)mask) - 1;
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(ws.piece_head, (unsigned)cnt);
+            base = __shfl(base, leader);
+            unsigned long long below = lane ? (mask & ((1ull << lane) - 1ull)) : 0ull;
+            unsigned id = base + (unsigned)__popcll(below);
+            int gwant = __shfl(want, in_group ? lead : lane);
+            unsigned gid = __shfl(id, in_group ? lead : lane);
+            if (gwant) {
+                if (gid >= npieces) {
+                    exhausted = 1;
+                } else if ((unsigned)ws.pieces[gid].ray >= nrays) {
+                    /* never planned (cannot happen with the planner's sizing): skip */
+                    have = 0;
+                } else {
+                    pid = (int)gid;
+                    const Piece pc = ws.pieces[pid];
+                    const LongRay &R = ws.rays[pc.ray];
+                    slot = R.slot;
+                    const DParamsM *q = reqs + slot;
+                    L.pix = pid;
+                    L.phase = PH_STOKES;
+                    L.nseg = 1;
+                    L.seg = 0;
+                    for (int i = 0; i < 3; ++i) {
+                        double o = pc.interval ? R.s0out[i] : R.cout[i];
+                        L.cout[i] = o;
+                        L.cin[i] = o;
+                    }
+                    L.flags = 0; L.n_rhs = 0; L.n_steps = 0;
+                    L.tau_bg = R.tau_bg; L.thick = 0.0;
+                    for (int i = 0; i < 4; ++i) {
+                        L.x[i] = (role == i + 1) ? 1.0f : 0.0f;
+                        L.xlo[i] = 0.f;
+                    }
+                    L.t = pc.ta;
+                    L.L = pc.tb;
+                    L.dt = pc.dt0;
+                    L.first = 1;
+                    sfloor = split_scale_floor(R, role);
+                    ml_stokes_loop(L, q->S, q->c);
+                    have = 1;
+                    if (L.phase == PH_DONE) have = 2;   /* empty piece */
+                }
+            }
+        }
+        if (!__any(have) && __all(exhausted)) break;
+        if (have == 1) {
+            float r[4];
+            const DParamsM *q = reqs + slot;
+            JSX_OPAQUE_V(q);
+            split_piece_rhs(L, q->m, q->im, q->c, role == 0 ? 1.0f : 0.0f, r);
+            q = reqs + slot;
+            JSX_OPAQUE_V(q);
+            SplitGroupMax red;
+            red.lead = lead;
+            red.floor = sfloor;
+            ml_consume(L, q->S, q->im, q->c, r, red);
+            if (L.phase == PH_DONE) have = 2;
+        }
+        if (have == 2) {
+            PieceOut &o = ws.pout[pid];
+            if (role == 0) {
+                for (int i = 0; i < 4; ++i) o.P[i] = ml_xval(L, i);
+                o.n_rhs = L.n_rhs;
+                o.n_steps = L.n_steps;
+            } else {
+                for (int i = 0; i < 4; ++i) o.Phi[4 * (role - 1) + i] = ml_xval(L, i);
+            }
+            if (L.flags) atomicOr(&o.flags, L.flags);
+            have = 0;
+        }
+    }
+}
+
+/* ---- 6. compose pieces and write the pixel ---- */
+__global__ __launch_bounds__(256) void jsx_k_split_combine(const DOut *__restrict__ outs, SplitWS ws) {
+    const unsigned r = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned n = *ws.n_long;
+    if (n > (unsigned)ws.cap) n = ws.cap;
+    if (r >= n) return;
+    const LongRay &R = ws.rays[r];
+    double S[4] = {R.x[0], R.x[1], R.x[2], R.x[3]};
+    uint32_t fl = R.flags, nr = R.n_rhs, nst = R.n_steps;
+    if (R.piece0 + R.npieces > ws.piece_cap) {
+        fl |= JSX_PIX_FAIL_MAXTRIALS;   /* planner overflow: cannot happen with the sizing in use */
+    } else {
+        for (int k = 0; k < R.npieces; ++k) {
+            const PieceOut &o = ws.pout[R.piece0 + k];
+            double T[4];
+            for (int i = 0; i < 4; ++i)
+                T[i] = o.Phi[i] * S[0] + o.Phi[4 + i] * S[1] + o.Phi[8 + i] * S[2] + o.Phi[12 + i] * S[3] + o.P[i];
+            for (int i = 0; i < 4; ++i) S[i] = T[i];
+            fl |= o.flags;
+            nr += o.n_rhs;
+            nst += o.n_steps;
+        }
+    }
+    const DOut &O = outs[R.slot];
+    const int64_t q = R.pix;
+    if ((fl & JSX_PIX_HIT) && S[0] != S[0]) fl |= JSX_PIX_NAN;
+    if (O.tau) O.tau[q] = R.tau_bg;
+    if (O.I) O.I[q] = S[0];
+    if (O.Q) O.Q[q] = S[1];
+    if (O.U) O.U[q] = S[2];
+    if (O.V) O.V[q] = S[3];
+    if (O.l) O.l[q] = R.thick;
+    if (O.flags) O.flags[q] = fl;
+    if (O.n_rhs) O.n_rhs[q] = nr;
+    if (O.n_steps) O.n_steps[q] = nst;
+}
+
+/* zero the split counters (a kernel: graph-safe) */
+__global__ void jsx_k_split_reset(SplitWS ws) {
+    if (threadIdx.x == 0) {
+        *ws.n_long = 0u;
+        *ws.n_pieces = 0u;
+        *ws.piece_head = 0u;
+    }
+}
+
+#endif /* JSX_SPLIT_H */

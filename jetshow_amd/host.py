@@ -20,7 +20,9 @@ ABI_SYMBOLS = [
     "jsx_ctx_create", "jsx_ctx_destroy", "jsx_last_error", "jsx_abi_version",
     "jsx_grid_upload", "jsx_grid_release", "jsx_alloc_outputs", "jsx_free_outputs",
     "jsx_observe", "jsx_observe_host", "jsx_hit_host", "jsx_image_pixel_sizes",
-    "jsx_mas_to_pc", "jsx_observe_batch",
+    "jsx_mas_to_pc", "jsx_observe_batch", "jsx_ctx_set_option", "jsx_ctx_get_option",
+    "jsx_server_create", "jsx_server_submit", "jsx_server_poll", "jsx_server_wait", "jsx_server_stats",
+    "jsx_server_destroy",
 ]
 
 
@@ -67,6 +69,20 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
 
 
+def _host_planes(image):
+    """Host output planes [n_across, n_along] and the jsx_outputs view of them."""
+    shape = (image.n_across, image.n_along)
+    res = {k: np.zeros(shape) for k in ("tau", "I", "Q", "U", "V", "l")}
+    for k in ("flags", "n_rhs", "n_steps"):
+        res[k] = np.zeros(shape, dtype=np.uint32)
+    o = abi.Outputs()
+    for k in ("tau", "I", "Q", "U", "V", "l"):
+        setattr(o, k, res[k].ctypes.data_as(C.POINTER(C.c_double)))
+    for k in ("flags", "n_rhs", "n_steps"):
+        setattr(o, k, res[k].ctypes.data_as(C.POINTER(C.c_uint32)))
+    return res, o
+
+
 class Context:
     """A solver context on one GPU (jsx_ctx)."""
 
@@ -110,17 +126,30 @@ class Context:
 
     def observe(self, jet, image, params):
         """Run the hot path for one image; returns numpy planes [n_across, n_along]."""
-        shape = (image.n_across, image.n_along)
-        res = {k: np.zeros(shape) for k in ("tau", "I", "Q", "U", "V", "l")}
-        for k in ("flags", "n_rhs", "n_steps"):
-            res[k] = np.zeros(shape, dtype=np.uint32)
-        o = abi.Outputs()
-        for k in ("tau", "I", "Q", "U", "V", "l"):
-            setattr(o, k, res[k].ctypes.data_as(C.POINTER(C.c_double)))
-        for k in ("flags", "n_rhs", "n_steps"):
-            setattr(o, k, res[k].ctypes.data_as(C.POINTER(C.c_uint32)))
+        res, o = _host_planes(image)
         self._check(lib().jsx_observe_host(self._h, C.byref(jet), C.byref(image), C.byref(params), C.byref(o)))
         return res
+
+    def set_option(self, name, value):
+        """Launch-shape / scheduling knob of the batched solver (jsx_ctx_set_option)."""
+        self._check(lib().jsx_ctx_set_option(self._h, name.encode(), C.c_double(value)))
+
+    def get_option(self, name):
+        v = C.c_double()
+        self._check(lib().jsx_ctx_get_option(self._h, name.encode(), C.byref(v)))
+        return v.value
+
+    def observe_batch(self, items):
+        """Many images through one persistent launch per (precision, stokes)
+        group: items = [(jet, image, params)]; returns a list of plane dicts."""
+        reqs = (abi.Request * len(items))()
+        out = []
+        for i, (jet, image, params) in enumerate(items):
+            planes, o = _host_planes(image)
+            reqs[i].jet, reqs[i].image, reqs[i].params, reqs[i].out_host = jet, image, params, o
+            out.append(planes)
+        self._check(lib().jsx_observe_batch(self._h, reqs, len(items)))
+        return out
 
     def hit(self, jet, rays, params=None):
         """Geometry::hit for rays [n, 6]; returns (n_seg [n], segments [n, 2, 6])."""
@@ -131,3 +160,68 @@ class Context:
         self._check(lib().jsx_hit_host(self._h, C.byref(jet), C.byref(params) if params is not None else None,
                                        _ptr(r), C.c_int64(n), _ptr(ns), _ptr(seg)))
         return ns, seg
+
+
+class Server:
+    """Continuous-batching request server on a context (jsx_server_*).
+
+    submit() queues an image and returns its id; its planes are filled in
+    host memory when wait(id) returns.  Grids may travel with the request
+    (uploaded on admission, released when done)."""
+
+    def __init__(self, ctx, precision=abi.PREC_MIXED, stokes=abi.STOKES_FULL, max_active=64, hbm_budget=0,
+                 epoch_budget=0):
+        self.ctx = ctx
+        self._h = C.c_void_p()
+        self._keep = {}
+        ctx._check(lib().jsx_server_create(ctx.handle, precision, stokes, max_active, C.c_size_t(hbm_budget),
+                                           epoch_budget, C.byref(self._h)))
+
+    def submit(self, jet, image, params, grids=()):
+        """grids: [(field, GridDesc, values)] with field in abi.FIELD_*."""
+        planes, o = _host_planes(image)
+        r = abi.Request()
+        r.jet, r.image, r.params, r.out_host = jet, image, params, o
+        pl = (abi.GridPayload * max(1, len(grids)))()
+        vals = []
+        for i, (field, desc, values) in enumerate(grids):
+            v = np.ascontiguousarray(values, dtype=np.float64)
+            vals.append(v)
+            pl[i].field, pl[i].desc = field, desc
+            pl[i].values = v.ctypes.data_as(C.POINTER(C.c_double))
+        rid = C.c_uint64()
+        self.ctx._check(lib().jsx_server_submit(self._h, C.byref(r), len(grids), pl, C.byref(rid)))
+        self._keep[rid.value] = (planes, vals, pl)
+        return rid.value
+
+    def poll(self, rid):
+        done = C.c_int32()
+        self.ctx._check(lib().jsx_server_poll(self._h, C.c_uint64(rid), C.byref(done), None))
+        return bool(done.value)
+
+    def wait(self, rid):
+        """Blocks until the request is done; returns (planes, timing dict)."""
+        t = abi.RequestTiming()
+        rc = lib().jsx_server_wait(self._h, C.c_uint64(rid), C.byref(t))
+        planes, _, _ = self._keep.pop(rid)
+        if rc:
+            raise JsxError(rc, "request %d failed: %s" % (rid, lib().jsx_last_error(self.ctx.handle).decode()))
+        return planes, {"t_submit": t.t_submit, "t_admit": t.t_admit, "t_done": t.t_done,
+                        "latency_s": t.t_done - t.t_submit, "n_pixels": t.n_pixels}
+
+    def stats(self):
+        ep = C.c_int64()
+        used = C.c_size_t()
+        lib().jsx_server_stats(self._h, C.byref(ep), C.byref(used))
+        return {"epochs": ep.value, "hbm_used": used.value}
+
+    def close(self):
+        if self._h:
+            lib().jsx_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

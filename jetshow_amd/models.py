@@ -115,7 +115,11 @@ def paraboloid_helical_model(n_side=512, pixel_pc=0.01, los=0.05, redshift=0.1, 
     im.lg_pixel_size_start_cm = im.lg_pixel_size_stop_cm = math.log10(pixel_pc * PC)
     im.los_angle = los
     p = abi.default_params(nu_ghz * 1e9 * (1.0 + redshift), stokes=stokes, precision=precision)
-    p.stokes_abs_tol, p.stokes_rel_tol = 1e-10, 1e-8
+    # Intensities in this image span ~1e-10 .. 1e-4 (cgs), so an absolute
+    # tolerance of 1e-10 would dominate the error norm and leave faint pixels
+    # with ~3e-4 relative error; relative control (abs 1e-20) converges the
+    # FP64 image to ~1e-8.
+    p.stokes_abs_tol, p.stokes_rel_tol = 1e-20, 1e-8
     return j, im, p
 
 

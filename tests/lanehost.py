@@ -1,0 +1,92 @@
+"""ctypes wrapper of tests/hostbuild/liblanehost.so -- the kernels' lane code
+compiled for the CPU (test infrastructure; see tests/hostbuild/lane_host.cpp).
+Built on first use with g++ (a few seconds)."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from jetshow_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SRC = os.path.join(HERE, "hostbuild", "lane_host.cpp")
+LIB = os.path.join(HERE, "hostbuild", "liblanehost.so")
+DEPS = [SRC, os.path.join(HERE, "hostbuild", "hip", "hip_runtime.h"), os.path.join(ROOT, "include", "jetshow_amd.h")]
+_lib = None
+
+
+def _sources():
+    csrc = os.path.join(ROOT, "jetshow_amd", "csrc")
+    return DEPS + [os.path.join(csrc, f) for f in os.listdir(csrc) if f.endswith((".h", ".hip"))]
+
+
+def build(force=False):
+    if not force and os.path.exists(LIB):
+        t = os.path.getmtime(LIB)
+        if all(os.path.getmtime(s) <= t for s in _sources()):
+            return LIB
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math", "-w",
+           "-I", os.path.join(HERE, "hostbuild"), "-I", os.path.join(ROOT, "include"), SRC, "-o", LIB]
+    subprocess.run(cmd, check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = C.CDLL(build())
+        _lib.lh_observe.restype = C.c_int
+        _lib.lh_observe.argtypes = [C.POINTER(abi.JetDesc), C.POINTER(abi.ImageDesc), C.POINTER(abi.SolverParams),
+                                    C.POINTER(C.c_int64), C.c_int64, C.c_int32, C.POINTER(abi.GridDesc),
+                                    C.POINTER(C.POINTER(C.c_double)), C.c_uint, C.c_float,
+                                    C.POINTER(abi.Outputs), C.POINTER(C.c_int32)]
+        _lib.lh_coefficients_mixed.restype = C.c_int
+        _lib.lh_coefficients_mixed.argtypes = [C.POINTER(abi.JetDesc), C.POINTER(abi.ImageDesc),
+                                               C.POINTER(abi.SolverParams), C.POINTER(C.c_double), C.c_int64,
+                                               C.POINTER(C.c_double)]
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def observe(jet, image, params, pixels, grids=None, defer_rhs=0, w_piece=256.0):
+    """Lane-by-lane run of the listed pixels; grids = [(GridDesc, values)] in
+    handle order 1..n.  Returns planes like oracle.observe plus 'n_pieces'."""
+    pix = np.ascontiguousarray(pixels, dtype=np.int64)
+    n = len(pix)
+    res = {k: np.zeros(n) for k in ("tau", "I", "Q", "U", "V", "l")}
+    for k in ("flags", "n_rhs", "n_steps"):
+        res[k] = np.zeros(n, dtype=np.uint32)
+    o = abi.Outputs()
+    for k in ("tau", "I", "Q", "U", "V", "l"):
+        setattr(o, k, _dp(res[k]))
+    for k in ("flags", "n_rhs", "n_steps"):
+        setattr(o, k, res[k].ctypes.data_as(C.POINTER(C.c_uint32)))
+    npcs = np.zeros(n, dtype=np.int32)
+    grids = grids or []
+    gd = (abi.GridDesc * max(1, len(grids)))()
+    vals = [np.ascontiguousarray(v, dtype=np.float64) for _, v in grids]
+    gv = (C.POINTER(C.c_double) * max(1, len(grids)))()
+    for i, (d, _) in enumerate(grids):
+        gd[i] = d
+        gv[i] = _dp(vals[i])
+    rc = lib().lh_observe(C.byref(jet), C.byref(image), C.byref(params), pix.ctypes.data_as(C.POINTER(C.c_int64)),
+                          n, len(grids), gd, gv, defer_rhs, w_piece, C.byref(o),
+                          npcs.ctypes.data_as(C.POINTER(C.c_int32)))
+    if rc:
+        raise RuntimeError("lh_observe failed: %d" % rc)
+    res["n_pieces"] = npcs
+    return res
+
+
+def coefficients_mixed(jet, image, params, pts_cm):
+    pts = np.ascontiguousarray(pts_cm, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros((len(pts), 11))
+    rc = lib().lh_coefficients_mixed(C.byref(jet), C.byref(image), C.byref(params), _dp(pts), len(pts), _dp(out))
+    if rc:
+        raise RuntimeError("lh_coefficients_mixed failed: %d" % rc)
+    return out

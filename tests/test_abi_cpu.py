@@ -32,7 +32,8 @@ def test_exports_every_declared_symbol():
 def test_struct_layout_matches_header():
     L = host.lib()
     import ctypes
-    for i, t in enumerate([abi.JetDesc, abi.ImageDesc, abi.SolverParams, abi.Outputs, abi.GridDesc, abi.Request]):
+    for i, t in enumerate([abi.JetDesc, abi.ImageDesc, abi.SolverParams, abi.Outputs, abi.GridDesc, abi.Request,
+                            abi.GridPayload, abi.RequestTiming]):
         assert L.jsx_sizeof(i) == ctypes.sizeof(t), t
     assert L.jsx_abi_version() == 1
 

@@ -36,9 +36,13 @@ def _check_close(g, o, idx, name, p, rtol=1e-8):
     flip an individual accept/reject decision of the step controller; the
     result then moves by up to the integration tolerance.  Required: identical
     flags; every value within 1e-8 relative or within 10x the local tolerance
-    band of its pass; and at least 99% of the hit pixels within 1e-8 relative."""
+    band of its pass; and at least 99% of the hit pixels within 1e-8 relative.
+    The tau of a pixel whose tau crossed tau_max is the value at the end of the
+    crossing step (Observation.cpp:176-205), which moves by up to one step
+    increment when the step sequence differs: there it must agree to 1e-5."""
     assert (g["flags"][idx] == o["flags"]).all(), name + ": flags differ from libm oracle"
     hit = (o["flags"] & abi.PIX_HIT) != 0
+    crossed = (o["flags"] & abi.PIX_TAU_CROSSED) != 0
     for k in ("tau", "I", "Q", "U", "V"):
         a, b = g[k][idx], o[k]
         scale = np.abs(b)
@@ -48,6 +52,7 @@ def _check_close(g, o, idx, name, p, rtol=1e-8):
         within = diff <= rtol * scale
         if k == "tau":
             band = 10 * (p.tau_abs_tol + p.tau_rel_tol * scale)
+            band = np.where(crossed, np.maximum(band, 1e-5 * scale), band)
         else:
             band = 10 * (p.stokes_abs_tol + p.stokes_rel_tol * scale)
         ok = within | (diff <= band)
@@ -82,8 +87,10 @@ def test_conical_stokes_i(ctx):
 
 
 def test_paraboloid_full_stokes(ctx):
-    j, im, p = models.paraboloid_helical_model()
-    g, o = _run(ctx, "paraboloid_helical", j, im, p, n_sub=1500)
+    # 64 x 64 pixels of 0.08 pc: the apex rays are Faraday-thick (up to ~1e6
+    # right-hand sides per ray) but the image runs in seconds on one lane each
+    j, im, p = models.paraboloid_helical_model(n_side=64, pixel_pc=0.08)
+    g, o = _run(ctx, "paraboloid_helical", j, im, p)
     hit = (o["flags"] & abi.PIX_HIT) != 0
     assert hit.sum() > 100
     assert np.abs(o["Q"][hit]).max() > 0 and np.abs(o["V"][hit]).max() > 0
@@ -91,7 +98,7 @@ def test_paraboloid_full_stokes(ctx):
 
 def test_paraboloid_stokes_i(ctx):
     j, im, p = models.paraboloid_helical_model(stokes=abi.STOKES_I)
-    _run(ctx, "paraboloid_stokes_i", j, im, p, n_sub=1500)
+    _run(ctx, "paraboloid_stokes_i", j, im, p, n_sub=800)
 
 
 def test_reference_main_workload(ctx):

@@ -53,7 +53,7 @@ def test_hit_matches_oracle(ctx, geom):
 
 @pytest.mark.parametrize("order", [0, 1])
 def test_persistent_equals_plain(ctx, order):
-    for name, (j, im, p) in [("paraboloid", models.paraboloid_helical_model(n_side=128, pixel_pc=0.04)),
+    for name, (j, im, p) in [("paraboloid", models.paraboloid_helical_model(n_side=64, pixel_pc=0.08)),
                              ("reference_main", W.reference_main_model())]:
         g0 = ctx.observe(j, im, p)
         q = models.copy_struct(p)
@@ -78,7 +78,7 @@ def _mixed_errors(g64, gm):
 def test_mixed_matches_fp64(ctx, persistent):
     """FP32 coefficients/stages with FP64 roots and step control: I within
     1e-5 relative of FP64 on every hit pixel, Q/U/V within 1e-5 of peak I."""
-    cases = [("paraboloid_full", models.paraboloid_helical_model(n_side=128, pixel_pc=0.04)),
+    cases = [("paraboloid_full", models.paraboloid_helical_model(n_side=64, pixel_pc=0.08)),
              ("conical_I", models.conical_toroidal_model(n_side=64, pixel_pc=0.02)),
              ("reference_main", W.reference_main_model())]
     for name, (j, im, p) in cases:
@@ -108,3 +108,80 @@ def test_pyjetshow_get_i_image(ctx):
     I_jy = o["I"] / (1e-23 * (1.0165 ** 3) / omega)
     assert np.allclose(I.reshape(-1), I_jy, rtol=1e-14, atol=0)
     assert (o["flags"] & abi.PIX_TAU_CROSSED).any()
+
+
+def test_split_long_rays_on_gpu(ctx):
+    """Mixed full-Stokes batch with parking after 4096 right-hand sides: the
+    Faraday-thick apex rays go through the time-parallel pieces; every hit
+    pixel's I stays within 1e-5 of FP64 and Q/U/V within 5e-5 of peak I."""
+    j, im, p = models.paraboloid_helical_model(n_side=64, pixel_pc=0.08)
+    g64 = ctx.observe(j, im, p)
+    pm = models.with_precision(p, abi.PREC_MIXED, rel_tol=1e-6)
+    pm.flags |= OPT_PERSISTENT | OPT_APEX_FIRST
+    old = ctx.get_option("defer_rhs")
+    ctx.set_option("defer_rhs", 4096)
+    try:
+        gm = ctx.observe(j, im, pm)
+        parked, pieces = ctx.get_option("stat_parked"), ctx.get_option("stat_pieces")
+    finally:
+        ctx.set_option("defer_rhs", old)
+    rel, quv, nhit = _mixed_errors(g64, gm)
+    print("split: %d parked rays, %d pieces; max rel I %.3g, QUV/peak %.3g" % (parked, pieces, rel, quv), flush=True)
+    assert parked > 0 and pieces > parked
+    assert rel <= 1e-5
+    assert quv <= 5e-5
+
+
+def test_batch_equals_single_images(ctx):
+    """jsx_observe_batch over three different models (one launch per
+    precision/stokes group) gives each image exactly what the single-image
+    persistent launch gives (FP64, and mixed without splitting)."""
+    cases = [models.conical_toroidal_model(n_side=64, pixel_pc=0.02),
+             models.paraboloid_helical_model(n_side=32, pixel_pc=0.16),
+             W.reference_main_model()]
+    old = ctx.get_option("defer_rhs")
+    ctx.set_option("defer_rhs", 0)
+    try:
+        for prec in (abi.PREC_FP64, abi.PREC_MIXED):
+            items = []
+            for j, im, p in cases:
+                q = models.with_precision(p, prec, rel_tol=1e-6) if prec == abi.PREC_MIXED else models.copy_struct(p)
+                q.stokes = abi.STOKES_FULL
+                q.flags |= OPT_PERSISTENT | OPT_APEX_FIRST
+                items.append((j, im, q))
+            outs = ctx.observe_batch(items)
+            for (j, im, q), b in zip(items, outs):
+                s = ctx.observe(j, im, q)
+                for k in ("tau", "I", "Q", "U", "V", "l", "flags", "n_rhs", "n_steps"):
+                    assert np.array_equal(s[k], b[k]), (prec, k)
+    finally:
+        ctx.set_option("defer_rhs", old)
+
+
+def test_server_same_alone_and_in_mix(ctx):
+    """Continuous batching: a request gives the same planes alone and when
+    submitted among others of different sizes and frequencies (the server's
+    kernel never splits rays, so equality is exact); every request of the mix
+    completes and reports its latency."""
+    from jetshow_amd.host import Server
+    j0, im0, p0 = models.paraboloid_helical_model(n_side=32, pixel_pc=0.16)
+    p0 = models.with_precision(p0, abi.PREC_MIXED, rel_tol=1e-6)
+    others = []
+    for k, (n, px, nu) in enumerate([(48, 0.1, 8.1), (16, 0.3, 15.4), (64, 0.05, 22.0), (24, 0.2, 5.0)]):
+        j, im, p = models.paraboloid_helical_model(n_side=n, pixel_pc=px, nu_ghz=nu)
+        others.append((j, im, models.with_precision(p, abi.PREC_MIXED, rel_tol=1e-6)))
+    with Server(ctx, abi.PREC_MIXED, abi.STOKES_FULL, max_active=3, epoch_budget=2000) as srv:
+        alone, _ = srv.wait(srv.submit(j0, im0, p0))
+        ids = [srv.submit(*others[0]), srv.submit(*others[1])]
+        mid = srv.submit(j0, im0, p0)
+        ids += [srv.submit(*o) for o in others[2:]]
+        mixed, t = srv.wait(mid)
+        lat = [srv.wait(i)[1]["latency_s"] for i in ids]
+        st = srv.stats()
+    print("server: %d epochs, latencies %s s" % (st["epochs"], ["%.3f" % x for x in lat]), flush=True)
+    for k in ("tau", "I", "Q", "U", "V", "l", "flags", "n_rhs", "n_steps"):
+        assert np.array_equal(alone[k], mixed[k]), k
+    ref = ctx.observe(j0, im0, models.copy_struct(p0))
+    rel, quv, _ = _mixed_errors(ref, mixed)
+    assert rel <= 1e-5 and quv <= 5e-5
+    assert st["epochs"] >= 2

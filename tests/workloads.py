@@ -9,7 +9,7 @@ from jetshow_amd import abi, models
 PC = abi.PC
 
 
-def pixel_subset(image, n, seed=0, extra_cols=8):
+def pixel_subset(image, n, seed=0, extra_cols=4):
     """A deterministic pixel subset: n random pixels plus every pixel of the
     first `extra_cols` columns of the middle rows (the apex, where rays are
     longest and tau crosses tau_max)."""
@@ -17,7 +17,7 @@ def pixel_subset(image, n, seed=0, extra_cols=8):
     rng = np.random.default_rng(seed)
     idx = set(rng.choice(na * nl, size=min(n, na * nl), replace=False).tolist())
     mid = na // 2
-    for r in range(max(0, mid - 4), min(na, mid + 4)):
+    for r in range(max(0, mid - 2), min(na, mid + 2)):
         for c in range(min(extra_cols, nl)):
             idx.add(r * nl + c)
     return np.array(sorted(idx), dtype=np.int64)
