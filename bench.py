@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n-side", type=int, default=512)
     ap.add_argument("--rel-tol", type=float, default=1e-6, help="mixed-precision Stokes rel. tolerance")
-    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -69,6 +69,10 @@ def calibrate():
         print(json.dumps({"calibration": name, "max_n_rhs": int(r["n_rhs"].max()),
                           "n_rhs": r["n_rhs"].reshape(-1).tolist()}), flush=True)
     ctx.close()
+
+
+def _log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
 def main():
@@ -114,11 +118,14 @@ def main():
         if rc:
             raise RuntimeError(lib.jsx_last_error(ctx.handle).decode())
 
-    def timed(params, steps, warmup):
+    def timed(params, steps, warmup, tag):
         t, o = planes()
-        for _ in range(warmup):
+        for i in range(warmup):
+            _log("%s warmup %d/%d" % (tag, i + 1, warmup))
             run(params, o)
+            stream.synchronize()
         torch.cuda.synchronize()
+        ctx.set_option("timing", 1)
         evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
         t0 = time.perf_counter()
         with torch.cuda.stream(stream):
@@ -129,19 +136,26 @@ def main():
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         lat = np.array([e0.elapsed_time(e1) for e0, e1 in evs])
+        _log("%s: %d steps, %.1f ms mean" % (tag, steps, lat.mean()))
         return t, lat, wall
 
-    # headline: batched mixed-precision full-Stokes solver
-    tm, lat_m, wall_m = timed(pm, a.steps, a.warmup)
-    ms_step = float(lat_m.sum() / a.steps)
-    value = npix * a.steps / (lat_m.sum() / 1e3)
+    # headline: batched mixed-precision full-Stokes solver (long rays split in time)
+    tm, lat_m, wall_m = timed(pm, a.steps, a.warmup, "mixed batched")
+    kern = {k: ctx.get_option(k) for k in ("stat_batch_ms", "stat_piece_ms", "stat_split_ms", "stat_launches",
+                                           "stat_rhs_batch", "stat_rhs_pieces", "stat_parked", "stat_pieces")}
+    ctx.set_option("timing", 0)
+    ms_step = float(lat_m.mean())
+    value = npix / (ms_step / 1e3)
     res = {k: v.cpu().numpy() for k, v in tm.items()}
     n_rhs_tot = float(res["n_rhs"].astype(np.float64).sum())
 
-    # plain FP64 kernel and accuracy of the mixed result against it
+    # plain FP64 kernel (one thread per pixel) and the accuracy of the mixed
+    # result against it; its time is set by the slowest (Faraday-thick) ray
     plain = {}
     if not a.no_plain:
-        t64, lat64, _ = timed(p64, max(1, a.plain_steps), 0)
+        _log("FP64 plain kernel, %d image(s): the slowest ray takes minutes" % max(1, a.plain_steps))
+        t64, lat64, _ = timed(p64, max(1, a.plain_steps), 0, "fp64 plain")
+        ctx.set_option("timing", 0)
         r64 = {k: v.cpu().numpy() for k, v in t64.items()}
         hit = (r64["flags"] & abi.PIX_HIT) != 0
         assert np.array_equal(hit, (res["flags"] & abi.PIX_HIT) != 0), "hit masks differ between FP64 and mixed"
@@ -156,7 +170,8 @@ def main():
         plain = {
             "fp64_plain_pixels_per_s": npix / (lat64.mean() / 1e3),
             "fp64_plain_ms_per_image": float(lat64.mean()),
-            "speedup_vs_fp64_plain": float(lat64.mean() / (lat_m.mean())),
+            "speedup_vs_fp64_plain": float(lat64.mean() / lat_m.mean()),
+            "fp64_plain_max_n_rhs": int(r64["n_rhs"].max()),
             "max_rel_err_I": float(rel.max()) if rel.size else 0.0,
             "p99_rel_err_I": float(np.percentile(rel, 99)) if rel.size else 0.0,
             "max_err_QUV_over_peak_I": float(qerr),
@@ -165,19 +180,31 @@ def main():
             "hit_mask_identical": True,
         }
 
-    # roofline of the dominant kernel (the mixed persistent kernel): operations
-    # per RHS (per lane) x RHS evaluations / kernel time
+    # roofline of the dominant kernel: algorithmic VALU operations (right-hand
+    # sides evaluated in that kernel x operations per right-hand side) over
+    # its mean launch time, both measured in this run
     ops = OPS_PER_RHS["mixed_full"]
-    f64_ops, f32_ops = ops["f64"] * n_rhs_tot, ops["f32"] * n_rhs_tot
-    kern_s = lat_m.mean() / 1e3
-    achieved = (f64_ops + f32_ops) / kern_s / 1e12
-    peak = (f64_ops + f32_ops) / (f64_ops / PEAK_TFLOPS["f64"] + f32_ops / PEAK_TFLOPS["f32"])
-    roofline = {"bound": "mfma", "pipe": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None,
-                "note": "compute-bound VALU code (no MFMA); peak = FP64/FP32 vector peaks weighted by op mix"}
+    f_rhs = ops["f64"] + ops["f32"]
+    peak = f_rhs / (ops["f64"] / PEAK_TFLOPS["f64"] + ops["f32"] / PEAK_TFLOPS["f32"])
+    per_kernel = {
+        "jsx_k_batch<PolMixed<4>>": (kern["stat_rhs_batch"], kern["stat_batch_ms"]),
+        "jsx_k_split_piece": (kern["stat_rhs_pieces"], kern["stat_piece_ms"]),
+    }
+    dom = max(per_kernel, key=lambda k: per_kernel[k][1])
+    rhs_k, ms_k = per_kernel[dom]
+    achieved = f_rhs * rhs_k / (ms_k / 1e3) / 1e12 if ms_k > 0 else 0.0
+    roofline = {"bound": "mfma", "pipe": "valu", "kernel": dom, "achieved": achieved, "peak": peak,
+                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                "note": "VALU-bound transcendental code, no MFMA and no grid reads (traffic null): peak = FP64/FP32 "
+                        "vector peaks weighted by the op mix; ops per RHS from rocprofv3 VALU counters"}
+    kernels = {"batch_ms": kern["stat_batch_ms"], "split_total_ms": kern["stat_split_ms"],
+               "piece_ms": kern["stat_piece_ms"], "rhs_batch": kern["stat_rhs_batch"],
+               "rhs_pieces": kern["stat_rhs_pieces"], "rays_split": kern["stat_parked"],
+               "pieces": kern["stat_pieces"], "image_achieved_TFLOPs": f_rhs * n_rhs_tot / (ms_step / 1e3) / 1e12}
 
     cpu = None
     if not a.no_cpu:
+        _log("CPU oracle sample (%.0f s)" % a.cpu_seconds)
         cpu = cpu_baseline(j, im, p64, a.cpu_seconds)
 
     out = {
@@ -192,12 +219,14 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64",
-        "data": "synthetic (analytic jet model, seeded)",
-        "config": {"workload": "paraboloid helical-B full-Stokes IQUV, 512x512 x 1 frequency, FP32+compensated "
-                               "persistent solver vs FP64",
+        "data": "synthetic (analytic jet model, deterministic)",
+        "config": {"workload": "paraboloid helical-B full-Stokes IQUV, %dx%d pixels of 0.01 pc, 15.4 GHz; "
+                               "batched FP32+compensated solver (FP64 roots/step control) vs plain FP64 kernel"
+                               % (a.n_side, a.n_side),
                    "n_pixels": npix, "rel_tol_mixed": max(a.rel_tol, 1e-6), "rel_tol_fp64": p64.stokes_rel_tol},
         "latency_ms": {"p50": float(np.percentile(lat_m, 50)), "p99": float(np.percentile(lat_m, 99))},
         "rhs_evaluations": n_rhs_tot,
+        "kernels": kernels,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -211,26 +240,29 @@ def main():
 
 
 def cpu_baseline(j, im, p64, seconds):
-    """The CPU oracle (C-library math) timed on a strided pixel sample of the
-    same image with all host cores; extrapolated to pixels/s."""
+    """The CPU oracle (C-library math, FP64, OpenMP dynamic schedule over
+    pixels) timed on interleaved pixel sets of the same image with the host's
+    cores: set k holds every 61st pixel from offset 7k (61 is coprime with the
+    row length, so every column -- the Faraday-thick apex columns included --
+    is sampled in proportion).  Pixels/s = pixels done / time."""
     from oracle import oracle
     ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     ncores = max(1, min(ncores, 16))
     npix = im.n_across * im.n_along
-    stride = 64
+    stride = 61
     done, t_used, count = 0, 0.0, 0
-    rng_off = 0
-    while t_used < seconds and rng_off < stride:
-        idx = np.arange(rng_off, npix, stride, dtype=np.int64)
+    off = 0
+    while t_used < seconds and off < stride:
+        idx = np.arange(off, npix, stride, dtype=np.int64)
         t0 = time.perf_counter()
         oracle.observe(j, im, p64, idx, math=oracle.MATH_LIBM, threads=ncores)
         t_used += time.perf_counter() - t0
         count += len(idx)
-        rng_off += 7
+        off += 7
         done += 1
     return {"value": count / t_used, "unit": "pixels/s", "cores": ncores, "kind": "port",
-            "sample": "%d strided pixel sets (every %dth pixel, offsets 0,7,...) = %d pixels of the same image, "
-                      "%.1f s" % (done, stride, count, t_used)}
+            "sample": "%d interleaved pixel sets (every %dth pixel from offsets 0, 7, ...) = %d pixels of the same "
+                      "image, FP64 oracle with glibc libm, %.1f s" % (done, stride, count, t_used)}
 
 
 if __name__ == "__main__":

@@ -24,7 +24,18 @@
 
 struct BatchQueues {
     unsigned long long head[JSX_NQ * 8];   /* one counter per queue, 64 B apart */
+    unsigned long long rhs;                /* right-hand sides evaluated by the launch */
 };
+
+/* sum of a 64-bit value over the wavefront, returned in every lane */
+__device__ inline unsigned long long d_wave_sum_u64(unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned lo = __shfl_xor((unsigned)(v & 0xffffffffull), off);
+        unsigned hi = __shfl_xor((unsigned)(v >> 32), off);
+        v += ((unsigned long long)hi << 32) | lo;
+    }
+    return v;
+}
 
 /* policy: FP64 machine */
 template <int NS>
@@ -56,6 +67,7 @@ struct PolFP64 {
     __device__ static void write(const LaneT &L, const DOut &o, int64_t pix) { lane_write(L, o, pix); }
     __device__ static bool done(const LaneT &L) { return L.phase == PH_DONE; }
     __device__ static int64_t pix(const LaneT &L) { return L.pix; }
+    __device__ static unsigned rhs(const LaneT &L) { return L.n_rhs; }
     /* FP64 rays are never split: results stay identical to the sequential oracle */
     __device__ static bool park(const LaneT &, int, const SplitWS &, unsigned) { return false; }
 };
@@ -90,6 +102,7 @@ struct PolMixed {
     __device__ static void write(const LaneT &L, const DOut &o, int64_t pix) { ml_write(L, o, pix); }
     __device__ static bool done(const LaneT &L) { return L.phase == PH_DONE; }
     __device__ static int64_t pix(const LaneT &L) { return L.pix; }
+    __device__ static unsigned rhs(const LaneT &L) { return L.n_rhs; }
     /* full-Stokes rays past the budget go to the time-parallel path */
     __device__ static bool park(const LaneT &L, int slot, const SplitWS &ws, unsigned defer_rhs) {
         if constexpr (NS == 4) {
@@ -128,6 +141,7 @@ __global__ __launch_bounds__(256) void jsx_k_batch(const typename P::Params *__r
     using Params = typename P::Params;
     LaneT L;
     int have = 0, exhausted = 0, req = 0;
+    unsigned long long rhs_acc = 0;
     const int lane = __lane_id();
     int myq = steal ? d_xcd_of_block() : 0;
     int qtry = 0;   /* queues tried since the last successful grab */
@@ -161,8 +175,12 @@ __global__ __launch_bounds__(256) void jsx_k_batch(const typename P::Params *__r
                     int64_t local = id - ray_off[req];
                     int64_t pix = d_ray_to_pixel(qp->im, local, order);
                     P::begin(L, *qp, pix);
-                    if (P::done(L)) P::write(L, outs[req], pix);
-                    else have = 1;
+                    if (P::done(L)) {
+                        P::write(L, outs[req], pix);
+                        rhs_acc += P::rhs(L);
+                    } else {
+                        have = 1;
+                    }
                 }
             }
         }
@@ -178,18 +196,23 @@ __global__ __launch_bounds__(256) void jsx_k_batch(const typename P::Params *__r
             }
             if (P::done(L)) {
                 P::write(L, outs[req], P::pix(L));
+                rhs_acc += P::rhs(L);
                 have = 0;
             } else if (P::park(L, req, ws, defer_rhs)) {
+                rhs_acc += P::rhs(L);
                 have = 0;
             }
         }
     }
+    unsigned long long tot = d_wave_sum_u64(rhs_acc);
+    if (lane == 0 && tot) atomicAdd(&qs->rhs, tot);
 }
 
 /* zero the queue heads (a kernel, so the sequence can be captured in a graph) */
 __global__ void jsx_k_reset_queues(BatchQueues *qs) {
     int i = threadIdx.x;
     if (i < JSX_NQ * 8) qs->head[i] = 0ull;
+    if (i == 0) qs->rhs = 0ull;
 }
 
 #endif /* JSX_BATCH_H */

@@ -71,6 +71,12 @@ struct jsx_ctx {
     float split_w_min = 256.0f;
     SplitWS split{};
     void *split_mem = nullptr;
+    /* optional event timing of the batched launches (option "timing") */
+    int timing = 0;
+    struct EvSet { hipEvent_t e[5]; };
+    std::vector<EvSet> evs;     /* created on first use, reused */
+    int ev_n = 0;               /* launches recorded since timing was (re)enabled */
+    static const int kEvMax = 64;
 };
 
 static int fail(jsx_ctx *ctx, int code, const char *fmt, ...) {
@@ -138,6 +144,8 @@ extern "C" int jsx_ctx_destroy(jsx_ctx *ctx) {
     for (int i = 0; i < jsx_ctx::kSlots; ++i) (void)hipEventDestroy(ctx->slot_ev[i]);
     (void)hipFree(ctx->d_queues); (void)hipFree(ctx->d_reqs); (void)hipFree(ctx->d_off); (void)hipFree(ctx->d_outs);
     (void)hipFree(ctx->split_mem);
+    for (auto &es : ctx->evs)
+        for (int k = 0; k < 5; ++k) (void)hipEventDestroy(es.e[k]);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return JSX_OK;
@@ -167,6 +175,9 @@ extern "C" int jsx_ctx_set_option(jsx_ctx *ctx, const char *name, double v) {
         if (!(v > 0)) return fail(ctx, JSX_EINVAL, "split_w_min must be positive");
         ctx->split_w_min = (float)v;
         if (ctx->split_mem) ctx->split.w_min = (float)v;
+    } else if (k == "timing") {
+        ctx->timing = v != 0.0;
+        ctx->ev_n = 0;
     } else {
         return fail(ctx, JSX_EINVAL, "unknown option '%s'", name);
     }
@@ -190,6 +201,29 @@ extern "C" int jsx_ctx_get_option(jsx_ctx *ctx, const char *name, double *v) {
         }
         unsigned parked = c[0] < (unsigned)ctx->split.cap ? c[0] : (unsigned)ctx->split.cap;
         *v = k == "stat_parked" ? (double)parked : (double)c[1];
+    } else if (k == "stat_rhs_batch" || k == "stat_rhs_pieces") {
+        unsigned long long r = 0;
+        HIPCHECK(ctx, hipDeviceSynchronize());
+        if (k == "stat_rhs_batch" && ctx->d_queues)
+            HIPCHECK(ctx, hipMemcpy(&r, &ctx->d_queues->rhs, sizeof r, hipMemcpyDeviceToHost));
+        if (k == "stat_rhs_pieces" && ctx->split_mem)
+            HIPCHECK(ctx, hipMemcpy(&r, ctx->split.rhs_pieces, sizeof r, hipMemcpyDeviceToHost));
+        *v = (double)r;
+    } else if (k == "stat_launches") {
+        *v = ctx->ev_n;
+    } else if (k == "stat_batch_ms" || k == "stat_piece_ms" || k == "stat_split_ms") {
+        /* mean over the launches recorded since timing was enabled */
+        int a = k == "stat_batch_ms" ? 0 : (k == "stat_piece_ms" ? 2 : 1);
+        int b = k == "stat_batch_ms" ? 1 : (k == "stat_piece_ms" ? 3 : 4);
+        double sum = 0.0;
+        int n = ctx->ev_n < jsx_ctx::kEvMax ? ctx->ev_n : jsx_ctx::kEvMax;
+        for (int i = 0; i < n; ++i) {
+            float ms = 0.f;
+            HIPCHECK(ctx, hipEventSynchronize(ctx->evs[i].e[b]));
+            HIPCHECK(ctx, hipEventElapsedTime(&ms, ctx->evs[i].e[a], ctx->evs[i].e[b]));
+            sum += ms;
+        }
+        *v = n ? sum / n : 0.0;
     } else {
         return fail(ctx, JSX_EINVAL, "unknown option '%s'", name);
     }
@@ -501,7 +535,7 @@ static int split_workspace(jsx_ctx *ctx) {
     const int cap = 16384, piece_cap = 1 << 17;
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
-    size_t o_cnt = take(4 * sizeof(unsigned));
+    size_t o_cnt = take(4 * sizeof(unsigned) + sizeof(unsigned long long));
     size_t o_rays = take(sizeof(LongRay) * cap);
     size_t o_prof = take(sizeof(float) * (size_t)cap * JSX_SPLIT_NSAMP);
     size_t o_pcs = take(sizeof(Piece) * piece_cap);
@@ -512,6 +546,7 @@ static int split_workspace(jsx_ctx *ctx) {
     w.n_long = (unsigned *)(b + o_cnt);
     w.n_pieces = w.n_long + 1;
     w.piece_head = w.n_long + 2;
+    w.rhs_pieces = (unsigned long long *)(b + o_cnt + 4 * sizeof(unsigned));
     w.rays = (LongRay *)(b + o_rays);
     w.cap = cap;
     w.prof = (float *)(b + o_prof);
@@ -572,12 +607,23 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
     const bool full = stokes == JSX_STOKES_FULL;
     SplitWS ws{};
     const bool split = precision == JSX_PREC_MIXED && full && ctx->defer_rhs > 0;
+    jsx_ctx::EvSet *ev = nullptr;
+    if (ctx->timing && ctx->ev_n < jsx_ctx::kEvMax) {
+        if ((int)ctx->evs.size() <= ctx->ev_n) {
+            jsx_ctx::EvSet es;
+            for (int k = 0; k < 5; ++k) HIPCHECK(ctx, hipEventCreate(&es.e[k]));
+            ctx->evs.push_back(es);
+        }
+        ev = &ctx->evs[ctx->ev_n++];
+    }
+    auto mark = [&](int k) { if (ev) (void)hipEventRecord(ev->e[k], st); };
     if (split) {
         int rc = split_workspace(ctx);
         if (rc) return rc;
         ws = ctx->split;
         hipLaunchKernelGGL(jsx_k_split_reset, dim3(1), dim3(64), 0, st, ws);
     }
+    mark(0);
     if (precision == JSX_PREC_MIXED) {
         const DParamsM *R = (const DParamsM *)ctx->d_reqs;
         if (full)
@@ -586,14 +632,21 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
         else
             hipLaunchKernelGGL(jsx_k_batch<PolMixed<1>>, dim3(blocks), dim3(256), 0, st, R, ctx->d_off, ctx->d_outs, nreq,
                                total, ctx->d_queues, ctx->queue_block, order, ctx->steal, ws, 0u);
+        mark(1);
         if (split) {
             const int cap = ws.cap;
             hipLaunchKernelGGL(jsx_k_split_sample, dim3((cap * 64 + 255) / 256), dim3(256), 0, st, R, ws);
             hipLaunchKernelGGL(jsx_k_split_alloc, dim3(1), dim3(1024), 0, st, ws);
             hipLaunchKernelGGL(jsx_k_split_cut, dim3(cap), dim3(256), 0, st, ws);
+            mark(2);
             hipLaunchKernelGGL(jsx_k_split_piece, dim3(blocks), dim3(256), 0, st, R, ws);
+            mark(3);
             hipLaunchKernelGGL(jsx_k_split_combine, dim3((cap + 255) / 256), dim3(256), 0, st, ctx->d_outs, ws);
+        } else {
+            mark(2);
+            mark(3);
         }
+        mark(4);
     } else {
         const DParams *R = (const DParams *)ctx->d_reqs;
         if (full)
@@ -602,6 +655,10 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
         else
             hipLaunchKernelGGL(jsx_k_batch<PolFP64<1>>, dim3(blocks), dim3(256), 0, st, R, ctx->d_off, ctx->d_outs, nreq,
                                total, ctx->d_queues, ctx->queue_block, order, ctx->steal, ws, 0u);
+        mark(1);
+        mark(2);
+        mark(3);
+        mark(4);
     }
     HIPCHECK(ctx, hipGetLastError());
     return JSX_OK;

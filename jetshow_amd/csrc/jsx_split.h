@@ -71,6 +71,7 @@ struct SplitWS {
     float *prof;                  /* cap x NSAMP cumulative stiffness               */
     unsigned *n_pieces;           /* pieces planned                                 */
     unsigned *piece_head;         /* next piece to integrate                        */
+    unsigned long long *rhs_pieces; /* right-hand sides of the pieces (one lane each) */
     Piece *pieces;
     PieceOut *pout;
     int piece_cap;
@@ -395,6 +396,7 @@ __global__ __launch_bounds__(256) void jsx_k_split_piece(const DParamsM *__restr
     if (nrays > (unsigned)ws.cap) nrays = ws.cap;
     MLane<4> L;
     int have = 0, exhausted = in_group ? 0 : 1, slot = 0;
+    unsigned long long rhs_acc = 0;
     int pid = 0;
     double sfloor = 0.0;   /* error-scale floor of this lane's role (see ml_step_error) */
     for (;;) {
@@ -469,6 +471,7 @@ __global__ __launch_bounds__(256) void jsx_k_split_piece(const DParamsM *__restr
                 for (int i = 0; i < 4; ++i) o.P[i] = ml_xval(L, i);
                 o.n_rhs = L.n_rhs;
                 o.n_steps = L.n_steps;
+                rhs_acc += L.n_rhs;
             } else {
                 for (int i = 0; i < 4; ++i) o.Phi[4 * (role - 1) + i] = ml_xval(L, i);
             }
@@ -476,6 +479,13 @@ __global__ __launch_bounds__(256) void jsx_k_split_piece(const DParamsM *__restr
             have = 0;
         }
     }
+    unsigned long long tot = rhs_acc;
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned lo = __shfl_xor((unsigned)(tot & 0xffffffffull), off);
+        unsigned hi = __shfl_xor((unsigned)(tot >> 32), off);
+        tot += ((unsigned long long)hi << 32) | lo;
+    }
+    if (lane == 0 && tot) atomicAdd(ws.rhs_pieces, tot);
 }
 
 /* ---- 6. compose pieces and write the pixel ---- */
@@ -521,6 +531,7 @@ __global__ void jsx_k_split_reset(SplitWS ws) {
         *ws.n_long = 0u;
         *ws.n_pieces = 0u;
         *ws.piece_head = 0u;
+        *ws.rhs_pieces = 0ull;
     }
 }
 

@@ -48,6 +48,7 @@ static inline int __popcll(unsigned long long v) { return __builtin_popcountll(v
 static inline int __ffsll(long long v) { return __builtin_ffsll(v); }
 template <class T> static inline T __shfl(T v, int, int = 64) { return v; }
 template <class T> static inline T __shfl_up(T v, unsigned, int = 64) { return v; }
+template <class T> static inline T __shfl_xor(T, int, int = 64) { return T(0); }
 static inline void __syncthreads() {}
 template <class T, class U> static inline T atomicAdd(T *p, U v) { T o = *p; *p = o + (T)v; return o; }
 template <class T, class U> static inline T atomicSub(T *p, U v) { T o = *p; *p = o - (T)v; return o; }
@@ -84,6 +85,9 @@ static inline hipError_t hipEventCreateWithFlags(hipEvent_t *e, int) { *e = null
 static inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
 static inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
 static inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+static inline hipError_t hipEventCreate(hipEvent_t *e) { *e = nullptr; return hipSuccess; }
+static inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) { *ms = 0.f; return hipSuccess; }
+static inline hipError_t hipDeviceSynchronize() { return hipSuccess; }
 static inline hipError_t hipGetLastError() { return hipSuccess; }
 static inline const char *hipGetErrorString(hipError_t) { return "host build"; }
 static inline hipError_t hipStreamBeginCapture(hipStream_t, int) { return hipErrorNoDevice; }
