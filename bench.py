@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--json-out", default=None)
+    ap.add_argument("--workload", default="c1", choices=["c1", "c3"],
+                    help="c1: paraboloid full-Stokes image (headline); c3: batched ABC sweep of cone models")
+    ap.add_argument("--models", type=int, default=256, help="c3: models in the sweep (x2 frequencies)")
     ap.add_argument("--calibrate", action="store_true",
                     help="single-wavefront launches (2 rays) of the FP64 and mixed full-Stokes kernels for "
                          "rocprofv3 instruction counters; prints the RHS evaluations per launch")
@@ -84,6 +87,9 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (every ray is independent; "
               "see DESIGN.md for how images and models would shard). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "c3":
+        sweep_bench(a)
+        return
     import ctypes as C
 
     import torch
@@ -237,6 +243,126 @@ def main():
         with open(a.json_out, "w") as f:
             f.write(line + "\n")
     ctx.close()
+
+
+def sweep_bench(a):
+    """Batched ABC-style sweep: `models` cone models x 2 frequencies, full
+    Stokes, mixed precision, all images in one persistent launch per step
+    (device output planes, no host copies in the timed region)."""
+    import ctypes as C
+
+    import torch
+
+    from jetshow_amd import abi, host, models
+
+    lib = host.lib()
+    dev = torch.device("cuda:0")
+    torch.cuda.init()
+    ctx = host.Context(0)
+    reqs_py = models.abc_sweep_models(a.models)
+    n = len(reqs_py)
+    sizes = [im.n_across * im.n_along for _, im, _, _ in reqs_py]
+    total = int(sum(sizes))
+    buf = {k: torch.zeros(total, dtype=torch.float64, device=dev) for k in ("tau", "I", "Q", "U", "V", "l")}
+    for k in ("flags", "n_rhs", "n_steps"):
+        buf[k] = torch.zeros(total, dtype=torch.int32, device=dev)
+    reqs = (abi.Request * n)()
+    off = 0
+    for i, (j, im, p, _) in enumerate(reqs_py):
+        p.flags |= OPT_PERSISTENT | OPT_APEX_FIRST
+        reqs[i].jet, reqs[i].image, reqs[i].params = j, im, p
+        o = abi.Outputs()
+        for k in ("tau", "I", "Q", "U", "V", "l"):
+            setattr(o, k, C.cast(C.c_void_p(buf[k].data_ptr() + 8 * off), C.POINTER(C.c_double)))
+        for k in ("flags", "n_rhs", "n_steps"):
+            setattr(o, k, C.cast(C.c_void_p(buf[k].data_ptr() + 4 * off), C.POINTER(C.c_uint32)))
+        reqs[i].out_host = o
+        off += sizes[i]
+    stream = torch.cuda.Stream()
+    sh = C.c_void_p(stream.cuda_stream)
+
+    def run():
+        rc = lib.jsx_observe_batch_device(ctx.handle, reqs, n, sh)
+        if rc:
+            raise RuntimeError(lib.jsx_last_error(ctx.handle).decode())
+
+    for i in range(a.warmup):
+        _log("sweep warmup %d/%d" % (i + 1, a.warmup))
+        run()
+        stream.synchronize()
+    torch.cuda.synchronize()
+    ctx.set_option("timing", 1)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
+    for e0, e1 in evs:
+        e0.record(stream)
+        run()
+        e1.record(stream)
+    torch.cuda.synchronize()
+    lat = np.array([e0.elapsed_time(e1) for e0, e1 in evs])
+    kern = {k: ctx.get_option(k) for k in ("stat_batch_ms", "stat_piece_ms", "stat_rhs_batch", "stat_rhs_pieces",
+                                           "stat_parked", "stat_pieces")}
+    ctx.set_option("timing", 0)
+    flags = buf["flags"].cpu().numpy()
+    nrhs = buf["n_rhs"].cpu().numpy().astype(np.int64)
+    ms = float(lat.mean())
+    ops = OPS_PER_RHS["mixed_full"]
+    f_rhs = ops["f64"] + ops["f32"]
+    peak = f_rhs / (ops["f64"] / PEAK_TFLOPS["f64"] + ops["f32"] / PEAK_TFLOPS["f32"])
+    ach = f_rhs * kern["stat_rhs_batch"] / (kern["stat_batch_ms"] / 1e3) / 1e12 if kern["stat_batch_ms"] else 0.0
+    cpu = None
+    if not a.no_cpu:
+        cpu = sweep_cpu_baseline(reqs_py, a.cpu_seconds)
+    out = {
+        "metric": "integrated pixels/s at <=1e-5 rel Stokes-I err vs FP64; p50/p99 solve latency",
+        "value": total / (ms / 1e3), "unit": "pixels/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic (cone models drawn from the priors, seed 20261203)",
+        "config": {"workload": "ABC sweep: %d cone models x 2 frequencies (8.1, 15.4 GHz), full Stokes, log pixel "
+                               "grids, sides in {64,128,256,512}, one persistent launch" % a.models,
+                   "n_images": n, "n_pixels": total, "hit_pixels": int(((flags & 1) != 0).sum())},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "rhs_evaluations": float(nrhs.sum()),
+        "kernels": {"batch_ms": kern["stat_batch_ms"], "piece_ms": kern["stat_piece_ms"],
+                    "rhs_batch": kern["stat_rhs_batch"], "rhs_pieces": kern["stat_rhs_pieces"],
+                    "rays_split": kern["stat_parked"], "pieces": kern["stat_pieces"]},
+        "roofline": {"bound": "mfma", "pipe": "valu", "kernel": "jsx_k_batch<PolMixed<4>>", "achieved": ach,
+                     "peak": peak, "unit": "TFLOP/s", "frac": ach / peak, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    line = json.dumps(out)
+    print(line, flush=True)
+    if a.json_out:
+        with open(a.json_out, "w") as f:
+            f.write(line + "\n")
+    ctx.close()
+
+
+def sweep_cpu_baseline(reqs_py, seconds):
+    """CPU oracle (FP64 counterpart of each request) on interleaved pixel
+    subsets of the sweep's first 64 models, ~`seconds` of CPU work."""
+    from jetshow_amd import abi, models
+    from oracle import oracle
+    ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    ncores = max(1, min(ncores, 16))
+    sub = [r for r in reqs_py if r[3] < 64]
+    t_used, count, rounds = 0.0, 0, 0
+    stride = 61
+    off = 0
+    while t_used < seconds and off < stride:
+        for j, im, p, _ in sub:
+            q = models.copy_struct(p)
+            q.precision = abi.PREC_FP64
+            q.stokes_rel_tol = 1e-8
+            idx = np.arange(off, im.n_across * im.n_along, stride, dtype=np.int64)
+            t0 = time.perf_counter()
+            oracle.observe(j, im, q, idx, math=oracle.MATH_LIBM, threads=ncores)
+            t_used += time.perf_counter() - t0
+            count += len(idx)
+        off += 7
+        rounds += 1
+    return {"value": count / t_used, "unit": "pixels/s", "cores": ncores, "kind": "port",
+            "sample": "%d round(s) of every %dth pixel of the 128 images of the first 64 models, FP64 oracle with "
+                      "glibc libm, %.1f s" % (rounds, stride, t_used)}
 
 
 def cpu_baseline(j, im, p64, seconds):

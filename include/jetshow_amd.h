@@ -248,6 +248,12 @@ typedef struct jsx_request {
  * planes are filled on return. */
 int jsx_observe_batch(jsx_ctx *ctx, const jsx_request *reqs, int32_t n_reqs);
 
+/* Same with device output planes: reqs[i].out_host holds DEVICE pointers the
+ * caller owns.  One persistent launch per run of consecutive requests that
+ * share (precision, stokes), enqueued on `stream` (NULL = the context's
+ * stream); returns without synchronising. */
+int jsx_observe_batch_device(jsx_ctx *ctx, const jsx_request *reqs, int32_t n_reqs, void *stream);
+
 /* ---- request server: continuous batching of images from many callers.
  * Replaces JetModelZoom's loop of blocking get_i_image calls
  * (jetmodel.py:92-99, called per ABC proposal from abc_sampler.py / the

@@ -943,6 +943,40 @@ extern "C" int jsx_observe_batch(jsx_ctx *ctx, const jsx_request *reqs, int32_t 
     return JSX_OK;
 }
 
+extern "C" int jsx_observe_batch_device(jsx_ctx *ctx, const jsx_request *reqs, int32_t n_reqs, void *stream) {
+    if (!ctx || (!reqs && n_reqs) || n_reqs < 0) return JSX_EINVAL;
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    int32_t i0 = 0;
+    while (i0 < n_reqs) {
+        int32_t i1 = i0;
+        std::vector<BatchItem> items;
+        while (i1 < n_reqs && reqs[i1].params.precision == reqs[i0].params.precision &&
+               reqs[i1].params.stokes == reqs[i0].params.stokes) {
+            const jsx_request &r = reqs[i1];
+            if (r.image.n_across <= 0 || r.image.n_along <= 0 || (r.image.n_across % 2))
+                return fail(ctx, JSX_EINVAL, "request %d: image needs n_along > 0 and even n_across > 0", i1);
+            if (!r.out_host.I) return fail(ctx, JSX_EINVAL, "request %d: outputs.I must be set", i1);
+            BatchItem it;
+            it.npix = r.image.n_across * r.image.n_along;
+            int rc;
+            if ((rc = lower_model(ctx, r.jet, r.params.flags, it.m))) return rc;
+            if ((rc = lower_solver(ctx, r.params, it.S))) return rc;
+            if ((rc = lower_image(ctx, r.image, it.im, nullptr))) return rc;
+            if (r.params.precision == JSX_PREC_MIXED) lower_mixed(r.jet, r.params, it.m, it.im, it.c);
+            else std::memset(&it.c, 0, sizeof it.c);
+            it.out = lower_out(r.out_host);
+            items.push_back(it);
+            ++i1;
+        }
+        int rc = launch_batch(ctx, st, reqs[i0].params.precision, reqs[i0].params.stokes, items, 1);
+        if (rc) return rc;
+        i0 = i1;
+    }
+    return JSX_OK;
+}
+
 extern "C" size_t jsx_sizeof(int which) {
     switch (which) {
     case 0: return sizeof(jsx_jet_desc);

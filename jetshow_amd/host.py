@@ -20,7 +20,7 @@ ABI_SYMBOLS = [
     "jsx_ctx_create", "jsx_ctx_destroy", "jsx_last_error", "jsx_abi_version",
     "jsx_grid_upload", "jsx_grid_release", "jsx_alloc_outputs", "jsx_free_outputs",
     "jsx_observe", "jsx_observe_host", "jsx_hit_host", "jsx_image_pixel_sizes",
-    "jsx_mas_to_pc", "jsx_observe_batch", "jsx_ctx_set_option", "jsx_ctx_get_option",
+    "jsx_mas_to_pc", "jsx_observe_batch", "jsx_observe_batch_device", "jsx_ctx_set_option", "jsx_ctx_get_option",
     "jsx_server_create", "jsx_server_submit", "jsx_server_poll", "jsx_server_wait", "jsx_server_stats",
     "jsx_server_destroy",
 ]

@@ -136,3 +136,59 @@ def with_precision(params, precision, rel_tol=None, abs_tol=None):
 
 def copy_struct(s):
     return type(s).from_buffer_copy(s)
+
+
+def abc_sweep_models(n_models=4096, seed=20261203, freqs_ghz=(8.1, 15.4), redshift=0.1, sides=(64, 128, 256, 512),
+                     precision=abi.PREC_MIXED, rel_tol=1e-6, mas_to_pc=None):
+    """The batched ABC-style sweep: cone jets with parameters drawn from the
+    priors, each imaged at every frequency in `freqs_ghz` on a JetModelZoom-
+    style log-spaced pixel grid (pixel sizes 10^-3 .. 10^-1 mas along the jet,
+    jetmodel.py:212, converted at `redshift`), full Stokes.
+
+    Priors (numpy default_rng(seed), drawn in this order per model): LOS
+    U[0.02, 0.3] rad; half-angle U[0.005, 0.05] rad; Gamma U[2, 30]; B0
+    log-U[0.01, 1] G; N0 log-U[10, 1e4] cm^-3; s U[2, 3]; image side uniform
+    over `sides` (square images).  Fields: helical B (pitch 45 deg) of
+    magnitude B0 (|p|/pc)^-1, relativistic N0 (|p|/pc)^-2, both in the plasma
+    frame; ConstFlat Gamma along +z; cone clipped to 0 <= z <= 10 pc.
+
+    Returns a list of (JetDesc, ImageDesc, SolverParams, model_index)."""
+    if mas_to_pc is None:
+        from .host import mas_to_pc as _m2p
+        mas_to_pc = _m2p
+    rng = np.random.default_rng(seed)
+    pc_in_mas = mas_to_pc(redshift)
+    out = []
+    for k in range(n_models):
+        los = rng.uniform(0.02, 0.3)
+        alpha = rng.uniform(0.005, 0.05)
+        gamma = rng.uniform(2.0, 30.0)
+        b0 = 10 ** rng.uniform(-2.0, 0.0)
+        n0 = 10 ** rng.uniform(1.0, 4.0)
+        s = rng.uniform(2.0, 3.0)
+        side = int(sides[rng.integers(0, len(sides))])
+        j = _jet()
+        j.geometry = abi.GEOM_CONE
+        j.geo_param = alpha
+        j.geo_big_scale = 100.0 * PC
+        j.z_clip_min, j.z_clip_max = 0.0, 10.0 * PC
+        j.bfield = abi.B_HELICAL_R
+        j.b_in_plasma_frame = 1
+        j.b0, j.b_exp, j.b_pitch = b0, 1.0, math.pi / 4.0
+        j.nfield = abi.N_BK
+        j.n_in_plasma_frame = 1
+        j.n0, j.n_exp = n0, 2.0
+        j.vfield = abi.V_CONST_FLAT
+        j.gamma0 = gamma
+        j.s = s
+        im = abi.ImageDesc()
+        im.n_across = im.n_along = side
+        im.lg_pixel_size_start_cm = math.log10(1e-3 * pc_in_mas * PC)
+        im.lg_pixel_size_stop_cm = math.log10(1e-1 * pc_in_mas * PC)
+        im.los_angle = los
+        for f in freqs_ghz:
+            p = abi.default_params(f * 1e9 * (1.0 + redshift), stokes=abi.STOKES_FULL, precision=precision)
+            p.stokes_abs_tol = 1e-20
+            p.stokes_rel_tol = rel_tol if precision == abi.PREC_MIXED else 1e-8
+            out.append((j, im, p, k))
+    return out
