@@ -187,8 +187,14 @@ int jsx_abi_version(void);
  *   "defer_rhs"       right-hand sides after which a mixed full-Stokes ray is
  *                     split into time-parallel pieces (0 disables)
  *   "split_w_min"     smallest stiffness per piece
- * Read-only: "stat_parked" rays split by the last batched launch (waits for it),
- *            "stat_pieces" pieces it integrated. */
+ *   "split_floor_max" cap of the propagator-column error loosening (1 = off;
+ *                     >1 trades Q/U/V accuracy for fewer piece steps)
+ *   "timing"          1: record HIP events around the batched kernels
+ * Read-only (wait for the device): "stat_parked" rays split by the last
+ * batched launch, "stat_pieces" its pieces, "stat_rhs_batch" / "stat_rhs_pieces"
+ * right-hand sides evaluated by its batch / piece kernel, and, with "timing",
+ * "stat_batch_ms", "stat_piece_ms", "stat_split_ms" (all split kernels) means
+ * over the launches since timing was enabled ("stat_launches"). */
 int jsx_ctx_set_option(jsx_ctx *ctx, const char *name, double value);
 int jsx_ctx_get_option(jsx_ctx *ctx, const char *name, double *value);
 

@@ -69,6 +69,7 @@ struct jsx_ctx {
     /* time-parallel path for long full-Stokes rays (mixed precision) */
     unsigned defer_rhs = 8192;          /* 0 disables parking                  */
     float split_w_min = 256.0f;
+    float split_floor_max = 1.0f;
     SplitWS split{};
     void *split_mem = nullptr;
     /* optional event timing of the batched launches (option "timing") */
@@ -175,6 +176,10 @@ extern "C" int jsx_ctx_set_option(jsx_ctx *ctx, const char *name, double v) {
         if (!(v > 0)) return fail(ctx, JSX_EINVAL, "split_w_min must be positive");
         ctx->split_w_min = (float)v;
         if (ctx->split_mem) ctx->split.w_min = (float)v;
+    } else if (k == "split_floor_max") {
+        if (!(v >= 1.0)) return fail(ctx, JSX_EINVAL, "split_floor_max must be >= 1");
+        ctx->split_floor_max = (float)v;
+        if (ctx->split_mem) ctx->split.floor_max = (float)v;
     } else if (k == "timing") {
         ctx->timing = v != 0.0;
         ctx->ev_n = 0;
@@ -193,6 +198,7 @@ extern "C" int jsx_ctx_get_option(jsx_ctx *ctx, const char *name, double *v) {
     else if (k == "steal") *v = ctx->steal;
     else if (k == "defer_rhs") *v = ctx->defer_rhs;
     else if (k == "split_w_min") *v = ctx->split_w_min;
+    else if (k == "split_floor_max") *v = ctx->split_floor_max;
     else if (k == "stat_parked" || k == "stat_pieces") {
         unsigned c[2] = {0, 0};
         if (ctx->split_mem) {
@@ -557,6 +563,7 @@ static int split_workspace(jsx_ctx *ctx) {
     w.target_pieces = ctx->n_cu * ctx->persist_blocks_per_cu * 4 * (64 / JSX_SPLIT_GROUP) * 2;
     if (w.target_pieces > piece_cap / 2) w.target_pieces = piece_cap / 2;
     w.w_min = ctx->split_w_min;
+    w.floor_max = ctx->split_floor_max;
     return JSX_OK;
 }
 

@@ -77,6 +77,7 @@ struct SplitWS {
     int piece_cap;
     int target_pieces;            /* concurrency the planner aims at                */
     float w_min;                  /* smallest stiffness per piece worth a split     */
+    float floor_max;              /* cap of the column error-scale estimate (1: off) */
 };
 
 /* ---- 1. parking (called by the batch kernel for PolMixed<4>) ---- */
@@ -355,21 +356,23 @@ struct SplitGroupMax {
     }
 };
 
-/* Error-scale floor of a piece lane (see ml_step_error).  Relative to the
- * ray's intensity, an error dPhi in propagator column j reaches the result
- * multiplied by the Stokes component S_j entering the piece, so column j may
- * carry |I| / |S_j| times the absolute error of the I column (floor 1).  S is
- * estimated by the state where the ray was parked; the ratio is clamped to
- * [1, 1e3] so that a polarization that builds up later along the ray is still
- * held to 1e-3 of the tolerance.  The particular solution is held relative
- * to that intensity. */
-__device__ inline double split_scale_floor(const LongRay &R, int role) {
+/* Error-scale floor of a piece lane (see ml_step_error).  A propagator
+ * column starts at a unit vector and multiplies the Stokes vector entering
+ * its piece, so its entries are held to an absolute error (floor 1); the
+ * particular solution is held relative to the ray's intensity where it was
+ * parked.  Column j could carry |I| / |S_j| times more error (it reaches the
+ * result scaled by S_j), but S_j entering a later piece is unknown when the
+ * pieces run: estimated from the parked state, the loosening breaks where
+ * polarization builds up after the park point (Q/U/V errors of 4e-4 of peak
+ * I at a cap of 1e3, against 6e-6 unloosened).  `floor_max` caps the ratio;
+ * 1 (the default) disables the estimate. */
+__device__ inline double split_scale_floor(const LongRay &R, int role, float floor_max) {
     const double I = fabs(R.x[0]);
     if (role == 0) return I;
-    if (role == 1) return 1.0;
+    if (role == 1 || floor_max <= 1.0f) return 1.0;
     const double sj = fabs(R.x[role - 1]);
-    double f = sj > 0.0 ? I / sj : 1e3;
-    return f < 1.0 ? 1.0 : (f > 1e3 ? 1e3 : f);
+    double f = sj > 0.0 ? I / sj : (double)floor_max;
+    return f < 1.0 ? 1.0 : (f > floor_max ? (double)floor_max : f);
 }
 
 __device__ inline void split_piece_rhs(const MLane<4> &L, const DModel &m, const DImage &im, const MConsts &c,
@@ -444,7 +447,7 @@ __global__ __launch_bounds__(256) void jsx_k_split_piece(const DParamsM *__restr
                     L.L = pc.tb;
                     L.dt = pc.dt0;
                     L.first = 1;
-                    sfloor = split_scale_floor(R, role);
+                    sfloor = split_scale_floor(R, role, ws.floor_max);
                     ml_stokes_loop(L, q->S, q->c);
                     have = 1;
                     if (L.phase == PH_DONE) have = 2;   /* empty piece */

@@ -145,7 +145,14 @@ def test_grid_fields(ctx, vector_b):
         for i in range(len(table)):
             if table[i] is None:
                 table[i] = (descs[0], vals[0])
-        _run(ctx, "grid_vector%d" % vector_b, j, im, p, grids=table)
+        # No C-library cross-check here: the piecewise-linear grid fields have a
+        # kink at every cell edge, which the Dormand-Prince error estimate does
+        # not see, so tau and I depend on step placement at the 1e-3 level (at
+        # the reference tolerance the two oracle math modes land 1.7e-3 and
+        # 4.2e-3 from the converged value on one pixel) and a last-ulp change
+        # of hypot moves them by that much.  Bit-exact agreement with the
+        # shared-math oracle is the check.
+        _run(ctx, "grid_vector%d" % vector_b, j, im, p, grids=table, libm=False)
     finally:
         for h in handles:
             ctx.release_grid(h)

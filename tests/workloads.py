@@ -53,10 +53,12 @@ def toroidal_lab_frame_model():
 
 
 def cylinder_model(ref_bug=False):
+    """Cylinder r = 0.3 pc clipped to 0.5 <= z <= 5 pc (the clip keeps the
+    rays away from |p| = 0, where the |p|^-1 / |p|^-2 profiles diverge)."""
     j, im, p = models.conical_toroidal_model(n_side=32, pixel_pc=0.05)
     j.geometry = abi.GEOM_CYLINDER
     j.geo_param = 0.3 * PC
-    j.z_clip_min, j.z_clip_max = 0.0, 5.0 * PC
+    j.z_clip_min, j.z_clip_max = 0.5 * PC, 5.0 * PC
     if ref_bug:
         p.flags |= abi.OPT_CYLINDER_REF_BUG
     return j, im, p
