@@ -28,12 +28,14 @@ sys.path.insert(0, ROOT)
 OPT_PERSISTENT = 0x100
 OPT_APEX_FIRST = 0x200
 
-# FP64 / FP32 VALU operations per right-hand-side evaluation of each kernel,
-# measured with rocprofv3 SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_{F64,F32} counters
-# on single-ray launches (FMA counted as 2); see profiles/.
+# FP64 / FP32 VALU operations per right-hand-side evaluation (per lane, one
+# loop trip of the kernel), measured with rocprofv3 SQ_INSTS_VALU_{ADD,MUL,FMA,
+# TRANS}_{F64,F32} on single-wavefront launches (bench.py --calibrate; FMA = 2);
+# the counter table is in profiles/valu_ops_per_rhs_calibration.md.  The grid /
+# Stokes-I entries are not calibrated yet and fall back to the full-Stokes ones.
 OPS_PER_RHS = {
-    "mixed_full": {"f64": 74.0, "f32": 190.0},
-    "fp64_full": {"f64": 760.0, "f32": 0.0},
+    "mixed_full": {"f64": 67.0, "f32": 272.0},
+    "fp64_full": {"f64": 1290.1, "f32": 0.0},
 }
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 
@@ -50,8 +52,13 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--json-out", default=None)
-    ap.add_argument("--workload", default="c1", choices=["c1", "c3"],
-                    help="c1: paraboloid full-Stokes image (headline); c3: batched ABC sweep of cone models")
+    ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "c4"],
+                    help="c1: paraboloid full-Stokes image (headline); c2: grid-defined fields, 1024^2 at two "
+                         "frequencies; c3: batched ABC sweep of cone models; c4: request server under Poisson "
+                         "arrivals")
+    ap.add_argument("--rate", type=float, default=200.0, help="c4: mean arrivals per second")
+    ap.add_argument("--duration", type=float, default=20.0, help="c4: seconds of arrivals")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"], help="c2: solver precision")
     ap.add_argument("--models", type=int, default=256, help="c3: models in the sweep (x2 frequencies)")
     ap.add_argument("--calibrate", action="store_true",
                     help="single-wavefront launches (2 rays) of the FP64 and mixed full-Stokes kernels for "
@@ -62,13 +69,21 @@ def parse():
 def calibrate():
     """One wavefront per launch: every loop trip of that wavefront is one RHS
     evaluation for each active lane, so counters / max(n_rhs) = VALU operations
-    per RHS evaluation per lane."""
-    from jetshow_amd import abi, host, models
+    per RHS evaluation per lane.  Launches, in order: FP64 full Stokes, mixed
+    full Stokes (paraboloid, analytic fields), FP64 Stokes I, mixed Stokes I
+    (grid-defined fields)."""
+    from jetshow_amd import abi, grf, host, models
     ctx = host.Context(0)
     j, im, p = models.paraboloid_helical_model(n_side=2, pixel_pc=0.3)
     im.n_along = 1
-    for name, q in (("fp64_full", p), ("mixed_full", models.with_precision(p, abi.PREC_MIXED, rel_tol=1e-6))):
-        r = ctx.observe(j, im, q)
+    runs = [("fp64_full", j, im, p), ("mixed_full", j, im, models.with_precision(p, abi.PREC_MIXED, rel_tol=1e-6))]
+    (B, N, G), geo = grf.jet_grids(96, 24)
+    hs = [ctx.upload_grid(v, *geo, nan_value=nv) for v, nv in ((B, 0.0), (N, 0.0), (G, 1.0))]
+    jg, img, pg = models.grid_jet_model(hs, n_side=2, pixel_pc=0.3)
+    img.n_along = 1
+    runs += [("fp64_I_grid", jg, img, pg), ("mixed_I_grid", jg, img, models.with_precision(pg, abi.PREC_MIXED))]
+    for name, jj, ii, q in runs:
+        r = ctx.observe(jj, ii, q)
         print(json.dumps({"calibration": name, "max_n_rhs": int(r["n_rhs"].max()),
                           "n_rhs": r["n_rhs"].reshape(-1).tolist()}), flush=True)
     ctx.close()
@@ -89,6 +104,12 @@ def main():
         sys.exit(2)
     if a.workload == "c3":
         sweep_bench(a)
+        return
+    if a.workload == "c2":
+        grid_bench(a)
+        return
+    if a.workload == "c4":
+        serve_bench(a)
         return
     import ctypes as C
 
@@ -335,6 +356,230 @@ def sweep_bench(a):
         with open(a.json_out, "w") as f:
             f.write(line + "\n")
     ctx.close()
+
+
+def _device_requests(items, dev):
+    """Requests with device output planes: one torch buffer per plane for all
+    images; returns (Request array, buffers, sizes)."""
+    import ctypes as C
+
+    import torch
+
+    from jetshow_amd import abi
+    n = len(items)
+    sizes = [im.n_across * im.n_along for _, im, _ in items]
+    total = int(sum(sizes))
+    buf = {k: torch.zeros(total, dtype=torch.float64, device=dev) for k in ("tau", "I", "Q", "U", "V", "l")}
+    for k in ("flags", "n_rhs", "n_steps"):
+        buf[k] = torch.zeros(total, dtype=torch.int32, device=dev)
+    reqs = (abi.Request * n)()
+    off = 0
+    for i, (j, im, p) in enumerate(items):
+        reqs[i].jet, reqs[i].image, reqs[i].params = j, im, p
+        o = abi.Outputs()
+        for k in ("tau", "I", "Q", "U", "V", "l"):
+            setattr(o, k, C.cast(C.c_void_p(buf[k].data_ptr() + 8 * off), C.POINTER(C.c_double)))
+        for k in ("flags", "n_rhs", "n_steps"):
+            setattr(o, k, C.cast(C.c_void_p(buf[k].data_ptr() + 4 * off), C.POINTER(C.c_uint32)))
+        reqs[i].out_host = o
+        off += sizes[i]
+    return reqs, buf, sizes
+
+
+def grid_bench(a):
+    """Grid-defined fields: B, N, Gamma on 2048 x 512 (z, r) node grids of
+    seeded k^-11/3 random fields around the conical profiles, resident in HBM;
+    Stokes I on 1024^2 pixels at 8.1 and 15.4 GHz, both images in one
+    persistent launch per step."""
+    import ctypes as C
+
+    import torch
+
+    from jetshow_amd import abi, grf, host, models
+
+    lib = host.lib()
+    dev = torch.device("cuda:0")
+    torch.cuda.init()
+    ctx = host.Context(0)
+    (B, N, G), geo = grf.jet_grids(2048, 512)
+    hs = [ctx.upload_grid(v, *geo, nan_value=nv) for v, nv in ((B, 0.0), (N, 0.0), (G, 1.0))]
+    prec = abi.PREC_FP64 if a.precision == "fp64" else abi.PREC_MIXED
+    items = []
+    for nu in (8.1, 15.4):
+        j, im, p = models.grid_jet_model(hs, nu_ghz=nu, precision=prec)
+        p.flags |= OPT_PERSISTENT | OPT_APEX_FIRST
+        items.append((j, im, p))
+    reqs, buf, sizes = _device_requests(items, dev)
+    total = int(sum(sizes))
+    stream = torch.cuda.Stream()
+    sh = C.c_void_p(stream.cuda_stream)
+
+    def run():
+        rc = lib.jsx_observe_batch_device(ctx.handle, reqs, len(items), sh)
+        if rc:
+            raise RuntimeError(lib.jsx_last_error(ctx.handle).decode())
+
+    for i in range(a.warmup):
+        _log("grid warmup %d/%d" % (i + 1, a.warmup))
+        run()
+        stream.synchronize()
+    torch.cuda.synchronize()
+    ctx.set_option("timing", 1)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
+    for e0, e1 in evs:
+        e0.record(stream)
+        run()
+        e1.record(stream)
+    torch.cuda.synchronize()
+    lat = np.array([e0.elapsed_time(e1) for e0, e1 in evs])
+    kern = {k: ctx.get_option(k) for k in ("stat_batch_ms", "stat_rhs_batch")}
+    ctx.set_option("timing", 0)
+    flags = buf["flags"].cpu().numpy()
+    ms = float(lat.mean())
+    key = "fp64_I_grid" if prec == abi.PREC_FP64 else "mixed_I_grid"
+    ops = OPS_PER_RHS.get(key, OPS_PER_RHS["fp64_full" if prec == abi.PREC_FP64 else "mixed_full"])
+    f_rhs = ops["f64"] + ops["f32"]
+    peak = f_rhs / (ops["f64"] / PEAK_TFLOPS["f64"] + (ops["f32"] / PEAK_TFLOPS["f32"] if ops["f32"] else 0.0))
+    ach = f_rhs * kern["stat_rhs_batch"] / (kern["stat_batch_ms"] / 1e3) / 1e12 if kern["stat_batch_ms"] else 0.0
+    grid_gbs = 72.0 * kern["stat_rhs_batch"] / (kern["stat_batch_ms"] / 1e3) / 1e9 if kern["stat_batch_ms"] else 0.0
+    cpu = None
+    if not a.no_cpu:
+        cpu = grid_cpu_baseline(items, (B, N, G), geo, a.cpu_seconds)
+    out = {
+        "metric": "integrated pixels/s at <=1e-5 rel Stokes-I err vs FP64; p50/p99 solve latency",
+        "value": total / (ms / 1e3), "unit": "pixels/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if prec == abi.PREC_FP64 else "f32+f64",
+        "data": "synthetic (seeded k^-11/3 random-field grids, seeds 1, 2, 3)",
+        "config": {"workload": "grid-defined fields (2048x512 node grids in HBM), cylinder r 0.5 pc z<=20 pc, "
+                               "Stokes I, 1024x1024 pixels of 0.002 pc at 8.1 and 15.4 GHz",
+                   "n_pixels": total, "hit_pixels": int(((flags & 1) != 0).sum())},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "roofline": {"bound": "mfma", "pipe": "valu", "kernel": "jsx_k_batch", "achieved": ach, "peak": peak,
+                     "unit": "TFLOP/s", "frac": ach / peak, "traffic": None,
+                     "grid_lookup_GBps_algorithmic": grid_gbs},
+        "cpu_baseline": cpu,
+    }
+    line = json.dumps(out)
+    print(line, flush=True)
+    if a.json_out:
+        with open(a.json_out, "w") as f:
+            f.write(line + "\n")
+    ctx.close()
+
+
+def serve_bench(a):
+    """Request server under Poisson arrivals (seed 4): half the requests are
+    cone models drawn as in the ABC sweep, half grid-defined models with their
+    own B/N/Gamma grids (sizes 128x512, 256x1024 or 512x2048 nodes, uploaded
+    on admission); one or two frequencies each (one request per frequency);
+    full Stokes, mixed precision; HBM admission budget 64 GiB."""
+    import threading
+
+    import torch
+
+    from jetshow_amd import abi, grf, host, models
+
+    torch.cuda.init()
+    ctx = host.Context(0)
+    rng = np.random.default_rng(4)
+    n_arr = rng.poisson(a.rate * a.duration)
+    t_arr = np.sort(rng.uniform(0.0, a.duration, n_arr))
+    cones = models.abc_sweep_models(max(64, n_arr), seed=20261203, freqs_ghz=(8.1,))
+    grid_sets = {}
+    for nz, nr in ((512, 128), (1024, 256), (2048, 512)):
+        (B, N, G), geo = grf.jet_grids(nz, nr)
+        descs = []
+        for nv in (0.0, 0.0, 1.0):
+            d = abi.GridDesc()
+            d.nz, d.nr = nz, nr
+            d.z0, d.dz, d.r0, d.dr = geo
+            d.nan_value = nv
+            descs.append(d)
+        grid_sets[(nz, nr)] = list(zip((abi.FIELD_B, abi.FIELD_N, abi.FIELD_V), descs, (B, N, G)))
+    plan = []
+    for i in range(n_arr):
+        nf = 1 + int(rng.integers(0, 2))
+        freqs = (8.1, 15.4)[:nf]
+        if rng.uniform() < 0.5:
+            j, im, p, _ = cones[i % len(cones)]
+            for f in freqs:
+                q = models.copy_struct(p)
+                q.nu = f * 1e9 * 1.1
+                plan.append((t_arr[i], j, im, q, ()))
+        else:
+            key = list(grid_sets)[int(rng.integers(0, 3))]
+            side = int((64, 128, 256)[int(rng.integers(0, 3))])
+            for f in freqs:
+                j, im, p = models.grid_jet_model((0, 0, 0), n_side=side, pixel_pc=2.0 / side, nu_ghz=f,
+                                                 precision=abi.PREC_MIXED, stokes=abi.STOKES_FULL)
+                p.stokes_rel_tol = 1e-6
+                plan.append((t_arr[i], j, im, p, grid_sets[key]))
+    srv = host.Server(ctx, abi.PREC_MIXED, abi.STOKES_FULL, max_active=64, hbm_budget=64 << 30)
+    ids = []
+    _log("serving %d requests over %.0f s" % (len(plan), a.duration))
+    t0 = time.perf_counter()
+    for t_rel, j, im, p, grids in plan:
+        now = time.perf_counter() - t0
+        if t_rel > now:
+            time.sleep(t_rel - now)
+        ids.append((srv.submit(j, im, p, grids=grids), im.n_across * im.n_along))
+    lats, px = [], 0
+    for rid, n in ids:
+        _, t = srv.wait(rid)
+        lats.append(t["latency_s"])
+        px += n
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lats = np.array(lats) * 1e3
+    out = {
+        "metric": "integrated pixels/s at <=1e-5 rel Stokes-I err vs FP64; p50/p99 solve latency",
+        "value": px / wall, "unit": "pixels/s", "n_gpus": 1, "steps": len(plan), "warmup": 0,
+        "ms_per_step": float(lats.mean()), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic (Poisson arrivals seed 4; cone priors seed 20261203; grid seeds 1-3)",
+        "config": {"workload": "request server: Poisson %.0f req/s for %.0f s, half cone models (sides 64-512), "
+                               "half grid models (grids 128x512..512x2048, sides 64-256), 1-2 frequencies, "
+                               "full Stokes" % (a.rate, a.duration),
+                   "requests": len(plan), "arrivals": int(n_arr), "pixels": int(px), "epochs": st["epochs"]},
+        "latency_ms": {"p50": float(np.percentile(lats, 50)), "p99": float(np.percentile(lats, 99)),
+                       "max": float(lats.max())},
+        "wall_s": wall,
+    }
+    print(json.dumps(out), flush=True)
+    if a.json_out:
+        with open(a.json_out, "w") as f:
+            f.write(json.dumps(out) + "\n")
+    ctx.close()
+
+
+def grid_cpu_baseline(items, grids, geo, seconds):
+    """CPU oracle on interleaved pixel sets of the two grid images."""
+    from jetshow_amd import abi, models
+    from oracle import oracle
+    ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    ncores = max(1, min(ncores, 16))
+    table = []
+    for v, nv in zip(grids, (0.0, 0.0, 1.0)):
+        d = abi.GridDesc()
+        d.nz, d.nr = v.shape
+        d.z0, d.dz, d.r0, d.dr = geo
+        d.nan_value = nv
+        table.append((d, v))
+    t_used, count, off, stride = 0.0, 0, 0, 997
+    while t_used < seconds and off < stride:
+        for j, im, p in items:
+            q = models.copy_struct(p)
+            q.precision = abi.PREC_FP64
+            idx = np.arange(off, im.n_across * im.n_along, stride, dtype=np.int64)
+            t0 = time.perf_counter()
+            oracle.observe(j, im, q, idx, math=oracle.MATH_LIBM, grids=table, threads=ncores)
+            t_used += time.perf_counter() - t0
+            count += len(idx)
+        off += 13
+    return {"value": count / t_used, "unit": "pixels/s", "cores": ncores, "kind": "port",
+            "sample": "every 997th pixel of both images from offsets 0, 13, ...: %d pixels, FP64 oracle with glibc "
+                      "libm, %.1f s" % (count, t_used)}
 
 
 def sweep_cpu_baseline(reqs_py, seconds):

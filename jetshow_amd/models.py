@@ -192,3 +192,35 @@ def abc_sweep_models(n_models=4096, seed=20261203, freqs_ghz=(8.1, 15.4), redshi
             p.stokes_rel_tol = rel_tol if precision == abi.PREC_MIXED else 1e-8
             out.append((j, im, p, k))
     return out
+
+
+def grid_jet_model(handles, n_side=1024, pixel_pc=0.002, los=0.1, redshift=0.1, nu_ghz=15.4,
+                   precision=abi.PREC_FP64, stokes=abi.STOKES_I):
+    """Grid-defined fields (SimulationInterpolater path): a cylinder r = 0.5 pc
+    clipped to 0 <= z <= 20 pc, with tangled |B| (grid), N (grid) and Gamma
+    (grid, along +z) interpolated from the node grids referenced by
+    `handles` = (B, N, Gamma) -- see grf.jet_grids; s = 2.5; LOS 0.1 rad;
+    uniform pixels; tau_max 1e4; Stokes I with relative control (abs 1e-20,
+    rel 1e-8)."""
+    j = _jet()
+    j.geometry = abi.GEOM_CYLINDER
+    j.geo_param = 0.5 * PC
+    j.geo_big_scale = 100.0 * PC
+    j.z_clip_min, j.z_clip_max = 0.0, 20.0 * PC
+    hb, hn, hv = handles[:3]
+    j.bfield = abi.B_GRID_SCALAR
+    j.grid_b = hb
+    j.b_in_plasma_frame = 1
+    j.nfield = abi.N_GRID
+    j.grid_n = hn
+    j.n_in_plasma_frame = 1
+    j.vfield = abi.V_GRID
+    j.grid_v = hv
+    j.s = 2.5
+    im = abi.ImageDesc()
+    im.n_across = im.n_along = n_side
+    im.lg_pixel_size_start_cm = im.lg_pixel_size_stop_cm = math.log10(pixel_pc * PC)
+    im.los_angle = los
+    p = abi.default_params(nu_ghz * 1e9 * (1.0 + redshift), stokes=stokes, precision=precision)
+    p.stokes_abs_tol, p.stokes_rel_tol = 1e-20, 1e-8 if precision == abi.PREC_FP64 else 1e-7
+    return j, im, p
