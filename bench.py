@@ -57,7 +57,7 @@ def parse():
                          "frequencies; c3: batched ABC sweep of cone models; c4: request server under Poisson "
                          "arrivals")
     ap.add_argument("--rate", type=float, default=200.0, help="c4: mean arrivals per second")
-    ap.add_argument("--duration", type=float, default=20.0, help="c4: seconds of arrivals")
+    ap.add_argument("--duration", type=float, default=10.0, help="c4: seconds of arrivals")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"], help="c2: solver precision")
     ap.add_argument("--models", type=int, default=256, help="c3: models in the sweep (x2 frequencies)")
     ap.add_argument("--calibrate", action="store_true",
@@ -474,8 +474,6 @@ def serve_bench(a):
     own B/N/Gamma grids (sizes 128x512, 256x1024 or 512x2048 nodes, uploaded
     on admission); one or two frequencies each (one request per frequency);
     full Stokes, mixed precision; HBM admission budget 64 GiB."""
-    import threading
-
     import torch
 
     from jetshow_amd import abi, grf, host, models
@@ -523,7 +521,7 @@ def serve_bench(a):
         now = time.perf_counter() - t0
         if t_rel > now:
             time.sleep(t_rel - now)
-        ids.append((srv.submit(j, im, p, grids=grids), im.n_across * im.n_along))
+        ids.append((srv.submit(j, im, p, grids=grids, keep=("I", "flags")), im.n_across * im.n_along))
     lats, px = [], 0
     for rid, n in ids:
         _, t = srv.wait(rid)

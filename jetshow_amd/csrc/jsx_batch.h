@@ -57,10 +57,10 @@ struct PolFP64 {
     template <class Q>
     __device__ static void rhs_consume_uniform(LaneT &L, const Q *q0) {
         double r[NS];
-        const Q *q = q0;
+        const JSX_CONST Q *q = (const JSX_CONST Q *)q0;
         JSX_OPAQUE_S(q);
         lane_rhs(L, q->m, q->im, q->S, r);
-        q = q0;
+        q = (const JSX_CONST Q *)q0;
         JSX_OPAQUE_S(q);
         lane_consume(L, q->S, q->im, r);
     }
@@ -92,10 +92,10 @@ struct PolMixed {
     template <class Q>
     __device__ static void rhs_consume_uniform(LaneT &L, const Q *q0) {
         float r[NS];
-        const Q *q = q0;
+        const JSX_CONST Q *q = (const JSX_CONST Q *)q0;
         JSX_OPAQUE_S(q);
         ml_rhs(L, q->m, q->im, q->c, r);
-        q = q0;
+        q = (const JSX_CONST Q *)q0;
         JSX_OPAQUE_S(q);
         ml_consume(L, q->S, q->im, q->c, r);
     }

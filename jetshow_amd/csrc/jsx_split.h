@@ -457,15 +457,29 @@ __global__ __launch_bounds__(256) void jsx_k_split_piece(const DParamsM *__restr
         if (!__any(have) && __all(exhausted)) break;
         if (have == 1) {
             float r[4];
-            const DParamsM *q = reqs + slot;
-            JSX_OPAQUE_V(q);
-            split_piece_rhs(L, q->m, q->im, q->c, role == 0 ? 1.0f : 0.0f, r);
-            q = reqs + slot;
-            JSX_OPAQUE_V(q);
             SplitGroupMax red;
             red.lead = lead;
             red.floor = sfloor;
-            ml_consume(L, q->S, q->im, q->c, r, red);
+            const float emit = role == 0 ? 1.0f : 0.0f;
+            /* pieces are queued ray by ray, so a wavefront's groups nearly
+               always serve one request: then read its constants with scalar
+               loads, else with vector loads */
+            const int s0 = __builtin_amdgcn_readfirstlane(slot);
+            if (__all(slot == s0)) {
+                const JSX_CONST DParamsM *q = (const JSX_CONST DParamsM *)(reqs + s0);
+                JSX_OPAQUE_S(q);
+                split_piece_rhs(L, q->m, q->im, q->c, emit, r);
+                q = (const JSX_CONST DParamsM *)(reqs + s0);
+                JSX_OPAQUE_S(q);
+                ml_consume(L, q->S, q->im, q->c, r, red);
+            } else {
+                const DParamsM *q = reqs + slot;
+                JSX_OPAQUE_V(q);
+                split_piece_rhs(L, q->m, q->im, q->c, emit, r);
+                q = reqs + slot;
+                JSX_OPAQUE_V(q);
+                ml_consume(L, q->S, q->im, q->c, r, red);
+            }
             if (L.phase == PH_DONE) have = 2;
         }
         if (have == 2) {

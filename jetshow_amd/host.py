@@ -69,17 +69,24 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
 
 
-def _host_planes(image):
-    """Host output planes [n_across, n_along] and the jsx_outputs view of them."""
+PLANES = ("tau", "I", "Q", "U", "V", "l", "flags", "n_rhs", "n_steps")
+
+
+def _host_planes(image, keep=PLANES):
+    """Host output planes [n_across, n_along] (those named in `keep`; I always)
+    and the jsx_outputs view of them (NULL for the others)."""
     shape = (image.n_across, image.n_along)
-    res = {k: np.zeros(shape) for k in ("tau", "I", "Q", "U", "V", "l")}
-    for k in ("flags", "n_rhs", "n_steps"):
-        res[k] = np.zeros(shape, dtype=np.uint32)
+    res = {}
     o = abi.Outputs()
-    for k in ("tau", "I", "Q", "U", "V", "l"):
-        setattr(o, k, res[k].ctypes.data_as(C.POINTER(C.c_double)))
-    for k in ("flags", "n_rhs", "n_steps"):
-        setattr(o, k, res[k].ctypes.data_as(C.POINTER(C.c_uint32)))
+    for k in PLANES:
+        if k != "I" and k not in keep:
+            continue
+        if k in ("flags", "n_rhs", "n_steps"):
+            res[k] = np.zeros(shape, dtype=np.uint32)
+            setattr(o, k, res[k].ctypes.data_as(C.POINTER(C.c_uint32)))
+        else:
+            res[k] = np.zeros(shape)
+            setattr(o, k, res[k].ctypes.data_as(C.POINTER(C.c_double)))
     return res, o
 
 
@@ -177,9 +184,10 @@ class Server:
         ctx._check(lib().jsx_server_create(ctx.handle, precision, stokes, max_active, C.c_size_t(hbm_budget),
                                            epoch_budget, C.byref(self._h)))
 
-    def submit(self, jet, image, params, grids=()):
-        """grids: [(field, GridDesc, values)] with field in abi.FIELD_*."""
-        planes, o = _host_planes(image)
+    def submit(self, jet, image, params, grids=(), keep=PLANES):
+        """grids: [(field, GridDesc, values)] with field in abi.FIELD_*;
+        keep: the output planes to return (I always)."""
+        planes, o = _host_planes(image, keep)
         r = abi.Request()
         r.jet, r.image, r.params, r.out_host = jet, image, params, o
         pl = (abi.GridPayload * max(1, len(grids)))()
