@@ -232,6 +232,15 @@ int jsx_observe_host(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_image_desc
 int jsx_hit_host(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_solver_params *params,
                  const double *rays, int64_t n, int32_t *n_seg, double *segments);
 
+/* Observation::run_stripe + getStripe("tau") (src/Observation.cpp:310-347,
+ * src/Image.cpp:31-40): optical depth of the middle row's pixels
+ * n_along/2 .. n_along-1 with the fixed-step RK4 integrate_tau
+ * (src/Observation.cpp:58-124; params->n steps per segment, params->tau_max).
+ * Writes n_along - n_along/2 doubles to tau_host; synchronous.  Used to size
+ * images before the full run (example_simple.py:105-237). */
+int jsx_tau_stripe(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_image_desc *image,
+                   const jsx_solver_params *params, double *tau_host);
+
 /* Pixel sizes [n_across][n_along] in cm, Image::getPixelSizes
  * (src/Image.cpp:56-67, 202-204). Host only, no GPU needed. */
 int jsx_image_pixel_sizes(const jsx_image_desc *image, double *sizes);

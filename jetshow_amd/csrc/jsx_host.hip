@@ -826,6 +826,37 @@ extern "C" int jsx_hit_host(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_sol
     return JSX_OK;
 }
 
+/* ---------------- tau stripe ---------------- */
+extern "C" int jsx_tau_stripe(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_image_desc *image,
+                              const jsx_solver_params *params, double *tau_host) {
+    if (!ctx || !jet || !image || !params || !tau_host) return fail(ctx, JSX_EINVAL, "null argument");
+    if (image->n_across <= 0 || image->n_along <= 0 || (image->n_across % 2))
+        return fail(ctx, JSX_EINVAL, "image needs n_along > 0 and even n_across > 0");
+    if (params->n <= 0) return fail(ctx, JSX_EINVAL, "n must be positive");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    DModel m;
+    DSolver S;
+    DImage D;
+    int rc;
+    if ((rc = lower_model(ctx, *jet, params->flags, m))) return rc;
+    if ((rc = lower_solver(ctx, *params, S))) return rc;
+    if ((rc = lower_image(ctx, *image, D, nullptr))) return rc;
+    const int64_t n_out = image->n_along - image->n_along / 2;
+    DParams *dp = nullptr;
+    if ((rc = stage_params(ctx, m, D, S, ctx->stream, &dp))) return rc;
+    double *d_tau = nullptr;
+    HIPCHECK(ctx, hipMalloc(&d_tau, sizeof(double) * (size_t)n_out));
+    hipLaunchKernelGGL(jsx_k_tau_stripe, dim3((unsigned)((n_out + 255) / 256)), dim3(256), 0, ctx->stream, dp, d_tau,
+                       n_out);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(tau_host, d_tau, sizeof(double) * (size_t)n_out, hipMemcpyDeviceToHost);
+    (void)hipFree(d_tau);
+    if (e != hipSuccess) return fail(ctx, JSX_EHIP, "tau stripe failed: %s", hipGetErrorString(e));
+    return JSX_OK;
+}
+
 /* ---------------- cosmology (host, once per image) ----------------
  * comoving_transfer_distance2 + mas_to_pc (src/utils.cpp:186-208, 373-378):
  * controlled Dormand-Prince (abs = rel = 1.49e-8, dt0 = 1e-9) over the Ctd

@@ -69,6 +69,59 @@ __global__ __launch_bounds__(256) void jsx_k_plain(const DParams *__restrict__ P
     lane_write(L, out, pix);
 }
 
+/* Observation::run_stripe (src/Observation.cpp:310-347, 58-124): optical
+ * depth of the middle row's pixels n_along/2 .. n_along-1 with the fixed-step
+ * integrate_tau -- classic RK4 over a const-step time range of dt = L/n (a step
+ * is taken while (step+1) dt - L <= DBL_EPSILON; times are step * dt), stopping
+ * at the first state with tau >= tau_max, whose value is added; later
+ * segments are dropped.  One thread per stripe pixel. */
+__global__ __launch_bounds__(256) void jsx_k_tau_stripe(const DParams *__restrict__ P, double *tau, int64_t n_out) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n_out) return;
+    const JSX_CONST DParams *c = d_launder(P);
+    const int64_t pix = (c->im.na / 2) * c->im.nl + c->im.nl / 2 + q;
+    double o[3];
+    d_pixel_origin(c->im, pix, o);
+    double dir[3] = {c->im.dir[0], c->im.dir[1], c->im.dir[2]};
+    DSeg segs[2];
+    int clipped = 0;
+    const int ns = d_model_hit(c->m, o, dir, segs, &clipped);
+    const double n_los[3] = {-1. * dir[0], -1. * dir[1], -1. * dir[2]};
+    double bg = 0.0;
+    for (int si = 0; si < ns; ++si) {
+        double pin[3], dv[3];
+        for (int i = 0; i < 3; ++i) {
+            pin[i] = si ? segs[1].pin[i] : segs[0].pin[i];
+            dv[i] = (si ? segs[1].pout[i] : segs[0].pout[i]) - pin[i];
+        }
+        const double length = d_norm3(dv);
+        const double dt = length / c->S.n;
+        double x = 0.0, t = 0.0;
+        long step = 0;
+        int found = x >= c->S.tau_max;
+        while (!found && (double)(step + 1) * dt + 0.0 - length <= JSX_DBL_EPS) {
+            double k[4];
+            const double ts[4] = {t, t + dt * 0.5, t + dt * 0.5, t + dt * 1.0};
+            for (int s = 0; s < 4; ++s) {
+                double p[3];
+                for (int i = 0; i < 3; ++i) p[i] = pin[i] + ts[s] * dir[i];
+                DCoef cf;
+                c = d_launder(P);
+                d_coefficients<0>(c->m, p, n_los, c->S.nu, cf);
+                k[s] = cf.kI;
+            }
+            x = x + dt * (1.0 / 6.0) * k[0] + dt * (1.0 / 3.0) * k[1] + dt * (1.0 / 3.0) * k[2] +
+                dt * (1.0 / 6.0) * k[3];
+            step++;
+            t = 0.0 + (double)step * dt;
+            found = x >= c->S.tau_max;
+        }
+        bg += x;
+        if (found) break;
+    }
+    tau[q] = bg;
+}
+
 /* Geometry::hit for explicit rays */
 __global__ void jsx_k_hit(DModel m, const double *rays, int64_t n, int32_t *n_seg, double *segs) {
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -22,7 +22,7 @@ ABI_SYMBOLS = [
     "jsx_observe", "jsx_observe_host", "jsx_hit_host", "jsx_image_pixel_sizes",
     "jsx_mas_to_pc", "jsx_observe_batch", "jsx_observe_batch_device", "jsx_ctx_set_option", "jsx_ctx_get_option",
     "jsx_server_create", "jsx_server_submit", "jsx_server_poll", "jsx_server_wait", "jsx_server_stats",
-    "jsx_server_destroy",
+    "jsx_server_destroy", "jsx_tau_stripe",
 ]
 
 
@@ -156,6 +156,14 @@ class Context:
             reqs[i].jet, reqs[i].image, reqs[i].params, reqs[i].out_host = jet, image, params, o
             out.append(planes)
         self._check(lib().jsx_observe_batch(self._h, reqs, len(items)))
+        return out
+
+    def tau_stripe(self, jet, image, params):
+        """Observation::run_stripe + getStripe("tau"): optical depth of the
+        middle row's pixels n_along/2 .. n_along-1 (fixed-step RK4, params.n
+        steps per segment)."""
+        out = np.zeros(image.n_along - image.n_along // 2)
+        self._check(lib().jsx_tau_stripe(self._h, C.byref(jet), C.byref(image), C.byref(params), _ptr(out)))
         return out
 
     def hit(self, jet, rays, params=None):

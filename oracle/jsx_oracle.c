@@ -1160,6 +1160,68 @@ int orc_observe(const jsx_jet_desc *jet, const jsx_image_desc *image, const jsx_
     return 0;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Observation::run_stripe + Image::getStripe("tau")                          */
+/* (src/Observation.cpp:310-347, 58-124; src/Image.cpp:31-40): optical depth   */
+/* along the middle row's pixels k = n_along/2 .. n_along-1 with the           */
+/* fixed-step integrate_tau: classic RK4 (Boost runge_kutta4: stage times       */
+/* t + {0, 1/2, 1/2, 1} dt, x + (dt/6) k1 + (dt/3) k2 + (dt/3) k3 + (dt/6) k4)  */
+/* over a const-step time range of dt = length / n (a step is taken while     */
+/* (step + 1) dt - length <= DBL_EPSILON; times are step * dt), stopping at the */
+/* first state with tau >= tau_max, whose value is added; later segments are   */
+/* dropped.  Unhit pixels keep 0.                                             */
+static double stripe_pixel(const orc_model *m, const jsx_solver_params *P, const double *o, const double *d) {
+    orc_seg segs[2];
+    int clipped = 0;
+    int nseg = model_hit(m, o, d, segs, &clipped);
+    double background_tau = 0.;
+    uint32_t n_rhs = 0;
+    double n_los[3] = {-1. * d[0], -1. * d[1], -1. * d[2]};
+    for (int si = 0; si < nseg; ++si) {
+        double dv[3];
+        sub3(segs[si].pout, segs[si].pin, dv);
+        double length = norm3(dv);
+        double dt = length / P->n;
+        orc_sys S = {m, {segs[si].pin[0], segs[si].pin[1], segs[si].pin[2]}, {d[0], d[1], d[2]},
+                     {n_los[0], n_los[1], n_los[2]}, P->nu, 0, &n_rhs, 0};
+        double x = 0.0, t = 0.0;
+        long step = 0;
+        int found = x >= P->tau_max;
+        while (!found && (double)(step + 1) * dt + 0.0 - length <= DBL_EPSILON) {
+            double k1, k2, k3, k4;
+            sys_rhs(&S, &x, &k1, t);
+            sys_rhs(&S, &x, &k2, t + dt * 0.5);
+            sys_rhs(&S, &x, &k3, t + dt * 0.5);
+            sys_rhs(&S, &x, &k4, t + dt * 1.0);
+            x = x + dt * (1.0 / 6.0) * k1 + dt * (1.0 / 3.0) * k2 + dt * (1.0 / 3.0) * k3 + dt * (1.0 / 6.0) * k4;
+            step++;
+            t = 0.0 + (double)step * dt;
+            found = x >= P->tau_max;
+        }
+        background_tau += x;
+        if (found) break;
+    }
+    return background_tau;
+}
+
+int orc_tau_stripe(const jsx_jet_desc *jet, const jsx_image_desc *image, const jsx_solver_params *params,
+                   const orc_grid *grids, int32_t n_grids, int32_t math_mode, double *tau) {
+    orc_plane pl;
+    memset(&pl, 0, sizeof pl);
+    int rc = plane_build(image, &pl);
+    if (rc) { plane_free(&pl); return rc; }
+    orc_model m;
+    model_init(&m, jet, grids, n_grids, math_mode, params->flags);
+    int64_t i = pl.na / 2;
+    for (int64_t k = pl.nl / 2; k < pl.nl; ++k) {
+        double c[3], o[3];
+        plane_pixel(&pl, i, k, c, o);
+        tau[k - pl.nl / 2] = stripe_pixel(&m, params, o, pl.dir);
+    }
+    plane_free(&pl);
+    return 0;
+}
+
 int orc_hit(const jsx_jet_desc *jet, const jsx_solver_params *params, const double *rays, int64_t n,
             int32_t *n_seg, double *segments) {
     orc_model m;

@@ -42,6 +42,9 @@ int orc_observe(const jsx_jet_desc *jet, const jsx_image_desc *image,
                 int32_t math_mode, int32_t n_threads);
 
 /* Geometry::hit for explicit rays (n x 6: origin, direction). */
+/* Observation::run_stripe + getStripe("tau"): n_along - n_along/2 values. */
+int orc_tau_stripe(const jsx_jet_desc *jet, const jsx_image_desc *image, const jsx_solver_params *params,
+                   const orc_grid *grids, int32_t n_grids, int32_t math_mode, double *tau);
 int orc_hit(const jsx_jet_desc *jet, const jsx_solver_params *params,
             const double *rays, int64_t n, int32_t *n_seg, double *segments);
 

@@ -101,6 +101,16 @@ def hit(jet, rays, params=None):
     return ns, seg
 
 
+def tau_stripe(jet, image, params, math=MATH_JSX, grids=None):
+    """Observation::run_stripe + getStripe("tau"): n_along - n_along/2 values."""
+    out = np.zeros(image.n_along - image.n_along // 2)
+    garr, ng, keep = make_grids(grids or [])
+    rc = lib().orc_tau_stripe(C.byref(jet), C.byref(image), C.byref(params), garr, ng, int(math), _p(out))
+    if rc:
+        raise ValueError("orc_tau_stripe failed: %d" % rc)
+    return out
+
+
 def image(img):
     n = img.n_across * img.n_along
     sizes = np.zeros(n)

@@ -1,0 +1,128 @@
+"""Tau stripe (Observation::run_stripe, fixed-step RK4 integrate_tau) and the
+image-parameter search built on it."""
+import math
+
+import numpy as np
+import pytest
+
+from jetshow_amd import abi, models, stripe
+from oracle import oracle
+
+PC = abi.PC
+
+
+def _slab(n_side=16, pixel_pc=0.2, los=0.5):
+    """Cylinder r = 0.3 pc, 1 <= z <= 20 pc, constant N, constant tangled B,
+    Gamma = 1: the coefficients are the same at every point inside."""
+    j, im, p = models.conical_toroidal_model(n_side=n_side, pixel_pc=pixel_pc, los=los)
+    j.geometry = abi.GEOM_CYLINDER
+    j.geo_param = 0.3 * PC
+    j.z_clip_min, j.z_clip_max = 1.0 * PC, 20.0 * PC
+    j.bfield = abi.B_RANDOM_SCALAR
+    j.b0, j.b_exp = 0.5, 0.0
+    j.nfield = abi.N_CONST
+    j.n0 = 300.0
+    j.vfield = abi.V_CONST_FLAT
+    j.gamma0 = 1.0
+    return j, im, p
+
+
+def test_stripe_constant_slab_closed_form():
+    """tau = k_I * L per segment (RK4 is exact for a constant integrand), with
+    Boost's const-step range taking n steps, or n - 1 when n * (L/n)
+    rounds above L by more than DBL_EPSILON."""
+    j, im, p = _slab()
+    tau = oracle.tau_stripe(j, im, p)
+    assert len(tau) == 8
+    sizes, coords, rays = oracle.image(im)
+    i = im.n_across // 2
+    hit_any = False
+    for k in range(im.n_along // 2, im.n_along):
+        ray = rays[i * im.n_along + k]
+        ns, seg = oracle.hit(j, ray[None, :], p)
+        if ns[0] == 0:
+            assert tau[k - im.n_along // 2] == 0.0
+            continue
+        hit_any = True
+        L = np.linalg.norm(seg[0, 0, 3:] - seg[0, 0, :3])
+        mid = 0.5 * (seg[0, 0, 3:] + seg[0, 0, :3])
+        kI = oracle.coefficients(j, p.nu, mid, -ray[3:] / np.linalg.norm(ray[3:]))[0]
+        dt = L / p.n
+        steps = tau[k - im.n_along // 2] / (kI * dt)
+        assert abs(steps - round(steps)) < 1e-9 and round(steps) in (p.n - 1, p.n), steps
+    assert hit_any
+
+
+def test_stripe_truncates_at_tau_max():
+    j, im, p = _slab()
+    full = oracle.tau_stripe(j, im, p)
+    q = models.copy_struct(p)
+    q.tau_max = 0.5 * full.max()
+    cut = oracle.tau_stripe(j, im, q)
+    thick = full > q.tau_max
+    assert thick.any()
+    # stops at the first RK4 state with tau >= tau_max: within one step of it
+    assert (cut[thick] >= q.tau_max).all()
+    assert (cut[thick] <= q.tau_max + full[thick] / (p.n - 1) * 1.0000001).all()
+    assert np.array_equal(cut[~thick], full[~thick])
+
+
+def test_stripe_is_composite_simpson():
+    """For tau' = k_I(t), an RK4 step is Simpson's rule (k2 = k3 at the step
+    midpoint): the stripe must equal sum dt/6 (f(t) + 4 f(t + dt/2) + f(t + dt))
+    over the steps taken, with f sampled through the oracle's coefficient
+    function on the conical power-law jet."""
+    j, im, p = models.conical_toroidal_model(n_side=64, pixel_pc=0.02)
+    tau = oracle.tau_stripe(j, im, p)
+    sizes, coords, rays = oracle.image(im)
+    i = im.n_across // 2
+    checked = 0
+    for k in range(im.n_along // 2, im.n_along, 5):
+        ray = rays[i * im.n_along + k]
+        ns, seg = oracle.hit(j, ray[None, :], p)
+        if ns[0] == 0:
+            continue
+        d = ray[3:] / np.linalg.norm(ray[3:])
+        pin = seg[0, 0, :3]
+        L = np.linalg.norm(seg[0, 0, 3:] - pin)
+        dt = L / p.n
+        f = lambda t: oracle.coefficients(j, p.nu, pin + t * d, -d)[0]
+        steps = p.n if p.n * dt - L <= np.finfo(float).eps else p.n - 1
+        simpson = sum(dt / 6 * (f(s * dt) + 4 * f(s * dt + dt / 2) + f(s * dt + dt)) for s in range(steps))
+        assert abs(tau[k - im.n_along // 2] - simpson) <= 1e-12 * simpson, (k, tau[k - im.n_along // 2], simpson)
+        checked += 1
+    assert checked >= 3
+
+
+def test_analyze_tau_stripe_rule():
+    x = np.linspace(0, 1, 200)
+    # tau = 1 at 40% of the stripe, 1e-4 at 90%: pixels too small (tau=1 too far out)
+    tau = 10 ** (4 * (0.4 - x) / 0.5)
+    d = stripe.analyze_tau_stripe(tau)
+    assert d["increase_pixel_size"] and not d["decrease_pixels_size"]
+    assert not d["increase_number_of_pixels"] and not d["decrease_number_of_pixels"]
+    # tau = 1 within the first pixels: pixels too large
+    d = stripe.analyze_tau_stripe(10 ** (-8 * x))
+    assert d["decrease_pixels_size"]
+    # thin end (1e-4) only reached at the last pixel: more pixels
+    d = stripe.analyze_tau_stripe(10 ** (2 - 6 * x) * 0.99)
+    assert d["increase_number_of_pixels"]
+
+
+@pytest.mark.gpu
+def test_stripe_gpu_matches_oracle(ctx):
+    from tests import workloads as W
+    for j, im, p in (models.conical_toroidal_model(), _slab(), W.reference_main_model()):
+        g = ctx.tau_stripe(j, im, p)
+        o = oracle.tau_stripe(j, im, p)
+        assert np.array_equal(g, o)
+
+
+@pytest.mark.gpu
+def test_find_image_params_gpu_vs_oracle(ctx):
+    j, im, p = models.conical_toroidal_model()
+    a = stripe.find_image_params(ctx, j, p, 64, 0.05 * PC, im.los_angle, max_iter=6)
+    b = stripe.find_image_params(None, j, p, 64, 0.05 * PC, im.los_angle, max_iter=6,
+                                 stripe=lambda jj, ii, pp: oracle.tau_stripe(jj, ii, pp))
+    assert a[:2] == b[:2]
+    assert len(a[2]) >= 1
