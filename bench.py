@@ -31,11 +31,12 @@ OPT_APEX_FIRST = 0x200
 # FP64 / FP32 VALU operations per right-hand-side evaluation (per lane, one
 # loop trip of the kernel), measured with rocprofv3 SQ_INSTS_VALU_{ADD,MUL,FMA,
 # TRANS}_{F64,F32} on single-wavefront launches (bench.py --calibrate; FMA = 2);
-# the counter table is in profiles/valu_ops_per_rhs_calibration.md.  The grid /
-# Stokes-I entries are not calibrated yet and fall back to the full-Stokes ones.
+# the counter table is in profiles/valu_ops_per_rhs_calibration.md.
 OPS_PER_RHS = {
     "mixed_full": {"f64": 67.0, "f32": 272.0},
     "fp64_full": {"f64": 1290.1, "f32": 0.0},
+    "fp64_I_grid": {"f64": 906.3, "f32": 0.0},
+    "mixed_I_grid": {"f64": 475.6, "f32": 41.4},
 }
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 
@@ -46,7 +47,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n-side", type=int, default=512)
-    ap.add_argument("--rel-tol", type=float, default=1e-6, help="mixed-precision Stokes rel. tolerance")
+    ap.add_argument("--rel-tol", type=float, default=1e-7, help="mixed-precision Stokes rel. tolerance")
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -250,7 +251,7 @@ def main():
         "config": {"workload": "paraboloid helical-B full-Stokes IQUV, %dx%d pixels of 0.01 pc, 15.4 GHz; "
                                "batched FP32+compensated solver (FP64 roots/step control) vs plain FP64 kernel"
                                % (a.n_side, a.n_side),
-                   "n_pixels": npix, "rel_tol_mixed": max(a.rel_tol, 1e-6), "rel_tol_fp64": p64.stokes_rel_tol},
+                   "n_pixels": npix, "rel_tol_mixed": max(a.rel_tol, 1e-7), "rel_tol_fp64": p64.stokes_rel_tol},
         "latency_ms": {"p50": float(np.percentile(lat_m, 50)), "p99": float(np.percentile(lat_m, 99))},
         "rhs_evaluations": n_rhs_tot,
         "kernels": kernels,

@@ -25,6 +25,7 @@
 struct BatchQueues {
     unsigned long long head[JSX_NQ * 8];   /* one counter per queue, 64 B apart */
     unsigned long long rhs;                /* right-hand sides evaluated by the launch */
+    unsigned drained;                      /* set once every queue has run dry */
 };
 
 /* sum of a 64-bit value over the wavefront, returned in every lane */
@@ -141,6 +142,11 @@ __global__ __launch_bounds__(256) void jsx_k_batch(const typename P::Params *__r
     using Params = typename P::Params;
     LaneT L;
     int have = 0, exhausted = 0, req = 0;
+    /* Long rays are parked for the time-parallel path only once the id queues
+       have run dry: before that the other lanes have work and a long ray costs
+       nothing extra; after it, it would set the launch's tail.  Wave-uniform. */
+    int drained = 0;
+    unsigned trip = 0;
     unsigned long long rhs_acc = 0;
     const int lane = __lane_id();
     int myq = steal ? d_xcd_of_block() : 0;
@@ -187,6 +193,15 @@ __global__ __launch_bounds__(256) void jsx_k_batch(const typename P::Params *__r
         if (!__any(have) && __all(exhausted || have)) {
             if (__all(exhausted)) break;
         }
+        if (!drained) {
+            if (__any(exhausted)) {
+                drained = 1;
+                if (lane == 0) __hip_atomic_store(&qs->drained, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else if ((++trip & 15u) == 0u) {
+                unsigned v = __hip_atomic_load(&qs->drained, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                drained = __builtin_amdgcn_readfirstlane((int)v);
+            }
+        }
         if (have) {
             int r0 = __builtin_amdgcn_readfirstlane(req);
             if (__all(req == r0)) {
@@ -198,7 +213,7 @@ __global__ __launch_bounds__(256) void jsx_k_batch(const typename P::Params *__r
                 P::write(L, outs[req], P::pix(L));
                 rhs_acc += P::rhs(L);
                 have = 0;
-            } else if (P::park(L, req, ws, defer_rhs)) {
+            } else if (drained && P::park(L, req, ws, defer_rhs)) {
                 rhs_acc += P::rhs(L);
                 have = 0;
             }
@@ -212,7 +227,10 @@ __global__ __launch_bounds__(256) void jsx_k_batch(const typename P::Params *__r
 __global__ void jsx_k_reset_queues(BatchQueues *qs) {
     int i = threadIdx.x;
     if (i < JSX_NQ * 8) qs->head[i] = 0ull;
-    if (i == 0) qs->rhs = 0ull;
+    if (i == 0) {
+        qs->rhs = 0ull;
+        qs->drained = 0u;
+    }
 }
 
 #endif /* JSX_BATCH_H */

@@ -472,7 +472,7 @@ static int stage_params(jsx_ctx *ctx, const DModel &m, const DImage &D, const DS
 #define JSX_MIXED_TAU_FLOOR 1e-7
 #endif
 #ifndef JSX_MIXED_STOKES_FLOOR_FULL
-#define JSX_MIXED_STOKES_FLOOR_FULL 1e-6
+#define JSX_MIXED_STOKES_FLOOR_FULL 1e-7
 #endif
 #ifndef JSX_MIXED_STOKES_FLOOR_I
 #define JSX_MIXED_STOKES_FLOOR_I 1e-7
@@ -512,10 +512,9 @@ static void lower_mixed(const jsx_jet_desc &j, const jsx_solver_params &p, const
     /* FP32 stage values put a floor under the usable relative tolerance: the
        embedded error estimate dt * sum(dc_j k_j) carries the stages' rounding,
        ~1e-7 |k| x |dc| ~ 8e-9 dt |k|, i.e. an error-norm noise of ~8e-9 / rel,
-       harmless (< 0.1) down to rel = 1e-7.  Full Stokes keeps 1e-6: there the
-       per-component norm also weighs the small Q, U, V, whose noise relative
-       to their own scale is larger, and 1e-7 would triple the rejected steps
-       of Faraday-rotating rays for no gain in I */
+       harmless (< 0.1) down to rel = 1e-7.  (Q, U, V are controlled relative
+       to the polarized intensity, see ml_step_error, which keeps their noise
+       in the same range.) */
     const double ft = JSX_MIXED_TAU_FLOOR;
     const double fs = p.stokes == JSX_STOKES_FULL ? JSX_MIXED_STOKES_FLOOR_FULL : JSX_MIXED_STOKES_FLOOR_I;
     c.tau_abs = p.tau_abs_tol > ft ? p.tau_abs_tol : ft;
@@ -538,7 +537,7 @@ struct BatchItem {
 /* one allocation for the split path's tables (kept for the context's life) */
 static int split_workspace(jsx_ctx *ctx) {
     if (ctx->split_mem) return JSX_OK;
-    const int cap = 16384, piece_cap = 1 << 17;
+    const int cap = 65536, piece_cap = 1 << 19;
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
     size_t o_cnt = take(4 * sizeof(unsigned) + sizeof(unsigned long long));

@@ -30,6 +30,18 @@
 
 #include "jsx_kernels.h"
 
+/* Multiply-adds of the FP32 coefficient and stage arithmetic may be fused
+ * (the translation unit is compiled with -ffp-contract=off so that the FP64
+ * path reproduces the oracle bit for bit; the mixed path has no such
+ * constraint, and fused FP32 multiply-adds cut its VALU instruction count).
+ * The pragma is lexical: the shared geometry and the FP64 step control that
+ * these functions call stay uncontracted. */
+#if defined(__clang__)
+#define JSX_MIXED_FMA _Pragma("clang fp contract(fast)")
+#else
+#define JSX_MIXED_FMA
+#endif
+
 /* host-derived FP32 constants of the mixed path */
 struct MConsts {
     float A_k, A_e, A_kv, A_ev, A_F, A_C;  /* per-parsec coefficient prefactors           */
@@ -45,6 +57,7 @@ struct MConsts {
 
 /* x^e for x > 0 (see header) */
 __device__ __forceinline__ float m_powf(float x, double e) {
+    JSX_MIXED_FMA
     int k;
     float m = frexpf(x, &k);                 /* x = m 2^k, m in [0.5, 1) */
     double l2 = (double)k + (double)__builtin_amdgcn_logf(m);
@@ -64,6 +77,7 @@ struct MCoef { float kI, kQ, kU, kV, kF, kC, hQ, eI, eQ, eU, eV; };
 /* p_pc: point in parsec (FP64); p_cm needed only for grid lookups */
 template <int FULL>
 __device__ inline void m_coefficients(const DModel &m, const MConsts &c, const double *p_pc, MCoef &o) {
+    JSX_MIXED_FMA
     float px = (float)p_pc[0], py = (float)p_pc[1], pz = (float)p_pc[2];
     double p_cm[3];
     int need_cm = (m.bfield == JSX_B_GRID_SCALAR || m.bfield == JSX_B_GRID_VECTOR || m.nfield == JSX_N_GRID ||
@@ -289,12 +303,14 @@ __device__ inline double ml_seg_length(const MLane<NS> &L) {
 /* h * sum_j c_j k_j, the 5th-order increment of a step of size h */
 template <int NS>
 __device__ inline float ml_increment(const MLane<NS> &L, int i, double h) {
+    JSX_MIXED_FMA
     return (float)h * ((float)DP_C1 * L.k1[i] + (float)DP_C3 * L.k3[i] + (float)DP_C4 * L.k4[i] +
                        (float)DP_C5 * L.k5[i] + (float)DP_C6 * L.k6[i]);
 }
 
 template <int NS>
 __device__ inline void ml_prepare_stage(MLane<NS> &L, int s) {
+    JSX_MIXED_FMA
     float dt = (float)L.dt;
     double t = L.t, dtd = L.dt;
     switch (s) {
@@ -481,6 +497,7 @@ __device__ inline void ml_point_pc(const MLane<NS> &L, const DImage &im, const M
 
 template <int NS>
 __device__ inline void ml_rhs(const MLane<NS> &L, const DModel &m, const DImage &im, const MConsts &c, float *r) {
+    JSX_MIXED_FMA
     double p[3];
     ml_point_pc(L, im, c, p);
     MCoef k;
@@ -513,12 +530,13 @@ __device__ inline void ml_rhs(const MLane<NS> &L, const DModel &m, const DImage 
  * its error matters relative to the ray's intensity (floor |I| where the ray
  * was split).  Without the floors both would keep resolving decays and
  * build-ups to 1e-6 of their own small size, at a fraction of the step the
- * whole ray takes there. */
+ * whole ray takes there.  `rel_min` raises the relative tolerance (pieces). */
 template <int NS>
-__device__ inline double ml_step_error(const MLane<NS> &L, const MConsts &c, double floor = 0.0) {
+__device__ inline double ml_step_error(const MLane<NS> &L, const MConsts &c, double floor = 0.0, double rel_min = 0.0) {
     double dt = L.dt, err = 0.0;
     int nl = L.phase == PH_TAU ? 1 : NS;
     double ea = L.phase == PH_TAU ? c.tau_abs : c.st_abs, er = L.phase == PH_TAU ? c.tau_rel : c.st_rel;
+    er = er > rel_min ? er : rel_min;
     double xe[NS], sc[NS], pmax = 0.0;
     for (int i = 0; i < nl; ++i) {
         xe[i] = dt * (DP_DC1 * (double)L.k1[i] + DP_DC3 * (double)L.k3[i] + DP_DC4 * (double)L.k4[i] +

@@ -340,11 +340,16 @@ __global__ __launch_bounds__(256) void jsx_k_split_cut(SplitWS ws) {
 }
 
 /* ---- 5. pieces: groups of 5 lanes share one step sequence ---- */
+/* Relative tolerance of the pieces: at least 1e-6.  The pieces' error scales
+ * already hold I to ~2e-7 of FP64 on the rays that get split (Faraday-thick,
+ * stability-bound steps), where a tighter tolerance would only add steps. */
+#define JSX_SPLIT_REL_MIN 1e-6
+
 struct SplitGroupMax {
     int lead;
     double floor;
     __device__ double operator()(const MLane<4> &L, const MConsts &c) const {
-        double e = ml_step_error(L, c, floor);
+        double e = ml_step_error(L, c, floor, JSX_SPLIT_REL_MIN);
         double v = (e == e) ? e : __builtin_inf();
         double m = v;
 #pragma unroll
@@ -377,6 +382,7 @@ __device__ inline double split_scale_floor(const LongRay &R, int role, float flo
 
 __device__ inline void split_piece_rhs(const MLane<4> &L, const DModel &m, const DImage &im, const MConsts &c,
                                        float emit, float *r) {
+    JSX_MIXED_FMA
     double p[3];
     ml_point_pc(L, im, c, p);
     MCoef k;

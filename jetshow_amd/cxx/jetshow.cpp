@@ -228,6 +228,24 @@ void Observation::run(int n, double tau_max, double dt_max, double tau_min, std:
     if (output_type == "I") { P.erase("Q"); P.erase("U"); P.erase("V"); }
 }
 
+void Observation::run_stripe(int n, double tau_max, double tau_min) {
+    (void)tau_min;
+    Context &C = context ? *context : Context::instance();
+    jsx_solver_params p = params;
+    p.n = n;
+    p.tau_max = tau_max;
+    p.nu = nu;
+    jsx_jet_desc jd = jet->descriptor();
+    jsx_image_desc id = imagePlane->descriptor();
+    std::vector<double> stripe((size_t)(id.n_along - id.n_along / 2), 0.0);
+    C.check(jsx_tau_stripe(C.get(), &jd, &id, &p, stripe.data()));
+    /* the stripe lands in the "tau" plane's middle row, where getStripe reads */
+    auto &tau = imagePlane->planes["tau"];
+    tau.assign((size_t)(id.n_across * id.n_along), 0.0);
+    const size_t row = (size_t)(id.n_across / 2);
+    for (size_t k = 0; k < stripe.size(); ++k) tau[row * id.n_along + id.n_along / 2 + k] = stripe[k];
+}
+
 std::vector<std::vector<double>> Observation::getImage(std::string value) { return imagePlane->getImage(value); }
 std::vector<double> Observation::getStripe(std::string value) { return imagePlane->getStripe(value); }
 std::pair<unsigned long int, unsigned long int> Observation::getImageSize() { return imagePlane->image_size; }

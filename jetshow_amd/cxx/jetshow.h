@@ -360,6 +360,11 @@ public:
      * "adaptive"; output_type "I" or "full". */
     void run(int n, double tau_max, double dt_max, double tau_min, std::string integration_type,
              std::string output_type, int n_max, double tau_n_min, double tau_n_max);
+    /* src/Observation.cpp:310-347: optical depth of the middle row's jet-side
+     * pixels with the fixed-step RK4 integrate_tau (n steps per segment);
+     * getStripe("tau") returns it.  tau_min is accepted and unused, as in the
+     * reference. */
+    void run_stripe(int n, double tau_max, double tau_min);
     std::vector<std::vector<double>> getImage(std::string value);
     std::vector<double> getStripe(std::string value);
     std::pair<unsigned long int, unsigned long int> getImageSize();

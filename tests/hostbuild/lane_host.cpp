@@ -106,7 +106,7 @@ void run_piece(const DParamsM &q, const LongRay &R, const Piece &pc, PieceOut &p
             for (int role = 0; role < JSX_SPLIT_GROUP; ++role) {
                 MLane<4> tmp = L[role];
                 for (int i = 0; i < 4; ++i) tmp.k7[i] = r[role][i];
-                double e = ml_step_error(tmp, q.c, split_scale_floor(R, role, 1.0f));
+                double e = ml_step_error(tmp, q.c, split_scale_floor(R, role, 1.0f), JSX_SPLIT_REL_MIN);
                 e = (e == e) ? e : __builtin_inf();
                 emax = e > emax ? e : emax;
             }
