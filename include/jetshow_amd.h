@@ -187,12 +187,17 @@ int jsx_abi_version(void);
  *   "defer_rhs"       right-hand sides after which a mixed full-Stokes ray is
  *                     split into time-parallel pieces (0 disables)
  *   "split_w_min"     smallest stiffness per piece
+ *   "piece_layout"    0: a split piece is integrated by a group of 5 lanes
+ *                     (P and the 4 propagator columns); 1: by one lane holding
+ *                     all 20 components (one coefficient evaluation per stage)
  *   "split_floor_max" cap of the propagator-column error loosening (1 = off;
  *                     >1 trades Q/U/V accuracy for fewer piece steps)
  *   "timing"          1: record HIP events around the batched kernels
  * Read-only (wait for the device): "stat_parked" rays split by the last
  * batched launch, "stat_pieces" its pieces, "stat_rhs_batch" / "stat_rhs_pieces"
- * right-hand sides evaluated by its batch / piece kernel, and, with "timing",
+ * right-hand sides evaluated by its batch / piece kernel, "stat_trips" /
+ * "stat_trips_uniform" the batch kernel's wavefront loop trips (all / with
+ * every lane on one request: scalar constant loads), and, with "timing",
  * "stat_batch_ms", "stat_piece_ms", "stat_split_ms" (all split kernels) means
  * over the launches since timing was enabled ("stat_launches"). */
 int jsx_ctx_set_option(jsx_ctx *ctx, const char *name, double value);

@@ -25,6 +25,8 @@
 struct BatchQueues {
     unsigned long long head[JSX_NQ * 8];   /* one counter per queue, 64 B apart */
     unsigned long long rhs;                /* right-hand sides evaluated by the launch */
+    unsigned long long trips;              /* wavefront loop trips that evaluated RHS */
+    unsigned long long trips_uniform;      /* ... of them with all lanes on one request */
     unsigned drained;                      /* set once every queue has run dry */
 };
 
@@ -147,7 +149,7 @@ __global__ __launch_bounds__(256) void jsx_k_batch(const typename P::Params *__r
        nothing extra; after it, it would set the launch's tail.  Wave-uniform. */
     int drained = 0;
     unsigned trip = 0;
-    unsigned long long rhs_acc = 0;
+    unsigned long long rhs_acc = 0, n_trips = 0, n_uniform = 0;
     const int lane = __lane_id();
     int myq = steal ? d_xcd_of_block() : 0;
     int qtry = 0;   /* queues tried since the last successful grab */
@@ -202,9 +204,11 @@ __global__ __launch_bounds__(256) void jsx_k_batch(const typename P::Params *__r
                 drained = __builtin_amdgcn_readfirstlane((int)v);
             }
         }
+        if (__any(have)) n_trips++;
         if (have) {
             int r0 = __builtin_amdgcn_readfirstlane(req);
             if (__all(req == r0)) {
+                n_uniform++;
                 P::rhs_consume_uniform(L, reqs + r0);
             } else {
                 P::rhs_consume(L, reqs + req);
@@ -220,7 +224,11 @@ __global__ __launch_bounds__(256) void jsx_k_batch(const typename P::Params *__r
         }
     }
     unsigned long long tot = d_wave_sum_u64(rhs_acc);
-    if (lane == 0 && tot) atomicAdd(&qs->rhs, tot);
+    if (lane == 0) {
+        if (tot) atomicAdd(&qs->rhs, tot);
+        if (n_trips) atomicAdd(&qs->trips, n_trips);
+        if (n_uniform) atomicAdd(&qs->trips_uniform, n_uniform);
+    }
 }
 
 /* zero the queue heads (a kernel, so the sequence can be captured in a graph) */
@@ -229,6 +237,8 @@ __global__ void jsx_k_reset_queues(BatchQueues *qs) {
     if (i < JSX_NQ * 8) qs->head[i] = 0ull;
     if (i == 0) {
         qs->rhs = 0ull;
+        qs->trips = 0ull;
+        qs->trips_uniform = 0ull;
         qs->drained = 0u;
     }
 }

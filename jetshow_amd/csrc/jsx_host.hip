@@ -70,6 +70,7 @@ struct jsx_ctx {
     unsigned defer_rhs = 8192;          /* 0 disables parking                  */
     float split_w_min = 256.0f;
     float split_floor_max = 1.0f;
+    int piece_layout = 0;               /* 0: 5-lane groups; 1: one lane, 20 components */
     SplitWS split{};
     void *split_mem = nullptr;
     /* optional event timing of the batched launches (option "timing") */
@@ -152,6 +153,8 @@ extern "C" int jsx_ctx_destroy(jsx_ctx *ctx) {
     return JSX_OK;
 }
 
+static void split_target(jsx_ctx *ctx);
+
 /* ---------------- options ---------------- */
 extern "C" int jsx_ctx_set_option(jsx_ctx *ctx, const char *name, double v) {
     if (!ctx || !name) return JSX_EINVAL;
@@ -160,10 +163,7 @@ extern "C" int jsx_ctx_set_option(jsx_ctx *ctx, const char *name, double v) {
     if (k == "blocks_per_cu") {
         if (!(v >= 1 && v <= 16)) return fail(ctx, JSX_EINVAL, "blocks_per_cu must be in [1, 16]");
         ctx->persist_blocks_per_cu = (int)v;
-        if (ctx->split_mem) {
-            int t = ctx->n_cu * ctx->persist_blocks_per_cu * 4 * (64 / JSX_SPLIT_GROUP) * 2;
-            ctx->split.target_pieces = t < ctx->split.piece_cap / 2 ? t : ctx->split.piece_cap / 2;
-        }
+        if (ctx->split_mem) split_target(ctx);
     } else if (k == "queue_block") {
         if (!(v >= 1 && v <= (double)(1 << 20))) return fail(ctx, JSX_EINVAL, "queue_block must be in [1, 2^20]");
         ctx->queue_block = (int64_t)v;
@@ -180,6 +180,10 @@ extern "C" int jsx_ctx_set_option(jsx_ctx *ctx, const char *name, double v) {
         if (!(v >= 1.0)) return fail(ctx, JSX_EINVAL, "split_floor_max must be >= 1");
         ctx->split_floor_max = (float)v;
         if (ctx->split_mem) ctx->split.floor_max = (float)v;
+    } else if (k == "piece_layout") {
+        if (!(v == 0.0 || v == 1.0)) return fail(ctx, JSX_EINVAL, "piece_layout must be 0 or 1");
+        ctx->piece_layout = (int)v;
+        if (ctx->split_mem) split_target(ctx);
     } else if (k == "timing") {
         ctx->timing = v != 0.0;
         ctx->ev_n = 0;
@@ -199,6 +203,7 @@ extern "C" int jsx_ctx_get_option(jsx_ctx *ctx, const char *name, double *v) {
     else if (k == "defer_rhs") *v = ctx->defer_rhs;
     else if (k == "split_w_min") *v = ctx->split_w_min;
     else if (k == "split_floor_max") *v = ctx->split_floor_max;
+    else if (k == "piece_layout") *v = ctx->piece_layout;
     else if (k == "stat_parked" || k == "stat_pieces") {
         unsigned c[2] = {0, 0};
         if (ctx->split_mem) {
@@ -207,11 +212,15 @@ extern "C" int jsx_ctx_get_option(jsx_ctx *ctx, const char *name, double *v) {
         }
         unsigned parked = c[0] < (unsigned)ctx->split.cap ? c[0] : (unsigned)ctx->split.cap;
         *v = k == "stat_parked" ? (double)parked : (double)c[1];
-    } else if (k == "stat_rhs_batch" || k == "stat_rhs_pieces") {
+    } else if (k == "stat_rhs_batch" || k == "stat_rhs_pieces" || k == "stat_trips" || k == "stat_trips_uniform") {
         unsigned long long r = 0;
         HIPCHECK(ctx, hipDeviceSynchronize());
         if (k == "stat_rhs_batch" && ctx->d_queues)
             HIPCHECK(ctx, hipMemcpy(&r, &ctx->d_queues->rhs, sizeof r, hipMemcpyDeviceToHost));
+        if (k == "stat_trips" && ctx->d_queues)
+            HIPCHECK(ctx, hipMemcpy(&r, &ctx->d_queues->trips, sizeof r, hipMemcpyDeviceToHost));
+        if (k == "stat_trips_uniform" && ctx->d_queues)
+            HIPCHECK(ctx, hipMemcpy(&r, &ctx->d_queues->trips_uniform, sizeof r, hipMemcpyDeviceToHost));
         if (k == "stat_rhs_pieces" && ctx->split_mem)
             HIPCHECK(ctx, hipMemcpy(&r, ctx->split.rhs_pieces, sizeof r, hipMemcpyDeviceToHost));
         *v = (double)r;
@@ -534,6 +543,16 @@ struct BatchItem {
     int64_t npix;
 };
 
+/* pieces the planner aims at: about two per piece worker the piece kernel
+   keeps resident (5-lane groups at two wavefronts per SIMD, or single lanes
+   at one wavefront per SIMD) */
+static void split_target(jsx_ctx *ctx) {
+    SplitWS &w = ctx->split;
+    long t = ctx->piece_layout ? (long)ctx->n_cu * 4 * 64 * 2
+                               : (long)ctx->n_cu * ctx->persist_blocks_per_cu * 4 * (64 / JSX_SPLIT_GROUP) * 2;
+    w.target_pieces = (int)(t < w.piece_cap / 2 ? t : w.piece_cap / 2);
+}
+
 /* one allocation for the split path's tables (kept for the context's life) */
 static int split_workspace(jsx_ctx *ctx) {
     if (ctx->split_mem) return JSX_OK;
@@ -558,9 +577,7 @@ static int split_workspace(jsx_ctx *ctx) {
     w.pieces = (Piece *)(b + o_pcs);
     w.pout = (PieceOut *)(b + o_pout);
     w.piece_cap = piece_cap;
-    /* about two pieces per 5-lane group the piece kernel keeps resident */
-    w.target_pieces = ctx->n_cu * ctx->persist_blocks_per_cu * 4 * (64 / JSX_SPLIT_GROUP) * 2;
-    if (w.target_pieces > piece_cap / 2) w.target_pieces = piece_cap / 2;
+    split_target(ctx);
     w.w_min = ctx->split_w_min;
     w.floor_max = ctx->split_floor_max;
     return JSX_OK;
@@ -645,7 +662,10 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
             hipLaunchKernelGGL(jsx_k_split_alloc, dim3(1), dim3(1024), 0, st, ws);
             hipLaunchKernelGGL(jsx_k_split_cut, dim3(cap), dim3(256), 0, st, ws);
             mark(2);
-            hipLaunchKernelGGL(jsx_k_split_piece, dim3(blocks), dim3(256), 0, st, R, ws);
+            if (ctx->piece_layout)
+                hipLaunchKernelGGL(jsx_k_split_piece20, dim3(ctx->n_cu * 4), dim3(256), 0, st, R, ws);
+            else
+                hipLaunchKernelGGL(jsx_k_split_piece, dim3(blocks), dim3(256), 0, st, R, ws);
             mark(3);
             hipLaunchKernelGGL(jsx_k_split_combine, dim3((cap + 255) / 256), dim3(256), 0, st, ctx->d_outs, ws);
         } else {

@@ -559,6 +559,27 @@ struct MlNoReduce {
     __device__ double operator()(const MLane<NS> &L, const MConsts &c) const { return ml_step_error(L, c); }
 };
 
+/* step-size controller on the finished step's error norm (odeint's
+ * controlled_runge_kutta with the default error checker): reject and shrink
+ * when err > 1, else advance t and grow when err < 0.5; returns 1 when the
+ * step is accepted */
+template <int NS>
+__device__ __forceinline__ int ml_control(MLane<NS> &L, const DSolver &S, double maxdt, double err) {
+    if (err > 1.0) {
+        float fac = 0.9f * __builtin_amdgcn_exp2f(-(1.0f / 3.0f) * __builtin_amdgcn_logf((float)err));
+        L.dt = L.dt * (double)(fac > 0.2f ? fac : 0.2f);
+        if (maxdt != 0.0 && L.dt > maxdt) L.dt = maxdt;
+        return 0;
+    }
+    L.t = L.t + L.dt;
+    if (err < 0.5) {
+        double e2 = S.emin > err ? S.emin : err;
+        L.dt = L.dt * (double)(0.9f * __builtin_amdgcn_exp2f(-0.2f * __builtin_amdgcn_logf((float)e2)));
+        if (maxdt != 0.0 && L.dt > maxdt) L.dt = maxdt;
+    }
+    return 1;
+}
+
 template <int NS, class Red = MlNoReduce>
 __device__ inline void ml_consume(MLane<NS> &L, const DSolver &S, const DImage &im, const MConsts &c, const float *r,
                                   Red red = Red()) {
@@ -583,22 +604,7 @@ __device__ inline void ml_consume(MLane<NS> &L, const DSolver &S, const DImage &
     for (int i = 0; i < NS; ++i) L.k7[i] = r[i];
     double err = red(L, c);
     const double h = L.dt;   /* the step just taken; the controller rescales L.dt */
-    int ok;
-    double maxdt = L.phase == PH_TAU ? c.dt_max_pc : 0.0;
-    if (err > 1.0) {
-        float fac = 0.9f * __builtin_amdgcn_exp2f(-(1.0f / 3.0f) * __builtin_amdgcn_logf((float)err));
-        L.dt = L.dt * (double)(fac > 0.2f ? fac : 0.2f);
-        if (maxdt != 0.0 && L.dt > maxdt) L.dt = maxdt;
-        ok = 0;
-    } else {
-        L.t = L.t + L.dt;
-        if (err < 0.5) {
-            double e2 = S.emin > err ? S.emin : err;
-            L.dt = L.dt * (double)(0.9f * __builtin_amdgcn_exp2f(-0.2f * __builtin_amdgcn_logf((float)e2)));
-            if (maxdt != 0.0 && L.dt > maxdt) L.dt = maxdt;
-        }
-        ok = 1;
-    }
+    const int ok = ml_control(L, S, L.phase == PH_TAU ? c.dt_max_pc : 0.0, err);
     if (L.tries++ >= S.max_fail) { ml_fail(L); return; }
     if (!ok) { ml_try_begin(L, S, c); return; }
     /* accepted: Kahan-compensated state update with the increment x_new - x */

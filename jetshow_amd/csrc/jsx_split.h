@@ -511,6 +511,243 @@ __global__ __launch_bounds__(256) void jsx_k_split_piece(const DParamsM *__restr
     if (lane == 0 && tot) atomicAdd(ws.rhs_pieces, tot);
 }
 
+/* ---- 5'. pieces, one lane each: P and the four propagator columns as one
+ *      20-component state (the other Stokes-state layout).  One coefficient
+ *      evaluation per stage serves all five vectors, at the cost of ~5x the
+ *      stage storage per lane (held in VGPRs and AGPRs at one wavefront per
+ *      SIMD); each piece keeps its own step sequence. ---- */
+#define JSX_PIECE_NS 20
+
+/* error norm of a 20-component piece lane: per vector v (components 4v..4v+3)
+ * the scales of ml_step_error (Q/U/V floored at the vector's polarized
+ * maximum) and the vector's floor (split_scale_floor of role v) */
+struct PieceErr20 {
+    double floor[5];
+    __device__ double operator()(const MLane<JSX_PIECE_NS> &L, const MConsts &c) const {
+        const double dt = L.dt;
+        double er = c.st_rel > JSX_SPLIT_REL_MIN ? c.st_rel : JSX_SPLIT_REL_MIN;
+        const double ea = c.st_abs;
+        double err = 0.0;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+            double xe[4], sc[4], pmax = 0.0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int i = 4 * v + q;
+                xe[q] = dt * (DP_DC1 * (double)L.k1[i] + DP_DC3 * (double)L.k3[i] + DP_DC4 * (double)L.k4[i] +
+                              DP_DC5 * (double)L.k5[i] + DP_DC6 * (double)L.k6[i] + DP_DC7 * (double)L.k7[i]);
+                sc[q] = fabs(ml_xval(L, i)) + dt * fabs((double)L.k1[i]);
+                if (q > 0) pmax = sc[q] > pmax ? sc[q] : pmax;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                double sv = q > 0 ? pmax : sc[q];
+                sv = sv > floor[v] ? sv : floor[v];
+                double e = fabs(xe[q]) / (ea + er * sv);
+                e = (e == e) ? e : __builtin_inf();
+                err = err < e ? e : err;
+            }
+        }
+        return err;
+    }
+};
+
+__device__ inline void piece20_rhs(const MLane<JSX_PIECE_NS> &L, const DModel &m, const DImage &im,
+                                   const MConsts &c, float *r) {
+    JSX_MIXED_FMA
+    double p[3];
+    ml_point_pc(L, im, c, p);
+    MCoef k;
+    m_coefficients<1>(m, c, p, k);
+    const float *x = L.xt;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        const float *y = x + 4 * v;
+        float *o = r + 4 * v;
+        const float e = v == 0 ? 1.0f : 0.0f;
+        o[0] = e * k.eI - k.kI * y[0] - k.kQ * y[1] - k.kU * y[2] - k.kV * y[3];
+        o[1] = e * k.eQ - k.kI * y[1] - k.kQ * y[0] - k.kF * y[2] - k.hQ * y[3];
+        o[2] = e * k.eU - k.kI * y[2] - k.kU * y[0] + k.kF * y[1] - k.kC * y[3];
+        o[3] = e * k.eV - k.kI * y[3] - k.kV * y[0] + k.hQ * y[1] + k.kC * y[2];
+    }
+}
+
+/* the piece lane's transitions: one Stokes pass over [ta, tb], no segments,
+ * no tau pass (the general machine's mutual recursion would keep the 20-
+ * component lane in memory) */
+__device__ __forceinline__ void p20_loop(MLane<JSX_PIECE_NS> &L) {
+    if (!d_less_with_sign(L.t, L.L, L.dt)) { L.phase = PH_DONE; return; }
+    if (d_less_with_sign(L.L, L.t + L.dt, L.dt)) L.dt = L.L - L.t;
+    L.tries = 0;
+    if (L.first) {
+#pragma unroll
+        for (int i = 0; i < JSX_PIECE_NS; ++i) L.xt[i] = L.x[i];
+        L.tq = L.t;
+        L.sub = SUB_INIT;
+        L.first = 0;
+        return;
+    }
+    L.n_steps++;
+    ml_prepare_stage(L, 2);
+}
+
+__device__ __forceinline__ void p20_consume(MLane<JSX_PIECE_NS> &L, const DSolver &S, const MConsts &c,
+                                            const float *r, const PieceErr20 &red) {
+    L.n_rhs++;
+#pragma unroll
+    for (int i = 0; i < JSX_PIECE_NS; ++i)
+        if (r[i] != r[i]) L.flags |= JSX_PIX_NAN;
+    switch (L.sub) {
+    case SUB_INIT:
+#pragma unroll
+        for (int i = 0; i < JSX_PIECE_NS; ++i) L.k1[i] = r[i];
+        L.n_steps++;
+        ml_prepare_stage(L, 2);
+        return;
+    case 2:
+#pragma unroll
+        for (int i = 0; i < JSX_PIECE_NS; ++i) L.k2[i] = r[i];
+        ml_prepare_stage(L, 3); return;
+    case 3:
+#pragma unroll
+        for (int i = 0; i < JSX_PIECE_NS; ++i) L.k3[i] = r[i];
+        ml_prepare_stage(L, 4); return;
+    case 4:
+#pragma unroll
+        for (int i = 0; i < JSX_PIECE_NS; ++i) L.k4[i] = r[i];
+        ml_prepare_stage(L, 5); return;
+    case 5:
+#pragma unroll
+        for (int i = 0; i < JSX_PIECE_NS; ++i) L.k5[i] = r[i];
+        ml_prepare_stage(L, 6); return;
+    case 6:
+#pragma unroll
+        for (int i = 0; i < JSX_PIECE_NS; ++i) L.k6[i] = r[i];
+        ml_prepare_stage(L, 7); return;
+    default: break;
+    }
+#pragma unroll
+    for (int i = 0; i < JSX_PIECE_NS; ++i) L.k7[i] = r[i];
+    const double h = L.dt;
+    const int ok = ml_control(L, S, 0.0, red(L, c));
+    if (L.tries++ >= S.max_fail) {
+        L.flags |= JSX_PIX_FAIL_MAXTRIALS;
+        L.phase = PH_DONE;
+        return;
+    }
+    if (!ok) {
+        L.n_steps++;
+        ml_prepare_stage(L, 2);
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < JSX_PIECE_NS; ++i) {
+        m_kahan_add(L.x[i], L.xlo[i], ml_increment(L, i, h));
+        L.k1[i] = L.k7[i];
+    }
+    p20_loop(L);
+}
+
+__global__ __launch_bounds__(256) void jsx_k_split_piece20(const DParamsM *__restrict__ reqs, SplitWS ws) {
+    const int lane = __lane_id();
+    const unsigned npieces = *ws.n_pieces;
+    unsigned nrays = *ws.n_long;
+    if (nrays > (unsigned)ws.cap) nrays = ws.cap;
+    MLane<JSX_PIECE_NS> L;
+    PieceErr20 red;
+    int have = 0, exhausted = 0, slot = 0, pid = 0;
+    unsigned long long rhs_acc = 0;
+    for (;;) {
+        int want = !have && !exhausted;
+        unsigned long long mask = __ballot(want);
+        if (mask) {
+            int cnt = __popcll(mask);
+            int leader = __ffsll((long long)mask) - 1;
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(ws.piece_head, (unsigned)cnt);
+            base = __shfl(base, leader);
+            unsigned long long below = lane ? (mask & ((1ull << lane) - 1ull)) : 0ull;
+            unsigned id = base + (unsigned)__popcll(below);
+            if (want) {
+                if (id >= npieces) {
+                    exhausted = 1;
+                } else if ((unsigned)ws.pieces[id].ray < nrays) {
+                    pid = (int)id;
+                    const Piece pc = ws.pieces[pid];
+                    const LongRay &R = ws.rays[pc.ray];
+                    slot = R.slot;
+                    const DParamsM *q = reqs + slot;
+                    L.pix = pid;
+                    L.phase = PH_STOKES;
+                    L.nseg = 1;
+                    L.seg = 0;
+                    for (int i = 0; i < 3; ++i) {
+                        double o = pc.interval ? R.s0out[i] : R.cout[i];
+                        L.cout[i] = o;
+                        L.cin[i] = o;
+                    }
+                    L.flags = 0; L.n_rhs = 0; L.n_steps = 0;
+                    L.tau_bg = R.tau_bg; L.thick = 0.0;
+#pragma unroll
+                    for (int i = 0; i < JSX_PIECE_NS; ++i) {
+                        L.x[i] = (i >= 4 && (i & 3) == (i >> 2) - 1) ? 1.0f : 0.0f;   /* columns e_0..e_3 */
+                        L.xlo[i] = 0.f;
+                    }
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) red.floor[v] = split_scale_floor(R, v, ws.floor_max);
+                    L.t = pc.ta;
+                    L.L = pc.tb;
+                    L.dt = pc.dt0;
+                    L.first = 1;
+                    (void)q;
+                    p20_loop(L);
+                    have = L.phase == PH_DONE ? 2 : 1;
+                }
+            }
+        }
+        if (!__any(have) && __all(exhausted)) break;
+        if (have == 1) {
+            float r[JSX_PIECE_NS];
+            const int s0 = __builtin_amdgcn_readfirstlane(slot);
+            if (__all(slot == s0)) {
+                const JSX_CONST DParamsM *q = (const JSX_CONST DParamsM *)(reqs + s0);
+                JSX_OPAQUE_S(q);
+                piece20_rhs(L, q->m, q->im, q->c, r);
+                q = (const JSX_CONST DParamsM *)(reqs + s0);
+                JSX_OPAQUE_S(q);
+                p20_consume(L, q->S, q->c, r, red);
+            } else {
+                const DParamsM *q = reqs + slot;
+                JSX_OPAQUE_V(q);
+                piece20_rhs(L, q->m, q->im, q->c, r);
+                q = reqs + slot;
+                JSX_OPAQUE_V(q);
+                p20_consume(L, q->S, q->c, r, red);
+            }
+            if (L.phase == PH_DONE) have = 2;
+        }
+        if (have == 2) {
+            PieceOut &o = ws.pout[pid];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) o.P[i] = ml_xval(L, i);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) o.Phi[i] = ml_xval(L, 4 + i);
+            o.flags = L.flags;
+            o.n_rhs = L.n_rhs;
+            o.n_steps = L.n_steps;
+            rhs_acc += L.n_rhs;
+            have = 0;
+        }
+    }
+    unsigned long long tot = rhs_acc;
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned lo = __shfl_xor((unsigned)(tot & 0xffffffffull), off);
+        unsigned hi = __shfl_xor((unsigned)(tot >> 32), off);
+        tot += ((unsigned long long)hi << 32) | lo;
+    }
+    if (lane == 0 && tot) atomicAdd(ws.rhs_pieces, tot);
+}
+
 /* ---- 6. compose pieces and write the pixel ---- */
 __global__ __launch_bounds__(256) void jsx_k_split_combine(const DOut *__restrict__ outs, SplitWS ws) {
     const unsigned r = blockIdx.x * blockDim.x + threadIdx.x;

@@ -191,3 +191,30 @@ def test_server_same_alone_and_in_mix(ctx):
     rel, quv, _ = _mixed_errors(ref, mixed)
     assert rel <= 1e-5 and quv <= 5e-5
     assert st["epochs"] >= 2
+
+
+def test_split_piece_layouts_agree(ctx):
+    """The two Stokes-state layouts of a piece (a 5-lane group, or one lane
+    with 20 components) evaluate the same coefficients and the same error norm,
+    so they take the same steps: the split rays agree to 1e-6 of peak I."""
+    j, im, p = models.paraboloid_helical_model(n_side=64, pixel_pc=0.08)
+    pm = models.with_precision(p, abi.PREC_MIXED, rel_tol=1e-6)
+    pm.flags |= OPT_PERSISTENT | OPT_APEX_FIRST
+    old = ctx.get_option("defer_rhs"), ctx.get_option("piece_layout")
+    ctx.set_option("defer_rhs", 4096)
+    try:
+        out = []
+        for layout in (0, 1):
+            ctx.set_option("piece_layout", layout)
+            out.append(ctx.observe(j, im, pm))
+            assert ctx.get_option("stat_parked") > 0
+    finally:
+        ctx.set_option("defer_rhs", old[0])
+        ctx.set_option("piece_layout", old[1])
+    a, b = out
+    peak = a["I"].max()
+    diff = max(np.abs(a[k] - b[k]).max() for k in ("I", "Q", "U", "V")) / peak
+    same = all(np.array_equal(a[k], b[k]) for k in ("I", "Q", "U", "V"))
+    print("piece layouts: max diff / peak I %.3g, bitwise identical %s" % (diff, same), flush=True)
+    assert np.array_equal(a["flags"], b["flags"])
+    assert diff <= 1e-6
