@@ -33,10 +33,10 @@ OPT_APEX_FIRST = 0x200
 # TRANS}_{F64,F32} on single-wavefront launches (bench.py --calibrate; FMA = 2);
 # the counter table is in profiles/valu_ops_per_rhs_calibration.md.
 OPS_PER_RHS = {
-    "mixed_full": {"f64": 67.0, "f32": 272.0},
+    "mixed_full": {"f64": 83.5, "f32": 312.0},
     "fp64_full": {"f64": 1290.1, "f32": 0.0},
     "fp64_I_grid": {"f64": 906.3, "f32": 0.0},
-    "mixed_I_grid": {"f64": 475.6, "f32": 41.4},
+    "mixed_I_grid": {"f64": 477.8, "f32": 99.2},
 }
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 
