@@ -188,8 +188,15 @@ int jsx_abi_version(void);
  *                     split into time-parallel pieces (0 disables)
  *   "split_w_min"     smallest stiffness per piece
  *   "piece_layout"    0: a split piece is integrated by a group of 5 lanes
- *                     (P and the 4 propagator columns); 1: by one lane holding
- *                     all 20 components (one coefficient evaluation per stage)
+ *                     (P and the 4 propagator columns); 1 (default): by one
+ *                     lane holding all 20 components (one coefficient
+ *                     evaluation per stage)
+ *   "triage"          1 (default): a first pass writes the rays that need no
+ *                     integration and queues only the others; 0: the queues
+ *                     deal every ray id
+ *   "park_lanes"      rays are split only once the queues are dry and at most
+ *                     this many lanes of the launch still hold a ray (0: half a
+ *                     wavefront per SIMD of the device)
  *   "split_floor_max" cap of the propagator-column error loosening (1 = off;
  *                     >1 trades Q/U/V accuracy for fewer piece steps)
  *   "timing"          1: record HIP events around the batched kernels
@@ -198,7 +205,8 @@ int jsx_abi_version(void);
  * right-hand sides evaluated by its batch / piece kernel, "stat_trips" /
  * "stat_trips_uniform" the batch kernel's wavefront loop trips (all / with
  * every lane on one request: scalar constant loads), and, with "timing",
- * "stat_batch_ms", "stat_piece_ms", "stat_split_ms" (all split kernels) means
+ * "stat_triage_ms", "stat_batch_ms", "stat_piece_ms", "stat_split_ms" (all
+ * split kernels) means
  * over the launches since timing was enabled ("stat_launches"). */
 int jsx_ctx_set_option(jsx_ctx *ctx, const char *name, double value);
 int jsx_ctx_get_option(jsx_ctx *ctx, const char *name, double *value);

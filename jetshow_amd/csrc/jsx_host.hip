@@ -66,16 +66,22 @@ struct jsx_ctx {
     DOut *d_outs = nullptr;
     size_t n_cap = 0;
     std::vector<unsigned char> h_reqs;
+    /* triage pass: live-ray list (grow-only) */
+    int triage = 1;
+    int64_t *d_live = nullptr;
+    size_t live_cap = 0;
+    /* lanes still holding a ray below which long rays may be parked (0: auto) */
+    unsigned park_lanes = 0;
     /* time-parallel path for long full-Stokes rays (mixed precision) */
     unsigned defer_rhs = 8192;          /* 0 disables parking                  */
     float split_w_min = 256.0f;
     float split_floor_max = 1.0f;
-    int piece_layout = 0;               /* 0: 5-lane groups; 1: one lane, 20 components */
+    int piece_layout = 1;               /* 0: 5-lane groups; 1: one lane, 20 components */
     SplitWS split{};
     void *split_mem = nullptr;
     /* optional event timing of the batched launches (option "timing") */
     int timing = 0;
-    struct EvSet { hipEvent_t e[5]; };
+    struct EvSet { hipEvent_t e[6]; };
     std::vector<EvSet> evs;     /* created on first use, reused */
     int ev_n = 0;               /* launches recorded since timing was (re)enabled */
     static const int kEvMax = 64;
@@ -145,6 +151,7 @@ extern "C" int jsx_ctx_destroy(jsx_ctx *ctx) {
     (void)hipHostFree(ctx->h_params);
     for (int i = 0; i < jsx_ctx::kSlots; ++i) (void)hipEventDestroy(ctx->slot_ev[i]);
     (void)hipFree(ctx->d_queues); (void)hipFree(ctx->d_reqs); (void)hipFree(ctx->d_off); (void)hipFree(ctx->d_outs);
+    (void)hipFree(ctx->d_live);
     (void)hipFree(ctx->split_mem);
     for (auto &es : ctx->evs)
         for (int k = 0; k < 5; ++k) (void)hipEventDestroy(es.e[k]);
@@ -184,6 +191,11 @@ extern "C" int jsx_ctx_set_option(jsx_ctx *ctx, const char *name, double v) {
         if (!(v == 0.0 || v == 1.0)) return fail(ctx, JSX_EINVAL, "piece_layout must be 0 or 1");
         ctx->piece_layout = (int)v;
         if (ctx->split_mem) split_target(ctx);
+    } else if (k == "triage") {
+        ctx->triage = v != 0.0;
+    } else if (k == "park_lanes") {
+        if (!(v >= 0 && v < 4e9)) return fail(ctx, JSX_EINVAL, "park_lanes out of range");
+        ctx->park_lanes = (unsigned)v;
     } else if (k == "timing") {
         ctx->timing = v != 0.0;
         ctx->ev_n = 0;
@@ -204,6 +216,8 @@ extern "C" int jsx_ctx_get_option(jsx_ctx *ctx, const char *name, double *v) {
     else if (k == "split_w_min") *v = ctx->split_w_min;
     else if (k == "split_floor_max") *v = ctx->split_floor_max;
     else if (k == "piece_layout") *v = ctx->piece_layout;
+    else if (k == "triage") *v = ctx->triage;
+    else if (k == "park_lanes") *v = ctx->park_lanes;
     else if (k == "stat_parked" || k == "stat_pieces") {
         unsigned c[2] = {0, 0};
         if (ctx->split_mem) {
@@ -226,10 +240,10 @@ extern "C" int jsx_ctx_get_option(jsx_ctx *ctx, const char *name, double *v) {
         *v = (double)r;
     } else if (k == "stat_launches") {
         *v = ctx->ev_n;
-    } else if (k == "stat_batch_ms" || k == "stat_piece_ms" || k == "stat_split_ms") {
+    } else if (k == "stat_batch_ms" || k == "stat_piece_ms" || k == "stat_split_ms" || k == "stat_triage_ms") {
         /* mean over the launches recorded since timing was enabled */
-        int a = k == "stat_batch_ms" ? 0 : (k == "stat_piece_ms" ? 2 : 1);
-        int b = k == "stat_batch_ms" ? 1 : (k == "stat_piece_ms" ? 3 : 4);
+        int a = k == "stat_batch_ms" ? 0 : (k == "stat_piece_ms" ? 2 : (k == "stat_triage_ms" ? 5 : 1));
+        int b = k == "stat_batch_ms" ? 1 : (k == "stat_piece_ms" ? 3 : (k == "stat_triage_ms" ? 0 : 4));
         double sum = 0.0;
         int n = ctx->ev_n < jsx_ctx::kEvMax ? ctx->ev_n : jsx_ctx::kEvMax;
         for (int i = 0; i < n; ++i) {
@@ -620,6 +634,7 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
         off[i + 1] = off[i] + it.npix;
         outs[i] = it.out;
     }
+    if (off[n] == 0) return JSX_OK;
     HIPCHECK(ctx, hipMemcpyAsync(ctx->d_reqs, ctx->h_reqs.data(), n * esz, hipMemcpyHostToDevice, st));
     HIPCHECK(ctx, hipMemcpyAsync(ctx->d_off, off.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, st));
     HIPCHECK(ctx, hipMemcpyAsync(ctx->d_outs, outs.data(), sizeof(DOut) * n, hipMemcpyHostToDevice, st));
@@ -634,7 +649,7 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
     if (ctx->timing && ctx->ev_n < jsx_ctx::kEvMax) {
         if ((int)ctx->evs.size() <= ctx->ev_n) {
             jsx_ctx::EvSet es;
-            for (int k = 0; k < 5; ++k) HIPCHECK(ctx, hipEventCreate(&es.e[k]));
+            for (int k = 0; k < 6; ++k) HIPCHECK(ctx, hipEventCreate(&es.e[k]));
             ctx->evs.push_back(es);
         }
         ev = &ctx->evs[ctx->ev_n++];
@@ -646,15 +661,41 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
         ws = ctx->split;
         hipLaunchKernelGGL(jsx_k_split_reset, dim3(1), dim3(64), 0, st, ws);
     }
-    mark(0);
+    /* triage list: one int64 per ray of the launch */
+    int64_t *live = nullptr;
+    if (ctx->triage) {
+        if (ctx->live_cap < (size_t)total) {
+            (void)hipFree(ctx->d_live);
+            ctx->d_live = nullptr;
+            ctx->live_cap = 0;
+            HIPCHECK(ctx, hipMalloc(&ctx->d_live, sizeof(int64_t) * (size_t)total));
+            ctx->live_cap = (size_t)total;
+        }
+        live = ctx->d_live;
+    }
+    const int64_t tri_need = (total + 255) / 256;
+    const int tri_blocks = (int)(tri_need < (int64_t)ctx->n_cu * 16 ? tri_need : (int64_t)ctx->n_cu * 16);
+    /* default tail threshold: half a wavefront per SIMD of the chip */
+    const unsigned park_lanes = ctx->park_lanes ? ctx->park_lanes : (unsigned)ctx->n_cu * 4u * 32u;
+    mark(5);
     if (precision == JSX_PREC_MIXED) {
         const DParamsM *R = (const DParamsM *)ctx->d_reqs;
+        if (live) {
+            if (full)
+                hipLaunchKernelGGL(jsx_k_triage<PolMixed<4>>, dim3(tri_blocks), dim3(256), 0, st, R, ctx->d_off,
+                                   ctx->d_outs, nreq, total, order, ctx->d_queues, live);
+            else
+                hipLaunchKernelGGL(jsx_k_triage<PolMixed<1>>, dim3(tri_blocks), dim3(256), 0, st, R, ctx->d_off,
+                                   ctx->d_outs, nreq, total, order, ctx->d_queues, live);
+        }
+        mark(0);
         if (full)
             hipLaunchKernelGGL(jsx_k_batch<PolMixed<4>>, dim3(blocks), dim3(256), 0, st, R, ctx->d_off, ctx->d_outs, nreq,
-                               total, ctx->d_queues, ctx->queue_block, order, ctx->steal, ws, ctx->defer_rhs);
+                               total, ctx->d_queues, ctx->queue_block, order, ctx->steal, ws, ctx->defer_rhs, live,
+                               park_lanes);
         else
             hipLaunchKernelGGL(jsx_k_batch<PolMixed<1>>, dim3(blocks), dim3(256), 0, st, R, ctx->d_off, ctx->d_outs, nreq,
-                               total, ctx->d_queues, ctx->queue_block, order, ctx->steal, ws, 0u);
+                               total, ctx->d_queues, ctx->queue_block, order, ctx->steal, ws, 0u, live, park_lanes);
         mark(1);
         if (split) {
             const int cap = ws.cap;
@@ -675,12 +716,21 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
         mark(4);
     } else {
         const DParams *R = (const DParams *)ctx->d_reqs;
+        if (live) {
+            if (full)
+                hipLaunchKernelGGL(jsx_k_triage<PolFP64<4>>, dim3(tri_blocks), dim3(256), 0, st, R, ctx->d_off,
+                                   ctx->d_outs, nreq, total, order, ctx->d_queues, live);
+            else
+                hipLaunchKernelGGL(jsx_k_triage<PolFP64<1>>, dim3(tri_blocks), dim3(256), 0, st, R, ctx->d_off,
+                                   ctx->d_outs, nreq, total, order, ctx->d_queues, live);
+        }
+        mark(0);
         if (full)
             hipLaunchKernelGGL(jsx_k_batch<PolFP64<4>>, dim3(blocks), dim3(256), 0, st, R, ctx->d_off, ctx->d_outs, nreq,
-                               total, ctx->d_queues, ctx->queue_block, order, ctx->steal, ws, 0u);
+                               total, ctx->d_queues, ctx->queue_block, order, ctx->steal, ws, 0u, live, park_lanes);
         else
             hipLaunchKernelGGL(jsx_k_batch<PolFP64<1>>, dim3(blocks), dim3(256), 0, st, R, ctx->d_off, ctx->d_outs, nreq,
-                               total, ctx->d_queues, ctx->queue_block, order, ctx->steal, ws, 0u);
+                               total, ctx->d_queues, ctx->queue_block, order, ctx->steal, ws, 0u, live, park_lanes);
         mark(1);
         mark(2);
         mark(3);

@@ -4,9 +4,10 @@ Knobs (jsx_ctx_set_option): persistent workgroups per CU ("blocks_per_cu",
 i.e. waves per SIMD the launch asks for), ray ids per queue block
 ("queue_block", the work-stealing chunk), per-XCD queues with stealing vs one
 global queue ("steal"), the right-hand-side budget after which a full-Stokes
-ray is split into time-parallel pieces ("defer_rhs"), and the Stokes-state
-layout of a piece ("piece_layout": a 5-lane group with 4 components per lane,
-or one lane with all 20).
+ray is split into time-parallel pieces ("defer_rhs"), how few lanes may still
+hold a ray before splitting starts ("park_lanes"), and the Stokes-state layout
+of a piece ("piece_layout": a 5-lane group with 4 components per lane, or one
+lane with all 20).
 
 A sweep times each candidate on a workload (a list of (jet, image, params)
 sent through one batched launch), keeps the fastest, and caches it in a JSON
@@ -27,6 +28,7 @@ DEFAULT_SPACE = {
     "steal": [0, 1],
     "defer_rhs": [2048, 8192, 32768],
     "piece_layout": [0, 1],
+    "park_lanes": [0, 8192],
 }
 CACHE_PATH = os.environ.get("JSX_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "jetshow_amd",
                                                             "tune.json"))

@@ -164,6 +164,35 @@ def test_batch_equals_single_images(ctx):
         ctx.set_option("defer_rhs", old)
 
 
+def test_triage_list_same_planes(ctx):
+    """The triage pass (misses written first, live rays queued) changes only
+    which lane takes which ray: FP64 and unsplit mixed planes are identical
+    with and without it, over a batch of images whose hit fractions differ."""
+    cases = [models.conical_toroidal_model(n_side=64, pixel_pc=0.02),
+             models.paraboloid_helical_model(n_side=48, pixel_pc=0.1),
+             W.reference_main_model()]
+    old = ctx.get_option("defer_rhs"), ctx.get_option("triage")
+    ctx.set_option("defer_rhs", 0)
+    try:
+        for prec in (abi.PREC_FP64, abi.PREC_MIXED):
+            items = []
+            for j, im, p in cases:
+                q = models.with_precision(p, prec, rel_tol=1e-6) if prec == abi.PREC_MIXED else models.copy_struct(p)
+                q.stokes = abi.STOKES_FULL
+                q.flags |= OPT_PERSISTENT | OPT_APEX_FIRST
+                items.append((j, im, q))
+            res = []
+            for tri in (0, 1):
+                ctx.set_option("triage", tri)
+                res.append(ctx.observe_batch(items))
+            for a, b in zip(*res):
+                for k in ("tau", "I", "Q", "U", "V", "l", "flags", "n_rhs", "n_steps"):
+                    assert np.array_equal(a[k], b[k]), (prec, k)
+    finally:
+        ctx.set_option("defer_rhs", old[0])
+        ctx.set_option("triage", old[1])
+
+
 def test_server_same_alone_and_in_mix(ctx):
     """Continuous batching: a request gives the same planes alone and when
     submitted among others of different sizes and frequencies (the server's
