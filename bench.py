@@ -37,6 +37,11 @@ OPS_PER_RHS = {
     "fp64_full": {"f64": 1290.1, "f32": 0.0},
     "fp64_I_grid": {"f64": 906.3, "f32": 0.0},
     "mixed_I_grid": {"f64": 477.8, "f32": 99.2},
+    # per piece right-hand side (20 components, one lane) and per coefficient
+    # evaluation of an exponential piece (2 per step: coefficients, the two
+    # exponentials, error estimate and propagator update)
+    "piece20": {"f64": 146.9, "f32": 590.1},
+    "exp": {"f64": 1292.5, "f32": 228.0},
 }
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 
@@ -87,11 +92,58 @@ def calibrate():
         r = ctx.observe(jj, ii, q)
         print(json.dumps({"calibration": name, "max_n_rhs": int(r["n_rhs"].max()),
                           "n_rhs": r["n_rhs"].reshape(-1).tolist()}), flush=True)
+    # The piece kernels: the same two rays, each parked after 64 right-hand
+    # sides into a single piece (two lanes of one wavefront do all the work),
+    # by 20-component Dormand-Prince and by exponential propagators; the
+    # longest piece's right-hand sides are the wavefront's loop trips.  The
+    # other wavefronts of those launches only find the queue empty.
+    qm = models.with_precision(p, abi.PREC_MIXED, rel_tol=1e-6)
+    qm.flags |= OPT_PERSISTENT
+    for name, layout in (("mixed_full_piece20", 1), ("mixed_full_exp", 2)):
+        for k, v in (("defer_rhs", 64), ("park_lanes", 4e9 - 1), ("piece_layout", layout), ("piece_budget", 0),
+                     ("piece_target", 1)):
+            ctx.set_option(k, v)
+        r = ctx.observe(j, im, qm)
+        rhs = ctx.get_option("stat_rhs_pieces" if layout == 1 else "stat_rhs_exp")
+        print(json.dumps({"calibration": name, "pieces": int(ctx.get_option("stat_pieces")),
+                          "piece_rhs": int(rhs), "max_piece_rhs": int(ctx.get_option("stat_piece_steps_max")),
+                          "n_rhs": r["n_rhs"].reshape(-1).tolist()}), flush=True)
     ctx.close()
 
 
 def _log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+KERNEL_STATS = ("stat_batch_ms", "stat_piece_ms", "stat_piece_dp_ms", "stat_piece_exp_ms", "stat_split_ms",
+                "stat_launches", "stat_rhs_batch", "stat_rhs_pieces", "stat_rhs_exp", "stat_parked",
+                "stat_parked_exp", "stat_pieces")
+
+
+def _peak(ops):
+    f = ops["f64"] + ops["f32"]
+    return f / (ops["f64"] / PEAK_TFLOPS["f64"] + ops["f32"] / PEAK_TFLOPS["f32"])
+
+
+def kernel_rooflines(kern, batch_ops="mixed_full", batch_name="jsx_k_batch<PolMixed<4>>"):
+    """Per kernel of the batched launch: algorithmic VALU operations (right-hand
+    sides it evaluated x operations per right-hand side of that kernel) over
+    its mean time by HIP events, against the FP64/FP32 vector peaks weighted
+    by its op mix.  Returns (table, name of the longest kernel)."""
+    rows = {
+        batch_name: (kern["stat_rhs_batch"], kern["stat_batch_ms"], batch_ops),
+        "jsx_k_split_piece20": (kern.get("stat_rhs_pieces", 0.0), kern.get("stat_piece_dp_ms", 0.0), "piece20"),
+        "jsx_k_split_magnus": (kern.get("stat_rhs_exp", 0.0), kern.get("stat_piece_exp_ms", 0.0), "exp"),
+    }
+    table = {}
+    for name, (rhs, ms, key) in rows.items():
+        ops = OPS_PER_RHS[key]
+        f = ops["f64"] + ops["f32"]
+        ach = f * rhs / (ms / 1e3) / 1e12 if ms > 0 else 0.0
+        table[name] = {"ms": ms, "rhs": rhs, "ops_per_rhs": f, "achieved_TFLOPs": ach, "peak_TFLOPs": _peak(ops),
+                       "frac": ach / _peak(ops)}
+    dom = max(table, key=lambda k: table[k]["ms"])
+    return table, dom
 
 
 def main():
@@ -169,8 +221,7 @@ def main():
 
     # headline: batched mixed-precision full-Stokes solver (long rays split in time)
     tm, lat_m, wall_m = timed(pm, a.steps, a.warmup, "mixed batched")
-    kern = {k: ctx.get_option(k) for k in ("stat_batch_ms", "stat_piece_ms", "stat_split_ms", "stat_launches",
-                                           "stat_rhs_batch", "stat_rhs_pieces", "stat_parked", "stat_pieces")}
+    kern = {k: ctx.get_option(k) for k in KERNEL_STATS}
     ctx.set_option("timing", 0)
     ms_step = float(lat_m.mean())
     value = npix / (ms_step / 1e3)
@@ -211,24 +262,17 @@ def main():
     # roofline of the dominant kernel: algorithmic VALU operations (right-hand
     # sides evaluated in that kernel x operations per right-hand side) over
     # its mean launch time, both measured in this run
-    ops = OPS_PER_RHS["mixed_full"]
-    f_rhs = ops["f64"] + ops["f32"]
-    peak = f_rhs / (ops["f64"] / PEAK_TFLOPS["f64"] + ops["f32"] / PEAK_TFLOPS["f32"])
-    per_kernel = {
-        "jsx_k_batch<PolMixed<4>>": (kern["stat_rhs_batch"], kern["stat_batch_ms"]),
-        "jsx_k_split_piece": (kern["stat_rhs_pieces"], kern["stat_piece_ms"]),
-    }
-    dom = max(per_kernel, key=lambda k: per_kernel[k][1])
-    rhs_k, ms_k = per_kernel[dom]
-    achieved = f_rhs * rhs_k / (ms_k / 1e3) / 1e12 if ms_k > 0 else 0.0
-    roofline = {"bound": "mfma", "pipe": "valu", "kernel": dom, "achieved": achieved, "peak": peak,
-                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+    table, dom = kernel_rooflines(kern)
+    t = table[dom]
+    roofline = {"bound": "mfma", "pipe": "valu", "kernel": dom, "achieved": t["achieved_TFLOPs"],
+                "peak": t["peak_TFLOPs"], "unit": "TFLOP/s", "frac": t["frac"], "traffic": None,
                 "note": "VALU-bound transcendental code, no MFMA and no grid reads (traffic null): peak = FP64/FP32 "
-                        "vector peaks weighted by the op mix; ops per RHS from rocprofv3 VALU counters"}
+                        "vector peaks weighted by the kernel's op mix; ops per RHS from rocprofv3 VALU counters"}
     kernels = {"batch_ms": kern["stat_batch_ms"], "split_total_ms": kern["stat_split_ms"],
                "piece_ms": kern["stat_piece_ms"], "rhs_batch": kern["stat_rhs_batch"],
-               "rhs_pieces": kern["stat_rhs_pieces"], "rays_split": kern["stat_parked"],
-               "pieces": kern["stat_pieces"], "image_achieved_TFLOPs": f_rhs * n_rhs_tot / (ms_step / 1e3) / 1e12}
+               "rhs_pieces": kern["stat_rhs_pieces"], "rhs_exp": kern["stat_rhs_exp"],
+               "rays_split": kern["stat_parked"], "rays_exp": kern["stat_parked_exp"], "pieces": kern["stat_pieces"],
+               "per_kernel": table}
 
     cpu = None
     if not a.no_cpu:
@@ -321,16 +365,12 @@ def sweep_bench(a):
         e1.record(stream)
     torch.cuda.synchronize()
     lat = np.array([e0.elapsed_time(e1) for e0, e1 in evs])
-    kern = {k: ctx.get_option(k) for k in ("stat_batch_ms", "stat_piece_ms", "stat_rhs_batch", "stat_rhs_pieces",
-                                           "stat_parked", "stat_pieces")}
+    kern = {k: ctx.get_option(k) for k in KERNEL_STATS}
     ctx.set_option("timing", 0)
     flags = buf["flags"].cpu().numpy()
     nrhs = buf["n_rhs"].cpu().numpy().astype(np.int64)
     ms = float(lat.mean())
-    ops = OPS_PER_RHS["mixed_full"]
-    f_rhs = ops["f64"] + ops["f32"]
-    peak = f_rhs / (ops["f64"] / PEAK_TFLOPS["f64"] + ops["f32"] / PEAK_TFLOPS["f32"])
-    ach = f_rhs * kern["stat_rhs_batch"] / (kern["stat_batch_ms"] / 1e3) / 1e12 if kern["stat_batch_ms"] else 0.0
+    table, dom = kernel_rooflines(kern)
     cpu = None
     if not a.no_cpu:
         cpu = sweep_cpu_baseline(reqs_py, a.cpu_seconds)
@@ -346,9 +386,11 @@ def sweep_bench(a):
         "rhs_evaluations": float(nrhs.sum()),
         "kernels": {"batch_ms": kern["stat_batch_ms"], "piece_ms": kern["stat_piece_ms"],
                     "rhs_batch": kern["stat_rhs_batch"], "rhs_pieces": kern["stat_rhs_pieces"],
-                    "rays_split": kern["stat_parked"], "pieces": kern["stat_pieces"]},
-        "roofline": {"bound": "mfma", "pipe": "valu", "kernel": "jsx_k_batch<PolMixed<4>>", "achieved": ach,
-                     "peak": peak, "unit": "TFLOP/s", "frac": ach / peak, "traffic": None},
+                    "rhs_exp": kern["stat_rhs_exp"], "rays_split": kern["stat_parked"],
+                    "rays_exp": kern["stat_parked_exp"], "pieces": kern["stat_pieces"], "per_kernel": table},
+        "roofline": {"bound": "mfma", "pipe": "valu", "kernel": dom, "achieved": table[dom]["achieved_TFLOPs"],
+                     "peak": table[dom]["peak_TFLOPs"], "unit": "TFLOP/s", "frac": table[dom]["frac"],
+                     "traffic": None},
         "cpu_baseline": cpu,
     }
     line = json.dumps(out)

@@ -187,10 +187,21 @@ int jsx_abi_version(void);
  *   "defer_rhs"       right-hand sides after which a mixed full-Stokes ray is
  *                     split into time-parallel pieces (0 disables)
  *   "split_w_min"     smallest stiffness per piece
- *   "piece_layout"    0: a split piece is integrated by a group of 5 lanes
- *                     (P and the 4 propagator columns); 1 (default): by one
- *                     lane holding all 20 components (one coefficient
- *                     evaluation per stage)
+ *   "piece_layout"    how split pieces are integrated: 0 by Dormand-Prince
+ *                     in a group of 5 lanes (P and the 4 propagator columns);
+ *                     1 by Dormand-Prince in one lane holding all 20
+ *                     components; 2 by exponential (4th-order Magnus)
+ *                     propagators, one lane per piece; 3 (default) 1 or 2 per
+ *                     ray, from cost estimates sampled along its rest
+ *   "piece_target"    exponential pieces the planner aims at in the first
+ *                     round (0: one per 4 resident lanes)
+ *   "piece_budget"    right-hand sides after which an exponential piece hands
+ *                     the rest of its interval to new pieces in the next of 4
+ *                     rounds (0: one round, no budget)
+ *   "mg_cost"         layout 3: ratio of estimated costs below which a ray's
+ *                     pieces are exponential
+ *   "mg_profile"      cuts of exponential rays placed by 0 (default) the
+ *                     stiffness profile, 1 the sampled exponential-step profile
  *   "triage"          1 (default): a first pass writes the rays that need no
  *                     integration and queues only the others; 0: the queues
  *                     deal every ray id
@@ -201,13 +212,16 @@ int jsx_abi_version(void);
  *                     >1 trades Q/U/V accuracy for fewer piece steps)
  *   "timing"          1: record HIP events around the batched kernels
  * Read-only (wait for the device): "stat_parked" rays split by the last
- * batched launch, "stat_pieces" its pieces, "stat_rhs_batch" / "stat_rhs_pieces"
- * right-hand sides evaluated by its batch / piece kernel, "stat_trips" /
- * "stat_trips_uniform" the batch kernel's wavefront loop trips (all / with
- * every lane on one request: scalar constant loads), and, with "timing",
- * "stat_triage_ms", "stat_batch_ms", "stat_piece_ms", "stat_split_ms" (all
- * split kernels) means
- * over the launches since timing was enabled ("stat_launches"). */
+ * batched launch ("stat_parked_exp" of them with exponential pieces),
+ * "stat_pieces" its pieces, "stat_piece_steps_max" / "stat_piece_steps_mean"
+ * right-hand sides per piece, "stat_rhs_batch" / "stat_rhs_pieces" /
+ * "stat_rhs_exp" right-hand sides evaluated by its batch / Dormand-Prince
+ * piece / exponential piece kernel, "stat_trips" / "stat_trips_uniform" the
+ * batch kernel's wavefront loop trips (all / with every lane on one request:
+ * scalar constant loads), and, with "timing", "stat_triage_ms",
+ * "stat_batch_ms", "stat_piece_ms" (both piece kernels: "stat_piece_dp_ms" +
+ * "stat_piece_exp_ms"), "stat_split_ms" (all split kernels) means over the
+ * launches since timing was enabled ("stat_launches"). */
 int jsx_ctx_set_option(jsx_ctx *ctx, const char *name, double value);
 int jsx_ctx_get_option(jsx_ctx *ctx, const char *name, double *value);
 

@@ -43,16 +43,6 @@ struct BatchQueues {
     unsigned drained;                      /* set once every queue has run dry */
 };
 
-/* sum of a 64-bit value over the wavefront, returned in every lane */
-__device__ inline unsigned long long d_wave_sum_u64(unsigned long long v) {
-    for (int off = 32; off > 0; off >>= 1) {
-        unsigned lo = __shfl_xor((unsigned)(v & 0xffffffffull), off);
-        unsigned hi = __shfl_xor((unsigned)(v >> 32), off);
-        v += ((unsigned long long)hi << 32) | lo;
-    }
-    return v;
-}
-
 /* policy: FP64 machine */
 template <int NS>
 struct PolFP64 {

@@ -72,16 +72,25 @@ struct jsx_ctx {
     size_t live_cap = 0;
     /* lanes still holding a ray below which long rays may be parked (0: auto) */
     unsigned park_lanes = 0;
+    /* pieces the split planner aims at (0: auto, about two per resident piece lane) */
+    int piece_target = 0;
+    /* exponential pieces: right-hand sides a piece may use in each of the
+       first rounds before handing the rest of its interval on (0: one round) */
+    unsigned piece_budget = 64;
     /* time-parallel path for long full-Stokes rays (mixed precision) */
-    unsigned defer_rhs = 8192;          /* 0 disables parking                  */
+    unsigned defer_rhs = 1024;          /* 0 disables parking                  */
     float split_w_min = 256.0f;
     float split_floor_max = 1.0f;
-    int piece_layout = 1;               /* 0: 5-lane groups; 1: one lane, 20 components */
+    int piece_layout = 3;               /* 0: 5-lane groups; 1: one lane, 20 components;
+                                           2: one lane, exponential (Magnus) propagator;
+                                           3: 1 or 2 per ray by its rotation */
+    float mg_cost = 0.5f;               /* layout 3: cost ratio below which a ray's pieces are exponential */
+    int mg_profile = 0;                 /* cuts of exponential rays: 0 stiffness, 1 step estimate */
     SplitWS split{};
     void *split_mem = nullptr;
     /* optional event timing of the batched launches (option "timing") */
     int timing = 0;
-    struct EvSet { hipEvent_t e[6]; };
+    struct EvSet { hipEvent_t e[7]; };
     std::vector<EvSet> evs;     /* created on first use, reused */
     int ev_n = 0;               /* launches recorded since timing was (re)enabled */
     static const int kEvMax = 64;
@@ -188,9 +197,24 @@ extern "C" int jsx_ctx_set_option(jsx_ctx *ctx, const char *name, double v) {
         ctx->split_floor_max = (float)v;
         if (ctx->split_mem) ctx->split.floor_max = (float)v;
     } else if (k == "piece_layout") {
-        if (!(v == 0.0 || v == 1.0)) return fail(ctx, JSX_EINVAL, "piece_layout must be 0 or 1");
+        if (!(v == 0.0 || v == 1.0 || v == 2.0 || v == 3.0))
+            return fail(ctx, JSX_EINVAL, "piece_layout must be 0, 1, 2 or 3");
         ctx->piece_layout = (int)v;
         if (ctx->split_mem) split_target(ctx);
+    } else if (k == "piece_target") {
+        if (!(v >= 0 && v <= (double)(1 << 19))) return fail(ctx, JSX_EINVAL, "piece_target must be in [0, 2^19]");
+        ctx->piece_target = (int)v;
+        if (ctx->split_mem) split_target(ctx);
+    } else if (k == "mg_profile") {
+        ctx->mg_profile = v != 0.0;
+        if (ctx->split_mem) ctx->split.mg_profile = ctx->mg_profile;
+    } else if (k == "mg_cost") {
+        if (!(v > 0)) return fail(ctx, JSX_EINVAL, "mg_cost must be positive");
+        ctx->mg_cost = (float)v;
+        if (ctx->split_mem) ctx->split.mg_cost = (float)v;
+    } else if (k == "piece_budget") {
+        if (!(v >= 0 && v < 1e9)) return fail(ctx, JSX_EINVAL, "piece_budget out of range");
+        ctx->piece_budget = (unsigned)v;
     } else if (k == "triage") {
         ctx->triage = v != 0.0;
     } else if (k == "park_lanes") {
@@ -217,6 +241,10 @@ extern "C" int jsx_ctx_get_option(jsx_ctx *ctx, const char *name, double *v) {
     else if (k == "split_floor_max") *v = ctx->split_floor_max;
     else if (k == "piece_layout") *v = ctx->piece_layout;
     else if (k == "triage") *v = ctx->triage;
+    else if (k == "piece_target") *v = ctx->piece_target;
+    else if (k == "piece_budget") *v = ctx->piece_budget;
+    else if (k == "mg_cost") *v = ctx->mg_cost;
+    else if (k == "mg_profile") *v = ctx->mg_profile;
     else if (k == "park_lanes") *v = ctx->park_lanes;
     else if (k == "stat_parked" || k == "stat_pieces") {
         unsigned c[2] = {0, 0};
@@ -225,8 +253,34 @@ extern "C" int jsx_ctx_get_option(jsx_ctx *ctx, const char *name, double *v) {
             HIPCHECK(ctx, hipMemcpy(c, ctx->split.n_long, 2 * sizeof(unsigned), hipMemcpyDeviceToHost));
         }
         unsigned parked = c[0] < (unsigned)ctx->split.cap ? c[0] : (unsigned)ctx->split.cap;
-        *v = k == "stat_parked" ? (double)parked : (double)c[1];
-    } else if (k == "stat_rhs_batch" || k == "stat_rhs_pieces" || k == "stat_trips" || k == "stat_trips_uniform") {
+        unsigned pieces = c[1] < (unsigned)ctx->split.piece_cap ? c[1] : (unsigned)ctx->split.piece_cap;
+        *v = k == "stat_parked" ? (double)parked : (double)pieces;
+    } else if (k == "stat_parked_exp") {
+        unsigned c = 0;
+        if (ctx->split_mem) {
+            HIPCHECK(ctx, hipDeviceSynchronize());
+            HIPCHECK(ctx, hipMemcpy(&c, ctx->split.n_mg, sizeof c, hipMemcpyDeviceToHost));
+        }
+        *v = c;
+    } else if (k == "stat_piece_steps_max" || k == "stat_piece_steps_mean") {
+        /* step attempts (right-hand sides) of the last launch's pieces: the
+           longest piece bounds the piece kernel */
+        unsigned np = 0;
+        double mx = 0.0, sum = 0.0;
+        if (ctx->split_mem) {
+            HIPCHECK(ctx, hipDeviceSynchronize());
+            HIPCHECK(ctx, hipMemcpy(&np, ctx->split.n_pieces, sizeof np, hipMemcpyDeviceToHost));
+            if (np > (unsigned)ctx->split.piece_cap) np = (unsigned)ctx->split.piece_cap;
+            std::vector<PieceOut> po(np);
+            if (np) HIPCHECK(ctx, hipMemcpy(po.data(), ctx->split.pout, sizeof(PieceOut) * np, hipMemcpyDeviceToHost));
+            for (unsigned i = 0; i < np; ++i) {
+                mx = po[i].n_rhs > mx ? po[i].n_rhs : mx;
+                sum += po[i].n_rhs;
+            }
+        }
+        *v = k == "stat_piece_steps_max" ? mx : (np ? sum / np : 0.0);
+    } else if (k == "stat_rhs_batch" || k == "stat_rhs_pieces" || k == "stat_trips" || k == "stat_trips_uniform" ||
+               k == "stat_rhs_exp") {
         unsigned long long r = 0;
         HIPCHECK(ctx, hipDeviceSynchronize());
         if (k == "stat_rhs_batch" && ctx->d_queues)
@@ -237,13 +291,22 @@ extern "C" int jsx_ctx_get_option(jsx_ctx *ctx, const char *name, double *v) {
             HIPCHECK(ctx, hipMemcpy(&r, &ctx->d_queues->trips_uniform, sizeof r, hipMemcpyDeviceToHost));
         if (k == "stat_rhs_pieces" && ctx->split_mem)
             HIPCHECK(ctx, hipMemcpy(&r, ctx->split.rhs_pieces, sizeof r, hipMemcpyDeviceToHost));
+        if (k == "stat_rhs_exp" && ctx->split_mem)
+            HIPCHECK(ctx, hipMemcpy(&r, ctx->split.rhs_mg, sizeof r, hipMemcpyDeviceToHost));
         *v = (double)r;
     } else if (k == "stat_launches") {
         *v = ctx->ev_n;
-    } else if (k == "stat_batch_ms" || k == "stat_piece_ms" || k == "stat_split_ms" || k == "stat_triage_ms") {
-        /* mean over the launches recorded since timing was enabled */
-        int a = k == "stat_batch_ms" ? 0 : (k == "stat_piece_ms" ? 2 : (k == "stat_triage_ms" ? 5 : 1));
-        int b = k == "stat_batch_ms" ? 1 : (k == "stat_piece_ms" ? 3 : (k == "stat_triage_ms" ? 0 : 4));
+    } else if (k == "stat_batch_ms" || k == "stat_piece_ms" || k == "stat_split_ms" || k == "stat_triage_ms" ||
+               k == "stat_piece_dp_ms" || k == "stat_piece_exp_ms") {
+        /* mean over the launches recorded since timing was enabled; events:
+           5 triage 0 batch 1 planner 2 Dormand-Prince pieces 6 exponential
+           pieces 3 combine 4 */
+        int a = 1, b = 4;
+        if (k == "stat_batch_ms") { a = 0; b = 1; }
+        else if (k == "stat_piece_ms") { a = 2; b = 3; }
+        else if (k == "stat_triage_ms") { a = 5; b = 0; }
+        else if (k == "stat_piece_dp_ms") { a = 2; b = 6; }
+        else if (k == "stat_piece_exp_ms") { a = 6; b = 3; }
         double sum = 0.0;
         int n = ctx->ev_n < jsx_ctx::kEvMax ? ctx->ev_n : jsx_ctx::kEvMax;
         for (int i = 0; i < n; ++i) {
@@ -557,14 +620,20 @@ struct BatchItem {
     int64_t npix;
 };
 
-/* pieces the planner aims at: about two per piece worker the piece kernel
-   keeps resident (5-lane groups at two wavefronts per SIMD, or single lanes
-   at one wavefront per SIMD) */
+/* pieces the planner aims at.  Dormand-Prince pieces: about two per piece
+   worker the piece kernel keeps resident (5-lane groups at two wavefronts per
+   SIMD, or single lanes at one wavefront per SIMD).  Exponential pieces: a
+   quarter of the resident lanes in the first round ("piece_target"
+   overrides), the budgeted rounds cut the costly ones further. */
 static void split_target(jsx_ctx *ctx) {
     SplitWS &w = ctx->split;
-    long t = ctx->piece_layout ? (long)ctx->n_cu * 4 * 64 * 2
-                               : (long)ctx->n_cu * ctx->persist_blocks_per_cu * 4 * (64 / JSX_SPLIT_GROUP) * 2;
+    long t = ctx->piece_layout != 0 ? (long)ctx->n_cu * 4 * 64 * 2
+                                    : (long)ctx->n_cu * ctx->persist_blocks_per_cu * 4 * (64 / JSX_SPLIT_GROUP) * 2;
+    long tm = (long)ctx->n_cu * 64;
+    if (ctx->piece_target > 0) tm = ctx->piece_target;
     w.target_pieces = (int)(t < w.piece_cap / 2 ? t : w.piece_cap / 2);
+    w.target_mg = (int)(tm < w.piece_cap / 2 ? tm : w.piece_cap / 2);
+    w.layout = ctx->piece_layout;
 }
 
 /* one allocation for the split path's tables (kept for the context's life) */
@@ -573,7 +642,7 @@ static int split_workspace(jsx_ctx *ctx) {
     const int cap = 65536, piece_cap = 1 << 19;
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
-    size_t o_cnt = take(4 * sizeof(unsigned) + sizeof(unsigned long long));
+    size_t o_cnt = take(8 * sizeof(unsigned) + 2 * sizeof(unsigned long long));
     size_t o_rays = take(sizeof(LongRay) * cap);
     size_t o_prof = take(sizeof(float) * (size_t)cap * JSX_SPLIT_NSAMP);
     size_t o_pcs = take(sizeof(Piece) * piece_cap);
@@ -584,7 +653,12 @@ static int split_workspace(jsx_ctx *ctx) {
     w.n_long = (unsigned *)(b + o_cnt);
     w.n_pieces = w.n_long + 1;
     w.piece_head = w.n_long + 2;
-    w.rhs_pieces = (unsigned long long *)(b + o_cnt + 4 * sizeof(unsigned));
+    w.round_end = w.n_long + 3;
+    w.dp_end = w.n_long + 4;
+    w.mg_head = w.n_long + 5;
+    w.n_mg = w.n_long + 6;
+    w.rhs_pieces = (unsigned long long *)(b + o_cnt + 8 * sizeof(unsigned));
+    w.rhs_mg = w.rhs_pieces + 1;
     w.rays = (LongRay *)(b + o_rays);
     w.cap = cap;
     w.prof = (float *)(b + o_prof);
@@ -594,6 +668,8 @@ static int split_workspace(jsx_ctx *ctx) {
     split_target(ctx);
     w.w_min = ctx->split_w_min;
     w.floor_max = ctx->split_floor_max;
+    w.mg_cost = ctx->mg_cost;
+    w.mg_profile = ctx->mg_profile;
     return JSX_OK;
 }
 
@@ -649,7 +725,7 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
     if (ctx->timing && ctx->ev_n < jsx_ctx::kEvMax) {
         if ((int)ctx->evs.size() <= ctx->ev_n) {
             jsx_ctx::EvSet es;
-            for (int k = 0; k < 6; ++k) HIPCHECK(ctx, hipEventCreate(&es.e[k]));
+            for (int k = 0; k < 7; ++k) HIPCHECK(ctx, hipEventCreate(&es.e[k]));
             ctx->evs.push_back(es);
         }
         ev = &ctx->evs[ctx->ev_n++];
@@ -703,14 +779,24 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
             hipLaunchKernelGGL(jsx_k_split_alloc, dim3(1), dim3(1024), 0, st, ws);
             hipLaunchKernelGGL(jsx_k_split_cut, dim3(cap), dim3(256), 0, st, ws);
             mark(2);
-            if (ctx->piece_layout)
-                hipLaunchKernelGGL(jsx_k_split_piece20, dim3(ctx->n_cu * 4), dim3(256), 0, st, R, ws);
-            else
+            if (ctx->piece_layout == 0)
                 hipLaunchKernelGGL(jsx_k_split_piece, dim3(blocks), dim3(256), 0, st, R, ws);
+            else if (ctx->piece_layout != 2)
+                hipLaunchKernelGGL(jsx_k_split_piece20, dim3(ctx->n_cu * 4), dim3(256), 0, st, R, ws);
+            mark(6);
+            if (ctx->piece_layout >= 2) {
+                const int rounds = ctx->piece_budget ? 4 : 1;
+                for (int r = 0; r < rounds; ++r) {
+                    if (r) hipLaunchKernelGGL(jsx_k_split_round, dim3(1), dim3(64), 0, st, ws);
+                    hipLaunchKernelGGL(jsx_k_split_magnus, dim3(ctx->n_cu * 8), dim3(256), 0, st, R, ws,
+                                       r + 1 < rounds ? ctx->piece_budget : 0u);
+                }
+            }
             mark(3);
             hipLaunchKernelGGL(jsx_k_split_combine, dim3((cap + 255) / 256), dim3(256), 0, st, ctx->d_outs, ws);
         } else {
             mark(2);
+            mark(6);
             mark(3);
         }
         mark(4);
@@ -733,6 +819,7 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
                                total, ctx->d_queues, ctx->queue_block, order, ctx->steal, ws, 0u, live, park_lanes);
         mark(1);
         mark(2);
+        mark(6);
         mark(3);
         mark(4);
     }

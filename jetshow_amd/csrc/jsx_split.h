@@ -26,18 +26,28 @@
  *     (so every piece costs about the same), prefix offsets;
  *  4. jsx_k_split_cut: piece boundaries at equal-w quantiles of each of the
  *     ray's (one or two) remaining path intervals;
- *  5. jsx_k_split_piece: groups of 5 lanes (particular solution and the four
- *     propagator columns) integrate one piece each, with one shared step
- *     sequence (the group's largest error norm decides);
+ *  5. pieces, by Dormand-Prince or exponential propagators (option
+ *     "piece_layout"; with layout 3 the planner picks per ray from two cost
+ *     estimates sampled along the rest of the ray, see jsx_k_split_sample):
+ *     jsx_k_split_piece: groups of 5 lanes (particular solution and the four
+ *     propagator columns) integrate one piece each by Dormand-Prince, with one
+ *     shared step sequence (the group's largest error norm decides);
+ *     jsx_k_split_piece20: one lane holds all 20 components, same steps;
+ *     jsx_k_split_magnus: one lane accumulates the piece's propagator from
+ *     exact exponentials of 4th-order Magnus steps (jsx_magnus.h);
  *  6. jsx_k_split_combine: one lane per parked ray composes its pieces in
  *     order and writes the pixel.
  */
 #ifndef JSX_SPLIT_H
 #define JSX_SPLIT_H
 
-#include "jsx_mixed.h"
+#include "jsx_magnus.h"
 
 #define JSX_SPLIT_NSAMP 512     /* coefficient samples per parked ray (8 per lane) */
+/* Relative tolerance of the pieces: at least 1e-6.  The pieces' error scales
+ * already hold I to ~2e-7 of FP64 on the rays that get split (Faraday-thick,
+ * stability-bound steps), where a tighter tolerance would only add steps. */
+#define JSX_SPLIT_REL_MIN 1e-6
 #define JSX_SPLIT_GROUP 5       /* lanes per piece                                 */
 
 struct LongRay {
@@ -51,11 +61,14 @@ struct LongRay {
     double x[4];                  /* Stokes vector at t                              */
     int piece0, npieces;          /* filled by the planner                           */
     float w_total, w_split;       /* cumulative stiffness: total, at the interval end */
+    float w_dp, w_mg;             /* cost estimates: Dormand-Prince, exponential      */
+    int method;                   /* piece method chosen by the planner (1 or 2)     */
 };
 
 struct Piece {
     int ray, interval;            /* interval 0: rest of the current segment; 1: segment 0 */
     double ta, tb, dt0;
+    int next, pad;                /* the ray's following piece (-1: last) */
 };
 
 struct PieceOut {
@@ -70,15 +83,37 @@ struct SplitWS {
     int cap;
     float *prof;                  /* cap x NSAMP cumulative stiffness               */
     unsigned *n_pieces;           /* pieces planned                                 */
-    unsigned *piece_head;         /* next piece to integrate                        */
-    unsigned long long *rhs_pieces; /* right-hand sides of the pieces (one lane each) */
+    unsigned *piece_head;         /* next Dormand-Prince piece to integrate         */
+    unsigned *dp_end;             /* Dormand-Prince pieces are [0, dp_end)           */
+    unsigned *mg_head;            /* next exponential piece to integrate            */
+    unsigned *round_end;          /* exponential pieces [.., round_end) are this round's */
+    unsigned *n_mg;               /* rays given exponential pieces                  */
+    unsigned long long *rhs_pieces; /* right-hand sides of the Dormand-Prince pieces  */
+    unsigned long long *rhs_mg;   /* coefficient evaluations of the exponential pieces */
     Piece *pieces;
     PieceOut *pout;
     int piece_cap;
-    int target_pieces;            /* concurrency the planner aims at                */
+    int target_pieces;            /* Dormand-Prince pieces the planner aims at      */
+    int target_mg;                /* exponential pieces of the first round          */
+    int layout;                   /* 0/1: Dormand-Prince only; 2: exponential only;
+                                     3: per ray by its rotation                       */
+    float mg_cost;                /* layout 3: cost ratio below which a ray's pieces
+                                     are exponential                                 */
+    int mg_profile;               /* cuts of exponential rays: 0 stiffness, 1 their
+                                     own step estimate                               */
     float w_min;                  /* smallest stiffness per piece worth a split     */
     float floor_max;              /* cap of the column error-scale estimate (1: off) */
 };
+
+/* sum of a 64-bit value over the wavefront, returned in every lane */
+__device__ inline unsigned long long d_wave_sum_u64(unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned lo = __shfl_xor((unsigned)(v & 0xffffffffull), off);
+        unsigned hi = __shfl_xor((unsigned)(v >> 32), off);
+        v += ((unsigned long long)hi << 32) | lo;
+    }
+    return v;
+}
 
 /* ---- 1. parking (called by the batch kernel for PolMixed<4>) ---- */
 __device__ inline bool split_should_park(const MLane<4> &L, unsigned defer_rhs) {
@@ -129,6 +164,65 @@ __device__ inline float split_stiffness(const MCoef &k) {
 }
 
 /* ---- 2. sampling: one wavefront per parked ray ---- */
+/* Exponential steps a sample interval of length du would take: a 4th-order
+ * Magnus step's error is led by its commutator terms, sqrt(3)/12 h^2 times
+ * ([G2, G1], A2 s1 - A1 s2) for the generators and sources at its Gauss
+ * points, so over du the step count scales as the cube root of that term for
+ * the interval's end samples over the tolerance (sources relative to sP). */
+__device__ inline float split_mg_density(const MCoef &a, const MCoef &b, double du, double rel, double sP) {
+    JSX_MIXED_FMA
+    MgGen ga, gb, cm;
+    mg_gen(a, ga);
+    mg_gen(b, gb);
+    mg_comm(gb, ga, cm);
+    double e = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        const double x = fabs(cm.e[i]), y = fabs(cm.r[i]);
+        e = x > e ? x : e;
+        e = y > e ? y : e;
+    }
+    if (sP > 0.0) {
+        const double sa[4] = {(double)a.eI, (double)a.eQ, (double)a.eU, (double)a.eV};
+        const double sb[4] = {(double)b.eI, (double)b.eQ, (double)b.eU, (double)b.eV};
+        double t1[4], t2[4];
+        mg_apply(gb, sa, t1);
+        mg_apply(ga, sb, t2);
+        for (int i = 0; i < 4; ++i) {
+            const double d = fabs((-(double)b.kI * sa[i] - t1[i]) - (-(double)a.kI * sb[i] - t2[i])) / sP;
+            e = d > e ? d : e;
+        }
+    }
+    const double err = 0.14433756729740643 * du * du * e / rel;
+    /* With n steps over du, each step is damped by its own optical depth
+     * (errors shrink by e^-mu) and, when it spans several rotations, the
+     * commutator term averages over them (~1/(rotation per step)); one
+     * fixed-point pass from the undamped count. */
+    const double n0 = cbrt(err);
+    double n1 = n0;
+    if (n0 > 0.0) {
+        const double tau = 0.5 * ((double)a.kI + (double)b.kI) * du;
+        const double rot = 0.5 * (fabs((double)a.kF) + fabs((double)b.kF) + fabs((double)a.kC) + fabs((double)b.kC) +
+                                  fabs((double)a.hQ) + fabs((double)b.hQ)) * du;
+        const double mu = tau / n0, bs = rot / n0;
+        const double damp = exp(-(mu < 50.0 ? mu : 50.0)) / (bs > 1.0 ? bs : 1.0);
+        n1 = cbrt(err * damp);
+    }
+    const float n = (float)n1;
+    return (n == n && n < 1e30f) ? n : 1e30f;
+}
+
+/* ---- 2. sampling and method: one wavefront per parked ray ----
+ * Two cost densities per sample: the Dormand-Prince stiffness (step count
+ * bound by rotation and absorption) and the exponential steps
+ * (split_mg_density); the profile of the chosen method's density places the
+ * cuts.  Layout 3 chooses per ray.  Dormand-Prince needs about 7 right-hand
+ * sides per radian of birefringent rotation (k_F, k_C, h_Q) and 2.6 per unit
+ * of absorption depth (k_I + |eta|) on the rays that get split (CPU lane runs
+ * of the bench images); an exponential step takes 2 and costs about 1.6
+ * Dormand-Prince piece right-hand sides.  Rotation-dominated rays (rotation
+ * >= 4 x depth, where the sampled estimate of exponential steps is
+ * pessimistic by up to 10x: the steps average over many rotations) always
+ * take exponential pieces; the others when 2 w_mg < mg_cost (7 rot + 2.6 tau). */
 __global__ __launch_bounds__(256) void jsx_k_split_sample(const DParamsM *__restrict__ reqs, SplitWS ws) {
     const int lane = __lane_id();
     const unsigned r = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
@@ -140,11 +234,14 @@ __global__ __launch_bounds__(256) void jsx_k_split_sample(const DParamsM *__rest
     const double total = (R.L - R.t) + (R.seg ? R.L0 : 0.0);
     const int per = JSX_SPLIT_NSAMP / 64;
     const double du = total / JSX_SPLIT_NSAMP;
-    float w[JSX_SPLIT_NSAMP / 64];
-    float acc = 0.f;
+    const double rel = q->c.st_rel > JSX_SPLIT_REL_MIN ? q->c.st_rel : JSX_SPLIT_REL_MIN;
+    const double sP = fabs(R.x[0]);
+    float w[JSX_SPLIT_NSAMP / 64], wm[JSX_SPLIT_NSAMP / 64];
+    float acc = 0.f, accm = 0.f, rot = 0.f, tau = 0.f;
     /* floor: a piece's step count never drops below what its length needs when
        the coefficients vanish (~16 steps over the whole remaining path) */
     const float floor_w = 16.0f / JSX_SPLIT_NSAMP;
+    MCoef prev;
 #pragma unroll
     for (int j = 0; j < per; ++j) {
         int kk = lane * per + j;
@@ -156,6 +253,40 @@ __global__ __launch_bounds__(256) void jsx_k_split_sample(const DParamsM *__rest
         if (!(c == c) || c > 1e30f) c = 1e30f;
         acc += c + floor_w;
         w[j] = acc;
+        const float rr = (fabsf(k.kF) + fabsf(k.kC) + fabsf(k.hQ)) * (float)du;
+        const float tt = (fabsf(k.kI) + __builtin_sqrtf(k.kQ * k.kQ + k.kU * k.kU + k.kV * k.kV)) * (float)du;
+        rot += (rr == rr && rr < 1e30f) ? rr : 1e30f;
+        tau += (tt == tt && tt < 1e30f) ? tt : 1e30f;
+        if (j > 0) {
+            const float m = split_mg_density(prev, k, du, rel, sP) + floor_w;
+            if (j == 1) {
+                accm += m;          /* the lane's first interval: its neighbour's density */
+                wm[0] = accm;
+            }
+            accm += m;
+            wm[j] = accm;
+        }
+        prev = k;
+    }
+    /* the method: both totals over the wavefront */
+    float tw = acc, tm = accm;
+    for (int off = 32; off > 0; off >>= 1) {
+        tw += __shfl_xor(tw, off);
+        tm += __shfl_xor(tm, off);
+        rot += __shfl_xor(rot, off);
+        tau += __shfl_xor(tau, off);
+    }
+    const bool mg_better = rot >= 4.0f * tau || 2.0f * tm < ws.mg_cost * (7.0f * rot + 2.6f * tau);
+    const int method = ws.layout == 2 ? 2 : (ws.layout == 3 ? (mg_better ? 2 : 1) : 1);
+    if (lane == 0) {
+        R.method = method;
+        R.w_dp = tw;
+        R.w_mg = tm;
+    }
+    if (method == 2 && ws.mg_profile) {
+#pragma unroll
+        for (int j = 0; j < per; ++j) w[j] = wm[j];
+        acc = accm;
     }
     /* inclusive scan of the lane totals across the wavefront */
     float tot = acc;
@@ -187,36 +318,59 @@ __global__ __launch_bounds__(256) void jsx_k_split_sample(const DParamsM *__rest
 }
 
 /* ---- 3. piece counts (one workgroup) ---- */
+__device__ inline int split_count(const LongRay &R, float wp) {
+    float np = ceilf(R.w_total / wp);
+    int k = np < 1.f ? 1 : (np > 65536.f ? 65536 : (int)np);
+    if (R.seg) k = k < 2 ? 2 : k;
+    return k;
+}
+
+/* Pieces per ray in proportion to its stiffness, for the Dormand-Prince rays
+ * (aiming at target_pieces) and the exponential rays (target_mg) separately;
+ * Dormand-Prince pieces take slots [0, dp_end), exponential ones follow. */
 __global__ __launch_bounds__(1024) void jsx_k_split_alloc(SplitWS ws) {
-    __shared__ float s_sum[1024];
-    __shared__ float s_wp;
+    __shared__ float s_sum[2][1024];
+    __shared__ float s_wp[2];
     __shared__ int s_tot[1024];
+    __shared__ int s_base, s_nmg;
     const int tid = threadIdx.x;
     unsigned n = *ws.n_long;
     if (n > (unsigned)ws.cap) n = ws.cap;
-    float local = 0.f;
-    for (unsigned r = tid; r < n; r += blockDim.x) local += ws.rays[r].w_total;
-    s_sum[tid] = local;
+    if (tid == 0) s_nmg = 0;
+    __syncthreads();
+    float local[2] = {0.f, 0.f};
+    int nmg = 0;
+    for (unsigned r = tid; r < n; r += blockDim.x) {
+        const LongRay &R = ws.rays[r];
+        const int m = R.method;
+        local[m == 2] += R.w_total;
+        nmg += m == 2;
+    }
+    if (nmg) atomicAdd(&s_nmg, nmg);
+    s_sum[0][tid] = local[0];
+    s_sum[1][tid] = local[1];
     __syncthreads();
     for (int h = 512; h > 0; h >>= 1) {
-        if (tid < h) s_sum[tid] += s_sum[tid + h];
+        if (tid < h) {
+            s_sum[0][tid] += s_sum[0][tid + h];
+            s_sum[1][tid] += s_sum[1][tid + h];
+        }
         __syncthreads();
     }
     if (tid == 0) {
-        float wp = s_sum[0] / (float)ws.target_pieces;
-        s_wp = wp > ws.w_min ? wp : ws.w_min;
+        for (int g = 0; g < 2; ++g) {
+            const int tgt = g ? ws.target_mg : ws.target_pieces;
+            float wp = s_sum[g][0] / (float)(tgt > 0 ? tgt : 1);
+            s_wp[g] = wp > ws.w_min ? wp : ws.w_min;
+        }
     }
     __syncthreads();
     /* grow the per-piece stiffness until the pieces fit the table */
     for (int iter = 0; iter < 32; ++iter) {
-        float wp = s_wp;
         int cnt = 0;
         for (unsigned r = tid; r < n; r += blockDim.x) {
             const LongRay &R = ws.rays[r];
-            float np = ceilf(R.w_total / wp);
-            int k = np < 1.f ? 1 : (np > 65536.f ? 65536 : (int)np);
-            if (R.seg) k = k < 2 ? 2 : k;
-            cnt += k;
+            cnt += split_count(R, s_wp[R.method == 2]);
         }
         s_tot[tid] = cnt;
         __syncthreads();
@@ -227,42 +381,52 @@ __global__ __launch_bounds__(1024) void jsx_k_split_alloc(SplitWS ws) {
         int total = s_tot[0];
         __syncthreads();
         if (total <= ws.piece_cap) break;
-        if (tid == 0) s_wp = wp * 2.0f;
+        if (tid == 0) {
+            s_wp[0] *= 2.0f;
+            s_wp[1] *= 2.0f;
+        }
         __syncthreads();
     }
-    /* exclusive prefix of the piece counts in ray order (serial over chunks) */
-    const float wp = s_wp;
-    __shared__ int s_base;
+    /* exclusive prefix of the piece counts in ray order, Dormand-Prince rays
+       first, then exponential rays (serial over chunks) */
     if (tid == 0) s_base = 0;
     __syncthreads();
-    for (unsigned r0 = 0; r0 < n; r0 += blockDim.x) {
-        unsigned r = r0 + tid;
-        int k = 0;
-        if (r < n) {
-            const LongRay &R = ws.rays[r];
-            float np = ceilf(R.w_total / wp);
-            k = np < 1.f ? 1 : (np > 65536.f ? 65536 : (int)np);
-            if (R.seg) k = k < 2 ? 2 : k;
-        }
-        s_tot[tid] = k;
-        __syncthreads();
-        for (int off = 1; off < (int)blockDim.x; off <<= 1) {
-            int v = tid >= off ? s_tot[tid - off] : 0;
+    for (int g = 0; g < 2; ++g) {
+        if (g == 1 && tid == 0) *ws.dp_end = (unsigned)s_base;
+        for (unsigned r0 = 0; r0 < n; r0 += blockDim.x) {
+            unsigned r = r0 + tid;
+            int k = 0;
+            if (r < n) {
+                const LongRay &R = ws.rays[r];
+                if ((R.method == 2) == (g == 1)) k = split_count(R, s_wp[g]);
+            }
+            s_tot[tid] = k;
             __syncthreads();
-            s_tot[tid] += v;
+            for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+                int v = tid >= off ? s_tot[tid - off] : 0;
+                __syncthreads();
+                s_tot[tid] += v;
+                __syncthreads();
+            }
+            if (r < n && k > 0) {
+                ws.rays[r].piece0 = s_base + s_tot[tid] - k;
+                ws.rays[r].npieces = k;
+            }
+            __syncthreads();
+            if (tid == blockDim.x - 1) s_base += s_tot[tid];
             __syncthreads();
         }
-        if (r < n) {
-            ws.rays[r].piece0 = s_base + s_tot[tid] - k;
-            ws.rays[r].npieces = k;
-        }
-        __syncthreads();
-        if (tid == blockDim.x - 1) s_base += s_tot[tid];
-        __syncthreads();
     }
     if (tid == 0) {
         int tot = s_base;
-        *ws.n_pieces = (unsigned)(tot < ws.piece_cap ? tot : ws.piece_cap);
+        const unsigned np = (unsigned)(tot < ws.piece_cap ? tot : ws.piece_cap);
+        unsigned de = *ws.dp_end;
+        de = de < np ? de : np;
+        *ws.dp_end = de;
+        *ws.n_pieces = np;
+        *ws.mg_head = de;
+        *ws.round_end = np;
+        *ws.n_mg = (unsigned)s_nmg;
     }
 }
 
@@ -334,16 +498,14 @@ __global__ __launch_bounds__(256) void jsx_k_split_cut(SplitWS ws) {
            fraction of their length and adapt */
         pc.dt0 = (k == 0) ? R.dt : (pc.tb - pc.ta) / 100.0;
         if (!(pc.dt0 > 0.0)) pc.dt0 = (pc.tb - pc.ta) > 0.0 ? (pc.tb - pc.ta) : 1e-30;
+        pc.next = k + 1 < np ? p0 + k + 1 : -1;
+        pc.pad = 0;
         ws.pieces[p0 + k] = pc;
         ws.pout[p0 + k].flags = 0u;
     }
 }
 
 /* ---- 5. pieces: groups of 5 lanes share one step sequence ---- */
-/* Relative tolerance of the pieces: at least 1e-6.  The pieces' error scales
- * already hold I to ~2e-7 of FP64 on the rays that get split (Faraday-thick,
- * stability-bound steps), where a tighter tolerance would only add steps. */
-#define JSX_SPLIT_REL_MIN 1e-6
 
 struct SplitGroupMax {
     int lead;
@@ -400,7 +562,7 @@ __global__ __launch_bounds__(256) void jsx_k_split_piece(const DParamsM *__restr
     const int role = lane - grp * JSX_SPLIT_GROUP;
     const int lead = grp * JSX_SPLIT_GROUP;
     const bool in_group = lane < (64 / JSX_SPLIT_GROUP) * JSX_SPLIT_GROUP;
-    const unsigned npieces = *ws.n_pieces;
+    const unsigned npieces = *ws.dp_end;
     unsigned nrays = *ws.n_long;
     if (nrays > (unsigned)ws.cap) nrays = ws.cap;
     MLane<4> L;
@@ -650,7 +812,7 @@ __device__ __forceinline__ void p20_consume(MLane<JSX_PIECE_NS> &L, const DSolve
 
 __global__ __launch_bounds__(256) void jsx_k_split_piece20(const DParamsM *__restrict__ reqs, SplitWS ws) {
     const int lane = __lane_id();
-    const unsigned npieces = *ws.n_pieces;
+    const unsigned npieces = *ws.dp_end;
     unsigned nrays = *ws.n_long;
     if (nrays > (unsigned)ws.cap) nrays = ws.cap;
     MLane<JSX_PIECE_NS> L;
@@ -739,13 +901,223 @@ __global__ __launch_bounds__(256) void jsx_k_split_piece20(const DParamsM *__res
             have = 0;
         }
     }
-    unsigned long long tot = rhs_acc;
-    for (int off = 32; off > 0; off >>= 1) {
-        unsigned lo = __shfl_xor((unsigned)(tot & 0xffffffffull), off);
-        unsigned hi = __shfl_xor((unsigned)(tot >> 32), off);
-        tot += ((unsigned long long)hi << 32) | lo;
-    }
+    unsigned long long tot = d_wave_sum_u64(rhs_acc);
     if (lane == 0 && tot) atomicAdd(ws.rhs_pieces, tot);
+}
+
+/* ---- 5c. pieces by exponential propagators (one lane per piece) ---- */
+struct MgLane {
+    double U[20];                 /* Phi column-major (U[4 j + i]), then P (U[16 + i]) */
+    double t, tb, h, p_floor;
+    double o[3];                  /* origin of the piece's pass [cm] */
+    uint32_t flags, n_rhs, n_steps, tries;
+    int done;
+};
+
+__device__ inline void mg_lane_begin(MgLane &L, const LongRay &R, const Piece &pc) {
+#pragma unroll
+    for (int i = 0; i < 20; ++i) L.U[i] = (i < 16 && (i >> 2) == (i & 3)) ? 1.0 : 0.0;
+    for (int i = 0; i < 3; ++i) L.o[i] = pc.interval ? R.s0out[i] : R.cout[i];
+    L.t = pc.ta;
+    L.tb = pc.tb;
+    const double len = pc.tb - pc.ta;
+    L.h = pc.dt0 > 0.0625 * len ? pc.dt0 : 0.0625 * len;
+    L.p_floor = fabs(R.x[0]);
+    L.flags = 0; L.n_rhs = 0; L.n_steps = 0; L.tries = 0;
+    L.done = !(pc.tb > pc.ta);
+}
+
+/* one attempted step: accept (propagator updated, t advanced) or shrink h */
+__device__ inline void mg_lane_step(MgLane &L, const DModel &m, const DImage &im, const MConsts &c, const DSolver &S) {
+    JSX_MIXED_FMA
+    double h = L.h;
+    int last = 0;
+    if (!(L.t + h < L.tb)) { h = L.tb - L.t; last = 1; }
+    MCoef A[2];
+#pragma unroll
+    for (int n = 0; n < 2; ++n) {
+        const double tg = L.t + (n ? 0.78867513459481287 : 0.21132486540518713) * h;
+        double p[3];
+        for (int i = 0; i < 3; ++i) p[i] = L.o[i] * c.inv_pc - tg * im.dir[i];
+        m_coefficients<1>(m, c, p, A[n]);
+    }
+    L.n_rhs += 2;
+    const double rel = c.st_rel > JSX_SPLIT_REL_MIN ? c.st_rel : JSX_SPLIT_REL_MIN;
+    double E[16], w[4], err;
+    const int ok = mg_step(A[0], A[1], h, rel, L.p_floor, L.U + 16, E, w, &err);
+    if (ok && err <= 1.0) {
+        double nU[20];
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const double *u = L.U + 4 * j;
+                double acc = E[i] * u[0] + E[4 + i] * u[1] + E[8 + i] * u[2] + E[12 + i] * u[3];
+                nU[4 * j + i] = j == 4 ? acc + w[i] : acc;
+            }
+#pragma unroll
+        for (int i = 0; i < 20; ++i) L.U[i] = nU[i];
+        L.t = last ? L.tb : L.t + h;
+        L.n_steps++;
+        L.tries = 0;
+        float fac = 2.0f;
+        if (err > 0.0) {
+            fac = 0.9f * __builtin_amdgcn_exp2f(-(1.0f / 3.0f) * __builtin_amdgcn_logf((float)err));
+            fac = fac < 2.0f ? fac : 2.0f;
+        }
+        L.h = h * (double)fac;
+        if (last) L.done = 1;
+    } else {
+        float fac = 0.2f;
+        if (ok && err == err) {
+            fac = 0.9f * __builtin_amdgcn_exp2f(-(1.0f / 3.0f) * __builtin_amdgcn_logf((float)err));
+            fac = fac > 0.2f ? fac : 0.2f;
+        }
+        L.h = h * (double)fac;
+        if (++L.tries > (uint32_t)S.max_fail) {
+            L.flags |= JSX_PIX_FAIL_MAXTRIALS;
+            L.done = 1;
+        }
+    }
+}
+
+/* A piece that has used `budget` right-hand sides (0: no budget) and would
+ * need more than `budget` again at its current step size stops where it is
+ * and hands the rest of its interval, cut into up to JSX_MG_SPAWN equal parts
+ * of about `budget` right-hand sides each, to the next round as new pieces
+ * chained after it; the last round runs without a budget.  Cost per unit length varies by orders of magnitude along
+ * a Faraday-thick ray (most steps sit in the first per-mille of the path past
+ * the apex), so the planner's sampled cuts leave some pieces far longer than
+ * the rest; the rounds bound the longest piece a lane holds. */
+#define JSX_MG_SPAWN 8
+
+/* write the continuation pieces at slots base.. and chain them after pid */
+__device__ inline void mg_spawn_at(const SplitWS &ws, int pid, const MgLane &L, unsigned base, int m) {
+    const Piece self = ws.pieces[pid];
+    const double len = (L.tb - L.t) / (double)m;
+    for (int i = 0; i < m; ++i) {
+        Piece c;
+        c.ray = self.ray;
+        c.interval = self.interval;
+        c.ta = L.t + (double)i * len;
+        c.tb = i + 1 == m ? L.tb : L.t + (double)(i + 1) * len;
+        c.dt0 = L.h;
+        c.next = i + 1 == m ? self.next : (int)base + i + 1;
+        c.pad = 0;
+        ws.pieces[base + i] = c;
+        ws.pout[base + i].flags = 0u;
+    }
+    ws.pieces[pid].next = (int)base;
+}
+
+/* slots reserved past the table's end by a failed reservation: the part
+ * inside the table is marked empty (no ray), so a later round skips it */
+__device__ inline void mg_spawn_void(const SplitWS &ws, unsigned base, int m) {
+    for (unsigned i = base; i < base + (unsigned)m && i < (unsigned)ws.piece_cap; ++i) ws.pieces[i].ray = -1;
+}
+
+/* continuation parts a piece asks for: the right-hand sides it still needs at
+ * its current step size, in parts of about `budget` (0 or 1: go on alone) */
+__device__ inline int mg_spawn_parts(const MgLane &L, unsigned budget) {
+    const double est = 2.0 * (L.tb - L.t) / L.h;
+    if (!(est > 2.0 * (double)budget)) return 0;
+    const double m = ceil(est / (double)budget);
+    return m < (double)JSX_MG_SPAWN ? (int)m : JSX_MG_SPAWN;
+}
+
+__global__ __launch_bounds__(256) void jsx_k_split_magnus(const DParamsM *__restrict__ reqs, SplitWS ws,
+                                                          unsigned budget) {
+    const int lane = __lane_id();
+    const unsigned npieces = *ws.round_end;
+    unsigned nrays = *ws.n_long;
+    unsigned *head = ws.mg_head;
+    if (nrays > (unsigned)ws.cap) nrays = ws.cap;
+    MgLane L;
+    int have = 0, exhausted = 0, slot = 0, pid = 0, spawn_ok = 0;
+    unsigned long long rhs_acc = 0;
+    for (;;) {
+        int want = !have && !exhausted;
+        unsigned long long mask = __ballot(want);
+        if (mask) {
+            int cnt = __popcll(mask);
+            int leader = __ffsll((long long)mask) - 1;
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(head, (unsigned)cnt);
+            base = __shfl(base, leader);
+            unsigned long long below = lane ? (mask & ((1ull << lane) - 1ull)) : 0ull;
+            unsigned id = base + (unsigned)__popcll(below);
+            if (want) {
+                if (id >= npieces) {
+                    exhausted = 1;
+                } else if ((unsigned)ws.pieces[id].ray < nrays) {
+                    pid = (int)id;
+                    const Piece pc = ws.pieces[pid];
+                    const LongRay &R = ws.rays[pc.ray];
+                    slot = R.slot;
+                    mg_lane_begin(L, R, pc);
+                    have = L.done ? 2 : 1;
+                    spawn_ok = budget != 0u;
+                }
+            }
+        }
+        if (!__any(have) && __all(exhausted)) break;
+        if (have == 1) {
+            const int s0 = __builtin_amdgcn_readfirstlane(slot);
+            if (__all(slot == s0)) {
+                const JSX_CONST DParamsM *q = (const JSX_CONST DParamsM *)(reqs + s0);
+                JSX_OPAQUE_S(q);
+                mg_lane_step(L, q->m, q->im, q->c, q->S);
+            } else {
+                const DParamsM *q = reqs + slot;
+                JSX_OPAQUE_V(q);
+                mg_lane_step(L, q->m, q->im, q->c, q->S);
+            }
+            if (L.done) have = 2;
+        }
+        /* continuation pieces: one reservation per wavefront */
+        {
+            const int m = (have == 1 && spawn_ok && L.n_rhs >= budget) ? mg_spawn_parts(L, budget) : 0;
+            const unsigned long long smask = __ballot(m > 0);
+            if (smask) {
+                /* inclusive scan of the part counts over the wavefront */
+                int incl = m;
+                for (int off = 1; off < 64; off <<= 1) {
+                    const int v = __shfl_up(incl, off);
+                    if (lane >= off) incl += v;
+                }
+                const int total = __shfl(incl, 63);
+                unsigned base = 0;
+                if (lane == 0) base = atomicAdd(ws.n_pieces, (unsigned)total);
+                base = __shfl(base, 0);
+                const unsigned mine = base + (unsigned)(incl - m);
+                if (m > 0) {
+                    if (mine + (unsigned)m <= (unsigned)ws.piece_cap) {
+                        mg_spawn_at(ws, pid, L, mine, m);
+                        L.tb = L.t;
+                        L.done = 1;
+                        have = 2;
+                    } else {
+                        mg_spawn_void(ws, mine, m);
+                        spawn_ok = 0;   /* table full: this piece runs to its end */
+                    }
+                }
+            }
+        }
+        if (have == 2) {
+            PieceOut &o = ws.pout[pid];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) o.P[i] = L.U[16 + i];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) o.Phi[i] = L.U[i];
+            o.flags = L.flags;
+            o.n_rhs = L.n_rhs;
+            o.n_steps = L.n_steps;
+            rhs_acc += L.n_rhs;
+            have = 0;
+        }
+    }
+    unsigned long long tot = d_wave_sum_u64(rhs_acc);
+    if (lane == 0 && tot) atomicAdd(ws.rhs_mg, tot);
 }
 
 /* ---- 6. compose pieces and write the pixel ---- */
@@ -760,8 +1132,9 @@ __global__ __launch_bounds__(256) void jsx_k_split_combine(const DOut *__restric
     if (R.piece0 + R.npieces > ws.piece_cap) {
         fl |= JSX_PIX_FAIL_MAXTRIALS;   /* planner overflow: cannot happen with the sizing in use */
     } else {
-        for (int k = 0; k < R.npieces; ++k) {
-            const PieceOut &o = ws.pout[R.piece0 + k];
+        int k = R.piece0;
+        for (int guard = 0; k >= 0 && guard < ws.piece_cap; ++guard, k = ws.pieces[k].next) {
+            const PieceOut &o = ws.pout[k];
             double T[4];
             for (int i = 0; i < 4; ++i)
                 T[i] = o.Phi[i] * S[0] + o.Phi[4 + i] * S[1] + o.Phi[8 + i] * S[2] + o.Phi[12 + i] * S[3] + o.P[i];
@@ -791,7 +1164,22 @@ __global__ void jsx_k_split_reset(SplitWS ws) {
         *ws.n_long = 0u;
         *ws.n_pieces = 0u;
         *ws.piece_head = 0u;
+        *ws.dp_end = 0u;
+        *ws.mg_head = 0u;
+        *ws.round_end = 0u;
+        *ws.n_mg = 0u;
         *ws.rhs_pieces = 0ull;
+        *ws.rhs_mg = 0ull;
+    }
+}
+
+/* next round of exponential pieces: the pieces spawned by the last one */
+__global__ void jsx_k_split_round(SplitWS ws) {
+    if (threadIdx.x == 0) {
+        const unsigned e = *ws.round_end;
+        const unsigned n = *ws.n_pieces;
+        *ws.mg_head = e;
+        *ws.round_end = n < (unsigned)ws.piece_cap ? n : (unsigned)ws.piece_cap;
     }
 }
 

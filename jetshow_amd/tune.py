@@ -6,8 +6,9 @@ i.e. waves per SIMD the launch asks for), ray ids per queue block
 global queue ("steal"), the right-hand-side budget after which a full-Stokes
 ray is split into time-parallel pieces ("defer_rhs"), how few lanes may still
 hold a ray before splitting starts ("park_lanes"), and the Stokes-state layout
-of a piece ("piece_layout": a 5-lane group with 4 components per lane, or one
-lane with all 20).
+of a piece and its method ("piece_layout": Dormand-Prince in one lane with all
+20 components, or the per-ray choice between that and exponential propagators)
+with the exponential pieces' per-round budget ("piece_budget").
 
 A sweep times each candidate on a workload (a list of (jet, image, params)
 sent through one batched launch), keeps the fastest, and caches it in a JSON
@@ -26,9 +27,10 @@ DEFAULT_SPACE = {
     "blocks_per_cu": [1, 2, 4, 8],
     "queue_block": [16, 64, 256],
     "steal": [0, 1],
-    "defer_rhs": [2048, 8192, 32768],
-    "piece_layout": [0, 1],
+    "defer_rhs": [1024, 4096, 16384],
+    "piece_layout": [1, 3],
     "park_lanes": [0, 8192],
+    "piece_budget": [64, 128],
 }
 CACHE_PATH = os.environ.get("JSX_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "jetshow_amd",
                                                             "tune.json"))

@@ -53,6 +53,7 @@ static inline void __syncthreads() {}
 template <class T, class U> static inline T atomicAdd(T *p, U v) { T o = *p; *p = o + (T)v; return o; }
 template <class T, class U> static inline T atomicSub(T *p, U v) { T o = *p; *p = o - (T)v; return o; }
 template <class T, class U> static inline T atomicOr(T *p, U v) { T o = *p; *p = o | (T)v; return o; }
+template <class T, class U> static inline T atomicCAS(T *p, U cmp, U v) { T o = *p; if (o == (T)cmp) *p = (T)v; return o; }
 #define __HIP_MEMORY_SCOPE_AGENT 3
 template <class T, class U> static inline void __hip_atomic_store(T *p, U v, int, int) { *p = (T)v; }
 template <class T> static inline T __hip_atomic_load(T *p, int, int) { return *p; }

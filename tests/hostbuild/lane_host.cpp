@@ -9,6 +9,8 @@
 
 namespace {
 
+std::vector<double> g_costs;   /* (Dormand-Prince, exponential) cost estimates of split rays */
+
 struct HostCtx {
     jsx_ctx *ctx = nullptr;
     HostCtx(int ngrids, const jsx_grid_desc *gd, const double *const *gv) {
@@ -121,9 +123,63 @@ void run_piece(const DParamsM &q, const LongRay &R, const Piece &pc, PieceOut &p
     po.n_steps = L[0].n_steps;
 }
 
+/* one piece by exponential propagators (the GPU's jsx_k_split_magnus lane) */
+void run_piece_magnus(const DParamsM &q, const LongRay &R, const Piece &pc, PieceOut &po) {
+    MgLane L;
+    mg_lane_begin(L, R, pc);
+    while (!L.done) mg_lane_step(L, q.m, q.im, q.c, q.S);
+    for (int i = 0; i < 4; ++i) po.P[i] = L.U[16 + i];
+    for (int i = 0; i < 16; ++i) po.Phi[i] = L.U[i];
+    po.flags = L.flags;
+    po.n_rhs = L.n_rhs;
+    po.n_steps = L.n_steps;
+}
+
+/* exponential pieces in budgeted rounds, as the launch sequence of
+ * jsx_k_split_magnus / jsx_k_split_round runs them (one lane after another) */
+void run_pieces_magnus_rounds(const DParamsM &q, const LongRay &R, SplitWS &ws, unsigned budget) {
+    unsigned head = 0, end = *ws.n_pieces;
+    const int rounds = budget ? 4 : 1;
+    for (int r = 0; r < rounds; ++r) {
+        const unsigned b = r + 1 < rounds ? budget : 0u;
+        for (unsigned id = head; id < end; ++id) {
+            if (ws.pieces[id].ray < 0) continue;
+            MgLane L;
+            mg_lane_begin(L, R, ws.pieces[id]);
+            int spawn_ok = b != 0u;
+            while (!L.done) {
+                mg_lane_step(L, q.m, q.im, q.c, q.S);
+                const int m = (!L.done && spawn_ok && L.n_rhs >= b) ? mg_spawn_parts(L, b) : 0;
+                if (m > 0) {
+                    const unsigned mine = atomicAdd(ws.n_pieces, (unsigned)m);
+                    if (mine + (unsigned)m <= (unsigned)ws.piece_cap) {
+                        mg_spawn_at(ws, (int)id, L, mine, m);
+                        L.tb = L.t;
+                        L.done = 1;
+                    } else {
+                        mg_spawn_void(ws, mine, m);
+                        spawn_ok = 0;
+                    }
+                }
+            }
+            PieceOut &po = ws.pout[id];
+            for (int i = 0; i < 4; ++i) po.P[i] = L.U[16 + i];
+            for (int i = 0; i < 16; ++i) po.Phi[i] = L.U[i];
+            po.flags = L.flags;
+            po.n_rhs = L.n_rhs;
+            po.n_steps = L.n_steps;
+        }
+        head = end;
+        end = *ws.n_pieces < (unsigned)ws.piece_cap ? *ws.n_pieces : (unsigned)ws.piece_cap;
+    }
+}
+
 /* the split pipeline for one ray, as the GPU kernels do it (sampling, piece
- * counts for a given per-piece stiffness, cuts at quantiles, pieces, combine) */
-void run_mixed_split(const DParamsM &q, int64_t pix, unsigned defer_rhs, float w_piece, Out1 &o, int *n_pieces) {
+ * counts for a given per-piece stiffness, cuts at quantiles, pieces, combine);
+ * method 0: Dormand-Prince pieces (5 lanes), 2: exponential propagators,
+ * 3: exponential propagators in budgeted rounds (64 right-hand sides) */
+void run_mixed_split(const DParamsM &q, int64_t pix, unsigned defer_rhs, float w_piece, int method, Out1 &o,
+                     int *n_pieces) {
     MLane<4> L;
     ml_begin(L, q.m, q.im, q.S, q.c, pix);
     *n_pieces = 0;
@@ -152,7 +208,9 @@ void run_mixed_split(const DParamsM &q, int64_t pix, unsigned defer_rhs, float w
     const double total = (R.L - R.t) + (R.seg ? R.L0 : 0.0);
     const double du = total / JSX_SPLIT_NSAMP;
     const float floor_w = 16.0f / JSX_SPLIT_NSAMP;
-    float acc = 0.f;
+    float acc = 0.f, accm = 0.f, acc_rot = 0.f, acc_tau = 0.f;
+    const double rel = q.c.st_rel > JSX_SPLIT_REL_MIN ? q.c.st_rel : JSX_SPLIT_REL_MIN;
+    MCoef prev;
     for (int kk = 0; kk < JSX_SPLIT_NSAMP; ++kk) {
         double p[3];
         split_point(R, q.im, q.c.inv_pc, (kk + 0.5) * du, p);
@@ -162,7 +220,15 @@ void run_mixed_split(const DParamsM &q, int64_t pix, unsigned defer_rhs, float w
         if (!(c == c) || c > 1e30f) c = 1e30f;
         acc += c + floor_w;
         prof[kk] = acc;
+        if (kk > 0) accm += split_mg_density(prev, k, du, rel, fabs(R.x[0])) + floor_w;
+        prev = k;
+        acc_tau += (fabsf(k.kI) + sqrtf(k.kQ * k.kQ + k.kU * k.kU + k.kV * k.kV)) * (float)du;
+        acc_rot += (fabsf(k.kF) + fabsf(k.kC) + fabsf(k.hQ)) * (float)du;
     }
+    g_costs.push_back(acc);
+    g_costs.push_back(accm);
+    g_costs.push_back(acc_rot);
+    g_costs.push_back(acc_tau);
     R.w_total = acc;
     if (R.seg) {
         int kb = (int)floor((R.L - R.t) / du);
@@ -176,11 +242,14 @@ void run_mixed_split(const DParamsM &q, int64_t pix, unsigned defer_rhs, float w
     if (R.seg) k = k < 2 ? 2 : k;
     R.piece0 = 0;
     R.npieces = k;
-    std::vector<Piece> pcs(k);
-    std::vector<PieceOut> pout(k);
+    const int cap = method == 3 ? k + 65536 : k;
+    std::vector<Piece> pcs(cap);
+    std::vector<PieceOut> pout(cap);
+    unsigned n_pcs = (unsigned)k;
     ws.pieces = pcs.data();
     ws.pout = pout.data();
-    ws.piece_cap = k;
+    ws.piece_cap = cap;
+    ws.n_pieces = &n_pcs;
     /* cut: same code as jsx_k_split_cut, one "thread" per piece */
     {
         const double restA = R.L - R.t;
@@ -221,9 +290,18 @@ void run_mixed_split(const DParamsM &q, int64_t pix, unsigned defer_rhs, float w
             if (pc.tb < pc.ta) pc.tb = pc.ta;
             pc.dt0 = (j == 0) ? R.dt : (pc.tb - pc.ta) / 100.0;
             if (!(pc.dt0 > 0.0)) pc.dt0 = (pc.tb - pc.ta) > 0.0 ? (pc.tb - pc.ta) : 1e-30;
+            pc.next = j + 1 < k ? j + 1 : -1;
+            pc.pad = 0;
         }
     }
-    for (int j = 0; j < k; ++j) run_piece(q, R, pcs[j], pout[j]);
+    if (method == 3) {
+        run_pieces_magnus_rounds(q, R, ws, 64u);
+    } else {
+        for (int j = 0; j < k; ++j) {
+            if (method == 2) run_piece_magnus(q, R, pcs[j], pout[j]);
+            else run_piece(q, R, pcs[j], pout[j]);
+        }
+    }
     /* combine: the GPU's jsx_k_split_combine for this one ray */
     DOut ov = o.view();
     DOut outs[1] = {ov};
@@ -232,7 +310,7 @@ void run_mixed_split(const DParamsM &q, int64_t pix, unsigned defer_rhs, float w
     unsigned one = 1;
     w2.n_long = &one;
     jsx_k_split_combine(outs, w2);
-    *n_pieces = k;
+    *n_pieces = (int)(n_pcs < (unsigned)cap ? n_pcs : (unsigned)cap);
 }
 
 int prepare(HostCtx &H, const jsx_jet_desc *jet, const jsx_image_desc *im, const jsx_solver_params *p, DParams &d,
@@ -251,23 +329,25 @@ int prepare(HostCtx &H, const jsx_jet_desc *jet, const jsx_image_desc *im, const
 /* Integrate the listed pixels one lane at a time with the kernels' own lane
  * code (FP64 or mixed, per params->precision).  defer_rhs > 0 (mixed full
  * Stokes only) sends rays past that many right-hand sides through the split
- * pipeline with w_piece stiffness per piece; n_pieces (optional) receives the
- * piece count per pixel (0 = not split). */
+ * pipeline with w_piece stiffness per piece and piece method `method` (0:
+ * Dormand-Prince, 2: exponential propagators); n_pieces (optional) receives
+ * the piece count per pixel (0 = not split). */
 extern "C" int lh_observe(const jsx_jet_desc *jet, const jsx_image_desc *im, const jsx_solver_params *p,
                           const int64_t *pix, int64_t n, int32_t ngrids, const jsx_grid_desc *gd,
-                          const double *const *gv, unsigned defer_rhs, float w_piece, const jsx_outputs *out,
-                          int32_t *n_pieces) {
+                          const double *const *gv, unsigned defer_rhs, float w_piece, int32_t method,
+                          const jsx_outputs *out, int32_t *n_pieces) {
     HostCtx H(ngrids, gd, gv);
     DParams d;
     DParamsM dm;
     int rc = prepare(H, jet, im, p, d, dm);
     if (rc) return rc;
+    g_costs.clear();
     const bool full = p->stokes == JSX_STOKES_FULL;
     for (int64_t i = 0; i < n; ++i) {
         Out1 o{};
         int np = 0;
         if (p->precision == JSX_PREC_MIXED) {
-            if (full && defer_rhs) run_mixed_split(dm, pix[i], defer_rhs, w_piece, o, &np);
+            if (full && defer_rhs) run_mixed_split(dm, pix[i], defer_rhs, w_piece, method, o, &np);
             else if (full) run_mixed<4>(dm, pix[i], o);
             else run_mixed<1>(dm, pix[i], o);
         } else {
@@ -297,4 +377,31 @@ extern "C" int lh_coefficients_mixed(const jsx_jet_desc *jet, const jsx_image_de
         for (int j = 0; j < 11; ++j) out[11 * i + j] = (double)v[j] / JSX_PC;   /* per parsec -> per cm */
     }
     return JSX_OK;
+}
+
+/* exp(-mu 1 + G(eta, rho)) (column-major) and phi1(...) v: the closed forms
+ * of the exponential pieces, for comparison with a general matrix exponential. */
+extern "C" int lh_magnus_expphi(double mu, const double *eta, const double *rho, const double *v, double *E,
+                                double *w) {
+    MgGen g;
+    for (int i = 0; i < 3; ++i) { g.e[i] = eta[i]; g.r[i] = rho[i]; }
+    return mg_expphi(mu, g, v, E, w);
+}
+
+/* [G(e1, r1), G(e2, r2)] as (eta', rho') */
+extern "C" void lh_magnus_comm(const double *e1, const double *r1, const double *e2, const double *r2, double *eo,
+                               double *ro) {
+    MgGen a, b, c;
+    for (int i = 0; i < 3; ++i) { a.e[i] = e1[i]; a.r[i] = r1[i]; b.e[i] = e2[i]; b.r[i] = r2[i]; }
+    mg_comm(a, b, c);
+    for (int i = 0; i < 3; ++i) { eo[i] = c.e[i]; ro[i] = c.r[i]; }
+}
+
+/* cost estimates (Dormand-Prince stiffness, exponential steps, rotation,
+ * optical depth) of the rays the last lh_observe call split, in order;
+ * returns how many rays */
+extern "C" int64_t lh_split_costs(double *out, int64_t cap) {
+    int64_t n = (int64_t)g_costs.size() / 4;
+    for (int64_t i = 0; i < 4 * n && i < 4 * cap; ++i) out[i] = g_costs[i];
+    return n;
 }

@@ -40,8 +40,14 @@ def lib():
         _lib.lh_observe.restype = C.c_int
         _lib.lh_observe.argtypes = [C.POINTER(abi.JetDesc), C.POINTER(abi.ImageDesc), C.POINTER(abi.SolverParams),
                                     C.POINTER(C.c_int64), C.c_int64, C.c_int32, C.POINTER(abi.GridDesc),
-                                    C.POINTER(C.POINTER(C.c_double)), C.c_uint, C.c_float,
+                                    C.POINTER(C.POINTER(C.c_double)), C.c_uint, C.c_float, C.c_int32,
                                     C.POINTER(abi.Outputs), C.POINTER(C.c_int32)]
+        _lib.lh_magnus_expphi.restype = C.c_int
+        _lib.lh_magnus_expphi.argtypes = [C.c_double] + [C.POINTER(C.c_double)] * 5
+        _lib.lh_split_costs.restype = C.c_int64
+        _lib.lh_split_costs.argtypes = [C.POINTER(C.c_double), C.c_int64]
+        _lib.lh_magnus_comm.restype = None
+        _lib.lh_magnus_comm.argtypes = [C.POINTER(C.c_double)] * 6
         _lib.lh_coefficients_mixed.restype = C.c_int
         _lib.lh_coefficients_mixed.argtypes = [C.POINTER(abi.JetDesc), C.POINTER(abi.ImageDesc),
                                                C.POINTER(abi.SolverParams), C.POINTER(C.c_double), C.c_int64,
@@ -53,9 +59,12 @@ def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-def observe(jet, image, params, pixels, grids=None, defer_rhs=0, w_piece=256.0):
+def observe(jet, image, params, pixels, grids=None, defer_rhs=0, w_piece=256.0, method=0):
     """Lane-by-lane run of the listed pixels; grids = [(GridDesc, values)] in
-    handle order 1..n.  Returns planes like oracle.observe plus 'n_pieces'."""
+    handle order 1..n.  defer_rhs > 0 splits rays past that many right-hand
+    sides into pieces of w_piece stiffness, integrated by Dormand-Prince
+    (method 0) or exponential propagators (method 2).  Returns planes like
+    oracle.observe plus 'n_pieces'."""
     pix = np.ascontiguousarray(pixels, dtype=np.int64)
     n = len(pix)
     res = {k: np.zeros(n) for k in ("tau", "I", "Q", "U", "V", "l")}
@@ -75,7 +84,7 @@ def observe(jet, image, params, pixels, grids=None, defer_rhs=0, w_piece=256.0):
         gd[i] = d
         gv[i] = _dp(vals[i])
     rc = lib().lh_observe(C.byref(jet), C.byref(image), C.byref(params), pix.ctypes.data_as(C.POINTER(C.c_int64)),
-                          n, len(grids), gd, gv, defer_rhs, w_piece, C.byref(o),
+                          n, len(grids), gd, gv, defer_rhs, w_piece, method, C.byref(o),
                           npcs.ctypes.data_as(C.POINTER(C.c_int32)))
     if rc:
         raise RuntimeError("lh_observe failed: %d" % rc)
@@ -90,3 +99,27 @@ def coefficients_mixed(jet, image, params, pts_cm):
     if rc:
         raise RuntimeError("lh_coefficients_mixed failed: %d" % rc)
     return out
+
+
+def magnus_expphi(mu, eta, rho, v):
+    """(E, w, ok): exp(-mu 1 + G(eta, rho)) and phi1(-mu 1 + G) v."""
+    E = np.zeros(16)
+    w = np.zeros(4)
+    ok = lib().lh_magnus_expphi(float(mu), _dp(np.ascontiguousarray(eta, float)), _dp(np.ascontiguousarray(rho, float)),
+                                _dp(np.ascontiguousarray(v, float)), _dp(E), _dp(w))
+    return E.reshape(4, 4).T, w, ok
+
+
+def magnus_comm(e1, r1, e2, r2):
+    eo, ro = np.zeros(3), np.zeros(3)
+    a = [np.ascontiguousarray(x, float) for x in (e1, r1, e2, r2)]
+    lib().lh_magnus_comm(*[_dp(x) for x in a], _dp(eo), _dp(ro))
+    return eo, ro
+
+
+def split_costs(cap=1 << 16):
+    """(Dormand-Prince stiffness, exponential steps, rotation, optical depth)
+    estimates of the rays the last observe() call split, in pixel order."""
+    out = np.zeros(4 * cap)
+    n = lib().lh_split_costs(_dp(out), cap)
+    return out[:4 * n].reshape(n, 4)

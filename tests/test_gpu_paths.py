@@ -138,6 +138,36 @@ def test_split_long_rays_on_gpu(ctx):
     assert quv <= 5e-5
 
 
+def test_split_exponential_pieces_on_gpu(ctx):
+    """piece_layout 2: the split rays' pieces are integrated by exponential
+    (Magnus) propagators; every hit pixel's I stays within 1e-5 of FP64 and
+    Q/U/V within 5e-5 of peak I, with fewer piece right-hand sides than the
+    Dormand-Prince pieces of the same split."""
+    j, im, p = models.paraboloid_helical_model(n_side=64, pixel_pc=0.08)
+    g64 = ctx.observe(j, im, p)
+    pm = models.with_precision(p, abi.PREC_MIXED, rel_tol=1e-6)
+    pm.flags |= OPT_PERSISTENT | OPT_APEX_FIRST
+    old = ctx.get_option("defer_rhs"), ctx.get_option("piece_layout")
+    ctx.set_option("defer_rhs", 4096)
+    try:
+        rhs = {}
+        for layout in (1, 2):
+            ctx.set_option("piece_layout", layout)
+            gm = ctx.observe(j, im, pm)
+            rhs[layout] = ctx.get_option("stat_rhs_pieces")
+            parked = ctx.get_option("stat_parked")
+    finally:
+        ctx.set_option("defer_rhs", old[0])
+        ctx.set_option("piece_layout", old[1])
+    rel, quv, nhit = _mixed_errors(g64, gm)
+    print("exponential pieces: %d parked rays; piece rhs %d (Dormand-Prince %d); max rel I %.3g, QUV/peak %.3g" % (
+        parked, rhs[2], rhs[1], rel, quv), flush=True)
+    assert parked > 0
+    assert rhs[2] < rhs[1]
+    assert rel <= 1e-5
+    assert quv <= 5e-5
+
+
 def test_batch_equals_single_images(ctx):
     """jsx_observe_batch over three different models (one launch per
     precision/stokes group) gives each image exactly what the single-image
