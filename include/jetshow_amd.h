@@ -216,9 +216,9 @@ int jsx_abi_version(void);
  * "stat_pieces" its pieces, "stat_piece_steps_max" / "stat_piece_steps_mean"
  * right-hand sides per piece, "stat_rhs_batch" / "stat_rhs_pieces" /
  * "stat_rhs_exp" right-hand sides evaluated by its batch / Dormand-Prince
- * piece / exponential piece kernel, "stat_trips" / "stat_trips_uniform" the
- * batch kernel's wavefront loop trips (all / with every lane on one request:
- * scalar constant loads), and, with "timing", "stat_triage_ms",
+ * piece / exponential piece kernel, "stat_trips" / "stat_trips_uniform" /
+ * "stat_trips_tail" the batch kernel's wavefront loop trips (all / with every
+ * lane on one request: scalar constant loads / after the queues ran dry), and, with "timing", "stat_triage_ms",
  * "stat_batch_ms", "stat_piece_ms" (both piece kernels: "stat_piece_dp_ms" +
  * "stat_piece_exp_ms"), "stat_split_ms" (all split kernels) means over the
  * launches since timing was enabled ("stat_launches"). */

@@ -21,7 +21,9 @@
  * count of lanes that still hold a ray (`active`); long full-Stokes rays are
  * parked for the time-parallel path only when that count has fallen to
  * `park_lanes`, i.e. when the machine is running out of work, so a launch
- * whose remaining rays still fill the chip finishes them sequentially.
+ * whose remaining rays still fill the chip finishes them sequentially; or
+ * when at most `park_wave` lanes of their own wavefront still hold a ray
+ * (the wavefront then runs mostly empty).
  *
  * Within a request, ids map to pixels either row-major or apex columns first
  * (longest rays first).
@@ -38,7 +40,10 @@ struct BatchQueues {
     unsigned long long rhs;                /* right-hand sides evaluated by the launch */
     unsigned long long trips;              /* wavefront loop trips that evaluated RHS */
     unsigned long long trips_uniform;      /* ... of them with all lanes on one request */
+    unsigned long long trips_tail;         /* ... of them after the queues ran dry */
     unsigned long long n_live;             /* rays the triage pass left for integration */
+    unsigned bucket_n[32];                 /* live rays per cost bucket (JSX_LPT_BUCKETS) */
+    unsigned bucket_fill[32];              /* scatter cursors */
     unsigned long long active;             /* lanes holding a ray (sum of per-wave deltas) */
     unsigned drained;                      /* set once every queue has run dry */
 };
@@ -74,6 +79,15 @@ struct PolFP64 {
     __device__ static bool done(const LaneT &L) { return L.phase == PH_DONE; }
     __device__ static int64_t pix(const LaneT &L) { return L.pix; }
     __device__ static unsigned rhs(const LaneT &L) { return L.n_rhs; }
+    /* stiffness k_I + |k_F| + |k_C| + |h_Q| + |k_pol| at a point [cm], per cm */
+    template <class Q>
+    __device__ static float stiffness(const Q &q, const double *p_cm) {
+        const double n_los[3] = {-1. * q.im.dir[0], -1. * q.im.dir[1], -1. * q.im.dir[2]};
+        DCoef c;
+        d_coefficients<NS == 4 ? 1 : 0>(q.m, p_cm, n_los, q.S.nu, c);
+        const double pol = sqrt(c.kQ * c.kQ + c.kU * c.kU + c.kV * c.kV);
+        return (float)(fabs(c.kI) + fabs(c.kF) + fabs(c.kC) + fabs(c.hQ) + pol);
+    }
     /* FP64 rays are never split: results stay identical to the sequential oracle */
     __device__ static bool park(const LaneT &, int, const SplitWS &, unsigned) { return false; }
 };
@@ -109,6 +123,13 @@ struct PolMixed {
     __device__ static bool done(const LaneT &L) { return L.phase == PH_DONE; }
     __device__ static int64_t pix(const LaneT &L) { return L.pix; }
     __device__ static unsigned rhs(const LaneT &L) { return L.n_rhs; }
+    template <class Q>
+    __device__ static float stiffness(const Q &q, const double *p_cm) {
+        const double p[3] = {p_cm[0] * q.c.inv_pc, p_cm[1] * q.c.inv_pc, p_cm[2] * q.c.inv_pc};
+        MCoef k;
+        m_coefficients<NS == 4 ? 1 : 0>(q.m, q.c, p, k);
+        return split_stiffness(k) * (float)q.c.inv_pc;   /* per pc -> per cm */
+    }
     /* full-Stokes rays past the budget go to the time-parallel path */
     __device__ static bool park(const LaneT &L, int slot, const SplitWS &ws, unsigned defer_rhs) {
         if constexpr (NS == 4) {
@@ -138,27 +159,64 @@ __device__ inline int64_t d_queue_id(int q, unsigned long long h, int64_t block)
     return (int64_t)(b * JSX_NQ * (unsigned long long)block + (unsigned long long)q * block + r);
 }
 
+/* Estimated cost of a live ray: its stiffness integrated over its segments
+ * with 4 midpoint samples each (~ the Dormand-Prince step count; 8 coefficient
+ * evaluations against ~10^3 for an average ray), as a bucket 0..31 of its
+ * log2 (31 = costliest).  The live list is ordered by bucket, costliest
+ * first, so the long rays start early and the launch's tail is short ones
+ * (longest-processing-time-first scheduling). */
+#define JSX_LPT_BUCKETS 32
+
+template <class P, class L_, class Q>
+__device__ inline int d_cost_bucket(const L_ &L, const Q &q) {
+    float w = 0.f;
+    for (int s = 0; s < L.nseg; ++s) {
+        const double *a = s ? L.s1in : L.s0in, *b = s ? L.s1out : L.s0out;
+        const double d[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        const float len = (float)d_norm3(d);
+        for (int i = 0; i < 4; ++i) {
+            const double f = 0.125 + 0.25 * i;
+            const double p[3] = {a[0] + f * d[0], a[1] + f * d[1], a[2] + f * d[2]};
+            const float k = P::stiffness(q, p);
+            w += (k == k && k < 1e30f) ? k * 0.25f * len : 1e30f;
+        }
+    }
+    const int e = (int)__builtin_amdgcn_logf(1.0f + w);   /* log2 */
+    return e < 0 ? 0 : (e > JSX_LPT_BUCKETS - 1 ? JSX_LPT_BUCKETS - 1 : e);
+}
+
 /* Triage: one pass over every ray id.  Rays finished by their first point are
    written; the rest go to `live`, each wavefront's survivors as one chunk. */
 template <class P>
 __global__ __launch_bounds__(256) void jsx_k_triage(const typename P::Params *__restrict__ reqs,
                                                     const int64_t *__restrict__ ray_off,
                                                     const DOut *__restrict__ outs, int n_req, int64_t total,
-                                                    int order, BatchQueues *qs, int64_t *__restrict__ live) {
+                                                    int order, BatchQueues *qs, int64_t *__restrict__ live,
+                                                    unsigned char *__restrict__ bucket) {
     using LaneT = typename P::LaneT;
+    __shared__ unsigned s_hist[JSX_LPT_BUCKETS];
     const int lane = __lane_id();
+    if (threadIdx.x < JSX_LPT_BUCKETS) s_hist[threadIdx.x] = 0u;
+    __syncthreads();
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t base_id = (int64_t)blockIdx.x * blockDim.x; base_id < total; base_id += stride) {
         const int64_t id = base_id + threadIdx.x;
-        int alive = 0;
+        int alive = 0, bk = 0;
         if (id < total) {
             LaneT L;
             const int req = d_find_request(ray_off, n_req, id);
             const typename P::Params *qp = reqs + req;
             const int64_t pix = d_ray_to_pixel(qp->im, id - ray_off[req], order);
             P::begin(L, *qp, pix);
-            if (P::done(L)) P::write(L, outs[req], pix);
-            else alive = 1;
+            if (P::done(L)) {
+                P::write(L, outs[req], pix);
+            } else {
+                alive = 1;
+                if (bucket) {
+                    bk = d_cost_bucket<P>(L, *qp);
+                    atomicAdd(&s_hist[bk], 1u);
+                }
+            }
         }
         const unsigned long long mask = __ballot(alive);
         if (mask) {
@@ -169,8 +227,49 @@ __global__ __launch_bounds__(256) void jsx_k_triage(const typename P::Params *__
             const unsigned hi = __shfl((unsigned)(at >> 32), leader);
             at = ((unsigned long long)hi << 32) | lo;
             const unsigned long long below = lane ? (mask & ((1ull << lane) - 1ull)) : 0ull;
-            if (alive) live[at + (unsigned long long)__popcll(below)] = id;
+            if (alive) {
+                live[at + (unsigned long long)__popcll(below)] = id;
+                if (bucket) bucket[at + (unsigned long long)__popcll(below)] = (unsigned char)bk;
+            }
         }
+    }
+    __syncthreads();
+    if (threadIdx.x < JSX_LPT_BUCKETS && s_hist[threadIdx.x]) atomicAdd(&qs->bucket_n[threadIdx.x], s_hist[threadIdx.x]);
+}
+
+/* Order the live list by cost bucket, costliest first: each entry goes to
+ * its bucket's range (bucket ranges in descending bucket order, offsets from
+ * the bucket counts of the triage pass); order inside a bucket is arbitrary. */
+__global__ __launch_bounds__(256) void jsx_k_lpt_scatter(BatchQueues *qs, const int64_t *__restrict__ live,
+                                                         const unsigned char *__restrict__ bucket,
+                                                         int64_t *__restrict__ sorted) {
+    __shared__ unsigned s_base[JSX_LPT_BUCKETS], s_cnt[JSX_LPT_BUCKETS], s_res[JSX_LPT_BUCKETS];
+    if (threadIdx.x == 0) {
+        unsigned acc = 0;
+        for (int b = JSX_LPT_BUCKETS - 1; b >= 0; --b) {
+            s_base[b] = acc;
+            acc += qs->bucket_n[b];
+        }
+    }
+    const int64_t n = (int64_t)qs->n_live;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    /* every thread of the block runs the same number of rounds */
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
+        const int64_t i = i0 + threadIdx.x;
+        if (threadIdx.x < JSX_LPT_BUCKETS) s_cnt[threadIdx.x] = 0u;
+        __syncthreads();
+        int b = -1;
+        unsigned rank = 0;
+        if (i < n) {
+            b = bucket[i];
+            rank = atomicAdd(&s_cnt[b], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x < JSX_LPT_BUCKETS)
+            s_res[threadIdx.x] = s_cnt[threadIdx.x] ? atomicAdd(&qs->bucket_fill[threadIdx.x], s_cnt[threadIdx.x]) : 0u;
+        __syncthreads();
+        if (b >= 0) sorted[s_base[b] + s_res[b] + rank] = live[i];
+        __syncthreads();
     }
 }
 
@@ -179,7 +278,8 @@ __global__ __launch_bounds__(256) void jsx_k_batch(const typename P::Params *__r
                                                    const int64_t *__restrict__ ray_off, const DOut *__restrict__ outs,
                                                    int n_req, int64_t total, BatchQueues *qs, int64_t block,
                                                    int order, int steal, SplitWS ws, unsigned defer_rhs,
-                                                   const int64_t *__restrict__ live, unsigned park_lanes) {
+                                                   const int64_t *__restrict__ live, unsigned park_lanes,
+                                                   int park_wave) {
     using LaneT = typename P::LaneT;
     using Params = typename P::Params;
     LaneT L;
@@ -198,7 +298,7 @@ __global__ __launch_bounds__(256) void jsx_k_batch(const typename P::Params *__r
     int drained = 0, may_park = 0;
     unsigned trip = 0;
     int held = 0;   /* lanes of this wavefront counted in qs->active */
-    unsigned long long rhs_acc = 0, n_trips = 0, n_uniform = 0;
+    unsigned long long rhs_acc = 0, n_trips = 0, n_uniform = 0, n_tail = 0;
     const int lane = __lane_id();
     int myq = steal ? d_xcd_of_block() : 0;
     int qtry = 0;   /* queues tried since the last successful grab */
@@ -264,11 +364,15 @@ __global__ __launch_bounds__(256) void jsx_k_batch(const typename P::Params *__r
             const long long act = (long long)__hip_atomic_load(&qs->active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             may_park = __builtin_amdgcn_readfirstlane((int)(act <= (long long)park_lanes));
         }
-        if (__any(have)) n_trips++;
+        if (__any(have)) {
+            n_trips++;
+            if (drained) n_tail++;
+        }
+        int uni = 0;
         if (have) {
             int r0 = __builtin_amdgcn_readfirstlane(req);
-            if (__all(req == r0)) {
-                n_uniform++;
+            uni = __all(req == r0);
+            if (uni) {
                 P::rhs_consume_uniform(L, reqs + r0);
             } else {
                 P::rhs_consume(L, reqs + req);
@@ -277,17 +381,19 @@ __global__ __launch_bounds__(256) void jsx_k_batch(const typename P::Params *__r
                 P::write(L, outs[req], P::pix(L));
                 rhs_acc += P::rhs(L);
                 have = 0;
-            } else if (may_park && P::park(L, req, ws, defer_rhs)) {
+            } else if ((may_park || (drained && held <= park_wave)) && P::park(L, req, ws, defer_rhs)) {
                 rhs_acc += P::rhs(L);
                 have = 0;
             }
         }
+        if (__any(uni)) n_uniform++;   /* counted by every lane: lane 0 reports */
     }
     unsigned long long tot = d_wave_sum_u64(rhs_acc);
     if (lane == 0) {
         if (tot) atomicAdd(&qs->rhs, tot);
         if (n_trips) atomicAdd(&qs->trips, n_trips);
         if (n_uniform) atomicAdd(&qs->trips_uniform, n_uniform);
+        if (n_tail) atomicAdd(&qs->trips_tail, n_tail);
     }
 }
 
@@ -299,7 +405,9 @@ __global__ void jsx_k_reset_queues(BatchQueues *qs) {
         qs->rhs = 0ull;
         qs->trips = 0ull;
         qs->trips_uniform = 0ull;
+        qs->trips_tail = 0ull;
         qs->n_live = 0ull;
+        for (int b = 0; b < 32; ++b) { qs->bucket_n[b] = 0u; qs->bucket_fill[b] = 0u; }
         qs->active = 0ull;
         qs->drained = 0u;
     }

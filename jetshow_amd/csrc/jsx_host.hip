@@ -66,12 +66,17 @@ struct jsx_ctx {
     DOut *d_outs = nullptr;
     size_t n_cap = 0;
     std::vector<unsigned char> h_reqs;
-    /* triage pass: live-ray list (grow-only) */
+    /* triage pass: live-ray list, its cost buckets and the list ordered by
+       them (grow-only) */
     int triage = 1;
-    int64_t *d_live = nullptr;
+    int lpt = 1;
+    int64_t *d_live = nullptr, *d_sorted = nullptr;
+    unsigned char *d_bucket = nullptr;
     size_t live_cap = 0;
-    /* lanes still holding a ray below which long rays may be parked (0: auto) */
+    /* lanes still holding a ray below which long rays may be parked (0: auto),
+       and the same per wavefront (0: off) */
     unsigned park_lanes = 0;
+    int park_wave = 0;
     /* pieces the split planner aims at (0: auto, about two per resident piece lane) */
     int piece_target = 0;
     /* exponential pieces: right-hand sides a piece may use in each of the
@@ -160,7 +165,7 @@ extern "C" int jsx_ctx_destroy(jsx_ctx *ctx) {
     (void)hipHostFree(ctx->h_params);
     for (int i = 0; i < jsx_ctx::kSlots; ++i) (void)hipEventDestroy(ctx->slot_ev[i]);
     (void)hipFree(ctx->d_queues); (void)hipFree(ctx->d_reqs); (void)hipFree(ctx->d_off); (void)hipFree(ctx->d_outs);
-    (void)hipFree(ctx->d_live);
+    (void)hipFree(ctx->d_live); (void)hipFree(ctx->d_sorted); (void)hipFree(ctx->d_bucket);
     (void)hipFree(ctx->split_mem);
     for (auto &es : ctx->evs)
         for (int k = 0; k < 5; ++k) (void)hipEventDestroy(es.e[k]);
@@ -217,6 +222,11 @@ extern "C" int jsx_ctx_set_option(jsx_ctx *ctx, const char *name, double v) {
         ctx->piece_budget = (unsigned)v;
     } else if (k == "triage") {
         ctx->triage = v != 0.0;
+    } else if (k == "lpt") {
+        ctx->lpt = v != 0.0;
+    } else if (k == "park_wave") {
+        if (!(v >= 0 && v <= 64)) return fail(ctx, JSX_EINVAL, "park_wave must be in [0, 64]");
+        ctx->park_wave = (int)v;
     } else if (k == "park_lanes") {
         if (!(v >= 0 && v < 4e9)) return fail(ctx, JSX_EINVAL, "park_lanes out of range");
         ctx->park_lanes = (unsigned)v;
@@ -241,11 +251,13 @@ extern "C" int jsx_ctx_get_option(jsx_ctx *ctx, const char *name, double *v) {
     else if (k == "split_floor_max") *v = ctx->split_floor_max;
     else if (k == "piece_layout") *v = ctx->piece_layout;
     else if (k == "triage") *v = ctx->triage;
+    else if (k == "lpt") *v = ctx->lpt;
     else if (k == "piece_target") *v = ctx->piece_target;
     else if (k == "piece_budget") *v = ctx->piece_budget;
     else if (k == "mg_cost") *v = ctx->mg_cost;
     else if (k == "mg_profile") *v = ctx->mg_profile;
     else if (k == "park_lanes") *v = ctx->park_lanes;
+    else if (k == "park_wave") *v = ctx->park_wave;
     else if (k == "stat_parked" || k == "stat_pieces") {
         unsigned c[2] = {0, 0};
         if (ctx->split_mem) {
@@ -280,7 +292,7 @@ extern "C" int jsx_ctx_get_option(jsx_ctx *ctx, const char *name, double *v) {
         }
         *v = k == "stat_piece_steps_max" ? mx : (np ? sum / np : 0.0);
     } else if (k == "stat_rhs_batch" || k == "stat_rhs_pieces" || k == "stat_trips" || k == "stat_trips_uniform" ||
-               k == "stat_rhs_exp") {
+               k == "stat_rhs_exp" || k == "stat_trips_tail") {
         unsigned long long r = 0;
         HIPCHECK(ctx, hipDeviceSynchronize());
         if (k == "stat_rhs_batch" && ctx->d_queues)
@@ -289,6 +301,8 @@ extern "C" int jsx_ctx_get_option(jsx_ctx *ctx, const char *name, double *v) {
             HIPCHECK(ctx, hipMemcpy(&r, &ctx->d_queues->trips, sizeof r, hipMemcpyDeviceToHost));
         if (k == "stat_trips_uniform" && ctx->d_queues)
             HIPCHECK(ctx, hipMemcpy(&r, &ctx->d_queues->trips_uniform, sizeof r, hipMemcpyDeviceToHost));
+        if (k == "stat_trips_tail" && ctx->d_queues)
+            HIPCHECK(ctx, hipMemcpy(&r, &ctx->d_queues->trips_tail, sizeof r, hipMemcpyDeviceToHost));
         if (k == "stat_rhs_pieces" && ctx->split_mem)
             HIPCHECK(ctx, hipMemcpy(&r, ctx->split.rhs_pieces, sizeof r, hipMemcpyDeviceToHost));
         if (k == "stat_rhs_exp" && ctx->split_mem)
@@ -738,16 +752,26 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
         hipLaunchKernelGGL(jsx_k_split_reset, dim3(1), dim3(64), 0, st, ws);
     }
     /* triage list: one int64 per ray of the launch */
-    int64_t *live = nullptr;
+    int64_t *live = nullptr, *sorted = nullptr;
+    unsigned char *bucket = nullptr;
     if (ctx->triage) {
         if (ctx->live_cap < (size_t)total) {
             (void)hipFree(ctx->d_live);
-            ctx->d_live = nullptr;
+            (void)hipFree(ctx->d_sorted);
+            (void)hipFree(ctx->d_bucket);
+            ctx->d_live = ctx->d_sorted = nullptr;
+            ctx->d_bucket = nullptr;
             ctx->live_cap = 0;
             HIPCHECK(ctx, hipMalloc(&ctx->d_live, sizeof(int64_t) * (size_t)total));
+            HIPCHECK(ctx, hipMalloc(&ctx->d_sorted, sizeof(int64_t) * (size_t)total));
+            HIPCHECK(ctx, hipMalloc(&ctx->d_bucket, (size_t)total));
             ctx->live_cap = (size_t)total;
         }
         live = ctx->d_live;
+        if (ctx->lpt) {
+            sorted = ctx->d_sorted;
+            bucket = ctx->d_bucket;
+        }
     }
     const int64_t tri_need = (total + 255) / 256;
     const int tri_blocks = (int)(tri_need < (int64_t)ctx->n_cu * 16 ? tri_need : (int64_t)ctx->n_cu * 16);
@@ -759,19 +783,24 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
         if (live) {
             if (full)
                 hipLaunchKernelGGL(jsx_k_triage<PolMixed<4>>, dim3(tri_blocks), dim3(256), 0, st, R, ctx->d_off,
-                                   ctx->d_outs, nreq, total, order, ctx->d_queues, live);
+                                   ctx->d_outs, nreq, total, order, ctx->d_queues, live, bucket);
             else
                 hipLaunchKernelGGL(jsx_k_triage<PolMixed<1>>, dim3(tri_blocks), dim3(256), 0, st, R, ctx->d_off,
-                                   ctx->d_outs, nreq, total, order, ctx->d_queues, live);
+                                   ctx->d_outs, nreq, total, order, ctx->d_queues, live, bucket);
+        }
+        if (sorted) {
+            hipLaunchKernelGGL(jsx_k_lpt_scatter, dim3(tri_blocks), dim3(256), 0, st, ctx->d_queues, live, bucket, sorted);
+            live = sorted;
         }
         mark(0);
         if (full)
             hipLaunchKernelGGL(jsx_k_batch<PolMixed<4>>, dim3(blocks), dim3(256), 0, st, R, ctx->d_off, ctx->d_outs, nreq,
                                total, ctx->d_queues, ctx->queue_block, order, ctx->steal, ws, ctx->defer_rhs, live,
-                               park_lanes);
+                               park_lanes, ctx->park_wave);
         else
             hipLaunchKernelGGL(jsx_k_batch<PolMixed<1>>, dim3(blocks), dim3(256), 0, st, R, ctx->d_off, ctx->d_outs, nreq,
-                               total, ctx->d_queues, ctx->queue_block, order, ctx->steal, ws, 0u, live, park_lanes);
+                               total, ctx->d_queues, ctx->queue_block, order, ctx->steal, ws, 0u, live, park_lanes,
+                               ctx->park_wave);
         mark(1);
         if (split) {
             const int cap = ws.cap;
@@ -805,18 +834,24 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
         if (live) {
             if (full)
                 hipLaunchKernelGGL(jsx_k_triage<PolFP64<4>>, dim3(tri_blocks), dim3(256), 0, st, R, ctx->d_off,
-                                   ctx->d_outs, nreq, total, order, ctx->d_queues, live);
+                                   ctx->d_outs, nreq, total, order, ctx->d_queues, live, bucket);
             else
                 hipLaunchKernelGGL(jsx_k_triage<PolFP64<1>>, dim3(tri_blocks), dim3(256), 0, st, R, ctx->d_off,
-                                   ctx->d_outs, nreq, total, order, ctx->d_queues, live);
+                                   ctx->d_outs, nreq, total, order, ctx->d_queues, live, bucket);
+        }
+        if (sorted) {
+            hipLaunchKernelGGL(jsx_k_lpt_scatter, dim3(tri_blocks), dim3(256), 0, st, ctx->d_queues, live, bucket, sorted);
+            live = sorted;
         }
         mark(0);
         if (full)
             hipLaunchKernelGGL(jsx_k_batch<PolFP64<4>>, dim3(blocks), dim3(256), 0, st, R, ctx->d_off, ctx->d_outs, nreq,
-                               total, ctx->d_queues, ctx->queue_block, order, ctx->steal, ws, 0u, live, park_lanes);
+                               total, ctx->d_queues, ctx->queue_block, order, ctx->steal, ws, 0u, live, park_lanes,
+                               ctx->park_wave);
         else
             hipLaunchKernelGGL(jsx_k_batch<PolFP64<1>>, dim3(blocks), dim3(256), 0, st, R, ctx->d_off, ctx->d_outs, nreq,
-                               total, ctx->d_queues, ctx->queue_block, order, ctx->steal, ws, 0u, live, park_lanes);
+                               total, ctx->d_queues, ctx->queue_block, order, ctx->steal, ws, 0u, live, park_lanes,
+                               ctx->park_wave);
         mark(1);
         mark(2);
         mark(6);

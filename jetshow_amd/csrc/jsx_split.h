@@ -1132,16 +1132,37 @@ __global__ __launch_bounds__(256) void jsx_k_split_combine(const DOut *__restric
     if (R.piece0 + R.npieces > ws.piece_cap) {
         fl |= JSX_PIX_FAIL_MAXTRIALS;   /* planner overflow: cannot happen with the sizing in use */
     } else {
+        /* Walk the chain.  Chains are runs of consecutive slots (a ray's
+           planned pieces, each round's continuation parts), so the next four
+           slots' links and outputs are loaded together and applied while the
+           links stay consecutive: one load latency per run of up to four. */
         int k = R.piece0;
-        for (int guard = 0; k >= 0 && guard < ws.piece_cap; ++guard, k = ws.pieces[k].next) {
-            const PieceOut &o = ws.pout[k];
-            double T[4];
-            for (int i = 0; i < 4; ++i)
-                T[i] = o.Phi[i] * S[0] + o.Phi[4 + i] * S[1] + o.Phi[8 + i] * S[2] + o.Phi[12 + i] * S[3] + o.P[i];
-            for (int i = 0; i < 4; ++i) S[i] = T[i];
-            fl |= o.flags;
-            nr += o.n_rhs;
-            nst += o.n_steps;
+        int guard = 0;
+        while (k >= 0 && guard < ws.piece_cap) {
+            int nx[4];
+            PieceOut o[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int kk = k + i < ws.piece_cap ? k + i : k;
+                nx[i] = ws.pieces[kk].next;
+                o[i] = ws.pout[kk];
+            }
+            int nk = -1;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                double T[4];
+                for (int c = 0; c < 4; ++c)
+                    T[c] = o[i].Phi[c] * S[0] + o[i].Phi[4 + c] * S[1] + o[i].Phi[8 + c] * S[2] +
+                           o[i].Phi[12 + c] * S[3] + o[i].P[c];
+                for (int c = 0; c < 4; ++c) S[c] = T[c];
+                fl |= o[i].flags;
+                nr += o[i].n_rhs;
+                nst += o[i].n_steps;
+                ++guard;
+                nk = nx[i];
+                if (i == 3 || nx[i] != k + i + 1 || k + i + 1 >= ws.piece_cap) break;
+            }
+            k = nk;
         }
     }
     const DOut &O = outs[R.slot];
