@@ -66,10 +66,26 @@ def parse():
     ap.add_argument("--duration", type=float, default=10.0, help="c4: seconds of arrivals")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"], help="c2: solver precision")
     ap.add_argument("--models", type=int, default=256, help="c3: models in the sweep (x2 frequencies)")
+    ap.add_argument("--set", action="append", default=[], metavar="OPTION=VALUE",
+                    help="solver option applied to the context before the run (jsx_ctx_set_option), repeatable")
     ap.add_argument("--calibrate", action="store_true",
                     help="single-wavefront launches (2 rays) of the FP64 and mixed full-Stokes kernels for "
                          "rocprofv3 instruction counters; prints the RHS evaluations per launch")
-    return ap.parse_args()
+    a = ap.parse_args()
+    a.options = {}
+    for kv in a.set:
+        k, v = kv.split("=", 1)
+        a.options[k] = float(v)
+    return a
+
+
+def new_context(a):
+    """Context with the options of --set applied (they appear in the JSON config)."""
+    from jetshow_amd import host
+    ctx = host.Context(0)
+    for k, v in a.options.items():
+        ctx.set_option(k, v)
+    return ctx
 
 
 def calibrate():
@@ -173,7 +189,7 @@ def main():
     lib = host.lib()
     dev = torch.device("cuda:0")
     torch.cuda.init()
-    ctx = host.Context(0)
+    ctx = new_context(a)
     j, im, p64 = models.paraboloid_helical_model(n_side=a.n_side)
     npix = im.n_across * im.n_along
     pm = models.with_precision(p64, abi.PREC_MIXED, rel_tol=a.rel_tol)
@@ -303,6 +319,8 @@ def main():
         "cpu_baseline": cpu,
     }
     out.update(plain)
+    if a.options:
+        out["config"]["options"] = a.options
     line = json.dumps(out)
     print(line, flush=True)
     if a.json_out:
@@ -324,7 +342,7 @@ def sweep_bench(a):
     lib = host.lib()
     dev = torch.device("cuda:0")
     torch.cuda.init()
-    ctx = host.Context(0)
+    ctx = new_context(a)
     reqs_py = models.abc_sweep_models(a.models)
     n = len(reqs_py)
     sizes = [im.n_across * im.n_along for _, im, _, _ in reqs_py]
@@ -393,6 +411,8 @@ def sweep_bench(a):
                      "traffic": None},
         "cpu_baseline": cpu,
     }
+    if a.options:
+        out["config"]["options"] = a.options
     line = json.dumps(out)
     print(line, flush=True)
     if a.json_out:
@@ -443,7 +463,7 @@ def grid_bench(a):
     lib = host.lib()
     dev = torch.device("cuda:0")
     torch.cuda.init()
-    ctx = host.Context(0)
+    ctx = new_context(a)
     (B, N, G), geo = grf.jet_grids(2048, 512)
     hs = [ctx.upload_grid(v, *geo, nan_value=nv) for v, nv in ((B, 0.0), (N, 0.0), (G, 1.0))]
     prec = abi.PREC_FP64 if a.precision == "fp64" else abi.PREC_MIXED
@@ -503,6 +523,8 @@ def grid_bench(a):
                      "grid_lookup_GBps_algorithmic": grid_gbs},
         "cpu_baseline": cpu,
     }
+    if a.options:
+        out["config"]["options"] = a.options
     line = json.dumps(out)
     print(line, flush=True)
     if a.json_out:
@@ -522,7 +544,7 @@ def serve_bench(a):
     from jetshow_amd import abi, grf, host, models
 
     torch.cuda.init()
-    ctx = host.Context(0)
+    ctx = new_context(a)
     rng = np.random.default_rng(4)
     n_arr = rng.poisson(a.rate * a.duration)
     t_arr = np.sort(rng.uniform(0.0, a.duration, n_arr))

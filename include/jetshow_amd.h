@@ -202,9 +202,13 @@ int jsx_abi_version(void);
  *                     pieces are exponential
  *   "mg_profile"      cuts of exponential rays placed by 0 (default) the
  *                     stiffness profile, 1 the sampled exponential-step profile
- *   "triage"          1 (default): a first pass writes the rays that need no
- *                     integration and queues only the others; 0: the queues
- *                     deal every ray id
+ *   "triage"          1: a first pass writes the rays that need no integration
+ *                     and queues only the others, costliest first (see
+ *                     "lpt"); 0: the queues deal every ray id in image order;
+ *                     2 (default): 1 for mixed-precision launches, 0 for FP64
+ *                     (the FP64 grid-field workload ran 24% slower with it)
+ *   "lpt"             1 (default): the triage list is ordered by estimated
+ *                     cost, costliest first; 0: in triage order
  *   "park_lanes"      rays are split only once the queues are dry and at most
  *                     this many lanes of the launch still hold a ray (0: half a
  *                     wavefront per SIMD of the device)

@@ -68,7 +68,7 @@ struct jsx_ctx {
     std::vector<unsigned char> h_reqs;
     /* triage pass: live-ray list, its cost buckets and the list ordered by
        them (grow-only) */
-    int triage = 1;
+    int triage = 2;                     /* 0 off, 1 on, 2 mixed-precision launches only */
     int lpt = 1;
     int64_t *d_live = nullptr, *d_sorted = nullptr;
     unsigned char *d_bucket = nullptr;
@@ -221,7 +221,8 @@ extern "C" int jsx_ctx_set_option(jsx_ctx *ctx, const char *name, double v) {
         if (!(v >= 0 && v < 1e9)) return fail(ctx, JSX_EINVAL, "piece_budget out of range");
         ctx->piece_budget = (unsigned)v;
     } else if (k == "triage") {
-        ctx->triage = v != 0.0;
+        if (!(v == 0.0 || v == 1.0 || v == 2.0)) return fail(ctx, JSX_EINVAL, "triage must be 0, 1 or 2");
+        ctx->triage = (int)v;
     } else if (k == "lpt") {
         ctx->lpt = v != 0.0;
     } else if (k == "park_wave") {
@@ -754,7 +755,8 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
     /* triage list: one int64 per ray of the launch */
     int64_t *live = nullptr, *sorted = nullptr;
     unsigned char *bucket = nullptr;
-    if (ctx->triage) {
+    const bool triage = ctx->triage == 1 || (ctx->triage == 2 && precision == JSX_PREC_MIXED);
+    if (triage) {
         if (ctx->live_cap < (size_t)total) {
             (void)hipFree(ctx->d_live);
             (void)hipFree(ctx->d_sorted);
