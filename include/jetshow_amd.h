@@ -194,7 +194,7 @@ int jsx_abi_version(void);
  *                     propagators, one lane per piece; 3 (default) 1 or 2 per
  *                     ray, from cost estimates sampled along its rest
  *   "piece_target"    exponential pieces the planner aims at in the first
- *                     round (0: one per 4 resident lanes)
+ *                     round (0: one per 2 resident lanes)
  *   "piece_budget"    right-hand sides after which an exponential piece hands
  *                     the rest of its interval to new pieces in the next of 4
  *                     rounds (0: one round, no budget)

@@ -637,14 +637,14 @@ struct BatchItem {
 
 /* pieces the planner aims at.  Dormand-Prince pieces: about two per piece
    worker the piece kernel keeps resident (5-lane groups at two wavefronts per
-   SIMD, or single lanes at one wavefront per SIMD).  Exponential pieces: a
-   quarter of the resident lanes in the first round ("piece_target"
-   overrides), the budgeted rounds cut the costly ones further. */
+   SIMD, or single lanes at one wavefront per SIMD).  Exponential pieces: half
+   of the resident lanes in the first round ("piece_target" overrides), the
+   budgeted rounds cut the costly ones further. */
 static void split_target(jsx_ctx *ctx) {
     SplitWS &w = ctx->split;
     long t = ctx->piece_layout != 0 ? (long)ctx->n_cu * 4 * 64 * 2
                                     : (long)ctx->n_cu * ctx->persist_blocks_per_cu * 4 * (64 / JSX_SPLIT_GROUP) * 2;
-    long tm = (long)ctx->n_cu * 64;
+    long tm = (long)ctx->n_cu * 128;
     if (ctx->piece_target > 0) tm = ctx->piece_target;
     w.target_pieces = (int)(t < w.piece_cap / 2 ? t : w.piece_cap / 2);
     w.target_mg = (int)(tm < w.piece_cap / 2 ? tm : w.piece_cap / 2);
