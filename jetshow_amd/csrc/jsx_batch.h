@@ -56,7 +56,7 @@ struct PolFP64 {
     template <class Q>
     __device__ static void begin(LaneT &L, const Q &q, int64_t pix) { lane_begin(L, q.m, q.im, q.S, pix); }
     template <class Q>
-    __device__ static void rhs_consume(LaneT &L, const Q *q0) {
+    __device__ __forceinline__ static void rhs_consume(LaneT &L, const Q *q0) {
         double r[NS];
         const Q *q = q0;
         JSX_OPAQUE_V(q);
@@ -66,7 +66,7 @@ struct PolFP64 {
         lane_consume(L, q->S, q->im, r);
     }
     template <class Q>
-    __device__ static void rhs_consume_uniform(LaneT &L, const Q *q0) {
+    __device__ __forceinline__ static void rhs_consume_uniform(LaneT &L, const Q *q0) {
         double r[NS];
         const JSX_CONST Q *q = (const JSX_CONST Q *)q0;
         JSX_OPAQUE_S(q);

@@ -199,9 +199,9 @@ __device__ inline double lane_dense(const Lane<NS> &L, double t) {
 
 /* ---- transitions: each returns when the lane needs a RHS or is done ---- */
 
-template <int NS> __device__ void lane_stokes_seg_begin(Lane<NS> &L, const DSolver &S);
-template <int NS> __device__ void lane_tau_range_next(Lane<NS> &L, const DSolver &S);
-
+/* The transitions are loops, not mutual recursion: a recursive call chain
+ * cannot be inlined, and a lane passed to an outlined function lives in
+ * scratch memory for the whole kernel. */
 template <int NS>
 __device__ inline void lane_finish(Lane<NS> &L) {
     L.phase = PH_DONE;
@@ -229,59 +229,25 @@ __device__ inline void lane_try_begin(Lane<NS> &L, const DSolver &S) {
 }
 
 template <int NS>
-__device__ inline void lane_stokes_begin(Lane<NS> &L, const DSolver &S) {
-    if (!(L.tau_bg > S.tau_min)) {
-        L.flags |= JSX_PIX_I_SKIPPED;
-        for (int i = 0; i < NS; ++i) L.x[i] = 0.0;
-        lane_finish(L);
-        return;
-    }
-    L.phase = PH_STOKES;
-    for (int i = 0; i < NS; ++i) L.x[i] = 0.0;
-    L.seg = L.nseg - 1;
-    lane_stokes_seg_begin(L, S);
-}
-
-template <int NS>
-__device__ inline void lane_tau_seg_end(Lane<NS> &L, const DSolver &S, int found) {
-    L.tau_bg = L.tau_bg + L.x[0];
-    L.thick = L.thick + L.L;
-    if (found || L.seg + 1 >= L.nseg) {
-        if (found) L.nseg = L.seg + 1;
-        lane_stokes_begin(L, S);
-        return;
-    }
-    L.seg = L.seg + 1;
+__device__ inline void lane_stokes_seg_setup(Lane<NS> &L, const DSolver &S) {
     lane_select_seg(L, L.seg);
     L.L = lane_seg_length(L);
-    L.x[0] = 0.0; L.t = 0.0; L.dt = L.L / S.n; L.deriv_ok = 0;
-    lane_tau_range_next(L, S);
+    double auto_n = S.n;
+    if (L.tau_bg > S.tau_n_min) auto_n = d_steps_schedule(S, L.tau_bg);
+    L.dt = L.L / auto_n;
+    L.t = 0.0;
+    L.first = 1;
 }
 
 template <int NS>
-__device__ void lane_tau_range_next(Lane<NS> &L, const DSolver &S) {
-    if (!d_less_with_sign(L.t, L.L, L.dt)) { lane_tau_seg_end(L, S, 0); return; }
-    if (d_less_with_sign(L.L, L.t + L.dt, L.dt)) { L.dt = L.L - L.t; L.deriv_ok = 0; }
-    L.t_old = L.t;
-    L.tries = 0;
-    if (!L.deriv_ok) {
-        L.xt[0] = L.x[0];
-        L.tq = L.t;
-        L.sub = SUB_INIT;
-        return;
-    }
-    lane_try_begin(L, S);
-}
-
-template <int NS>
-__device__ void lane_stokes_loop(Lane<NS> &L, const DSolver &S) {
+__device__ inline void lane_stokes_loop(Lane<NS> &L, const DSolver &S) {
     for (;;) {
         if (!d_less_with_sign(L.t, L.L, L.dt)) {
             /* segment finished: move to the nearer one */
             L.seg = L.seg - 1;
             if (L.seg < 0) { lane_finish(L); return; }
-            lane_stokes_seg_begin(L, S);
-            return;
+            lane_stokes_seg_setup(L, S);
+            continue;
         }
         if (d_less_with_sign(L.L, L.t + L.dt, L.dt)) L.dt = L.L - L.t;
         L.tries = 0;
@@ -298,15 +264,70 @@ __device__ void lane_stokes_loop(Lane<NS> &L, const DSolver &S) {
 }
 
 template <int NS>
-__device__ void lane_stokes_seg_begin(Lane<NS> &L, const DSolver &S) {
+__device__ inline void lane_stokes_seg_begin(Lane<NS> &L, const DSolver &S) {
+    lane_stokes_seg_setup(L, S);
+    lane_stokes_loop(L, S);
+}
+
+template <int NS>
+__device__ inline void lane_stokes_begin(Lane<NS> &L, const DSolver &S) {
+    if (!(L.tau_bg > S.tau_min)) {
+        L.flags |= JSX_PIX_I_SKIPPED;
+        for (int i = 0; i < NS; ++i) L.x[i] = 0.0;
+        lane_finish(L);
+        return;
+    }
+    L.phase = PH_STOKES;
+    for (int i = 0; i < NS; ++i) L.x[i] = 0.0;
+    L.seg = L.nseg - 1;
+    lane_stokes_seg_begin(L, S);
+}
+
+/* the next tau segment: reset the range (tau restarts at 0 per segment) */
+template <int NS>
+__device__ inline void lane_tau_next_seg(Lane<NS> &L, const DSolver &S) {
+    L.seg = L.seg + 1;
     lane_select_seg(L, L.seg);
     L.L = lane_seg_length(L);
-    double auto_n = S.n;
-    if (L.tau_bg > S.tau_n_min) auto_n = d_steps_schedule(S, L.tau_bg);
-    L.dt = L.L / auto_n;
-    L.t = 0.0;
-    L.first = 1;
-    lane_stokes_loop(L, S);
+    L.x[0] = 0.0; L.t = 0.0; L.dt = L.L / S.n; L.deriv_ok = 0;
+}
+
+template <int NS>
+__device__ inline void lane_tau_range_next(Lane<NS> &L, const DSolver &S) {
+    for (;;) {
+        if (!d_less_with_sign(L.t, L.L, L.dt)) {
+            /* segment ended below tau_max */
+            L.tau_bg = L.tau_bg + L.x[0];
+            L.thick = L.thick + L.L;
+            if (L.seg + 1 >= L.nseg) { lane_stokes_begin(L, S); return; }
+            lane_tau_next_seg(L, S);
+            continue;
+        }
+        if (d_less_with_sign(L.L, L.t + L.dt, L.dt)) { L.dt = L.L - L.t; L.deriv_ok = 0; }
+        L.t_old = L.t;
+        L.tries = 0;
+        if (!L.deriv_ok) {
+            L.xt[0] = L.x[0];
+            L.tq = L.t;
+            L.sub = SUB_INIT;
+            return;
+        }
+        lane_try_begin(L, S);
+        return;
+    }
+}
+
+template <int NS>
+__device__ inline void lane_tau_seg_end(Lane<NS> &L, const DSolver &S, int found) {
+    L.tau_bg = L.tau_bg + L.x[0];
+    L.thick = L.thick + L.L;
+    if (found || L.seg + 1 >= L.nseg) {
+        if (found) L.nseg = L.seg + 1;
+        lane_stokes_begin(L, S);
+        return;
+    }
+    lane_tau_next_seg(L, S);
+    lane_tau_range_next(L, S);
 }
 
 /* begin a ray: generate it, hit the geometry, set up the tau pass */

@@ -933,18 +933,23 @@ __device__ inline void mg_lane_step(MgLane &L, const DModel &m, const DImage &im
     double h = L.h;
     int last = 0;
     if (!(L.t + h < L.tb)) { h = L.tb - L.t; last = 1; }
-    MCoef A[2];
-#pragma unroll
-    for (int n = 0; n < 2; ++n) {
-        const double tg = L.t + (n ? 0.78867513459481287 : 0.21132486540518713) * h;
-        double p[3];
-        for (int i = 0; i < 3; ++i) p[i] = L.o[i] * c.inv_pc - tg * im.dir[i];
-        m_coefficients<1>(m, c, p, A[n]);
+    /* the two Gauss points (kept as two named values: an indexed pair would
+       live in scratch memory) */
+    MCoef A0, A1;
+    {
+        const double t0 = L.t + 0.21132486540518713 * h, t1 = L.t + 0.78867513459481287 * h;
+        double p0[3], p1[3];
+        for (int i = 0; i < 3; ++i) {
+            p0[i] = L.o[i] * c.inv_pc - t0 * im.dir[i];
+            p1[i] = L.o[i] * c.inv_pc - t1 * im.dir[i];
+        }
+        m_coefficients<1>(m, c, p0, A0);
+        m_coefficients<1>(m, c, p1, A1);
     }
     L.n_rhs += 2;
     const double rel = c.st_rel > JSX_SPLIT_REL_MIN ? c.st_rel : JSX_SPLIT_REL_MIN;
     double E[16], w[4], err;
-    const int ok = mg_step(A[0], A[1], h, rel, L.p_floor, L.U + 16, E, w, &err);
+    const int ok = mg_step(A0, A1, h, rel, L.p_floor, L.U + 16, E, w, &err);
     if (ok && err <= 1.0) {
         double nU[20];
 #pragma unroll
