@@ -209,6 +209,12 @@ int jsx_abi_version(void);
  *                     (the FP64 grid-field workload ran 24% slower with it)
  *   "lpt"             1 (default): the triage list is ordered by estimated
  *                     cost, costliest first; 0: in triage order
+ *   "park_bucket"     mixed full Stokes with "lpt": rays whose estimated cost
+ *                     bucket (log2 of the sampled stiffness) is at least this
+ *                     are split from their first Stokes step, without a
+ *                     sequential prefix (0, the default: off)
+ *   "park_wave"       long rays may also be split once at most this many
+ *                     lanes of their own wavefront still hold a ray (0: off)
  *   "park_lanes"      rays are split only once the queues are dry and at most
  *                     this many lanes of the launch still hold a ray (0: half a
  *                     wavefront per SIMD of the device)

@@ -242,7 +242,8 @@ __global__ __launch_bounds__(256) void jsx_k_triage(const typename P::Params *__
  * the bucket counts of the triage pass); order inside a bucket is arbitrary. */
 __global__ __launch_bounds__(256) void jsx_k_lpt_scatter(BatchQueues *qs, const int64_t *__restrict__ live,
                                                          const unsigned char *__restrict__ bucket,
-                                                         int64_t *__restrict__ sorted) {
+                                                         int64_t *__restrict__ sorted,
+                                                         unsigned char *__restrict__ sorted_bucket) {
     __shared__ unsigned s_base[JSX_LPT_BUCKETS], s_cnt[JSX_LPT_BUCKETS], s_res[JSX_LPT_BUCKETS];
     if (threadIdx.x == 0) {
         unsigned acc = 0;
@@ -268,7 +269,11 @@ __global__ __launch_bounds__(256) void jsx_k_lpt_scatter(BatchQueues *qs, const 
         if (threadIdx.x < JSX_LPT_BUCKETS)
             s_res[threadIdx.x] = s_cnt[threadIdx.x] ? atomicAdd(&qs->bucket_fill[threadIdx.x], s_cnt[threadIdx.x]) : 0u;
         __syncthreads();
-        if (b >= 0) sorted[s_base[b] + s_res[b] + rank] = live[i];
+        if (b >= 0) {
+            const unsigned at = s_base[b] + s_res[b] + rank;
+            sorted[at] = live[i];
+            sorted_bucket[at] = (unsigned char)b;
+        }
         __syncthreads();
     }
 }
@@ -279,11 +284,13 @@ __global__ __launch_bounds__(256) void jsx_k_batch(const typename P::Params *__r
                                                    int n_req, int64_t total, BatchQueues *qs, int64_t block,
                                                    int order, int steal, SplitWS ws, unsigned defer_rhs,
                                                    const int64_t *__restrict__ live, unsigned park_lanes,
-                                                   int park_wave) {
+                                                   int park_wave, const unsigned char *__restrict__ lbucket,
+                                                   int park_bucket) {
     using LaneT = typename P::LaneT;
     using Params = typename P::Params;
     LaneT L;
     int have = 0, exhausted = 0, req = 0;
+    int early = 0;   /* the ray's cost bucket marks it for splitting from its first Stokes step */
     /* entries handed out by the queues: the triage list when there is one */
     int64_t n_ids = total;
     if (live) {
@@ -328,6 +335,7 @@ __global__ __launch_bounds__(256) void jsx_k_batch(const typename P::Params *__r
             if (want && !exhausted) {
                 if (idx < n_ids) {
                     const int64_t id = live ? live[idx] : idx;
+                    early = (lbucket && park_bucket > 0) ? (int)(lbucket[idx] >= park_bucket) : 0;
                     req = d_find_request(ray_off, n_req, id);
                     const Params *qp = reqs + req;
                     int64_t local = id - ray_off[req];
@@ -381,7 +389,8 @@ __global__ __launch_bounds__(256) void jsx_k_batch(const typename P::Params *__r
                 P::write(L, outs[req], P::pix(L));
                 rhs_acc += P::rhs(L);
                 have = 0;
-            } else if ((may_park || (drained && held <= park_wave)) && P::park(L, req, ws, defer_rhs)) {
+            } else if ((early || may_park || (drained && held <= park_wave)) &&
+                       P::park(L, req, ws, early ? 0u : defer_rhs)) {
                 rhs_acc += P::rhs(L);
                 have = 0;
             }

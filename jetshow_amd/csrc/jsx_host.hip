@@ -71,7 +71,10 @@ struct jsx_ctx {
     int triage = 2;                     /* 0 off, 1 on, 2 mixed-precision launches only */
     int lpt = 1;
     int64_t *d_live = nullptr, *d_sorted = nullptr;
-    unsigned char *d_bucket = nullptr;
+    unsigned char *d_bucket = nullptr, *d_sbucket = nullptr;
+    /* rays in this cost bucket or above are split from their first Stokes
+       step (0: off) */
+    int park_bucket = 0;
     size_t live_cap = 0;
     /* lanes still holding a ray below which long rays may be parked (0: auto),
        and the same per wavefront (0: off) */
@@ -166,6 +169,7 @@ extern "C" int jsx_ctx_destroy(jsx_ctx *ctx) {
     for (int i = 0; i < jsx_ctx::kSlots; ++i) (void)hipEventDestroy(ctx->slot_ev[i]);
     (void)hipFree(ctx->d_queues); (void)hipFree(ctx->d_reqs); (void)hipFree(ctx->d_off); (void)hipFree(ctx->d_outs);
     (void)hipFree(ctx->d_live); (void)hipFree(ctx->d_sorted); (void)hipFree(ctx->d_bucket);
+    (void)hipFree(ctx->d_sbucket);
     (void)hipFree(ctx->split_mem);
     for (auto &es : ctx->evs)
         for (int k = 0; k < 5; ++k) (void)hipEventDestroy(es.e[k]);
@@ -225,6 +229,9 @@ extern "C" int jsx_ctx_set_option(jsx_ctx *ctx, const char *name, double v) {
         ctx->triage = (int)v;
     } else if (k == "lpt") {
         ctx->lpt = v != 0.0;
+    } else if (k == "park_bucket") {
+        if (!(v >= 0 && v < 32)) return fail(ctx, JSX_EINVAL, "park_bucket must be in [0, 31]");
+        ctx->park_bucket = (int)v;
     } else if (k == "park_wave") {
         if (!(v >= 0 && v <= 64)) return fail(ctx, JSX_EINVAL, "park_wave must be in [0, 64]");
         ctx->park_wave = (int)v;
@@ -253,6 +260,7 @@ extern "C" int jsx_ctx_get_option(jsx_ctx *ctx, const char *name, double *v) {
     else if (k == "piece_layout") *v = ctx->piece_layout;
     else if (k == "triage") *v = ctx->triage;
     else if (k == "lpt") *v = ctx->lpt;
+    else if (k == "park_bucket") *v = ctx->park_bucket;
     else if (k == "piece_target") *v = ctx->piece_target;
     else if (k == "piece_budget") *v = ctx->piece_budget;
     else if (k == "mg_cost") *v = ctx->mg_cost;
@@ -754,19 +762,21 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
     }
     /* triage list: one int64 per ray of the launch */
     int64_t *live = nullptr, *sorted = nullptr;
-    unsigned char *bucket = nullptr;
+    unsigned char *bucket = nullptr, *lbucket = nullptr;
     const bool triage = ctx->triage == 1 || (ctx->triage == 2 && precision == JSX_PREC_MIXED);
     if (triage) {
         if (ctx->live_cap < (size_t)total) {
             (void)hipFree(ctx->d_live);
             (void)hipFree(ctx->d_sorted);
             (void)hipFree(ctx->d_bucket);
+            (void)hipFree(ctx->d_sbucket);
             ctx->d_live = ctx->d_sorted = nullptr;
-            ctx->d_bucket = nullptr;
+            ctx->d_bucket = ctx->d_sbucket = nullptr;
             ctx->live_cap = 0;
             HIPCHECK(ctx, hipMalloc(&ctx->d_live, sizeof(int64_t) * (size_t)total));
             HIPCHECK(ctx, hipMalloc(&ctx->d_sorted, sizeof(int64_t) * (size_t)total));
             HIPCHECK(ctx, hipMalloc(&ctx->d_bucket, (size_t)total));
+            HIPCHECK(ctx, hipMalloc(&ctx->d_sbucket, (size_t)total));
             ctx->live_cap = (size_t)total;
         }
         live = ctx->d_live;
@@ -791,18 +801,20 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
                                    ctx->d_outs, nreq, total, order, ctx->d_queues, live, bucket);
         }
         if (sorted) {
-            hipLaunchKernelGGL(jsx_k_lpt_scatter, dim3(tri_blocks), dim3(256), 0, st, ctx->d_queues, live, bucket, sorted);
+            hipLaunchKernelGGL(jsx_k_lpt_scatter, dim3(tri_blocks), dim3(256), 0, st, ctx->d_queues, live, bucket, sorted,
+                               ctx->d_sbucket);
             live = sorted;
+            lbucket = ctx->d_sbucket;
         }
         mark(0);
         if (full)
             hipLaunchKernelGGL(jsx_k_batch<PolMixed<4>>, dim3(blocks), dim3(256), 0, st, R, ctx->d_off, ctx->d_outs, nreq,
                                total, ctx->d_queues, ctx->queue_block, order, ctx->steal, ws, ctx->defer_rhs, live,
-                               park_lanes, ctx->park_wave);
+                               park_lanes, ctx->park_wave, lbucket, ctx->park_bucket);
         else
             hipLaunchKernelGGL(jsx_k_batch<PolMixed<1>>, dim3(blocks), dim3(256), 0, st, R, ctx->d_off, ctx->d_outs, nreq,
                                total, ctx->d_queues, ctx->queue_block, order, ctx->steal, ws, 0u, live, park_lanes,
-                               ctx->park_wave);
+                               ctx->park_wave, lbucket, 0);
         mark(1);
         if (split) {
             const int cap = ws.cap;
@@ -842,18 +854,20 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
                                    ctx->d_outs, nreq, total, order, ctx->d_queues, live, bucket);
         }
         if (sorted) {
-            hipLaunchKernelGGL(jsx_k_lpt_scatter, dim3(tri_blocks), dim3(256), 0, st, ctx->d_queues, live, bucket, sorted);
+            hipLaunchKernelGGL(jsx_k_lpt_scatter, dim3(tri_blocks), dim3(256), 0, st, ctx->d_queues, live, bucket, sorted,
+                               ctx->d_sbucket);
             live = sorted;
+            lbucket = ctx->d_sbucket;
         }
         mark(0);
         if (full)
             hipLaunchKernelGGL(jsx_k_batch<PolFP64<4>>, dim3(blocks), dim3(256), 0, st, R, ctx->d_off, ctx->d_outs, nreq,
                                total, ctx->d_queues, ctx->queue_block, order, ctx->steal, ws, 0u, live, park_lanes,
-                               ctx->park_wave);
+                               ctx->park_wave, lbucket, 0);
         else
             hipLaunchKernelGGL(jsx_k_batch<PolFP64<1>>, dim3(blocks), dim3(256), 0, st, R, ctx->d_off, ctx->d_outs, nreq,
                                total, ctx->d_queues, ctx->queue_block, order, ctx->steal, ws, 0u, live, park_lanes,
-                               ctx->park_wave);
+                               ctx->park_wave, lbucket, 0);
         mark(1);
         mark(2);
         mark(6);
