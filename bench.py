@@ -44,6 +44,11 @@ OPS_PER_RHS = {
     "exp": {"f64": 1292.5, "f32": 228.0},
 }
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
+# HBM bytes per image of a kernel on the default headline image (library
+# defaults), from rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by a copy of
+# known size (profiles/c1_hbm_traffic_counters.md); reported as roofline
+# "traffic" when that kernel dominates the default headline run.
+TRAFFIC_BYTES_C1 = {"jsx_k_split_magnus": 188.5e6, "jsx_k_batch<PolMixed<4>>": 24.8e6}
 
 
 def parse():
@@ -68,6 +73,9 @@ def parse():
     ap.add_argument("--models", type=int, default=256, help="c3: models in the sweep (x2 frequencies)")
     ap.add_argument("--set", action="append", default=[], metavar="OPTION=VALUE",
                     help="solver option applied to the context before the run (jsx_ctx_set_option), repeatable")
+    ap.add_argument("--traffic", action="store_true",
+                    help="one untimed launch of the headline image, then a 256 MiB device copy as the reference "
+                         "for rocprofv3 FETCH_SIZE / WRITE_SIZE counter runs")
     ap.add_argument("--calibrate", action="store_true",
                     help="single-wavefront launches (2 rays) of the FP64 and mixed full-Stokes kernels for "
                          "rocprofv3 instruction counters; prints the RHS evaluations per launch")
@@ -127,6 +135,27 @@ def calibrate():
     ctx.close()
 
 
+def traffic_probe(a):
+    """HBM traffic counters need a launch of known bytes to scale them: the
+    headline image once (library defaults), then torch's copy of 256 MiB
+    (268435456 bytes read and written)."""
+    import torch
+
+    from jetshow_amd import abi, models
+    ctx = new_context(a)
+    j, im, p64 = models.paraboloid_helical_model(n_side=a.n_side)
+    pm = models.with_precision(p64, abi.PREC_MIXED, rel_tol=a.rel_tol)
+    pm.flags |= OPT_PERSISTENT | OPT_APEX_FIRST
+    ctx.observe(j, im, pm)
+    x = torch.ones(1 << 25, dtype=torch.float64, device="cuda:0")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"traffic_probe": "done", "copy_bytes": 1 << 28, "rhs_exp": ctx.get_option("stat_rhs_exp"),
+                      "pieces": ctx.get_option("stat_pieces")}), flush=True)
+    ctx.close()
+
+
 def _log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
@@ -166,6 +195,9 @@ def main():
     a = parse()
     if a.calibrate:
         calibrate()
+        return
+    if a.traffic:
+        traffic_probe(a)
         return
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every ray is independent; "
@@ -280,10 +312,15 @@ def main():
     # its mean launch time, both measured in this run
     table, dom = kernel_rooflines(kern)
     t = table[dom]
+    default_run = a.n_side == 512 and not a.options and abs(a.rel_tol - 1e-7) < 1e-12
+    traffic = TRAFFIC_BYTES_C1.get(dom) if default_run else None
     roofline = {"bound": "mfma", "pipe": "valu", "kernel": dom, "achieved": t["achieved_TFLOPs"],
-                "peak": t["peak_TFLOPs"], "unit": "TFLOP/s", "frac": t["frac"], "traffic": None,
-                "note": "VALU-bound transcendental code, no MFMA and no grid reads (traffic null): peak = FP64/FP32 "
-                        "vector peaks weighted by the kernel's op mix; ops per RHS from rocprofv3 VALU counters"}
+                "peak": t["peak_TFLOPs"], "unit": "TFLOP/s", "frac": t["frac"], "traffic": traffic,
+                "traffic_unit": "HBM bytes per image in this kernel (all its launches)",
+                "hbm_GBps": traffic / (t["ms"] / 1e3) / 1e9 if traffic else None,
+                "note": "VALU-bound transcendental code, no MFMA: peak = FP64/FP32 vector peaks weighted by the "
+                        "kernel's op mix; ops per RHS from rocprofv3 VALU counters; traffic from rocprofv3 "
+                        "FETCH_SIZE/WRITE_SIZE runs of the default image (profiles/c1_hbm_traffic_counters.md)"}
     kernels = {"batch_ms": kern["stat_batch_ms"], "split_total_ms": kern["stat_split_ms"],
                "piece_ms": kern["stat_piece_ms"], "rhs_batch": kern["stat_rhs_batch"],
                "rhs_pieces": kern["stat_rhs_pieces"], "rhs_exp": kern["stat_rhs_exp"],
