@@ -670,6 +670,9 @@ static int split_workspace(jsx_ctx *ctx) {
     size_t o_prof = take(sizeof(float) * (size_t)cap * JSX_SPLIT_NSAMP);
     size_t o_pcs = take(sizeof(Piece) * piece_cap);
     size_t o_pout = take(sizeof(PieceOut) * piece_cap);
+    size_t o_pout2 = take(sizeof(PieceOut) * piece_cap);
+    size_t o_na = take(sizeof(int) * piece_cap);
+    size_t o_nb = take(sizeof(int) * piece_cap);
     HIPCHECK(ctx, hipMalloc(&ctx->split_mem, off));
     char *b = (char *)ctx->split_mem;
     SplitWS &w = ctx->split;
@@ -687,6 +690,9 @@ static int split_workspace(jsx_ctx *ctx) {
     w.prof = (float *)(b + o_prof);
     w.pieces = (Piece *)(b + o_pcs);
     w.pout = (PieceOut *)(b + o_pout);
+    w.pout2 = (PieceOut *)(b + o_pout2);
+    w.next_a = (int *)(b + o_na);
+    w.next_b = (int *)(b + o_nb);
     w.piece_cap = piece_cap;
     split_target(ctx);
     w.w_min = ctx->split_w_min;
@@ -836,7 +842,12 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
                 }
             }
             mark(3);
-            hipLaunchKernelGGL(jsx_k_split_combine, dim3((cap + 255) / 256), dim3(256), 0, st, ctx->d_outs, ws);
+            /* four chain-jumping rounds when the exponential rounds made long chains */
+            const int jump = ctx->piece_layout >= 2 && ctx->piece_budget ? 4 : 0;
+            for (int r = 0; r < jump; ++r)
+                hipLaunchKernelGGL(jsx_k_split_jump, dim3(ctx->n_cu * 8), dim3(256), 0, st, ws, r);
+            hipLaunchKernelGGL(jsx_k_split_combine, dim3((cap + 255) / 256), dim3(256), 0, st, ctx->d_outs, ws,
+                               jump ? 1 : 0);
         } else {
             mark(2);
             mark(6);

@@ -92,6 +92,8 @@ struct SplitWS {
     unsigned long long *rhs_mg;   /* coefficient evaluations of the exponential pieces */
     Piece *pieces;
     PieceOut *pout;
+    PieceOut *pout2;              /* second buffer of the chain-jumping rounds      */
+    int *next_a, *next_b;         /* chain links of the jumping rounds              */
     int piece_cap;
     int target_pieces;            /* Dormand-Prince pieces the planner aims at      */
     int target_mg;                /* exponential pieces of the first round          */
@@ -1125,8 +1127,53 @@ __global__ __launch_bounds__(256) void jsx_k_split_magnus(const DParamsM *__rest
     if (lane == 0 && tot) atomicAdd(ws.rhs_mg, tot);
 }
 
+/* ---- 6a. chain jumping: after round r every piece holds the composition of
+ * the 2^r pieces of its chain that start at it, and links to the piece after
+ * them (synchronous rounds, ping-pong buffers).  Four rounds shorten every
+ * chain 16-fold before the per-ray walk; composing in a tree instead of
+ * left to right changes only the FP64 rounding.  Slots that are not in any
+ * chain (marked empty) may hold stale links: links outside the table end the
+ * chain. */
+__device__ inline void split_compose(const PieceOut &a, const PieceOut &b, PieceOut &o) {
+    /* o = b after a: Phi = Phi_b Phi_a, P = Phi_b P_a + P_b */
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            o.Phi[4 * j + i] = b.Phi[i] * a.Phi[4 * j] + b.Phi[4 + i] * a.Phi[4 * j + 1] +
+                               b.Phi[8 + i] * a.Phi[4 * j + 2] + b.Phi[12 + i] * a.Phi[4 * j + 3];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        o.P[i] = b.Phi[i] * a.P[0] + b.Phi[4 + i] * a.P[1] + b.Phi[8 + i] * a.P[2] + b.Phi[12 + i] * a.P[3] + b.P[i];
+    o.flags = a.flags | b.flags;
+    o.n_rhs = a.n_rhs + b.n_rhs;
+    o.n_steps = a.n_steps + b.n_steps;
+    o.pad = 0;
+}
+
+__global__ __launch_bounds__(256) void jsx_k_split_jump(SplitWS ws, int round) {
+    unsigned n = *ws.n_pieces;
+    if (n > (unsigned)ws.piece_cap) n = (unsigned)ws.piece_cap;
+    const PieceOut *src = (round & 1) ? ws.pout2 : ws.pout;
+    PieceOut *dst = (round & 1) ? ws.pout : ws.pout2;
+    const int *snext = round == 0 ? nullptr : ((round & 1) ? ws.next_a : ws.next_b);
+    int *dnext = (round & 1) ? ws.next_b : ws.next_a;
+    for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int nk = snext ? snext[k] : ws.pieces[k].next;
+        if (nk < 0 || nk >= (int)n) {
+            dst[k] = src[k];
+            dnext[k] = -1;
+        } else {
+            PieceOut o;
+            split_compose(src[k], src[nk], o);
+            dst[k] = o;
+            dnext[k] = snext ? snext[nk] : ws.pieces[nk].next;
+        }
+    }
+}
+
 /* ---- 6. compose pieces and write the pixel ---- */
-__global__ __launch_bounds__(256) void jsx_k_split_combine(const DOut *__restrict__ outs, SplitWS ws) {
+__global__ __launch_bounds__(256) void jsx_k_split_combine(const DOut *__restrict__ outs, SplitWS ws, int jumped) {
     const unsigned r = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned n = *ws.n_long;
     if (n > (unsigned)ws.cap) n = ws.cap;
@@ -1137,12 +1184,27 @@ __global__ __launch_bounds__(256) void jsx_k_split_combine(const DOut *__restric
     if (R.piece0 + R.npieces > ws.piece_cap) {
         fl |= JSX_PIX_FAIL_MAXTRIALS;   /* planner overflow: cannot happen with the sizing in use */
     } else {
-        /* Walk the chain.  Chains are runs of consecutive slots (a ray's
-           planned pieces, each round's continuation parts), so the next four
-           slots' links and outputs are loaded together and applied while the
-           links stay consecutive: one load latency per run of up to four. */
+        /* Walk the chain.  Without jumping, chains are runs of consecutive
+           slots (a ray's planned pieces, each round's continuation parts), so
+           the next four slots' links and outputs are loaded together and
+           applied while the links stay consecutive: one load latency per run
+           of up to four.  After jumping the links are in next_b. */
         int k = R.piece0;
         int guard = 0;
+        if (jumped) {
+            for (; k >= 0 && k < ws.piece_cap && guard < ws.piece_cap; ++guard) {
+                const PieceOut o = ws.pout[k];
+                double T[4];
+                for (int c = 0; c < 4; ++c)
+                    T[c] = o.Phi[c] * S[0] + o.Phi[4 + c] * S[1] + o.Phi[8 + c] * S[2] + o.Phi[12 + c] * S[3] + o.P[c];
+                for (int c = 0; c < 4; ++c) S[c] = T[c];
+                fl |= o.flags;
+                nr += o.n_rhs;
+                nst += o.n_steps;
+                k = ws.next_b[k];
+            }
+            k = -1;
+        }
         while (k >= 0 && guard < ws.piece_cap) {
             int nx[4];
             PieceOut o[4];

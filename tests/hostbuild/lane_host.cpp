@@ -302,14 +302,23 @@ void run_mixed_split(const DParamsM &q, int64_t pix, unsigned defer_rhs, float w
             else run_piece(q, R, pcs[j], pout[j]);
         }
     }
-    /* combine: the GPU's jsx_k_split_combine for this one ray */
+    /* combine: the GPU's jsx_k_split_combine for this one ray, after the four
+       chain-jumping rounds of jsx_k_split_jump when the rounds made chains */
+    std::vector<PieceOut> pout2(cap);
+    std::vector<int> na(cap), nb(cap);
+    ws.pout2 = pout2.data();
+    ws.next_a = na.data();
+    ws.next_b = nb.data();
+    const int jumped = method == 3;
+    if (jumped)
+        for (int r = 0; r < 4; ++r) jsx_k_split_jump(ws, r);
     DOut ov = o.view();
     DOut outs[1] = {ov};
     nl = 1;
     SplitWS w2 = ws;
     unsigned one = 1;
     w2.n_long = &one;
-    jsx_k_split_combine(outs, w2);
+    jsx_k_split_combine(outs, w2, jumped);
     *n_pieces = (int)(n_pcs < (unsigned)cap ? n_pcs : (unsigned)cap);
 }
 
