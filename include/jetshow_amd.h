@@ -194,7 +194,7 @@ int jsx_abi_version(void);
  *                     propagators, one lane per piece; 3 (default) 1 or 2 per
  *                     ray, from cost estimates sampled along its rest
  *   "piece_target"    exponential pieces the planner aims at in the first
- *                     round (0: one per 2 resident lanes)
+ *                     round (0: one per resident lane)
  *   "piece_budget"    right-hand sides after which an exponential piece hands
  *                     the rest of its interval to new pieces in the next of 4
  *                     rounds (0: one round, no budget)
@@ -224,7 +224,8 @@ int jsx_abi_version(void);
  * Read-only (wait for the device): "stat_parked" rays split by the last
  * batched launch ("stat_parked_exp" of them with exponential pieces),
  * "stat_pieces" its pieces, "stat_piece_steps_max" / "stat_piece_steps_mean"
- * right-hand sides per piece, "stat_rhs_batch" / "stat_rhs_pieces" /
+ * right-hand sides per piece record (after the chain-jumping rounds of
+ * exponential pieces a record holds the composition of up to 16 pieces), "stat_rhs_batch" / "stat_rhs_pieces" /
  * "stat_rhs_exp" right-hand sides evaluated by its batch / Dormand-Prince
  * piece / exponential piece kernel, "stat_trips" / "stat_trips_uniform" /
  * "stat_trips_tail" the batch kernel's wavefront loop trips (all / with every

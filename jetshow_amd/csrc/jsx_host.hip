@@ -86,7 +86,7 @@ struct jsx_ctx {
        first rounds before handing the rest of its interval on (0: one round) */
     unsigned piece_budget = 64;
     /* time-parallel path for long full-Stokes rays (mixed precision) */
-    unsigned defer_rhs = 1024;          /* 0 disables parking                  */
+    unsigned defer_rhs = 512;           /* 0 disables parking                  */
     float split_w_min = 256.0f;
     float split_floor_max = 1.0f;
     int piece_layout = 3;               /* 0: 5-lane groups; 1: one lane, 20 components;
@@ -211,7 +211,7 @@ extern "C" int jsx_ctx_set_option(jsx_ctx *ctx, const char *name, double v) {
         ctx->piece_layout = (int)v;
         if (ctx->split_mem) split_target(ctx);
     } else if (k == "piece_target") {
-        if (!(v >= 0 && v <= (double)(1 << 19))) return fail(ctx, JSX_EINVAL, "piece_target must be in [0, 2^19]");
+        if (!(v >= 0 && v <= (double)(1 << 20))) return fail(ctx, JSX_EINVAL, "piece_target must be in [0, 2^20]");
         ctx->piece_target = (int)v;
         if (ctx->split_mem) split_target(ctx);
     } else if (k == "mg_profile") {
@@ -645,14 +645,14 @@ struct BatchItem {
 
 /* pieces the planner aims at.  Dormand-Prince pieces: about two per piece
    worker the piece kernel keeps resident (5-lane groups at two wavefronts per
-   SIMD, or single lanes at one wavefront per SIMD).  Exponential pieces: half
-   of the resident lanes in the first round ("piece_target" overrides), the
+   SIMD, or single lanes at one wavefront per SIMD).  Exponential pieces: one
+   per resident lane in the first round ("piece_target" overrides), the
    budgeted rounds cut the costly ones further. */
 static void split_target(jsx_ctx *ctx) {
     SplitWS &w = ctx->split;
     long t = ctx->piece_layout != 0 ? (long)ctx->n_cu * 4 * 64 * 2
                                     : (long)ctx->n_cu * ctx->persist_blocks_per_cu * 4 * (64 / JSX_SPLIT_GROUP) * 2;
-    long tm = (long)ctx->n_cu * 128;
+    long tm = (long)ctx->n_cu * 256;
     if (ctx->piece_target > 0) tm = ctx->piece_target;
     w.target_pieces = (int)(t < w.piece_cap / 2 ? t : w.piece_cap / 2);
     w.target_mg = (int)(tm < w.piece_cap / 2 ? tm : w.piece_cap / 2);
@@ -662,7 +662,7 @@ static void split_target(jsx_ctx *ctx) {
 /* one allocation for the split path's tables (kept for the context's life) */
 static int split_workspace(jsx_ctx *ctx) {
     if (ctx->split_mem) return JSX_OK;
-    const int cap = 65536, piece_cap = 1 << 19;
+    const int cap = 65536, piece_cap = 1 << 20;
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
     size_t o_cnt = take(8 * sizeof(unsigned) + 2 * sizeof(unsigned long long));

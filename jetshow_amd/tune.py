@@ -27,7 +27,7 @@ DEFAULT_SPACE = {
     "blocks_per_cu": [1, 2, 4, 8],
     "queue_block": [16, 64, 256],
     "steal": [0, 1],
-    "defer_rhs": [1024, 4096, 16384],
+    "defer_rhs": [512, 2048, 8192],
     "piece_layout": [1, 3],
     "park_lanes": [0, 8192],
     "piece_budget": [64, 128],
