@@ -202,6 +202,10 @@ int jsx_abi_version(void);
  *                     pieces are exponential
  *   "mg_profile"      cuts of exponential rays placed by 0 (default) the
  *                     stiffness profile, 1 the sampled exponential-step profile
+ *   "mg_rel"          smallest relative tolerance of an exponential step's
+ *                     error estimate (default 2e-6; the estimate, the 2nd-order
+ *                     exponential's difference, overstates the 4th-order
+ *                     step's error)
  *   "triage"          1: a first pass writes the rays that need no integration
  *                     and queues only the others, costliest first (see
  *                     "lpt"); 0: the queues deal every ray id in image order;

@@ -94,6 +94,7 @@ struct jsx_ctx {
                                            3: 1 or 2 per ray by its rotation */
     float mg_cost = 0.5f;               /* layout 3: cost ratio below which a ray's pieces are exponential */
     int mg_profile = 0;                 /* cuts of exponential rays: 0 stiffness, 1 step estimate */
+    float mg_rel = 2e-6f;               /* smallest relative tolerance of exponential steps */
     SplitWS split{};
     void *split_mem = nullptr;
     /* optional event timing of the batched launches (option "timing") */
@@ -214,6 +215,10 @@ extern "C" int jsx_ctx_set_option(jsx_ctx *ctx, const char *name, double v) {
         if (!(v >= 0 && v <= (double)(1 << 20))) return fail(ctx, JSX_EINVAL, "piece_target must be in [0, 2^20]");
         ctx->piece_target = (int)v;
         if (ctx->split_mem) split_target(ctx);
+    } else if (k == "mg_rel") {
+        if (!(v >= 1e-9 && v <= 1e-3)) return fail(ctx, JSX_EINVAL, "mg_rel must be in [1e-9, 1e-3]");
+        ctx->mg_rel = (float)v;
+        if (ctx->split_mem) ctx->split.mg_rel = (float)v;
     } else if (k == "mg_profile") {
         ctx->mg_profile = v != 0.0;
         if (ctx->split_mem) ctx->split.mg_profile = ctx->mg_profile;
@@ -265,6 +270,7 @@ extern "C" int jsx_ctx_get_option(jsx_ctx *ctx, const char *name, double *v) {
     else if (k == "piece_budget") *v = ctx->piece_budget;
     else if (k == "mg_cost") *v = ctx->mg_cost;
     else if (k == "mg_profile") *v = ctx->mg_profile;
+    else if (k == "mg_rel") *v = ctx->mg_rel;
     else if (k == "park_lanes") *v = ctx->park_lanes;
     else if (k == "park_wave") *v = ctx->park_wave;
     else if (k == "stat_parked" || k == "stat_pieces") {
@@ -699,6 +705,7 @@ static int split_workspace(jsx_ctx *ctx) {
     w.floor_max = ctx->split_floor_max;
     w.mg_cost = ctx->mg_cost;
     w.mg_profile = ctx->mg_profile;
+    w.mg_rel = ctx->mg_rel;
     return JSX_OK;
 }
 

@@ -103,6 +103,7 @@ struct SplitWS {
                                      are exponential                                 */
     int mg_profile;               /* cuts of exponential rays: 0 stiffness, 1 their
                                      own step estimate                               */
+    float mg_rel;                 /* smallest relative tolerance of exponential steps */
     float w_min;                  /* smallest stiffness per piece worth a split     */
     float floor_max;              /* cap of the column error-scale estimate (1: off) */
 };
@@ -910,13 +911,13 @@ __global__ __launch_bounds__(256) void jsx_k_split_piece20(const DParamsM *__res
 /* ---- 5c. pieces by exponential propagators (one lane per piece) ---- */
 struct MgLane {
     double U[20];                 /* Phi column-major (U[4 j + i]), then P (U[16 + i]) */
-    double t, tb, h, p_floor;
+    double t, tb, h, p_floor, rel_min;
     double o[3];                  /* origin of the piece's pass [cm] */
     uint32_t flags, n_rhs, n_steps, tries;
     int done;
 };
 
-__device__ inline void mg_lane_begin(MgLane &L, const LongRay &R, const Piece &pc) {
+__device__ inline void mg_lane_begin(MgLane &L, const LongRay &R, const Piece &pc, double rel_min) {
 #pragma unroll
     for (int i = 0; i < 20; ++i) L.U[i] = (i < 16 && (i >> 2) == (i & 3)) ? 1.0 : 0.0;
     for (int i = 0; i < 3; ++i) L.o[i] = pc.interval ? R.s0out[i] : R.cout[i];
@@ -925,6 +926,7 @@ __device__ inline void mg_lane_begin(MgLane &L, const LongRay &R, const Piece &p
     const double len = pc.tb - pc.ta;
     L.h = pc.dt0 > 0.0625 * len ? pc.dt0 : 0.0625 * len;
     L.p_floor = fabs(R.x[0]);
+    L.rel_min = rel_min;
     L.flags = 0; L.n_rhs = 0; L.n_steps = 0; L.tries = 0;
     L.done = !(pc.tb > pc.ta);
 }
@@ -949,7 +951,7 @@ __device__ inline void mg_lane_step(MgLane &L, const DModel &m, const DImage &im
         m_coefficients<1>(m, c, p1, A1);
     }
     L.n_rhs += 2;
-    const double rel = c.st_rel > JSX_SPLIT_REL_MIN ? c.st_rel : JSX_SPLIT_REL_MIN;
+    const double rel = c.st_rel > L.rel_min ? c.st_rel : L.rel_min;
     double E[16], w[4], err;
     const int ok = mg_step(A0, A1, h, rel, L.p_floor, L.U + 16, E, w, &err);
     if (ok && err <= 1.0) {
@@ -1061,7 +1063,7 @@ __global__ __launch_bounds__(256) void jsx_k_split_magnus(const DParamsM *__rest
                     const Piece pc = ws.pieces[pid];
                     const LongRay &R = ws.rays[pc.ray];
                     slot = R.slot;
-                    mg_lane_begin(L, R, pc);
+                    mg_lane_begin(L, R, pc, (double)ws.mg_rel);
                     have = L.done ? 2 : 1;
                     spawn_ok = budget != 0u;
                 }
