@@ -48,7 +48,7 @@ PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 # defaults), from rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by a copy of
 # known size (profiles/c1_hbm_traffic_counters.md); reported as roofline
 # "traffic" when that kernel dominates the default headline run.
-TRAFFIC_BYTES_C1 = {"jsx_k_split_magnus": 188.5e6, "jsx_k_batch<PolMixed<4>>": 24.8e6}
+TRAFFIC_BYTES_C1 = {"jsx_k_split_magnus": 188.8e6, "jsx_k_batch<PolMixed<4>>": 23.9e6}
 
 
 def parse():
@@ -63,10 +63,12 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--json-out", default=None)
-    ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "c4"],
+    ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "c4", "vis"],
                     help="c1: paraboloid full-Stokes image (headline); c2: grid-defined fields, 1024^2 at two "
                          "frequencies; c3: batched ABC sweep of cone models; c4: request server under Poisson "
-                         "arrivals")
+                         "arrivals; vis: visibilities of the c1 Stokes-I image at scattered (u, v)")
+    ap.add_argument("--n-vis", type=int, default=131072, help="vis: visibilities per step")
+    ap.add_argument("--vis-precision", default="mixed", choices=["fp64", "mixed"], help="vis: precision")
     ap.add_argument("--rate", type=float, default=200.0, help="c4: mean arrivals per second")
     ap.add_argument("--duration", type=float, default=10.0, help="c4: seconds of arrivals")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"], help="c2: solver precision")
@@ -211,6 +213,9 @@ def main():
         return
     if a.workload == "c4":
         serve_bench(a)
+        return
+    if a.workload == "vis":
+        vis_bench(a)
         return
     import ctypes as C
 
@@ -557,6 +562,11 @@ def grid_bench(a):
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "roofline": {"bound": "mfma", "pipe": "valu", "kernel": "jsx_k_batch", "achieved": ach, "peak": peak,
                      "unit": "TFLOP/s", "frac": ach / peak, "traffic": None,
+                     "valu_issue": {"variant": variant, "instr_per_pair": VALU_INSTR_PER_PAIR[variant],
+                                    "achieved_T_per_s": VALU_INSTR_PER_PAIR[variant] * pairs / (ms / 1e3) / 1e12,
+                                    "peak_T_per_s": VALU_ISSUE_PEAK_T,
+                                    "frac": VALU_INSTR_PER_PAIR[variant] * pairs / (ms / 1e3) / 1e12 /
+                                    VALU_ISSUE_PEAK_T},
                      "grid_lookup_GBps_algorithmic": grid_gbs},
         "cpu_baseline": cpu,
     }
@@ -680,6 +690,161 @@ def grid_cpu_baseline(items, grids, geo, seconds):
     return {"value": count / t_used, "unit": "pixels/s", "cores": ncores, "kind": "port",
             "sample": "every 997th pixel of both images from offsets 0, 13, ...: %d pixels, FP64 oracle with glibc "
                       "libm, %.1f s" % (count, t_used)}
+
+
+# VALU floating-point operations per pixel-visibility pair of jsx_k_vis (FMA =
+# 2, a hardware sine or cosine = 1), from rocprofv3 SQ_INSTS_VALU_{ADD,MUL,FMA,
+# TRANS}_{F64,F32} of the c1-image launch (262144 pixels x 131072 visibilities)
+# divided by its pairs per lane (profiles/vis_c1_image_kernel_trace.md):
+# FP64 sine/cosine; mixed with the hardware sine/cosine (default) and with the
+# FP32 polynomials.
+OPS_PER_PAIR = {"fp64": {"f64": 43.0, "f32": 0.0}, "mixed_hw": {"f64": 4.0, "f32": 6.0},
+                "mixed_poly": {"f64": 6.0, "f32": 24.0}}
+# all VALU instructions per pair (SQ_INSTS_VALU of the same launches) and the
+# issue peak: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz lane-instructions/s
+VALU_INSTR_PER_PAIR = {"fp64": 53.5, "mixed_hw": 9.48, "mixed_poly": 33.9}
+VALU_ISSUE_PEAK_T = 256 * 4 * 16 * 2.4e9 / 1e12
+
+
+def vis_pixels_and_uv(a):
+    """The c1 image's pixel centres in radians (x across, y along the jet, as
+    JetModelZoom's d and r_ob) and n_vis (u, v) points in wavelengths: 32
+    baselines' elliptical tracks of an Earth-rotation synthesis out to
+    5e8 wavelengths (VLBA at 2 cm), deterministic."""
+    from jetshow_amd import abi, host, models
+    j, im, p64 = models.paraboloid_helical_model(n_side=a.n_side)
+    sizes = host.pixel_sizes(im)                         # [n_across, n_along] cm
+    s_pc = sizes / abi.PC
+    n_ac = im.n_across
+    along = np.cumsum(s_pc, axis=1) - s_pc / 2
+    across = (n_ac / 2 - np.arange(n_ac)[:, None] - 0.5) * s_pc
+    rad_per_pc = abi.MAS_TO_RAD / host.mas_to_pc(0.1)
+    x = (across * rad_per_pc).ravel()
+    y = (along * rad_per_pc).ravel()
+    r = np.random.default_rng(5)
+    nb = 32
+    per = -(-a.n_vis // nb)
+    h = np.linspace(-np.pi / 2, np.pi / 2, per)
+    uu, vv = [], []
+    for b in range(nb):
+        L = r.uniform(0.05, 1.0) * 5e8
+        ang = r.uniform(0, np.pi)
+        dec = r.uniform(0.3, 1.2)
+        uu.append(L * np.cos(h + ang))
+        vv.append(L * np.sin(h + ang) * np.sin(dec))
+    u = np.concatenate(uu)[:a.n_vis]
+    v = np.concatenate(vv)[:a.n_vis]
+    return (j, im, p64), x, y, u, v
+
+
+def vis_bench(a):
+    """Visibilities of the c1 Stokes-I image (262144 scattered pixels) at
+    n_vis (u, v) points: one jsx_visibilities_device launch per step, inputs
+    resident in HBM; the image comes from one untimed mixed-precision solve."""
+    import torch
+
+    from jetshow_amd import abi, models
+    from oracle import fourier as ofour
+    dev = torch.device("cuda:0")
+    torch.cuda.init()
+    ctx = new_context(a)
+    (j, im, p64), x, y, u, v = vis_pixels_and_uv(a)
+    pm = models.with_precision(p64, abi.PREC_MIXED, rel_tol=a.rel_tol)
+    pm.flags |= OPT_PERSISTENT | OPT_APEX_FIRST
+    img = ctx.observe(j, im, pm)["I"].ravel()
+    n_src, n_vis = img.size, u.size
+    d = {k: torch.from_numpy(np.ascontiguousarray(t)).to(dev) for k, t in
+         (("x", x), ("y", y), ("c", img), ("u", u), ("v", v))}
+    prec = abi.PREC_MIXED if a.vis_precision == "mixed" else abi.PREC_FP64
+    stream = torch.cuda.Stream()
+    outs = {p_: torch.zeros(2 * n_vis, dtype=torch.float64, device=dev) for p_ in (abi.PREC_MIXED, abi.PREC_FP64)}
+
+    def run(precision):
+        ctx.visibilities_device(d["x"].data_ptr(), d["y"].data_ptr(), d["c"].data_ptr(), None, n_src,
+                                d["u"].data_ptr(), d["v"].data_ptr(), n_vis, outs[precision].data_ptr(),
+                                precision=precision, stream=stream.cuda_stream)
+
+    for i in range(a.warmup):
+        _log("vis warmup %d/%d" % (i + 1, a.warmup))
+        run(prec)
+    stream.synchronize()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
+    t0 = time.perf_counter()
+    for e0, e1 in evs:
+        e0.record(stream)
+        run(prec)
+        e1.record(stream)
+    stream.synchronize()
+    wall = time.perf_counter() - t0
+    lat = np.array([e0.elapsed_time(e1) for e0, e1 in evs])
+    ms = float(lat.mean())
+    _log("vis: %.3f ms per launch (events), %.3f ms wall per step" % (ms, wall / a.steps * 1e3))
+    # accuracy of the timed precision against FP64 on the GPU and the oracle on a sample
+    other = abi.PREC_FP64 if prec == abi.PREC_MIXED else abi.PREC_MIXED
+    run(other)
+    stream.synchronize()
+    o = outs[prec].cpu().numpy()
+    vis = o[0::2] + 1j * o[1::2]
+    o64 = outs[abi.PREC_FP64].cpu().numpy()
+    vis64 = o64[0::2] + 1j * o64[1::2]
+    flux = float(np.abs(img).sum())
+    sample = np.arange(0, n_vis, max(1, n_vis // 256))
+    ref = ofour.direct_sum(x, y, img, u[sample], v[sample])
+    acc = {"max_err_vs_fp64_over_flux": float(np.max(np.abs(vis - vis64)) / flux),
+           "rel_l2_vs_fp64": float(np.linalg.norm(vis - vis64) / np.linalg.norm(vis64)),
+           "fp64_vs_oracle_sample_over_flux": float(np.max(np.abs(vis64[sample] - ref)) / flux),
+           "oracle_sample": int(sample.size)}
+    trig = int(ctx.get_option("vis_trig"))
+    variant = "fp64" if prec == abi.PREC_FP64 else ("mixed_hw" if trig == 1 else "mixed_poly")
+    ops = OPS_PER_PAIR[variant]
+    pairs = float(n_src) * n_vis
+    ach = (ops["f64"] + ops["f32"]) * pairs / (ms / 1e3) / 1e12
+    peak = _peak(ops)
+    cpu = None
+    if not a.no_cpu:
+        from threadpoolctl import threadpool_limits
+        t_used, done = 0.0, 0
+        with threadpool_limits(1):
+            while t_used < min(a.cpu_seconds, 10.0):
+                k = np.arange(done * 7 % n_vis, n_vis, 4099)[:16]
+                t1 = time.perf_counter()
+                ofour.direct_sum(x, y, img, u[k], v[k])
+                t_used += time.perf_counter() - t1
+                done += len(k)
+        cpu = {"value": done / t_used, "unit": "visibilities/s", "cores": 1, "kind": "port",
+               "sample": "%d visibilities (every 4099th, several offsets) of the same image, all %d pixels, FP64 "
+                         "numpy direct sum, %.1f s" % (done, n_src, t_used)}
+    out = {
+        "metric": "visibilities/s of a model image (direct sum over its pixels)",
+        "value": n_vis / (ms / 1e3), "unit": "visibilities/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if prec == abi.PREC_MIXED else "f64", "data": "synthetic (c1 model image, "
+        "Earth-rotation uv tracks)",
+        "config": {"workload": "visibilities of the c1 paraboloid Stokes-I image (%d pixels) at %d scattered (u, v), "
+                               "|uv| <= 5e8 wavelengths" % (n_src, n_vis),
+                   "n_pixels": n_src, "n_vis": n_vis, "pairs_per_s": pairs / (ms / 1e3)},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "accuracy": acc,
+        "roofline": {"bound": "mfma", "pipe": "valu", "kernel": "jsx_k_vis", "achieved": ach, "peak": peak,
+                     "unit": "TFLOP/s", "frac": ach / peak, "traffic": None,
+                     "valu_issue": {"variant": variant, "instr_per_pair": VALU_INSTR_PER_PAIR[variant],
+                                    "achieved_T_per_s": VALU_INSTR_PER_PAIR[variant] * pairs / (ms / 1e3) / 1e12,
+                                    "peak_T_per_s": VALU_ISSUE_PEAK_T,
+                                    "frac": VALU_INSTR_PER_PAIR[variant] * pairs / (ms / 1e3) / 1e12 /
+                                    VALU_ISSUE_PEAK_T},
+                     "note": "VALU-bound sine/cosine per pixel-visibility pair (no MFMA shape: the exponent couples "
+                             "every pixel with every visibility); events span jsx_k_vis and its chunk reduction"},
+        "cpu_baseline": cpu,
+    }
+    if a.options:
+        out["config"]["options"] = a.options
+    line = json.dumps(out)
+    print(line, flush=True)
+    if a.json_out:
+        with open(a.json_out, "w") as f:
+            f.write(line + "\n")
+    ctx.close()
 
 
 def sweep_cpu_baseline(reqs_py, seconds):

@@ -225,6 +225,9 @@ int jsx_abi_version(void);
  *   "split_floor_max" cap of the propagator-column error loosening (1 = off;
  *                     >1 trades Q/U/V accuracy for fewer piece steps)
  *   "timing"          1: record HIP events around the batched kernels
+ *   "vis_trig"        mixed-precision visibilities: 1 (default) the hardware
+ *                     sine and cosine; 0 FP32 polynomials after an exact
+ *                     quarter-turn reduction (same accuracy, 2x slower)
  * Read-only (wait for the device): "stat_parked" rays split by the last
  * batched launch ("stat_parked_exp" of them with exponential pieces),
  * "stat_pieces" its pieces, "stat_piece_steps_max" / "stat_piece_steps_mean"
@@ -289,6 +292,32 @@ int jsx_image_pixel_sizes(const jsx_image_desc *image, double *sizes);
 
 /* Angular-size distance factor, mas_to_pc (src/utils.cpp:186-208). Host only. */
 double jsx_mas_to_pc(double redshift);
+
+/* ---- visibilities of an image (the step after the path in the fitting loop) ----
+ * Replaces FINUFFT_NUNU.forward (fourier.py:39-94: finufft nufft2d3, isign -1,
+ * eps 1e-6) and FINUFFT.forward (fourier.py:11-34: nufft2d2, eps 1e-8) as
+ * JetModelZoom.ft calls them (jetmodel.py:114-132), with the exact direct sum
+ *
+ *     vis[k] = sum_j (c_re[j] + i c_im[j]) exp(s 2 pi i (u[k] x[j] + v[k] y[j])),
+ *
+ * s = +1 if isign >= 0 else -1; u, v in cycles per unit of x, y (wavelengths
+ * and radians), so the phase is in revolutions (finufft's frequencies are
+ * 2 pi u).  c_im may be NULL (a real image).  vis: n_vis complex doubles,
+ * interleaved (re, im).  precision JSX_PREC_FP64: FP64 sine and cosine;
+ * JSX_PREC_MIXED: FP64 phase and reduction, FP32 sine, cosine and partial
+ * sums over 64 pixels (error ~1e-7 of sum |c|).  n_src == 0 gives zeros.
+ * Host buffers, synchronous. */
+int jsx_visibilities(jsx_ctx *ctx, const double *x, const double *y, const double *c_re, const double *c_im,
+                     int64_t n_src, const double *u, const double *v, int64_t n_vis, int32_t isign,
+                     int32_t precision, double *vis);
+
+/* The same on device buffers, asynchronous on `stream` (NULL: the context's
+ * stream) -- e.g. the I plane of jsx_observe's device outputs as c_re.
+ * Calls on one context share a partial-sum buffer; a launch on another
+ * stream waits for the previous launch's use of it. */
+int jsx_visibilities_device(jsx_ctx *ctx, const double *x, const double *y, const double *c_re,
+                            const double *c_im, int64_t n_src, const double *u, const double *v, int64_t n_vis,
+                            int32_t isign, int32_t precision, double *vis, void *stream);
 
 /* ---- batched and served requests (no reference counterpart beyond
  *      get_i_image called in a loop by JetModelZoom, jetmodel.py:92-99) ----- */

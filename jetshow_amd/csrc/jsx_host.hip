@@ -21,6 +21,7 @@
 #include "jsx_kernels.h"
 #include "jsx_mixed.h"
 #include "jsx_batch.h"
+#include "jsx_fourier.h"
 
 
 #define JSX_OPT_PERSISTENT 0x100u   /* internal: persistent queue kernel        */
@@ -97,6 +98,12 @@ struct jsx_ctx {
     float mg_rel = 2e-6f;               /* smallest relative tolerance of exponential steps */
     SplitWS split{};
     void *split_mem = nullptr;
+    /* visibilities: chunk partial sums (grow-only) and the event after their
+       last reader, which a launch on another stream waits for */
+    double2 *d_vis_part = nullptr;
+    size_t vis_part_cap = 0;
+    hipEvent_t vis_ev = nullptr;
+    int vis_trig = 1;                   /* mixed visibilities: 0 polynomial sine/cosine, 1 hardware */
     /* optional event timing of the batched launches (option "timing") */
     int timing = 0;
     struct EvSet { hipEvent_t e[7]; };
@@ -172,8 +179,10 @@ extern "C" int jsx_ctx_destroy(jsx_ctx *ctx) {
     (void)hipFree(ctx->d_live); (void)hipFree(ctx->d_sorted); (void)hipFree(ctx->d_bucket);
     (void)hipFree(ctx->d_sbucket);
     (void)hipFree(ctx->split_mem);
+    (void)hipFree(ctx->d_vis_part);
+    if (ctx->vis_ev) (void)hipEventDestroy(ctx->vis_ev);
     for (auto &es : ctx->evs)
-        for (int k = 0; k < 5; ++k) (void)hipEventDestroy(es.e[k]);
+        for (int k = 0; k < 7; ++k) (void)hipEventDestroy(es.e[k]);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return JSX_OK;
@@ -190,6 +199,9 @@ extern "C" int jsx_ctx_set_option(jsx_ctx *ctx, const char *name, double v) {
         if (!(v >= 1 && v <= 16)) return fail(ctx, JSX_EINVAL, "blocks_per_cu must be in [1, 16]");
         ctx->persist_blocks_per_cu = (int)v;
         if (ctx->split_mem) split_target(ctx);
+    } else if (k == "vis_trig") {
+        if (!(v == 0.0 || v == 1.0)) return fail(ctx, JSX_EINVAL, "vis_trig must be 0 or 1");
+        ctx->vis_trig = (int)v;
     } else if (k == "queue_block") {
         if (!(v >= 1 && v <= (double)(1 << 20))) return fail(ctx, JSX_EINVAL, "queue_block must be in [1, 2^20]");
         ctx->queue_block = (int64_t)v;
@@ -258,6 +270,7 @@ extern "C" int jsx_ctx_get_option(jsx_ctx *ctx, const char *name, double *v) {
     std::string k(name);
     if (k == "blocks_per_cu") *v = ctx->persist_blocks_per_cu;
     else if (k == "queue_block") *v = (double)ctx->queue_block;
+    else if (k == "vis_trig") *v = ctx->vis_trig;
     else if (k == "steal") *v = ctx->steal;
     else if (k == "defer_rhs") *v = ctx->defer_rhs;
     else if (k == "split_w_min") *v = ctx->split_w_min;
@@ -1080,6 +1093,142 @@ extern "C" int jsx_tau_stripe(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_i
     (void)hipFree(d_tau);
     if (e != hipSuccess) return fail(ctx, JSX_EHIP, "tau stripe failed: %s", hipGetErrorString(e));
     return JSX_OK;
+}
+
+/* ---------------- visibilities (jsx_fourier.h) ----------------
+ * Launch shape: JSX_VIS_BLOCK x JSX_VIS_R visibilities per block; the pixel
+ * list is cut into chunks of at least 512 pixels until the grid has about
+ * eight blocks per CU, and chunk partial sums are added by jsx_k_vis_reduce
+ * in chunk order. */
+static int vis_check(jsx_ctx *ctx, const double *x, const double *y, const double *c_re, int64_t n_src,
+                     const double *u, const double *v, int64_t n_vis, int32_t precision, const double *vis) {
+    if (!ctx) return JSX_EINVAL;
+    if (n_src < 0 || n_vis < 0) return fail(ctx, JSX_EINVAL, "negative pixel or visibility count");
+    if (precision != JSX_PREC_FP64 && precision != JSX_PREC_MIXED)
+        return fail(ctx, JSX_EINVAL, "precision must be JSX_PREC_FP64 or JSX_PREC_MIXED");
+    if (n_src > 0 && (!x || !y || !c_re)) return fail(ctx, JSX_EINVAL, "null pixel buffer");
+    if (n_vis > 0 && (!u || !v || !vis)) return fail(ctx, JSX_EINVAL, "null visibility buffer");
+    if ((n_vis + (int64_t)JSX_VIS_BLOCK * JSX_VIS_R - 1) / ((int64_t)JSX_VIS_BLOCK * JSX_VIS_R) > 0x7fffffffLL)
+        return fail(ctx, JSX_EINVAL, "too many visibilities for one launch");
+    return JSX_OK;
+}
+
+static int vis_launch(jsx_ctx *ctx, const double *x, const double *y, const double *cre, const double *cim,
+                      int64_t n_src, const double *u, const double *v, int64_t n_vis, int32_t isign,
+                      int32_t precision, double *vis, hipStream_t st) {
+    if (n_vis == 0) return JSX_OK;
+    double2 *out = (double2 *)vis;
+    if (n_src == 0) {
+        hipLaunchKernelGGL(jsx_k_vis_zero, dim3((unsigned)((n_vis + 255) / 256)), dim3(256), 0, st, out, n_vis);
+        HIPCHECK(ctx, hipGetLastError());
+        return JSX_OK;
+    }
+    const int64_t per_block = (int64_t)JSX_VIS_BLOCK * JSX_VIS_R;
+    const int64_t n_tb = (n_vis + per_block - 1) / per_block;
+    const int64_t want = 8LL * ctx->n_cu;
+    int64_t n_chunks = (want + n_tb - 1) / n_tb;
+    const int64_t max_chunks = (n_src + 511) / 512;
+    if (n_chunks > max_chunks) n_chunks = max_chunks;
+    if (n_chunks > 4096) n_chunks = 4096;
+    if (n_chunks < 1) n_chunks = 1;
+    const int64_t chunk = (n_src + n_chunks - 1) / n_chunks;
+    n_chunks = (n_src + chunk - 1) / chunk;
+    double2 *dst = out;
+    if (n_chunks > 1) {
+        if (!ctx->vis_ev) HIPCHECK(ctx, hipEventCreateWithFlags(&ctx->vis_ev, hipEventDisableTiming));
+        const size_t need = (size_t)n_chunks * (size_t)n_vis;
+        if (need > ctx->vis_part_cap) {
+            HIPCHECK(ctx, hipEventSynchronize(ctx->vis_ev));
+            (void)hipFree(ctx->d_vis_part);
+            ctx->d_vis_part = nullptr;
+            ctx->vis_part_cap = 0;
+            if (hipMalloc(&ctx->d_vis_part, need * sizeof(double2)) != hipSuccess)
+                return fail(ctx, JSX_ENOMEM, "visibility partial sums: %zu bytes", need * sizeof(double2));
+            ctx->vis_part_cap = need;
+        }
+        /* the partial-sum buffer is shared by every launch of this context */
+        HIPCHECK(ctx, hipStreamWaitEvent(st, ctx->vis_ev, 0));
+        dst = ctx->d_vis_part;
+    }
+    const dim3 grid((unsigned)n_tb, (unsigned)n_chunks);
+    const double sgn = isign >= 0 ? 1.0 : -1.0;
+    const bool mixed = precision == JSX_PREC_MIXED;
+    const bool hw = mixed && ctx->vis_trig == 1;
+#define JSX_VIS_LAUNCH(M, C, H)                                                                                    \
+    hipLaunchKernelGGL((jsx_k_vis<M, C, H>), grid, dim3(JSX_VIS_BLOCK), 0, st, x, y, cre, cim, u, v, dst, n_src, n_vis, \
+                       chunk, sgn)
+    if (!mixed) {
+        if (cim) JSX_VIS_LAUNCH(false, true, false);
+        else JSX_VIS_LAUNCH(false, false, false);
+    } else if (hw) {
+        if (cim) JSX_VIS_LAUNCH(true, true, true);
+        else JSX_VIS_LAUNCH(true, false, true);
+    } else {
+        if (cim) JSX_VIS_LAUNCH(true, true, false);
+        else JSX_VIS_LAUNCH(true, false, false);
+    }
+#undef JSX_VIS_LAUNCH
+    HIPCHECK(ctx, hipGetLastError());
+    if (n_chunks > 1) {
+        hipLaunchKernelGGL(jsx_k_vis_reduce, dim3((unsigned)((n_vis + 255) / 256)), dim3(256), 0, st, dst, out, n_vis,
+                           (int)n_chunks);
+        HIPCHECK(ctx, hipGetLastError());
+        HIPCHECK(ctx, hipEventRecord(ctx->vis_ev, st));
+    }
+    return JSX_OK;
+}
+
+extern "C" int jsx_visibilities_device(jsx_ctx *ctx, const double *x, const double *y, const double *c_re,
+                                       const double *c_im, int64_t n_src, const double *u, const double *v,
+                                       int64_t n_vis, int32_t isign, int32_t precision, double *vis, void *stream) {
+    int rc = vis_check(ctx, x, y, c_re, n_src, u, v, n_vis, precision, vis);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    return vis_launch(ctx, x, y, c_re, c_im, n_src, u, v, n_vis, isign, precision, vis,
+                      stream ? (hipStream_t)stream : ctx->stream);
+}
+
+extern "C" int jsx_visibilities(jsx_ctx *ctx, const double *x, const double *y, const double *c_re,
+                                const double *c_im, int64_t n_src, const double *u, const double *v, int64_t n_vis,
+                                int32_t isign, int32_t precision, double *vis) {
+    int rc = vis_check(ctx, x, y, c_re, n_src, u, v, n_vis, precision, vis);
+    if (rc) return rc;
+    if (n_vis == 0) return JSX_OK;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    const size_t nsrc_arrays = c_im ? 4 : 3;
+    const size_t bytes = sizeof(double) * ((size_t)n_src * nsrc_arrays + (size_t)n_vis * 4);
+    char *mem = nullptr;
+    if (hipMalloc(&mem, bytes) != hipSuccess) return fail(ctx, JSX_ENOMEM, "visibility buffers: %zu bytes", bytes);
+    double *dx = (double *)mem, *dy = dx + n_src, *dcr = dy + n_src, *dci = c_im ? dcr + n_src : nullptr;
+    double *du = dcr + n_src * (c_im ? 2 : 1), *dv = du + n_vis, *dvis = dv + n_vis;
+    hipStream_t st = ctx->stream;
+    hipError_t e = hipSuccess;
+    const size_t sb = sizeof(double) * (size_t)n_src, vb = sizeof(double) * (size_t)n_vis;
+    if (n_src > 0) {
+        if (e == hipSuccess) e = hipMemcpyAsync(dx, x, sb, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(dy, y, sb, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(dcr, c_re, sb, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess && c_im) e = hipMemcpyAsync(dci, c_im, sb, hipMemcpyHostToDevice, st);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(du, u, vb, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dv, v, vb, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(st);
+        (void)hipFree(mem);
+        return fail(ctx, JSX_EHIP, "visibility upload: %s", hipGetErrorString(e));
+    }
+    rc = vis_launch(ctx, dx, dy, dcr, dci, n_src, du, dv, n_vis, isign, precision, dvis, st);
+    if (rc == JSX_OK) {
+        e = hipMemcpyAsync(vis, dvis, 2 * vb, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) rc = fail(ctx, JSX_EHIP, "visibilities: %s", hipGetErrorString(e));
+    } else {
+        (void)hipStreamSynchronize(st);
+    }
+    (void)hipFree(mem);
+    return rc;
 }
 
 /* ---------------- cosmology (host, once per image) ----------------

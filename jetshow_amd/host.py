@@ -22,7 +22,7 @@ ABI_SYMBOLS = [
     "jsx_observe", "jsx_observe_host", "jsx_hit_host", "jsx_image_pixel_sizes",
     "jsx_mas_to_pc", "jsx_observe_batch", "jsx_observe_batch_device", "jsx_ctx_set_option", "jsx_ctx_get_option",
     "jsx_server_create", "jsx_server_submit", "jsx_server_poll", "jsx_server_wait", "jsx_server_stats",
-    "jsx_server_destroy", "jsx_tau_stripe",
+    "jsx_server_destroy", "jsx_tau_stripe", "jsx_visibilities", "jsx_visibilities_device",
 ]
 
 
@@ -157,6 +157,36 @@ class Context:
             out.append(planes)
         self._check(lib().jsx_observe_batch(self._h, reqs, len(items)))
         return out
+
+    def visibilities(self, x, y, c, u, v, isign=-1, precision=abi.PREC_FP64):
+        """Exact direct-sum visibilities (jsx_visibilities):
+        vis[k] = sum_j c[j] exp(sign(isign) 2 pi i (u[k] x[j] + v[k] y[j])),
+        u, v in cycles per unit of x, y.  c real or complex; returns complex128
+        [len(u)]."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        y = np.ascontiguousarray(y, dtype=np.float64).reshape(-1)
+        c = np.asarray(c).reshape(-1)
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        if not (x.size == y.size == c.size) or u.size != v.size:
+            raise ValueError("x, y, c and u, v must have matching lengths")
+        cre = np.ascontiguousarray(c.real, dtype=np.float64)
+        cim = np.ascontiguousarray(c.imag, dtype=np.float64) if np.iscomplexobj(c) else None
+        out = np.zeros(u.size, dtype=np.complex128)
+        self._check(lib().jsx_visibilities(self._h, _ptr(x), _ptr(y), _ptr(cre), _ptr(cim), C.c_int64(x.size),
+                                           _ptr(u), _ptr(v), C.c_int64(u.size), C.c_int32(isign),
+                                           C.c_int32(precision), _ptr(out)))
+        return out
+
+    def visibilities_device(self, x, y, c_re, c_im, n_src, u, v, n_vis, vis, isign=-1, precision=abi.PREC_FP64,
+                            stream=None):
+        """jsx_visibilities_device on device addresses (ints, e.g. a torch
+        tensor's data_ptr()); asynchronous on `stream` (a raw hipStream_t or
+        None for the context's stream)."""
+        vp = lambda a: C.c_void_p(a) if a else None
+        self._check(lib().jsx_visibilities_device(self._h, vp(x), vp(y), vp(c_re), vp(c_im), C.c_int64(n_src),
+                                                  vp(u), vp(v), C.c_int64(n_vis), C.c_int32(isign),
+                                                  C.c_int32(precision), vp(vis), vp(stream)))
 
     def tau_stripe(self, jet, image, params):
         """Observation::run_stripe + getStripe("tau"): optical depth of the

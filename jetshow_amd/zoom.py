@@ -11,14 +11,15 @@ Python loop of blocking calls.
 
 Differences: the angular-size distance is the solver's own cosmology
 (`mas_to_pc`, src/utils.cpp:186-208, as pyjetshow uses it) instead of
-astropy's WMAP9 (astropy is not a dependency here); the Fourier transform
-(`ft`, FINUFFT) and the plotting helpers are not part of this path.
+astropy's WMAP9 (astropy is not a dependency here); `ft` evaluates the
+visibilities as the exact direct sum on the GPU (jetshow_amd.fourier) instead
+of finufft; the plotting helpers are not part of this path.
 """
 import math
 
 import numpy as np
 
-from . import abi, host, models
+from . import abi, fourier, host, models
 
 MAS_TO_RAD = abi.MAS_TO_RAD
 
@@ -113,6 +114,23 @@ class JetModelZoom:
     @property
     def flags(self):
         return self._flags
+
+    def ft(self, uv, precision=abi.PREC_MIXED):
+        """Visibilities of the current image at uv [n, 2] (wavelengths)
+        (jetmodel.py:114-132): the image rotated by `rot` (uv @ rot) and shifted
+        by (dx, dy) mas (a phase on the unrotated uv); pixel centres d, r_ob
+        converted to radians.  mixed precision: ~1e-7 of the total flux."""
+        uv = np.asarray(uv, dtype=np.float64).reshape(-1, 2)
+        rot = np.array([[math.cos(self.rot), -math.sin(self.rot)],
+                        [math.sin(self.rot), math.cos(self.rot)]])
+        shift = np.array([self.dx * MAS_TO_RAD, self.dy * MAS_TO_RAD])
+        result = np.exp(-2.0 * np.pi * 1j * (uv @ shift))
+        rad_per_pc = MAS_TO_RAD / self.pc_in_mas
+        x = self.d * rad_per_pc
+        y = self.r_ob * rad_per_pc
+        img = self.image()
+        ft = fourier.FINUFFT_NUNU(uv @ rot, x.ravel(), y.ravel(), ctx=self._context(), precision=precision)
+        return result * ft.forward(img.T.ravel())
 
 
 def run_many(zooms, ctx=None, precision=abi.PREC_FP64, persistent=True):

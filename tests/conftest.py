@@ -14,6 +14,12 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def ctx():
+    # torch's HIP runtime must initialise the device before the library's
+    # context does when one process uses both (torch.cuda reports no GPU
+    # otherwise); tests hand torch device buffers to the library
+    import torch
+    if torch.cuda.device_count() > 0:
+        torch.cuda.init()
     from jetshow_amd import host
     c = host.Context(0)
     yield c

@@ -71,6 +71,8 @@ enum { hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2, hipMemcpyDeviceToDe
 #define hipEventDisableTiming 2
 #define hipStreamCaptureModeThreadLocal 1
 struct hipDeviceProp_t { int multiProcessorCount; };
+struct double2 { double x, y; };
+static inline double2 make_double2(double a, double b) { return double2{a, b}; }
 
 static inline hipError_t hipMalloc(void *p, size_t n) { *(void **)p = calloc(1, n ? n : 1); return hipSuccess; }
 static inline hipError_t hipHostMalloc(void *p, size_t n, int) { return hipMalloc(p, n); }
@@ -89,6 +91,7 @@ static inline hipError_t hipEventCreateWithFlags(hipEvent_t *e, int) { *e = null
 static inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
 static inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
 static inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+static inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
 static inline hipError_t hipEventCreate(hipEvent_t *e) { *e = nullptr; return hipSuccess; }
 static inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) { *ms = 0.f; return hipSuccess; }
 static inline hipError_t hipDeviceSynchronize() { return hipSuccess; }

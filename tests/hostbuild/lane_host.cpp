@@ -414,3 +414,33 @@ extern "C" int64_t lh_split_costs(double *out, int64_t cap) {
     for (int64_t i = 0; i < 4 * n && i < 4 * cap; ++i) out[i] = g_costs[i];
     return n;
 }
+
+/* the mixed visibility kernel's sine and cosine of 2 pi ph (ph in revolutions) */
+extern "C" void lh_vis_sincos(const double *ph, int64_t n, float *s, float *c) {
+    for (int64_t i = 0; i < n; ++i) vis_sincos_rev_mixed(ph[i], s[i], c[i]);
+}
+
+/* one lane of the mixed visibility kernel, one chunk holding every pixel:
+ * FP32 partial sums over JSX_VIS_SUB pixels, FP64 beyond */
+extern "C" void lh_vis_mixed(const double *x, const double *y, const double *cre, int64_t n_src, const double *u,
+                             const double *v, int64_t n_vis, int32_t isign, double *vis) {
+    const double sgn = isign >= 0 ? 1.0 : -1.0;
+    for (int64_t k = 0; k < n_vis; ++k) {
+        double A = 0.0, B = 0.0;
+        for (int64_t jb = 0; jb < n_src; jb += JSX_VIS_SUB) {
+            const int64_t je = jb + JSX_VIS_SUB < n_src ? jb + JSX_VIS_SUB : n_src;
+            float a = 0.0f, b = 0.0f;
+            for (int64_t j = jb; j < je; ++j) {
+                const double ph = fma(u[k], x[j], v[k] * y[j]);
+                float s, c;
+                vis_sincos_rev_mixed(ph, s, c);
+                a = fmaf((float)cre[j], c, a);
+                b = fmaf((float)cre[j], s, b);
+            }
+            A += (double)a;
+            B += (double)b;
+        }
+        vis[2 * k] = A;
+        vis[2 * k + 1] = sgn * B;
+    }
+}

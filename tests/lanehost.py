@@ -123,3 +123,22 @@ def split_costs(cap=1 << 16):
     out = np.zeros(4 * cap)
     n = lib().lh_split_costs(_dp(out), cap)
     return out[:4 * n].reshape(n, 4)
+
+
+def vis_sincos(ph):
+    """The mixed visibility kernel's (sin, cos) of 2 pi ph, in FP32."""
+    ph = np.ascontiguousarray(ph, dtype=np.float64)
+    s = np.zeros(ph.size, dtype=np.float32)
+    c = np.zeros(ph.size, dtype=np.float32)
+    lib().lh_vis_sincos(_dp(ph), C.c_int64(ph.size), s.ctypes.data_as(C.c_void_p), c.ctypes.data_as(C.c_void_p))
+    return s, c
+
+
+def vis_mixed(x, y, c, u, v, isign=-1):
+    """The mixed visibility kernel's arithmetic for a real image, one lane per
+    visibility, on the CPU."""
+    a = [np.ascontiguousarray(t, dtype=np.float64).reshape(-1) for t in (x, y, c, u, v)]
+    out = np.zeros(2 * a[3].size)
+    lib().lh_vis_mixed(_dp(a[0]), _dp(a[1]), _dp(a[2]), C.c_int64(a[0].size), _dp(a[3]), _dp(a[4]),
+                       C.c_int64(a[3].size), C.c_int32(isign), _dp(out))
+    return out[0::2] + 1j * out[1::2]
