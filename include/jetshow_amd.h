@@ -41,7 +41,11 @@ typedef enum jsx_status {
 typedef enum jsx_geometry_kind {
     JSX_GEOM_CONE = 1,        /* src/Cone.cpp:16-118 (double-nappe cone)          */
     JSX_GEOM_PARABOLOID = 2,  /* src/Parabaloid.cpp:11-60                          */
-    JSX_GEOM_CYLINDER = 3     /* src/Cylinder.cpp:9-99 (r^2 fixed; see flags)      */
+    JSX_GEOM_CYLINDER = 3,    /* src/Cylinder.cpp:9-99 (r^2 fixed; see flags)      */
+    JSX_GEOM_CONVEX_HULL = 4  /* SimulationGeometry (src/Geometry.cpp:92-132) over
+                                 the convex hull of the simulation's points
+                                 (src/main.cpp:873-897), as face planes uploaded
+                                 with jsx_hull_upload (handle in geo_table)      */
 } jsx_geometry_kind;
 
 typedef enum jsx_bfield_kind {
@@ -85,7 +89,7 @@ typedef struct jsx_jet_desc {
     /* B field */
     int32_t b_in_plasma_frame;    /* VectorBField::in_plasma_frame_ (BField.h:36-44) */
     int32_t grid_b, grid_b2;      /* grid handles for JSX_B_GRID_*                 */
-    int32_t pad0;
+    int32_t geo_table;            /* jsx_hull_upload handle for JSX_GEOM_CONVEX_HULL */
     double b0, b_exp, b_pitch;    /* [G], exponent, helical pitch [rad]            */
     /* electron density */
     int32_t n_in_plasma_frame;    /* NField::in_plasma_frame_ (NField.cpp:7-14)    */
@@ -250,6 +254,16 @@ int jsx_grid_upload(jsx_ctx *ctx, const jsx_grid_desc *desc, const double *value
                     int32_t *handle_out);
 int jsx_grid_release(jsx_ctx *ctx, int32_t handle);
 
+/* Face planes of a convex polyhedron for JSX_GEOM_CONVEX_HULL, the GPU form of
+ * SimulationGeometry's AABB tree over the convex hull of the simulation points
+ * (src/Geometry.cpp:92-132, src/main.cpp:873-897): planes[4 i .. 4 i + 3] =
+ * (n_x, n_y, n_z, h) in cm, a point p inside when n . p <= h for every plane
+ * (n need not be unit; n_planes >= 4).  The hit is the line's entry and exit
+ * points, ordered along the ray direction (the reference orders its two
+ * intersection points by the sign of x, which is the same order for the rays of
+ * an image plane).  Released with jsx_grid_release. */
+int jsx_hull_upload(jsx_ctx *ctx, const double *planes, int32_t n_planes, int32_t *handle_out);
+
 /* ---- device buffers for callers without their own allocator ------------- */
 int jsx_alloc_outputs(jsx_ctx *ctx, int64_t n_pixels, int32_t stokes, jsx_outputs *out);
 int jsx_free_outputs(jsx_ctx *ctx, jsx_outputs *out);
@@ -292,6 +306,31 @@ int jsx_image_pixel_sizes(const jsx_image_desc *image, double *sizes);
 
 /* Angular-size distance factor, mas_to_pc (src/utils.cpp:186-208). Host only. */
 double jsx_mas_to_pc(double redshift);
+
+/* ---- the reference's text formats (host only, no GPU) ----------------------
+ * jsx_write_map: write_2dvector (src/utils.cpp:211-224) of a row-major plane
+ * [n_rows][n_cols] into `path`, appended (append != 0, as src/main.cpp:
+ * 1447-1462 opens the map files) or truncated: value / scale in the stream's
+ * default format (printf %g) followed by a space, a newline per row.
+ * jsx_write_stripe: write_vector (src/utils.cpp:227-236), "%g \n" per value. */
+int jsx_write_map(const char *path, const double *plane, int64_t n_rows, int64_t n_cols, double scale,
+                  int32_t append);
+int jsx_write_stripe(const char *path, const double *values, int64_t n, double scale, int32_t append);
+
+/* read_from_txt (src/utils.cpp:381-388): rows of three numbers (z [cm],
+ * r [cm], value) until the first that does not parse; *n_points = rows read,
+ * their values in zrv[3 * i + k] up to `cap` rows (cap 0: count only;
+ * JSX_EINVAL when cap > 0 is too small).  JSX_ENOTFOUND: no such file. */
+int jsx_read_field_text(const char *path, double *zrv, int64_t cap, int64_t *n_points);
+
+/* create_triangulation (src/SimulationInterpolater.cpp:105-117) for points
+ * that form a complete regular grid: coordinates divided by pc, distinct z and
+ * r values uniformly spaced (1e-9), every node present once.  Fills `desc`
+ * (and, when values != NULL, values[i * nr + j] at (z0 + i dz, r0 + j dr), cap
+ * >= nz * nr) for jsx_grid_upload.  JSX_EUNSUPPORTED: the points are not such a
+ * grid (scattered Delaunay data: resample it onto a regular grid first). */
+int jsx_grid_from_points(const double *zrv, int64_t n_points, double nan_value, jsx_grid_desc *desc,
+                         double *values, int64_t cap);
 
 /* ---- visibilities of an image (the step after the path in the fitting loop) ----
  * Replaces FINUFFT_NUNU.forward (fourier.py:39-94: finufft nufft2d3, isign -1,

@@ -12,7 +12,7 @@ STATUS_NAMES = {
     -7: "JSX_ENOTFOUND", -8: "JSX_ENODEV",
 }
 
-GEOM_CONE, GEOM_PARABOLOID, GEOM_CYLINDER = 1, 2, 3
+GEOM_CONE, GEOM_PARABOLOID, GEOM_CYLINDER, GEOM_CONVEX_HULL = 1, 2, 3, 4
 B_RANDOM_SCALAR, B_RANDOM_SCALAR_Z, B_TOROIDAL, B_TOROIDAL_R, B_HELICAL_R, \
     B_GRID_SCALAR, B_GRID_VECTOR, B_CONST_CYLINDER_Z = 1, 2, 3, 4, 5, 6, 7, 8
 N_CONST, N_BK, N_BK_Z, N_GRID = 1, 2, 3, 4
@@ -42,7 +42,7 @@ class JetDesc(C.Structure):
         ("geo_origin", C.c_double * 3), ("geo_axis", C.c_double * 3),
         ("geo_param", C.c_double), ("geo_big_scale", C.c_double),
         ("z_clip_min", C.c_double), ("z_clip_max", C.c_double),
-        ("b_in_plasma_frame", C.c_int32), ("grid_b", C.c_int32), ("grid_b2", C.c_int32), ("pad0", C.c_int32),
+        ("b_in_plasma_frame", C.c_int32), ("grid_b", C.c_int32), ("grid_b2", C.c_int32), ("geo_table", C.c_int32),
         ("b0", C.c_double), ("b_exp", C.c_double), ("b_pitch", C.c_double),
         ("n_in_plasma_frame", C.c_int32), ("grid_n", C.c_int32),
         ("n0", C.c_double), ("n_exp", C.c_double), ("thermal_fraction", C.c_double),

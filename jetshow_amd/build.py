@@ -26,7 +26,7 @@ FLAGS = [
 
 
 def sources():
-    return [os.path.join(CSRC, "jsx_host.hip")]
+    return [os.path.join(CSRC, "jsx_host.hip"), os.path.join(CSRC, "jsx_textio.cpp")]
 
 
 CXX_DIR = os.path.join(HERE, "cxx")

@@ -52,6 +52,8 @@ struct DModel {
     double e_ki, e_ei, e_kv, e_kCr;
     DGrid gb, gb2, gn, gv;
     int has_gb, has_gb2, has_gn, has_gv;
+    DGrid hull;          /* JSX_GEOM_CONVEX_HULL: face planes [nz][4] = (n, h) in cm */
+    int has_hull;
 };
 
 /* ---------------- 3-vector helpers ---------------- */
@@ -229,6 +231,33 @@ __device__ inline int d_clip_seg(const DModel &m, DSeg &s) {
     return 2;
 }
 
+/* SimulationGeometry::hit (src/Geometry.cpp:97-132) on the convex hull given
+ * by its face planes n . p <= h: the line o + t d enters where the last plane
+ * it crosses inwards is crossed (largest t over planes with n . d < 0) and
+ * leaves at the first outward crossing (smallest t over n . d > 0); a line
+ * parallel to a plane outside it misses.  One segment, entry to exit along d,
+ * when entry < exit. */
+__device__ inline int d_hull_hit(const DModel &m, const double *o, const double *d, DSeg *segs) {
+    double t_in = -INFINITY, t_out = INFINITY;
+    const double *pl = m.hull.v;
+    for (int i = 0; i < m.hull.nz; ++i) {
+        const double *q = pl + 4 * i;
+        const double nd = d_dot3(q, d);
+        const double gap = q[3] - d_dot3(q, o);
+        if (nd == 0.0) {
+            if (gap < 0.0) return 0;
+        } else {
+            const double t = gap / nd;
+            if (nd < 0.0) t_in = t > t_in ? t : t_in;
+            else t_out = t < t_out ? t : t_out;
+        }
+    }
+    if (!(t_in < t_out) || !(t_in > -INFINITY) || !(t_out < INFINITY)) return 0;
+    d_ray_point(o, d, t_in, segs[0].pin);
+    d_ray_point(o, d, t_out, segs[0].pout);
+    return 1;
+}
+
 /* Jet::hit + clip; *clipped set if the clip changed anything.  The two
  * possible segments are handled with constant indices so they stay in
  * registers. */
@@ -238,6 +267,7 @@ __device__ inline int d_model_hit(const DModel &m, const double *o, const double
     if (m.geometry == JSX_GEOM_CONE) n = d_cone_hit(m, o, d, segs);
     else if (m.geometry == JSX_GEOM_PARABOLOID) n = d_paraboloid_hit(m, o, d, segs);
     else if (m.geometry == JSX_GEOM_CYLINDER) n = d_cylinder_hit(m, o, d, segs);
+    else if (m.geometry == JSX_GEOM_CONVEX_HULL) n = d_hull_hit(m, o, d, segs);
     if (m.clip && n > 0) {
         DSeg a = segs[0], b = segs[1];
         int ra = d_clip_seg(m, a);

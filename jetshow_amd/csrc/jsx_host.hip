@@ -30,6 +30,7 @@ struct GridRec {
     jsx_grid_desc desc;
     double *dev;
     size_t bytes;
+    int hull = 0;        /* face planes of a convex hull (jsx_hull_upload) */
 };
 
 struct ImageRec {
@@ -385,6 +386,23 @@ extern "C" int jsx_grid_upload(jsx_ctx *ctx, const jsx_grid_desc *desc, const do
     return JSX_OK;
 }
 
+extern "C" int jsx_hull_upload(jsx_ctx *ctx, const double *planes, int32_t n_planes, int32_t *handle_out) {
+    if (!ctx || !planes || !handle_out) return fail(ctx, JSX_EINVAL, "null argument");
+    if (n_planes < 4) return fail(ctx, JSX_EINVAL, "a bounded convex hull needs at least 4 planes");
+    for (int64_t i = 0; i < 4 * (int64_t)n_planes; ++i)
+        if (!std::isfinite(planes[i])) return fail(ctx, JSX_EINVAL, "non-finite plane coefficient");
+    jsx_grid_desc d;
+    std::memset(&d, 0, sizeof d);
+    d.nz = n_planes;
+    d.nr = 4;
+    d.dz = d.dr = 1.0;
+    int rc = jsx_grid_upload(ctx, &d, planes, handle_out);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->grids[*handle_out].hull = 1;
+    return JSX_OK;
+}
+
 extern "C" int jsx_grid_release(jsx_ctx *ctx, int32_t handle) {
     if (!ctx) return JSX_EINVAL;
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -421,7 +439,7 @@ static int lower_grid(jsx_ctx *ctx, int32_t h, DGrid &g, int &has) {
 
 static int lower_model(jsx_ctx *ctx, const jsx_jet_desc &j, unsigned opt, DModel &m) {
     std::memset(&m, 0, sizeof m);
-    if (j.geometry < JSX_GEOM_CONE || j.geometry > JSX_GEOM_CYLINDER)
+    if (j.geometry < JSX_GEOM_CONE || j.geometry > JSX_GEOM_CONVEX_HULL)
         return fail(ctx, JSX_EUNSUPPORTED, "geometry kind %d", j.geometry);
     if (j.bfield < JSX_B_RANDOM_SCALAR || j.bfield > JSX_B_CONST_CYLINDER_Z)
         return fail(ctx, JSX_EUNSUPPORTED, "bfield kind %d", j.bfield);
@@ -464,6 +482,12 @@ static int lower_model(jsx_ctx *ctx, const jsx_jet_desc &j, unsigned opt, DModel
     m.e_kv = (s + 1.) / 2.;
     m.e_kCr = (s - 2.) / 2.;
     int rc;
+    if (j.geometry == JSX_GEOM_CONVEX_HULL) {
+        auto it = ctx->grids.find(j.geo_table);
+        if (it == ctx->grids.end() || !it->second.hull)
+            return fail(ctx, JSX_ENOTFOUND, "geo_table %d is not a hull handle", j.geo_table);
+        if ((rc = lower_grid(ctx, j.geo_table, m.hull, m.has_hull))) return rc;
+    }
     if (j.bfield == JSX_B_GRID_SCALAR || j.bfield == JSX_B_GRID_VECTOR) {
         if ((rc = lower_grid(ctx, j.grid_b, m.gb, m.has_gb))) return rc;
         if (j.bfield == JSX_B_GRID_VECTOR && (rc = lower_grid(ctx, j.grid_b2, m.gb2, m.has_gb2))) return rc;

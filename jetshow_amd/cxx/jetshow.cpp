@@ -41,6 +41,13 @@ Context &Context::instance() {
     static Context ctx(0, 0);
     return ctx;
 }
+int32_t Context::upload_hull(const std::vector<double> &planes) {
+    if (planes.size() % 4) throw Error(JSX_EINVAL, "hull planes come in fours");
+    int32_t h = 0;
+    check(jsx_hull_upload(ctx_, planes.data(), (int32_t)(planes.size() / 4), &h));
+    return h;
+}
+
 int32_t Context::upload_grid(const std::vector<double> &values, int nz, int nr, double z0, double dz, double r0,
                              double dr, double nan_value) {
     if ((size_t)nz * (size_t)nr != values.size()) throw Error(JSX_EINVAL, "grid size mismatch");
@@ -93,6 +100,11 @@ void Cylinder::lower(jsx_jet_desc &d) const {
     set3(d.geo_origin, origin_); set3(d.geo_axis, direction_);
     d.geo_param = r_; d.geo_big_scale = 100. * r_;
     d.z_clip_min = zmin_; d.z_clip_max = zmax_;
+}
+
+void SimulationGeometry::lower(jsx_jet_desc &d) const {
+    d.geometry = JSX_GEOM_CONVEX_HULL;
+    d.geo_table = h_;
 }
 
 /* ---- fields ---- */
@@ -251,5 +263,44 @@ std::vector<double> Observation::getStripe(std::string value) { return imagePlan
 std::pair<unsigned long int, unsigned long int> Observation::getImageSize() { return imagePlane->image_size; }
 
 double mas_to_pc(double z) { return jsx_mas_to_pc(z); }
+
+std::ostream &write_2dvector(std::ostream &os, std::vector<std::vector<double>> &v, double scale) {
+    for (auto &row : v) {
+        for (double x : row) os << x / scale << " ";
+        os << "\n";
+    }
+    return os;
+}
+
+std::ostream &write_vector(std::ostream &os, std::vector<double> &v, double scale) {
+    for (double x : v) os << x / scale << " " << "\n";
+    return os;
+}
+
+void read_from_txt(std::string fntxt, std::vector<std::vector<double>> &properties) {
+    int64_t n = 0;
+    int rc = jsx_read_field_text(fntxt.c_str(), nullptr, 0, &n);
+    if (rc == JSX_ENOTFOUND) return;   /* an unopened ifstream reads nothing */
+    if (rc) throw Error(rc, "cannot read " + fntxt);
+    std::vector<double> buf((size_t)(3 * n));
+    if (n && (rc = jsx_read_field_text(fntxt.c_str(), buf.data(), n, &n))) throw Error(rc, "cannot read " + fntxt);
+    for (int64_t i = 0; i < n; ++i) properties.push_back({buf[3 * i], buf[3 * i + 1], buf[3 * i + 2]});
+}
+
+int32_t create_grid(const std::string &fn, Context &ctx, double nan_value) {
+    int64_t n = 0;
+    int rc = jsx_read_field_text(fn.c_str(), nullptr, 0, &n);
+    if (rc) throw Error(rc, "cannot read " + fn);
+    std::vector<double> pts((size_t)(3 * n));
+    if (n && (rc = jsx_read_field_text(fn.c_str(), pts.data(), n, &n))) throw Error(rc, "cannot read " + fn);
+    jsx_grid_desc g;
+    rc = jsx_grid_from_points(pts.data(), n, nan_value, &g, nullptr, 0);
+    if (rc == JSX_EUNSUPPORTED) throw Unsupported(fn + ": the points are not a complete regular (z, r) grid");
+    if (rc) throw Error(rc, fn + ": bad field points");
+    std::vector<double> vals((size_t)g.nz * (size_t)g.nr);
+    if ((rc = jsx_grid_from_points(pts.data(), n, nan_value, &g, vals.data(), (int64_t)vals.size())))
+        throw Error(rc, fn + ": bad field points");
+    return ctx.upload_grid(vals, g.nz, g.nr, g.z0, g.dz, g.r0, g.dr, nan_value);
+}
 
 }  // namespace jetshow

@@ -20,6 +20,7 @@
 #include <cstdint>
 #include <list>
 #include <map>
+#include <ostream>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -145,6 +146,17 @@ private:
 };
 
 /* ---- fields (include/BField.h, NField.h, VField.h) ---- */
+/* SimulationGeometry (include/Geometry.h:58-66, src/Geometry.cpp:92-132): the
+ * convex hull of the simulation points, as face planes uploaded with
+ * Context::upload_hull (the hull replaces the reference's CGAL AABB tree) */
+class SimulationGeometry : public BaseGeometry {
+public:
+    explicit SimulationGeometry(int32_t hull_handle) : h_(hull_handle) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    int32_t h_;
+};
+
 class ScalarBField {
 public:
     virtual ~ScalarBField() = default;
@@ -343,6 +355,8 @@ public:
     jsx_ctx *get() const { return ctx_; }
     int32_t upload_grid(const std::vector<double> &values, int nz, int nr, double z0, double dz, double r0,
                         double dr, double nan_value);
+    /* planes: n_planes x (n_x, n_y, n_z, h) in cm, inside where n . p <= h */
+    int32_t upload_hull(const std::vector<double> &planes);
     static Context &instance();   /* process-wide default context on device 0 */
     void check(int rc) const;
 
@@ -380,6 +394,18 @@ private:
 
 /* mas_to_pc (src/utils.cpp:205-208) */
 double mas_to_pc(double z);
+
+/* text formats (include/utils.h:192-198, src/utils.cpp:211-236,381-388):
+ * write_2dvector / write_vector print value/scale with the stream's current
+ * format and a space, rows (values) end with a newline; read_from_txt reads
+ * rows of three numbers until the first that does not parse. */
+std::ostream &write_2dvector(std::ostream &os, std::vector<std::vector<double>> &v, double scale = 1.0);
+std::ostream &write_vector(std::ostream &os, std::vector<double> &v, double scale = 1.0);
+void read_from_txt(std::string fntxt, std::vector<std::vector<double>> &properties);
+/* create_triangulation (src/SimulationInterpolater.cpp:105-117) for a text
+ * file whose points form a complete regular (z, r) grid: the grid in HBM, its
+ * handle for the Simulation*Field classes (Unsupported for scattered points) */
+int32_t create_grid(const std::string &fn, Context &ctx, double nan_value = 0.0);
 
 }  // namespace jetshow
 

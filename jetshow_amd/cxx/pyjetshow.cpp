@@ -9,6 +9,7 @@
 #include <pybind11/stl.h>
 
 #include <cmath>
+#include <fstream>
 #include <vector>
 
 #include "jetshow.h"
@@ -106,4 +107,32 @@ PYBIND11_MODULE(pyjetshow, m) {
     args(&get_i_image, "get_i_image", "Obtain Stokes I image with random B-field", m);
     args(&get_i_image_np, "get_i_image_np", "get_i_image returning numpy arrays (tau, I_Jy, flags)", m);
     m.def("mas_to_pc", &mas_to_pc, "parsec per milliarcsecond at redshift z", "z"_a);
+    /* the text formats through the class API's stream writers */
+    m.def(
+        "write_2dvector",
+        [](const std::string &path, std::vector<std::vector<double>> v, double scale, bool append) {
+            std::fstream fs;
+            fs.open(path, std::ios::out | (append ? std::ios::app : std::ios::trunc));
+            if (!fs.is_open()) throw std::runtime_error("cannot open " + path);
+            write_2dvector(fs, v, scale);
+        },
+        "write_2dvector into a file (src/utils.cpp:211-224, src/main.cpp:1447-1462)", "path"_a, "v"_a,
+        "scale"_a = 1.0, "append"_a = true);
+    m.def(
+        "write_vector",
+        [](const std::string &path, std::vector<double> v, double scale, bool append) {
+            std::fstream fs;
+            fs.open(path, std::ios::out | (append ? std::ios::app : std::ios::trunc));
+            if (!fs.is_open()) throw std::runtime_error("cannot open " + path);
+            write_vector(fs, v, scale);
+        },
+        "write_vector into a file (src/utils.cpp:227-236)", "path"_a, "v"_a, "scale"_a = 1.0, "append"_a = true);
+    m.def(
+        "read_from_txt",
+        [](const std::string &path) {
+            std::vector<std::vector<double>> rows;
+            read_from_txt(path, rows);
+            return rows;
+        },
+        "rows of three numbers (src/utils.cpp:381-388)", "path"_a);
 }

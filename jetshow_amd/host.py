@@ -23,6 +23,7 @@ ABI_SYMBOLS = [
     "jsx_mas_to_pc", "jsx_observe_batch", "jsx_observe_batch_device", "jsx_ctx_set_option", "jsx_ctx_get_option",
     "jsx_server_create", "jsx_server_submit", "jsx_server_poll", "jsx_server_wait", "jsx_server_stats",
     "jsx_server_destroy", "jsx_tau_stripe", "jsx_visibilities", "jsx_visibilities_device",
+    "jsx_write_map", "jsx_write_stripe", "jsx_read_field_text", "jsx_grid_from_points", "jsx_hull_upload",
 ]
 
 
@@ -126,6 +127,15 @@ class Context:
         g.z0, g.dz, g.r0, g.dr, g.nan_value = z0, dz, r0, dr, nan_value
         h = C.c_int32()
         self._check(lib().jsx_grid_upload(self._h, C.byref(g), _ptr(v), C.byref(h)))
+        return h.value
+
+    def upload_hull(self, planes):
+        """Face planes [n, 4] = (n_x, n_y, n_z, h) in cm of a convex polyhedron
+        (n . p <= h inside) to HBM for GEOM_CONVEX_HULL; returns its handle
+        (jet.geo_table).  Released with release_grid."""
+        pl = np.ascontiguousarray(planes, dtype=np.float64).reshape(-1, 4)
+        h = C.c_int32()
+        self._check(lib().jsx_hull_upload(self._h, _ptr(pl), C.c_int32(pl.shape[0]), C.byref(h)))
         return h.value
 
     def release_grid(self, handle):

@@ -275,6 +275,34 @@ static int paraboloid_hit(const orc_model *m, const double *o, const double *d, 
     return 1;
 }
 
+/* SimulationGeometry::hit (Geometry.cpp:97-132) on the convex hull of the
+ * simulation points (main.cpp:873-897), given by its face planes n . p <= h
+ * (a grid table [n][4] in cm, handle geo_table).  The reference intersects the
+ * line with every hull facet and orders the two points by the sign of x; on a
+ * convex hull the same two points are the line's last inward and first
+ * outward plane crossings, ordered here along d. */
+static int hull_hit(const orc_model *m, const double *o, const double *d, orc_seg *segs) {
+    const orc_grid *g = grid_of(m, m->j->geo_table);
+    if (!g || g->desc.nr != 4) return 0;
+    double t_in = -INFINITY, t_out = INFINITY;
+    for (int i = 0; i < g->desc.nz; ++i) {
+        const double *q = g->values + 4 * i;
+        double nd = dot3(q, d);
+        double gap = q[3] - dot3(q, o);
+        if (nd == 0.0) {
+            if (gap < 0.0) return 0;
+        } else {
+            double t = gap / nd;
+            if (nd < 0.0) t_in = t > t_in ? t : t_in;
+            else t_out = t < t_out ? t : t_out;
+        }
+    }
+    if (!(t_in < t_out) || !(t_in > -INFINITY) || !(t_out < INFINITY)) return 0;
+    ray_point(o, d, t_in, segs[0].pin);
+    ray_point(o, d, t_out, segs[0].pout);
+    return 1;
+}
+
 /* Cylinder::hit (Cylinder.cpp:35-95), working in parsec as the reference
  * does.  The reference subtracts r/pc where r^2 is meant (:53); the fixed
  * form is the default, JSX_OPT_CYLINDER_REF_BUG selects the reference's. */
@@ -346,6 +374,7 @@ static int model_hit(const orc_model *m, const double *o, const double *d, orc_s
     case JSX_GEOM_CONE: n = cone_hit(m, o, d, segs); break;
     case JSX_GEOM_PARABOLOID: n = paraboloid_hit(m, o, d, segs); break;
     case JSX_GEOM_CYLINDER: n = cylinder_hit(m, o, d, segs); break;
+    case JSX_GEOM_CONVEX_HULL: n = hull_hit(m, o, d, segs); break;
     default: return 0;
     }
     if (m->j->z_clip_max > m->j->z_clip_min) {
@@ -1222,10 +1251,10 @@ int orc_tau_stripe(const jsx_jet_desc *jet, const jsx_image_desc *image, const j
     return 0;
 }
 
-int orc_hit(const jsx_jet_desc *jet, const jsx_solver_params *params, const double *rays, int64_t n,
-            int32_t *n_seg, double *segments) {
+int orc_hit(const jsx_jet_desc *jet, const jsx_solver_params *params, const orc_grid *grids, int32_t n_grids,
+            const double *rays, int64_t n, int32_t *n_seg, double *segments) {
     orc_model m;
-    model_init(&m, jet, NULL, 0, ORC_MATH_LIBM, params ? params->flags : 0);
+    model_init(&m, jet, grids, n_grids, ORC_MATH_LIBM, params ? params->flags : 0);
     for (int64_t q = 0; q < n; ++q) {
         double o[3] = {rays[6 * q], rays[6 * q + 1], rays[6 * q + 2]};
         double d[3] = {rays[6 * q + 3], rays[6 * q + 4], rays[6 * q + 5]};

@@ -45,7 +45,7 @@ int orc_observe(const jsx_jet_desc *jet, const jsx_image_desc *image,
 /* Observation::run_stripe + getStripe("tau"): n_along - n_along/2 values. */
 int orc_tau_stripe(const jsx_jet_desc *jet, const jsx_image_desc *image, const jsx_solver_params *params,
                    const orc_grid *grids, int32_t n_grids, int32_t math_mode, double *tau);
-int orc_hit(const jsx_jet_desc *jet, const jsx_solver_params *params,
+int orc_hit(const jsx_jet_desc *jet, const jsx_solver_params *params, const orc_grid *grids, int32_t n_grids,
             const double *rays, int64_t n, int32_t *n_seg, double *segments);
 
 /* Image plane: pixel sizes, pixel centre coordinates (across, along) and ray

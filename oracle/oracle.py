@@ -92,12 +92,16 @@ def observe(jet, image, params, pixel_idx=None, math=MATH_JSX, threads=0, grids=
     return res
 
 
-def hit(jet, rays, params=None):
+def hit(jet, rays, params=None, grids=None):
+    """Geometry::hit of rays [n, 6]; grids (list of (GridDesc, values), handle
+    h = grids[h - 1]) hold the hull planes of GEOM_CONVEX_HULL."""
     r = np.ascontiguousarray(rays, dtype=np.float64).reshape(-1, 6)
     n = r.shape[0]
     ns = np.zeros(n, dtype=np.int32)
     seg = np.zeros((n, 2, 6))
-    lib().orc_hit(C.byref(jet), C.byref(params) if params is not None else None, _p(r), C.c_int64(n), _p(ns), _p(seg))
+    arr, ng, keep = make_grids(grids)
+    lib().orc_hit(C.byref(jet), C.byref(params) if params is not None else None, arr, ng, _p(r), C.c_int64(n),
+                  _p(ns), _p(seg))
     return ns, seg
 
 
