@@ -67,7 +67,16 @@ struct jsx_ctx {
     int64_t *d_off = nullptr;
     DOut *d_outs = nullptr;
     size_t n_cap = 0;
-    std::vector<unsigned char> h_reqs;
+    /* request tables of batched launches: two pinned host buffers used in
+       turn (each with the event after its copy to the device), and the event
+       after the last launch (a launch on another stream waits for it before
+       reusing the device buffers) */
+    unsigned char *h_stage[2] = {nullptr, nullptr};
+    size_t h_stage_cap[2] = {0, 0};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    int stage_next = 0;
+    hipEvent_t batch_ev = nullptr;
+    hipStream_t batch_stream = nullptr;
     /* triage pass: live-ray list, its cost buckets and the list ordered by
        them (grow-only) */
     int triage = 2;                     /* 0 off, 1 on, 2 mixed-precision launches only */
@@ -177,6 +186,11 @@ extern "C" int jsx_ctx_destroy(jsx_ctx *ctx) {
     (void)hipHostFree(ctx->h_params);
     for (int i = 0; i < jsx_ctx::kSlots; ++i) (void)hipEventDestroy(ctx->slot_ev[i]);
     (void)hipFree(ctx->d_queues); (void)hipFree(ctx->d_reqs); (void)hipFree(ctx->d_off); (void)hipFree(ctx->d_outs);
+    for (int k = 0; k < 2; ++k) {
+        (void)hipHostFree(ctx->h_stage[k]);
+        if (ctx->stage_ev[k]) (void)hipEventDestroy(ctx->stage_ev[k]);
+    }
+    if (ctx->batch_ev) (void)hipEventDestroy(ctx->batch_ev);
     (void)hipFree(ctx->d_live); (void)hipFree(ctx->d_sorted); (void)hipFree(ctx->d_bucket);
     (void)hipFree(ctx->d_sbucket);
     (void)hipFree(ctx->split_mem);
@@ -754,8 +768,21 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
     size_t n = items.size();
     if (!n) return JSX_OK;
     size_t esz = precision == JSX_PREC_MIXED ? sizeof(DParamsM) : sizeof(DParams);
-    HIPCHECK(ctx, hipStreamSynchronize(st));
-    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    int64_t total_pix = 0;
+    for (size_t i = 0; i < n; ++i) total_pix += items[i].npix;
+    if (total_pix == 0) return JSX_OK;
+    /* Launches on one stream follow each other without a host wait; device
+       buffers are only reallocated after the device has drained them. */
+    const bool triage_on = ctx->triage == 1 || (ctx->triage == 2 && precision == JSX_PREC_MIXED);
+    const bool grow = !ctx->d_queues || ctx->reqs_cap < n * esz || ctx->n_cap < n ||
+                      (triage_on && ctx->live_cap < (size_t)total_pix);
+    if (grow) {
+        HIPCHECK(ctx, hipStreamSynchronize(st));
+        HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+        if (ctx->batch_stream) HIPCHECK(ctx, hipStreamSynchronize(ctx->batch_stream));
+    } else if (ctx->batch_ev && ctx->batch_stream != st) {
+        HIPCHECK(ctx, hipStreamWaitEvent(st, ctx->batch_ev, 0));
+    }
     if (!ctx->d_queues) HIPCHECK(ctx, hipMalloc(&ctx->d_queues, sizeof(BatchQueues)));
     if (ctx->reqs_cap < n * esz) {
         (void)hipFree(ctx->d_reqs);
@@ -768,28 +795,44 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
         HIPCHECK(ctx, hipMalloc(&ctx->d_off, sizeof(int64_t) * (n + 1)));
         HIPCHECK(ctx, hipMalloc(&ctx->d_outs, sizeof(DOut) * n));
     }
-    ctx->h_reqs.resize(n * esz);
-    std::vector<int64_t> off(n + 1, 0);
-    std::vector<DOut> outs(n);
+    /* request table, offsets and outputs in a pinned buffer (the one used two
+       launches ago: its copy has long completed) */
+    const size_t b_req = (n * esz + 255) & ~(size_t)255, b_off = (sizeof(int64_t) * (n + 1) + 255) & ~(size_t)255;
+    const size_t b_all = b_req + b_off + sizeof(DOut) * n;
+    const int sk = ctx->stage_next;
+    ctx->stage_next ^= 1;
+    if (!ctx->stage_ev[sk]) HIPCHECK(ctx, hipEventCreateWithFlags(&ctx->stage_ev[sk], hipEventDisableTiming));
+    HIPCHECK(ctx, hipEventSynchronize(ctx->stage_ev[sk]));
+    if (ctx->h_stage_cap[sk] < b_all) {
+        (void)hipHostFree(ctx->h_stage[sk]);
+        ctx->h_stage[sk] = nullptr;
+        ctx->h_stage_cap[sk] = 0;
+        HIPCHECK(ctx, hipHostMalloc((void **)&ctx->h_stage[sk], b_all, hipHostMallocDefault));
+        ctx->h_stage_cap[sk] = b_all;
+    }
+    unsigned char *hreq = ctx->h_stage[sk];
+    int64_t *off = (int64_t *)(hreq + b_req);
+    DOut *outs = (DOut *)(hreq + b_req + b_off);
+    off[0] = 0;
     for (size_t i = 0; i < n; ++i) {
         const BatchItem &it = items[i];
         if (precision == JSX_PREC_MIXED) {
             DParamsM q; q.m = it.m; q.im = it.im; q.S = it.S; q.c = it.c;
-            std::memcpy(&ctx->h_reqs[i * esz], &q, esz);
+            std::memcpy(hreq + i * esz, &q, esz);
         } else {
             DParams q; q.m = it.m; q.im = it.im; q.S = it.S;
-            std::memcpy(&ctx->h_reqs[i * esz], &q, esz);
+            std::memcpy(hreq + i * esz, &q, esz);
         }
         off[i + 1] = off[i] + it.npix;
         outs[i] = it.out;
     }
-    if (off[n] == 0) return JSX_OK;
-    HIPCHECK(ctx, hipMemcpyAsync(ctx->d_reqs, ctx->h_reqs.data(), n * esz, hipMemcpyHostToDevice, st));
-    HIPCHECK(ctx, hipMemcpyAsync(ctx->d_off, off.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, st));
-    HIPCHECK(ctx, hipMemcpyAsync(ctx->d_outs, outs.data(), sizeof(DOut) * n, hipMemcpyHostToDevice, st));
+    HIPCHECK(ctx, hipMemcpyAsync(ctx->d_reqs, hreq, n * esz, hipMemcpyHostToDevice, st));
+    HIPCHECK(ctx, hipMemcpyAsync(ctx->d_off, off, sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, st));
+    HIPCHECK(ctx, hipMemcpyAsync(ctx->d_outs, outs, sizeof(DOut) * n, hipMemcpyHostToDevice, st));
+    HIPCHECK(ctx, hipEventRecord(ctx->stage_ev[sk], st));
     hipLaunchKernelGGL(jsx_k_reset_queues, dim3(1), dim3(64), 0, st, ctx->d_queues);
     int blocks = ctx->n_cu * ctx->persist_blocks_per_cu;
-    int64_t total = off[n];
+    int64_t total = off[n];   /* == total_pix */
     int nreq = (int)n;
     const bool full = stokes == JSX_STOKES_FULL;
     SplitWS ws{};
@@ -930,6 +973,9 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
         mark(4);
     }
     HIPCHECK(ctx, hipGetLastError());
+    if (!ctx->batch_ev) HIPCHECK(ctx, hipEventCreateWithFlags(&ctx->batch_ev, hipEventDisableTiming));
+    HIPCHECK(ctx, hipEventRecord(ctx->batch_ev, st));
+    ctx->batch_stream = st;
     return JSX_OK;
 }
 
