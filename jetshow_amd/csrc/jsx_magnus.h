@@ -124,8 +124,16 @@ __device__ __forceinline__ void mg_g_series(double z, double ez, F use) {
         }
         return;
     }
-    double g = ez / ((double)(JSX_MG_GK_START + 1) + z);
-    for (int k = JSX_MG_GK_START; k > JSX_MG_NSER; --k) g = (ez - z * g) * mg_inv(k);
+    /* start of the backward recurrence: for z <= 0 each step down scales the
+       relative error of the start value by at most |z| / k, so the start
+       index K with prod_{k=NSER+1..K} |z| / k <= 1e-17 gives g_0..g_NSER-1 to
+       rounding; K is the largest any active lane of the wavefront needs (a
+       uniform loop keeps 1/k a scalar load) */
+    const double az = fabs(z);
+    const int K = __ballot(az > 12.0) ? JSX_MG_GK_START : __ballot(az > 8.0) ? 56 : __ballot(az > 4.0) ? 48
+                : __ballot(az > 2.0) ? 40 : __ballot(az > 1.0) ? 35 : __ballot(az > 0.1) ? 32 : 28;
+    double g = ez / ((double)(K + 1) + z);
+    for (int k = K; k > JSX_MG_NSER; --k) g = (ez - z * g) * mg_inv(k);
 #pragma unroll
     for (int k = JSX_MG_NSER; k > 0; --k) {
         g = (ez - z * g) * mg_inv(k);
