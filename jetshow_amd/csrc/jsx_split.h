@@ -391,34 +391,57 @@ __global__ __launch_bounds__(1024) void jsx_k_split_alloc(SplitWS ws) {
         __syncthreads();
     }
     /* exclusive prefix of the piece counts in ray order, Dormand-Prince rays
-       first, then exponential rays (serial over chunks) */
+       first, then exponential rays: each thread takes a contiguous run of
+       rays, one block scan of the run totals (wavefront scans and a scan of
+       the 16 wavefront totals) places the runs */
+    __shared__ int s_wtot[16];
+    __shared__ int s_gtot;
     if (tid == 0) s_base = 0;
+    const unsigned per = (n + blockDim.x - 1) / blockDim.x;
+    const unsigned ra = tid * per < n ? tid * per : n;
+    const unsigned rb = ra + per < n ? ra + per : n;
+    const int lane = tid & 63, wv = tid >> 6, nwv = (int)(blockDim.x >> 6);
     __syncthreads();
     for (int g = 0; g < 2; ++g) {
-        if (g == 1 && tid == 0) *ws.dp_end = (unsigned)s_base;
-        for (unsigned r0 = 0; r0 < n; r0 += blockDim.x) {
-            unsigned r = r0 + tid;
-            int k = 0;
-            if (r < n) {
-                const LongRay &R = ws.rays[r];
-                if ((R.method == 2) == (g == 1)) k = split_count(R, s_wp[g]);
-            }
-            s_tot[tid] = k;
-            __syncthreads();
-            for (int off = 1; off < (int)blockDim.x; off <<= 1) {
-                int v = tid >= off ? s_tot[tid - off] : 0;
-                __syncthreads();
-                s_tot[tid] += v;
-                __syncthreads();
-            }
-            if (r < n && k > 0) {
-                ws.rays[r].piece0 = s_base + s_tot[tid] - k;
-                ws.rays[r].npieces = k;
-            }
-            __syncthreads();
-            if (tid == blockDim.x - 1) s_base += s_tot[tid];
-            __syncthreads();
+        int run = 0;
+        for (unsigned r = ra; r < rb; ++r) {
+            const LongRay &R = ws.rays[r];
+            if ((R.method == 2) == (g == 1)) run += split_count(R, s_wp[g]);
         }
+        int incl = run;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(incl, off);
+            if (lane >= off) incl += v;
+        }
+        if (lane == 63) s_wtot[wv] = incl;
+        __syncthreads();
+        if (wv == 0) {
+            const int t = lane < nwv ? s_wtot[lane] : 0;
+            int ti = t;
+            for (int off = 1; off < 64; off <<= 1) {
+                const int v = __shfl_up(ti, off);
+                if (lane >= off) ti += v;
+            }
+            if (lane < nwv) s_wtot[lane] = ti - t;
+            if (lane == nwv - 1) s_gtot = ti;
+        }
+        __syncthreads();
+        int pos = s_base + s_wtot[wv] + incl - run;
+        for (unsigned r = ra; r < rb; ++r) {
+            const LongRay &R = ws.rays[r];
+            if ((R.method == 2) == (g == 1)) {
+                const int k = split_count(R, s_wp[g]);
+                ws.rays[r].piece0 = pos;
+                ws.rays[r].npieces = k;
+                pos += k;
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            s_base += s_gtot;
+            if (g == 0) *ws.dp_end = (unsigned)s_base;
+        }
+        __syncthreads();
     }
     if (tid == 0) {
         int tot = s_base;
