@@ -164,8 +164,18 @@ __device__ __forceinline__ double mg_phi1(double z) { return z != 0.0 ? expm1(z)
 /* Coefficients of exp(-mu 1 + G) = sum_k c_k G^k and phi1(-mu 1 + G) =
  * sum_k d_k G^k (k < 4).  Returns 0 when the step is unusable (growth beyond
  * e^40, or a non-finite coefficient), which the caller treats as a rejected
- * step. */
-__device__ inline int mg_coef(double mu, const MgGen &g, double *c, double *d) {
+ * step.
+ *
+ * Two forms: the Taylor series in G (a^2 + b^2 <= 1 only) and the closed form
+ * from the eigenvalues (any G: its small-argument pieces -- cosh a - 1,
+ * sinh(a)/a - 1, 1 - cos b, 1 - e^-mu cos b, and phi1's even and odd parts at
+ * a < 1/2 -- are evaluated without cancellation, and G with all eigenvalues
+ * zero takes the limits).  MODE 0 (the kernels): the series when every active
+ * lane of the wavefront can take it, else the closed form for all, so a
+ * wavefront never runs both; MODE 1: the closed form; MODE 2: per lane, the
+ * series exactly when a^2 + b^2 <= 1. */
+template <int MODE>
+__device__ inline int mg_coef_t(double mu, const MgGen &g, double *c, double *d) {
     JSX_MIXED_FMA
     const double ee = g.e[0] * g.e[0] + g.e[1] * g.e[1] + g.e[2] * g.e[2];
     const double rr = g.r[0] * g.r[0] + g.r[1] * g.r[1] + g.r[2] * g.r[2];
@@ -182,7 +192,8 @@ __device__ inline int mg_coef(double mu, const MgGen &g, double *c, double *d) {
     }
     const double s2 = a2 + b2;
     const double em = exp(-mu);
-    if (s2 <= 1.0) {
+    const bool series = MODE == 1 ? false : MODE == 2 ? s2 <= 1.0 : __ballot(!(s2 <= 1.0)) == 0ull;
+    if (series) {
         /* Taylor series in G, powers reduced with G^4 = p G^2 + q */
         double rc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -232,12 +243,13 @@ __device__ inline int mg_coef(double mu, const MgGen &g, double *c, double *d) {
         c[2] = (chm1 + em * omc) * is2;
         c[3] = (shm1 + em * scm1) * is2;
         /* phi1's even (E) and odd (O) parts at lambda = a and lambda = i b */
-        double Ea = 0.0, Oa = 0.0;
+        double Ea = 0.0, Oa = 0.0, g03[4] = {0.0, 0.0, 0.0, 0.0};
         if (a < 0.5) {
             /* sum_k a^{2k} g_{2k} / (2k)!, sum_k a^{2k} g_{2k+1} / (2k+1)! */
             if (fabs(mu) >= (double)(JSX_MG_NSER + 5)) {
                 double ak = 1.0;
                 mg_g_series(-mu, em, [&](int k, double gk) {
+                    if (k < 4) g03[k] = gk;
                     if (k & 1) {
                         Oa += ak * gk * mg_ifact(k);
                         ak *= a2;
@@ -247,6 +259,7 @@ __device__ inline int mg_coef(double mu, const MgGen &g, double *c, double *d) {
                 });
             } else {
                 mg_g_series(-mu, em, [&](int k, double gk) {
+                    if (k < 4) g03[k] = gk;
                     if (k & 1) Oa = Oa * a2 + gk * mg_ifact(k);
                     else Ea = Ea * a2 + gk * mg_ifact(k);
                 });
@@ -260,18 +273,27 @@ __device__ inline int mg_coef(double mu, const MgGen &g, double *c, double *d) {
         double Eb = 1.0, Ob = 0.5;
         if (mu * mu + b2 > 1e-300) {
             const double dz = 1.0 / (mu * mu + b2);
-            const double one_m = 1.0 - em * cb;
+            /* 1 - e^-mu cos b, both terms non-negative for mu >= 0 */
+            const double one_m = -expm1(-mu) + em * omc;
             Eb = (mu * one_m + b * em * sb) * dz;
             Ob = (one_m - mu * em * sc) * dz;
         }
-        d[0] = (b2 * Ea + a2 * Eb) * is2;
-        d[1] = (b2 * Oa + a2 * Ob) * is2;
-        d[2] = (Ea - Eb) * is2;
-        d[3] = (Oa - Ob) * is2;
+        if (s2 < 1e-200) {
+            /* all eigenvalues zero (G^4 = 0): the series' first four terms */
+            c[0] = em; c[1] = em; c[2] = 0.5 * em; c[3] = em * (1.0 / 6.0);
+            d[0] = g03[0]; d[1] = g03[1]; d[2] = 0.5 * g03[2]; d[3] = g03[3] * (1.0 / 6.0);
+        } else {
+            d[0] = (b2 * Ea + a2 * Eb) * is2;
+            d[1] = (b2 * Oa + a2 * Ob) * is2;
+            d[2] = (Ea - Eb) * is2;
+            d[3] = (Oa - Ob) * is2;
+        }
     }
     const double chk = c[0] + c[1] + c[2] + c[3] + d[0] + d[1] + d[2] + d[3];
     return chk == chk && fabs(chk) < 1e300;
 }
+
+__device__ inline int mg_coef(double mu, const MgGen &g, double *c, double *d) { return mg_coef_t<0>(mu, g, c, d); }
 
 /* y = (k0 + k1 G + k2 G^2 + k3 G^3) x */
 __device__ __forceinline__ void mg_poly(const MgGen &g, const double *k, const double *x, double *y) {
@@ -283,11 +305,13 @@ __device__ __forceinline__ void mg_poly(const MgGen &g, const double *k, const d
     for (int i = 0; i < 4; ++i) y[i] = k[0] * x[i] + k[1] * x1[i] + k[2] * x2[i] + k[3] * x3[i];
 }
 
-/* E = exp(-mu 1 + G) (column-major, E[4 j + i]) and w = phi1(-mu 1 + G) v */
+/* E = exp(-mu 1 + G) (column-major, E[4 j + i]) and w = phi1(-mu 1 + G) v,
+ * with mg_coef_t<MODE> */
+template <int MODE = 0>
 __device__ inline int mg_expphi(double mu, const MgGen &g, const double *v, double *E, double *w) {
     JSX_MIXED_FMA
     double c[4], d[4];
-    if (!mg_coef(mu, g, c, d)) return 0;
+    if (!mg_coef_t<MODE>(mu, g, c, d)) return 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const double x[4] = {j == 0 ? 1.0 : 0.0, j == 1 ? 1.0 : 0.0, j == 2 ? 1.0 : 0.0, j == 3 ? 1.0 : 0.0};

@@ -391,10 +391,11 @@ extern "C" int lh_coefficients_mixed(const jsx_jet_desc *jet, const jsx_image_de
 /* exp(-mu 1 + G(eta, rho)) (column-major) and phi1(...) v: the closed forms
  * of the exponential pieces, for comparison with a general matrix exponential. */
 extern "C" int lh_magnus_expphi(double mu, const double *eta, const double *rho, const double *v, double *E,
-                                double *w) {
+                                double *w, int32_t mode) {
     MgGen g;
     for (int i = 0; i < 3; ++i) { g.e[i] = eta[i]; g.r[i] = rho[i]; }
-    return mg_expphi(mu, g, v, E, w);
+    /* mode 1: the closed form for every generator; 2: the series where a^2 + b^2 <= 1 */
+    return mode == 1 ? mg_expphi<1>(mu, g, v, E, w) : mg_expphi<2>(mu, g, v, E, w);
 }
 
 /* [G(e1, r1), G(e2, r2)] as (eta', rho') */

@@ -101,12 +101,14 @@ def coefficients_mixed(jet, image, params, pts_cm):
     return out
 
 
-def magnus_expphi(mu, eta, rho, v):
-    """(E, w, ok): exp(-mu 1 + G(eta, rho)) and phi1(-mu 1 + G) v."""
+def magnus_expphi(mu, eta, rho, v, mode=2):
+    """(E, w, ok): exp(-mu 1 + G(eta, rho)) and phi1(-mu 1 + G) v; mode 1:
+    the closed form for every generator, 2: the Taylor series where
+    a^2 + b^2 <= 1 (what one lane of the kernels computes)."""
     E = np.zeros(16)
     w = np.zeros(4)
     ok = lib().lh_magnus_expphi(float(mu), _dp(np.ascontiguousarray(eta, float)), _dp(np.ascontiguousarray(rho, float)),
-                                _dp(np.ascontiguousarray(v, float)), _dp(E), _dp(w))
+                                _dp(np.ascontiguousarray(v, float)), _dp(E), _dp(w), C.c_int32(mode))
     return E.reshape(4, 4).T, w, ok
 
 

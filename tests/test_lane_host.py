@@ -113,21 +113,33 @@ def test_magnus_exponential_closed_form():
     the source column to 1e-10 relative.  And the Lorentz-algebra commutator
     [G(e1, r1), G(e2, r2)] = G(r2 x e1 - r1 x e2, e1 x e2 - r1 x r2)."""
     rng = np.random.default_rng(5)
+    cases = []
     for _ in range(1500):
         se, sr = 10 ** rng.uniform(-5, 2.5), 10 ** rng.uniform(-5, 3.3)
         e = rng.normal(size=3)
         e *= se / np.linalg.norm(e)
         r = rng.normal(size=3) * sr
         mu = max(10 ** rng.uniform(-6, 3), 1.01 * np.linalg.norm(e))   # k_I >= |eta|
+        cases.append((mu, e, r))
+    # the closed form's small-argument corners: tiny absorption with rotation
+    # far below / above it, all eigenvalues zero (|eta| = |rho|, eta . rho = 0)
+    for mu, se, sr in ((1e-12, 1e-14, 1e-13), (1e-9, 1e-12, 1e-6), (1e-6, 1e-8, 1e-12), (0.3, 1e-9, 1e-9),
+                       (1e-3, 0.0, 0.0)):
+        cases.append((mu, np.array([se, 0.0, 0.0]), np.array([0.0, sr, 0.0])))
+    cases.append((1e-4, np.array([0.0, 2e-5, 0.0]), np.array([2e-5, 0.0, 0.0])))
+    for mu, e, r in cases:
         v = rng.normal(size=4)
-        E, w, ok = LH.magnus_expphi(mu, e, r, v)
         A = np.zeros((5, 5))
         A[:4, :4] = -mu * np.eye(4) + _stokes_gen(e, r)
         A[:4, 4] = v
         X = expm(A)
-        assert ok
-        assert np.abs(E - X[:4, :4]).max() <= 1e-10, (mu, e, r)
-        assert np.abs(w - X[:4, 4]).max() <= 1e-10 * np.abs(X[:4, 4]).max(), (mu, e, r)
+        # both forms the kernels use: per-lane choice, and the closed form that a
+        # wavefront with any large generator applies to all of its lanes
+        for mode in (2, 1):
+            E, w, ok = LH.magnus_expphi(mu, e, r, v, mode=mode)
+            assert ok
+            assert np.abs(E - X[:4, :4]).max() <= 1e-10, (mode, mu, e, r)
+            assert np.abs(w - X[:4, 4]).max() <= 1e-10 * np.abs(X[:4, 4]).max(), (mode, mu, e, r)
     for _ in range(20):
         e1, r1, e2, r2 = rng.normal(size=(4, 3))
         eo, ro = LH.magnus_comm(e1, r1, e2, r2)
