@@ -220,13 +220,13 @@ __device__ inline int mg_coef_t(double mu, const MgGen &g, double *c, double *d)
     } else {
         const double a = sqrt(a2), b = sqrt(b2);
         if (a - mu > 40.0) return 0;
-        const double ep = exp(a - mu), en = exp(-a - mu);
         /* e^-mu (cosh a - 1), e^-mu (sinh a / a - 1) */
         double chm1, shm1;
         if (a < 0.5) {
             chm1 = em * mg_chm1(a2);
             shm1 = em * mg_shm1(a2);
         } else {
+            const double ep = exp(a - mu), en = exp(-a - mu);
             chm1 = 0.5 * (ep + en) - em;
             shm1 = 0.5 * (ep - en) / a - em;
         }
