@@ -233,3 +233,29 @@ def test_zoom_ft_and_uniform_class(ctx):
     pix = 1e-9
     got = fourier.FINUFFT(uvg, pix, ctx=ctx).forward(im)
     assert np.max(np.abs(got - ofour.uniform_forward(uvg, pix, im))) < 1e-12 * im.sum()
+
+
+@pytest.mark.gpu
+def test_visibilities_device_two_streams(ctx):
+    """Two chunked launches on different streams share the context's
+    partial-sum buffer: the second waits for the first (event), so both results
+    equal their host-entry values."""
+    import torch
+    sets = [_scatter(6000, 2100, seed=21), _scatter(9000, 1500, seed=22)]
+    print("vis two streams", flush=True)
+    ref = [ctx.visibilities(x, y, c, u, v, precision=abi.PREC_MIXED) for x, y, c, u, v in sets]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    outs, keep = [], []
+    for (x, y, c, u, v), st in zip(sets, streams):
+        d = [torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (x, y, c, u, v)]
+        torch.cuda.synchronize()
+        out = torch.zeros(2 * u.size, dtype=torch.float64, device="cuda")
+        ctx.visibilities_device(d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr(), None, x.size, d[3].data_ptr(),
+                                d[4].data_ptr(), u.size, out.data_ptr(), precision=abi.PREC_MIXED,
+                                stream=st.cuda_stream)
+        outs.append(out)
+        keep.append(d)
+    torch.cuda.synchronize()
+    for o, r in zip(outs, ref):
+        o = o.cpu().numpy()
+        assert np.array_equal(o[0::2] + 1j * o[1::2], r)
