@@ -724,6 +724,7 @@ static int split_workspace(jsx_ctx *ctx) {
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
     size_t o_cnt = take(8 * sizeof(unsigned) + 2 * sizeof(unsigned long long));
     size_t o_rays = take(sizeof(LongRay) * cap);
+    size_t o_rsum = take(sizeof(RaySum) * cap);
     size_t o_prof = take(sizeof(float) * (size_t)cap * JSX_SPLIT_NSAMP);
     size_t o_pcs = take(sizeof(Piece) * piece_cap);
     size_t o_pout = take(sizeof(PieceOut) * piece_cap);
@@ -743,6 +744,7 @@ static int split_workspace(jsx_ctx *ctx) {
     w.rhs_pieces = (unsigned long long *)(b + o_cnt + 8 * sizeof(unsigned));
     w.rhs_mg = w.rhs_pieces + 1;
     w.rays = (LongRay *)(b + o_rays);
+    w.rsum = (RaySum *)(b + o_rsum);
     w.cap = cap;
     w.prof = (float *)(b + o_prof);
     w.pieces = (Piece *)(b + o_pcs);

@@ -83,9 +83,17 @@ struct PieceOut {
     uint32_t flags, n_rhs, n_steps, pad;
 };
 
+/* what the piece counter needs of a parked ray, compact (8 B instead of the
+ * ray's record): its cost total and method, and whether it spans two segments */
+struct RaySum {
+    float w;                      /* cumulative cost over the remaining path      */
+    int kind;                     /* piece method | 16 if the path has two segments */
+};
+
 struct SplitWS {
     unsigned *n_long;             /* parked rays (may exceed cap: then clamp)        */
     LongRay *rays;
+    RaySum *rsum;                 /* cap summaries written by the sampler           */
     int cap;
     float *prof;                  /* cap x NSAMP cumulative stiffness               */
     unsigned *n_pieces;           /* pieces planned                                 */
@@ -308,7 +316,10 @@ __global__ __launch_bounds__(256) void jsx_k_split_sample(const DParamsM *__rest
 #pragma unroll
     for (int j = 0; j < per; ++j) prof[lane * per + j] = before + w[j];
     float wt = __shfl(tot, 63);
-    if (lane == 0) R.w_total = wt;
+    if (lane == 0) {
+        R.w_total = wt;
+        ws.rsum[r] = RaySum{wt, method | (R.seg ? 16 : 0)};
+    }
     /* stiffness up to the end of the current segment (interval 0): the
        cumulative value of the last sample inside it */
     if (R.seg) {
@@ -327,10 +338,10 @@ __global__ __launch_bounds__(256) void jsx_k_split_sample(const DParamsM *__rest
 }
 
 /* ---- 3. piece counts (one workgroup) ---- */
-__device__ inline int split_count(const LongRay &R, float wp) {
-    float np = ceilf(R.w_total / wp);
+__device__ inline int split_count(const RaySum &R, float wp) {
+    float np = ceilf(R.w / wp);
     int k = np < 1.f ? 1 : (np > 65536.f ? 65536 : (int)np);
-    if (R.seg) k = k < 2 ? 2 : k;
+    if (R.kind & 16) k = k < 2 ? 2 : k;
     return k;
 }
 
@@ -350,9 +361,9 @@ __global__ __launch_bounds__(1024) void jsx_k_split_alloc(SplitWS ws) {
     float local[2] = {0.f, 0.f};
     int nmg = 0;
     for (unsigned r = tid; r < n; r += blockDim.x) {
-        const LongRay &R = ws.rays[r];
-        const int m = R.method;
-        local[m == 2] += R.w_total;
+        const RaySum R = ws.rsum[r];
+        const int m = R.kind & 15;
+        local[m == 2] += R.w;
         nmg += m == 2;
     }
     if (nmg) atomicAdd(&s_nmg, nmg);
@@ -378,8 +389,8 @@ __global__ __launch_bounds__(1024) void jsx_k_split_alloc(SplitWS ws) {
     for (int iter = 0; iter < 32; ++iter) {
         int cnt = 0;
         for (unsigned r = tid; r < n; r += blockDim.x) {
-            const LongRay &R = ws.rays[r];
-            cnt += split_count(R, s_wp[R.method == 2]);
+            const RaySum R = ws.rsum[r];
+            cnt += split_count(R, s_wp[(R.kind & 15) == 2]);
         }
         s_tot[tid] = cnt;
         __syncthreads();
@@ -411,8 +422,8 @@ __global__ __launch_bounds__(1024) void jsx_k_split_alloc(SplitWS ws) {
     for (int g = 0; g < 2; ++g) {
         int run = 0;
         for (unsigned r = ra; r < rb; ++r) {
-            const LongRay &R = ws.rays[r];
-            if ((R.method == 2) == (g == 1)) run += split_count(R, s_wp[g]);
+            const RaySum R = ws.rsum[r];
+            if (((R.kind & 15) == 2) == (g == 1)) run += split_count(R, s_wp[g]);
         }
         int incl = run;
         for (int off = 1; off < 64; off <<= 1) {
@@ -434,8 +445,8 @@ __global__ __launch_bounds__(1024) void jsx_k_split_alloc(SplitWS ws) {
         __syncthreads();
         int pos = s_base + s_wtot[wv] + incl - run;
         for (unsigned r = ra; r < rb; ++r) {
-            const LongRay &R = ws.rays[r];
-            if ((R.method == 2) == (g == 1)) {
+            const RaySum R = ws.rsum[r];
+            if (((R.kind & 15) == 2) == (g == 1)) {
                 const int k = split_count(R, s_wp[g]);
                 ws.rays[r].piece0 = pos;
                 ws.rays[r].npieces = k;
