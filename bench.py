@@ -266,6 +266,13 @@ def main():
                 e0.record(stream)
                 run(params, o)
                 e1.record(stream)
+        # the launches are asynchronous: report progress while a long one runs
+        t_last = t0
+        while not evs[-1][1].query():
+            time.sleep(0.5)
+            if time.perf_counter() - t_last >= 20.0:
+                t_last = time.perf_counter()
+                _log("%s: running, %.0f s" % (tag, t_last - t0))
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         lat = np.array([e0.elapsed_time(e1) for e0, e1 in evs])
