@@ -569,11 +569,6 @@ def grid_bench(a):
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "roofline": {"bound": "mfma", "pipe": "valu", "kernel": "jsx_k_batch", "achieved": ach, "peak": peak,
                      "unit": "TFLOP/s", "frac": ach / peak, "traffic": None,
-                     "valu_issue": {"variant": variant, "instr_per_pair": VALU_INSTR_PER_PAIR[variant],
-                                    "achieved_T_per_s": VALU_INSTR_PER_PAIR[variant] * pairs / (ms / 1e3) / 1e12,
-                                    "peak_T_per_s": VALU_ISSUE_PEAK_T,
-                                    "frac": VALU_INSTR_PER_PAIR[variant] * pairs / (ms / 1e3) / 1e12 /
-                                    VALU_ISSUE_PEAK_T},
                      "grid_lookup_GBps_algorithmic": grid_gbs},
         "cpu_baseline": cpu,
     }
