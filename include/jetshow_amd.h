@@ -207,9 +207,11 @@ int jsx_abi_version(void);
  *   "mg_profile"      cuts of exponential rays placed by 0 (default) the
  *                     stiffness profile, 1 the sampled exponential-step profile
  *   "mg_rel"          smallest relative tolerance of an exponential step's
- *                     error estimate (default 2e-6; the estimate, the 2nd-order
- *                     exponential's difference, overstates the 4th-order
- *                     step's error)
+ *                     error estimate (default 5e-6; the estimate, the 2nd-order
+ *                     step's error, overstates the 4th-order step's error)
+ *   "mg_err"          that estimate: 0 (default) the first-order difference of
+ *                     the 2nd- and 4th-order steps (their commutator terms); 1
+ *                     the 2nd-order exponential formed and compared
  *   "triage"          1: a first pass writes the rays that need no integration
  *                     and queues only the others, costliest first (see
  *                     "lpt"); 0: the queues deal every ray id in image order;

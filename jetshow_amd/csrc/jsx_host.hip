@@ -105,7 +105,8 @@ struct jsx_ctx {
                                            3: 1 or 2 per ray by its rotation */
     float mg_cost = 0.5f;               /* layout 3: cost ratio below which a ray's pieces are exponential */
     int mg_profile = 0;                 /* cuts of exponential rays: 0 stiffness, 1 step estimate */
-    float mg_rel = 2e-6f;               /* smallest relative tolerance of exponential steps */
+    float mg_rel = 5e-6f;               /* smallest relative tolerance of exponential steps */
+    int mg_err = 0;                     /* their error estimate: 0 first order, 1 second exponential */
     SplitWS split{};
     void *split_mem = nullptr;
     /* visibilities: chunk partial sums (grow-only) and the event after their
@@ -246,6 +247,10 @@ extern "C" int jsx_ctx_set_option(jsx_ctx *ctx, const char *name, double v) {
         if (!(v >= 1e-9 && v <= 1e-3)) return fail(ctx, JSX_EINVAL, "mg_rel must be in [1e-9, 1e-3]");
         ctx->mg_rel = (float)v;
         if (ctx->split_mem) ctx->split.mg_rel = (float)v;
+    } else if (k == "mg_err") {
+        if (!(v == 0.0 || v == 1.0)) return fail(ctx, JSX_EINVAL, "mg_err must be 0 or 1");
+        ctx->mg_err = (int)v;
+        if (ctx->split_mem) ctx->split.mg_err = ctx->mg_err;
     } else if (k == "mg_profile") {
         ctx->mg_profile = v != 0.0;
         if (ctx->split_mem) ctx->split.mg_profile = ctx->mg_profile;
@@ -299,6 +304,7 @@ extern "C" int jsx_ctx_get_option(jsx_ctx *ctx, const char *name, double *v) {
     else if (k == "mg_cost") *v = ctx->mg_cost;
     else if (k == "mg_profile") *v = ctx->mg_profile;
     else if (k == "mg_rel") *v = ctx->mg_rel;
+    else if (k == "mg_err") *v = ctx->mg_err;
     else if (k == "park_lanes") *v = ctx->park_lanes;
     else if (k == "park_wave") *v = ctx->park_wave;
     else if (k == "stat_parked" || k == "stat_pieces") {
@@ -759,6 +765,7 @@ static int split_workspace(jsx_ctx *ctx) {
     w.mg_cost = ctx->mg_cost;
     w.mg_profile = ctx->mg_profile;
     w.mg_rel = ctx->mg_rel;
+    w.mg_err = ctx->mg_err;
     return JSX_OK;
 }
 

@@ -328,13 +328,18 @@ __device__ __forceinline__ void mg_gen(const MCoef &k, MgGen &g) {
 }
 
 /* One Magnus step over [t, t + h] from the coefficients at the two Gauss
- * points: E4, w4 (4th order) and the error estimate against the 2nd-order
+ * points: E4, w4 (4th order) and an estimate of the error of the 2nd-order
  * step of the same points, err = max(|E4 - E2| / rel, |w4 - w2| / (rel sP))
- * with sP the larger of p_floor and the particular solution's scale.  The
- * 2nd-order columns are formed one at a time and only compared.  Returns 0
- * if the step is unusable. */
+ * with sP the larger of p_floor and the particular solution's scale.
+ * est 1: E2 and w2 formed (a second exponential, compared column by column).
+ * est 0: their first-order difference, the commutator terms the 4th-order
+ * step adds -- dG = sqrt(3)/12 h^2 [A2, A1] in the generator and
+ * dv = v4 - v2 in the source -- carried through exp(-mu + G), whose norm
+ * e^{-mu + |eta|} bounds: |E4 - E2| ~ e^{-mu+|eta|} max|dG|,
+ * |w4 - w2| ~ e^{-mu+|eta|} (max|dv| + max|dG| max|w4|).  Returns 0 if the
+ * step is unusable. */
 __device__ inline int mg_step(const MCoef &k1, const MCoef &k2, double h, double rel, double p_floor,
-                              const double *P, double *E4, double *w4, double *err) {
+                              const double *P, double *E4, double *w4, double *err, int est = 0) {
     JSX_MIXED_FMA
     const double hh = 0.5 * h, cc = 0.14433756729740643 * h * h;   /* sqrt(3)/12 h^2 */
     const double mu = hh * ((double)k1.kI + (double)k2.kI);
@@ -359,6 +364,33 @@ __device__ inline int mg_step(const MCoef &k1, const MCoef &k2, double h, double
         const double a2s1 = -(double)k2.kI * s1[i] - t1[i];
         const double a1s2 = -(double)k1.kI * s2[i] - t2[i];
         v4[i] = v2[i] + cc * (a2s1 - a1s2);
+    }
+    if (est == 0) {
+        double c4[4], d4[4];
+        if (!mg_coef(mu, g4, c4, d4)) {
+            *err = __builtin_inf();
+            return 0;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const double x[4] = {j == 0 ? 1.0 : 0.0, j == 1 ? 1.0 : 0.0, j == 2 ? 1.0 : 0.0, j == 3 ? 1.0 : 0.0};
+            mg_poly(g4, c4, x, E4 + 4 * j);
+        }
+        mg_poly(g4, d4, v4, w4);
+        double dG = 0.0, dv = 0.0, ww = 0.0, sP = p_floor;
+        for (int i = 0; i < 3; ++i) dG = fmax(dG, fmax(fabs(cc * cm.e[i]), fabs(cc * cm.r[i])));
+        for (int i = 0; i < 4; ++i) {
+            dv = fmax(dv, fabs(v4[i] - v2[i]));
+            ww = fmax(ww, fabs(w4[i]));
+            sP = fmax(sP, fmax(fabs(P[i]), fabs(w4[i])));
+        }
+        const double eta = sqrt(g2.e[0] * g2.e[0] + g2.e[1] * g2.e[1] + g2.e[2] * g2.e[2]);
+        const double grow = exp(eta - mu);
+        const double e1 = grow * dG / rel;
+        const double eP = grow * (dv + dG * ww);
+        const double e2 = sP > 0.0 ? eP / (rel * sP) : (eP > 0.0 ? __builtin_inf() : 0.0);
+        *err = e1 > e2 ? e1 : e2;
+        return (*err == *err) ? 1 : 0;
     }
     double c4[4], d4[4], c2[4], d2[4];
     if (!mg_coef(mu, g4, c4, d4) || !mg_coef(mu, g2, c2, d2)) {

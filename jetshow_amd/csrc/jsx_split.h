@@ -118,6 +118,7 @@ struct SplitWS {
     int mg_profile;               /* cuts of exponential rays: 0 stiffness, 1 their
                                      own step estimate                               */
     float mg_rel;                 /* smallest relative tolerance of exponential steps */
+    int mg_err;                   /* their error estimate: 0 first order, 1 second exponential */
     float w_min;                  /* smallest stiffness per piece worth a split     */
     float floor_max;              /* cap of the column error-scale estimate (1: off) */
 };
@@ -955,9 +956,11 @@ struct MgLane {
     double o[3];                  /* origin of the piece's pass [cm] */
     uint32_t flags, n_rhs, n_steps, tries;
     int done;
+    int est;                      /* error estimate of mg_step (0 first order, 1 second exponential) */
 };
 
-__device__ inline void mg_lane_begin(MgLane &L, const LongRay &R, const Piece &pc, double rel_min) {
+__device__ inline void mg_lane_begin(MgLane &L, const LongRay &R, const Piece &pc, double rel_min, int est = 0) {
+    L.est = est;
 #pragma unroll
     for (int i = 0; i < 20; ++i) L.U[i] = (i < 16 && (i >> 2) == (i & 3)) ? 1.0 : 0.0;
     for (int i = 0; i < 3; ++i) L.o[i] = pc.interval ? R.s0out[i] : R.cout[i];
@@ -993,7 +996,7 @@ __device__ inline void mg_lane_step(MgLane &L, const DModel &m, const DImage &im
     L.n_rhs += 2;
     const double rel = c.st_rel > L.rel_min ? c.st_rel : L.rel_min;
     double E[16], w[4], err;
-    const int ok = mg_step(A0, A1, h, rel, L.p_floor, L.U + 16, E, w, &err);
+    const int ok = mg_step(A0, A1, h, rel, L.p_floor, L.U + 16, E, w, &err, L.est);
     if (ok && err <= 1.0) {
         double nU[20];
 #pragma unroll
@@ -1103,7 +1106,7 @@ __global__ __launch_bounds__(256) void jsx_k_split_magnus(const DParamsM *__rest
                     const Piece pc = ws.pieces[pid];
                     const LongRay &R = ws.rays[pc.ray];
                     slot = R.slot;
-                    mg_lane_begin(L, R, pc, (double)ws.mg_rel);
+                    mg_lane_begin(L, R, pc, (double)ws.mg_rel, ws.mg_err);
                     have = L.done ? 2 : 1;
                     spawn_ok = budget != 0u;
                 }
