@@ -41,7 +41,7 @@ OPS_PER_RHS = {
     # evaluation of an exponential piece (2 per step: coefficients, the two
     # exponentials, error estimate and propagator update)
     "piece20": {"f64": 146.9, "f32": 590.1},
-    "exp": {"f64": 1146.2, "f32": 217.2},
+    "exp": {"f64": 711.3, "f32": 217.0},
 }
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 # HBM bytes per image of a kernel on the default headline image (library
