@@ -189,7 +189,10 @@ int jsx_abi_version(void);
  *   "queue_block"     ray ids per queue block (>= 1)
  *   "steal"           1: per-XCD queues with stealing, 0: one global queue
  *   "defer_rhs"       right-hand sides after which a mixed full-Stokes ray is
- *                     split into time-parallel pieces (0 disables)
+ *                     split into time-parallel pieces (0 disables; default
+ *                     512; on the 512^2 paraboloid image 256 lets one
+ *                     pixel's Stokes I drift 1.6e-5 from FP64, 384 and up
+ *                     keep 5e-7)
  *   "split_w_min"     smallest stiffness per piece
  *   "piece_layout"    how split pieces are integrated: 0 by Dormand-Prince
  *                     in a group of 5 lanes (P and the 4 propagator columns);
