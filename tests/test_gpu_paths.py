@@ -141,8 +141,8 @@ def test_split_long_rays_on_gpu(ctx):
 def test_split_exponential_pieces_on_gpu(ctx):
     """piece_layout 2: the split rays' pieces are integrated by exponential
     (Magnus) propagators; every hit pixel's I stays within 1e-5 of FP64 and
-    Q/U/V within 5e-5 of peak I, with fewer piece right-hand sides than the
-    Dormand-Prince pieces of the same split."""
+    Q/U/V within 5e-5 of peak I, with fewer coefficient evaluations than the
+    Dormand-Prince pieces of the same split take right-hand sides."""
     j, im, p = models.paraboloid_helical_model(n_side=64, pixel_pc=0.08)
     g64 = ctx.observe(j, im, p)
     pm = models.with_precision(p, abi.PREC_MIXED, rel_tol=1e-6)
@@ -154,18 +154,55 @@ def test_split_exponential_pieces_on_gpu(ctx):
         for layout in (1, 2):
             ctx.set_option("piece_layout", layout)
             gm = ctx.observe(j, im, pm)
-            rhs[layout] = ctx.get_option("stat_rhs_pieces")
+            # Dormand-Prince pieces count right-hand sides, exponential
+            # pieces coefficient evaluations (two per step)
+            rhs[layout] = ctx.get_option("stat_rhs_pieces" if layout == 1 else "stat_rhs_exp")
             parked = ctx.get_option("stat_parked")
     finally:
         ctx.set_option("defer_rhs", old[0])
         ctx.set_option("piece_layout", old[1])
     rel, quv, nhit = _mixed_errors(g64, gm)
-    print("exponential pieces: %d parked rays; piece rhs %d (Dormand-Prince %d); max rel I %.3g, QUV/peak %.3g" % (
+    print("exponential pieces: %d parked rays; coefficient evaluations %d (Dormand-Prince rhs %d); max rel I %.3g, QUV/peak %.3g" % (
         parked, rhs[2], rhs[1], rel, quv), flush=True)
     assert parked > 0
-    assert rhs[2] < rhs[1]
+    assert 0 < rhs[2] < rhs[1]
     assert rel <= 1e-5
     assert quv <= 5e-5
+
+
+def test_exponential_error_estimates_on_gpu(ctx):
+    """Both step error estimates of the exponential pieces (mg_err 0: first
+    order in the commutator terms; 1: a second, 2nd-order exponential per
+    step) hold every hit pixel's I within 1e-5 of FP64 and Q/U/V within 5e-5
+    of peak I, at the default tolerance and at a 4x tighter one; the tighter
+    tolerance takes more coefficient evaluations."""
+    j, im, p = models.paraboloid_helical_model(n_side=64, pixel_pc=0.08)
+    g64 = ctx.observe(j, im, p)
+    pm = models.with_precision(p, abi.PREC_MIXED, rel_tol=1e-6)
+    pm.flags |= OPT_PERSISTENT | OPT_APEX_FIRST
+    names = ("defer_rhs", "piece_layout", "mg_err", "mg_rel")
+    old = [ctx.get_option(k) for k in names]
+    ctx.set_option("defer_rhs", 4096)
+    ctx.set_option("piece_layout", 2)
+    try:
+        for est in (0, 1):
+            ctx.set_option("mg_err", est)
+            evals = []
+            for tol in (old[3], old[3] / 4):
+                ctx.set_option("mg_rel", tol)
+                gm = ctx.observe(j, im, pm)
+                parked, n_exp = ctx.get_option("stat_parked"), ctx.get_option("stat_rhs_exp")
+                rel, quv, nhit = _mixed_errors(g64, gm)
+                print("mg_err %d mg_rel %.2g: %d parked, %d coefficient evaluations; max rel I %.3g, QUV/peak %.3g"
+                      % (est, tol, parked, n_exp, rel, quv), flush=True)
+                assert parked > 0 and n_exp > 0
+                assert rel <= 1e-5
+                assert quv <= 5e-5
+                evals.append(n_exp)
+            assert evals[1] > evals[0]
+    finally:
+        for k, v in zip(names, old):
+            ctx.set_option(k, v)
 
 
 def test_batch_equals_single_images(ctx):
