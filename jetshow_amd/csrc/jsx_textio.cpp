@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "../../include/jetshow_amd.h"
@@ -35,20 +36,20 @@ const double kPc = 3.0856775814671913e+18;   /* include/utils.h pc */
  * (strtod accepts the same decimal and exponent forms, and inf / nan) */
 struct Reader {
     FILE *f;
-    char buf[128];
+    std::string tok;
+    static bool space(int c) { return c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == '\v' || c == '\f'; }
     bool next(double &v) {
         int c;
-        do { c = fgetc(f); } while (c != EOF && (c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == '\v' || c == '\f'));
+        do { c = fgetc(f); } while (c != EOF && space(c));
         if (c == EOF) return false;
-        size_t n = 0;
-        while (c != EOF && !(c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == '\v' || c == '\f')) {
-            if (n + 1 < sizeof(buf)) buf[n++] = (char)c;
+        tok.clear();
+        while (c != EOF && !space(c)) {
+            tok.push_back((char)c);
             c = fgetc(f);
         }
-        buf[n] = 0;
         char *end = nullptr;
-        v = strtod(buf, &end);
-        return end != buf && *end == 0;
+        v = strtod(tok.c_str(), &end);
+        return end != tok.c_str() && *end == 0;
     }
 };
 
@@ -104,7 +105,7 @@ extern "C" int jsx_read_field_text(const char *path, double *zrv, int64_t cap, i
     if (!path || !n_points || cap < 0 || (cap > 0 && !zrv)) return JSX_EINVAL;
     FILE *f = fopen(path, "r");
     if (!f) return JSX_ENOTFOUND;
-    Reader rd{f, {0}};
+    Reader rd{f, {}};
     int64_t n = 0;
     double row[3];
     for (;;) {

@@ -79,6 +79,11 @@ def test_read_field_text_stops_at_first_bad_row(tmp_path):
     assert pyjetshow.read_from_txt(str(tmp_path / "missing.txt")) == []
     with pytest.raises(host.JsxError):
         textio.read_field_text(str(tmp_path / "missing.txt"))
+    # a number of any length is read whole, as stream extraction reads it
+    long_tok = "1" + "0" * 199
+    q = tmp_path / "long.txt"
+    q.write_text("%s 2 3\n4 5 %s.5e-190\n" % (long_tok, long_tok))
+    assert textio.read_field_text(str(q)).tolist() == [[1e199, 2, 3], [4, 5, float(long_tok + ".5e-190")]]
 
 
 def _grid_points(nz=7, nr=5, seed=3):
