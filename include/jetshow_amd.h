@@ -229,7 +229,7 @@ int jsx_abi_version(void);
  *   "park_wave"       long rays may also be split once at most this many
  *                     lanes of their own wavefront still hold a ray (0: off)
  *   "park_lanes"      rays are split only once the queues are dry and at most
- *                     this many lanes of the launch still hold a ray (0: half a
+ *                     this many lanes of the launch still hold a ray (0: one
  *                     wavefront per SIMD of the device)
  *   "split_floor_max" cap of the propagator-column error loosening (1 = off;
  *                     >1 trades Q/U/V accuracy for fewer piece steps)

@@ -889,8 +889,10 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
     }
     const int64_t tri_need = (total + 255) / 256;
     const int tri_blocks = (int)(tri_need < (int64_t)ctx->n_cu * 16 ? tri_need : (int64_t)ctx->n_cu * 16);
-    /* default tail threshold: half a wavefront per SIMD of the chip */
-    const unsigned park_lanes = ctx->park_lanes ? ctx->park_lanes : (unsigned)ctx->n_cu * 4u * 32u;
+    /* default tail threshold: one wavefront per SIMD of the chip (half a
+     * wavefront per SIMD left the long rays of a many-image sweep running
+     * sequentially for longer: 11% slower on 512 images, 2% on one) */
+    const unsigned park_lanes = ctx->park_lanes ? ctx->park_lanes : (unsigned)ctx->n_cu * 4u * 64u;
     mark(5);
     if (precision == JSX_PREC_MIXED) {
         const DParamsM *R = (const DParamsM *)ctx->d_reqs;
