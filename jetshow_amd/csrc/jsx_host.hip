@@ -646,6 +646,11 @@ static int stage_params(jsx_ctx *ctx, const DModel &m, const DImage &D, const DS
 #ifndef JSX_MIXED_STOKES_FLOOR_FULL
 #define JSX_MIXED_STOKES_FLOOR_FULL 1e-7
 #endif
+/* the smallest relative tolerance the mixed path holds the Stokes state to
+   (the state is FP32 + compensation, ~1e-14) */
+#ifndef JSX_MIXED_STATE_FLOOR
+#define JSX_MIXED_STATE_FLOOR 1e-10
+#endif
 #ifndef JSX_MIXED_STOKES_FLOOR_I
 #define JSX_MIXED_STOKES_FLOOR_I 1e-7
 #endif
@@ -668,8 +673,21 @@ static void lower_mixed(const jsx_jet_desc &j, const jsx_solver_params &p, const
     c.A_F = (float)(2. * PI * CP * CB / (CL * nu * nu) * PC);
     c.A_C = (float)(PI * CP * CB * CB / (CL * nu * nu * nu) * PC);
     c.r_kq = (float)m.r_kq; c.r_eq = (float)m.r_eq; c.f_kFr = (float)m.f_kFr;
-    c.kCr_pre = (float)m.kCr_pre; c.kCr_gm = (float)m.kCr_gm; c.kC_xfac = (float)pow(g, (s - 2.) / 2.);
-    c.e_ki = (float)m.e_ki; c.e_ei = (float)m.e_ei; c.e_kv = (float)m.e_kv; c.e_kCr = (float)m.e_kCr;
+    {   /* k_C_r through expm1 (jsx_mixed.h): y = (s-2)(ln gmin + ln(g x)/2), x = 2^l2 */
+        const double e = s - 2., lg = log(j.gamma_min) + 0.5 * log(g);
+        if (e != 0.0) {
+            c.kC_a = (float)(-(2. / e) * pow(j.gamma_min, -e));
+            c.kC_lin = 0.f;
+            c.kC_y0 = e * lg;
+            c.kC_y1 = 0.5 * e * 0.69314718055994530942;
+        } else {
+            c.kC_a = -2.f;
+            c.kC_lin = 1.f;
+            c.kC_y0 = lg;
+            c.kC_y1 = 0.5 * 0.69314718055994530942;
+        }
+    }
+    c.e_ki = (float)m.e_ki; c.e_ei = (float)m.e_ei; c.e_kv = (float)m.e_kv;
     c.n0 = (float)j.n0; c.b0 = (float)j.b0; c.n_negexp = (float)m.n_negexp; c.b_negexp = (float)m.b_negexp;
     double gam = j.gamma0 > 1.0 ? j.gamma0 : 1.0;
     double beta = sqrt(1. - 1. / (gam * gam));
@@ -693,6 +711,7 @@ static void lower_mixed(const jsx_jet_desc &j, const jsx_solver_params &p, const
     c.tau_rel = p.tau_rel_tol > ft ? p.tau_rel_tol : ft;
     c.st_abs = p.stokes_abs_tol;
     c.st_rel = p.stokes_rel_tol > fs ? p.stokes_rel_tol : fs;
+    c.st_rel_x = p.stokes_rel_tol > JSX_MIXED_STATE_FLOOR ? p.stokes_rel_tol : JSX_MIXED_STATE_FLOOR;
     c.dt_max_pc = p.dt_max / PC;
 }
 

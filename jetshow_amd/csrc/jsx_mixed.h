@@ -45,14 +45,16 @@
 /* host-derived FP32 constants of the mixed path */
 struct MConsts {
     float A_k, A_e, A_kv, A_ev, A_F, A_C;  /* per-parsec coefficient prefactors           */
-    float r_kq, r_eq, f_kFr, kCr_pre, kCr_gm, kC_xfac;
-    float e_ki, e_ei, e_kv, e_kCr;
+    float r_kq, r_eq, f_kFr, kC_a, kC_lin;
+    float e_ki, e_ei, e_kv;
     float n0, b0, n_negexp, b_negexp;
     float beta_flat, omb_flat, gam_flat;  /* ConstFlat beta, 1 - beta, Gamma              */
     float one_minus_mu;                   /* 1 - n.z for v along z                        */
     float n_los[3], axis[3], sin_pitch, cos_pitch, thermal_fraction;
     double inv_pc;
+    double kC_y0, kC_y1;                  /* k_C_r's exponent y = y0 + y1 log2(x)          */
     double tau_abs, tau_rel, st_abs, st_rel, dt_max_pc;
+    double st_rel_x;                      /* the caller's Stokes tolerance on |x|          */
 };
 
 /* x^e for x > 0 (see header) */
@@ -234,13 +236,18 @@ __device__ inline void m_coefficients(const DModel &m, const MConsts &c, const d
         return;
     }
     float kq = c.r_kq * kI, eq = c.r_eq * eI;
-    float p_kv = pw(c.e_kv), p_kC = pw(c.e_kCr);
+    float p_kv = pw(c.e_kv);
     float kv = -c.A_kv * nd * bn * D * cth * p_kv;
     float ev = -c.A_ev * nd * bn * D * D * cth * p_ki;
     float kFc = c.A_F * nd * bn * cth * D;
     float kCc = -c.A_C * nd * (bp * D) * (bp * D);
     float kF = c.f_kFr * kFc;
-    float kC = c.kCr_pre * (c.kCr_gm - c.kC_xfac * p_kC) * kCc;
+    /* k_C_r = 2/(s-2) (gmin^(2-s) - X^((s-2)/2)) k_C_c with X = nu_B_perp / nu'
+       (utils.cpp:93-98), without the cancellation of its two terms for s near
+       2: = -(2/(s-2)) gmin^(2-s) expm1(y) k_C_c, y = (s-2)(ln gmin + ln(X)/2)
+       formed in FP64 (kC_a = -(2/(s-2)) gmin^(2-s); at s = 2 the limit -2 y') */
+    float ykc = (float)(c.kC_y0 + c.kC_y1 * l2);
+    float kC = c.kC_a * (c.kC_lin != 0.f ? ykc : expm1f(ykc)) * kCc;
     if (c.thermal_fraction != 0.f) {
         float nth = c.thermal_fraction;
         kF = kF + nth * c.A_F * nd * bn * cth * D;
@@ -535,19 +542,31 @@ template <int NS>
 __device__ inline double ml_step_error(const MLane<NS> &L, const MConsts &c, double floor = 0.0, double rel_min = 0.0) {
     double dt = L.dt, err = 0.0;
     int nl = L.phase == PH_TAU ? 1 : NS;
+    /* odeint's norm |xerr| / (abs + rel (|x| + dt |dx/dt|)) with the relative
+       tolerance split: the caller's on |x| (st_rel_x), the FP32 floor on the
+       step's increment dt |dx/dt| (st_rel), which is what the stages' rounding
+       scales with -- so on long optically thin paths, where |x| >> dt |dx/dt|,
+       the state is held to the caller's tolerance as in FP64 */
     double ea = L.phase == PH_TAU ? c.tau_abs : c.st_abs, er = L.phase == PH_TAU ? c.tau_rel : c.st_rel;
+    double erx = L.phase == PH_TAU ? er : c.st_rel_x;
     er = er > rel_min ? er : rel_min;
-    double xe[NS], sc[NS], pmax = 0.0;
+    erx = erx > rel_min ? erx : rel_min;
+    double xe[NS], sx[NS], sk[NS], px = 0.0, pk = 0.0;
     for (int i = 0; i < nl; ++i) {
         xe[i] = dt * (DP_DC1 * (double)L.k1[i] + DP_DC3 * (double)L.k3[i] + DP_DC4 * (double)L.k4[i] +
                       DP_DC5 * (double)L.k5[i] + DP_DC6 * (double)L.k6[i] + DP_DC7 * (double)L.k7[i]);
-        sc[i] = fabs(ml_xval(L, i)) + dt * fabs((double)L.k1[i]);
-        if (i > 0) pmax = sc[i] > pmax ? sc[i] : pmax;
+        sx[i] = fabs(ml_xval(L, i));
+        sk[i] = dt * fabs((double)L.k1[i]);
+        if (i > 0) {
+            px = sx[i] > px ? sx[i] : px;
+            pk = sk[i] > pk ? sk[i] : pk;
+        }
     }
     for (int i = 0; i < nl; ++i) {
-        double s = i > 0 ? pmax : sc[i];
-        s = s > floor ? s : floor;
-        double e = fabs(xe[i]) / (ea + er * s);
+        double s = i > 0 ? erx * px + er * pk : erx * sx[i] + er * sk[i];
+        double sf = er * floor;
+        s = s > sf ? s : sf;
+        double e = fabs(xe[i]) / (ea + s);
         err = (err < e) ? e : err;
     }
     return err;

@@ -620,60 +620,70 @@ static void jet_frame(const orc_model *m, const double *p, const double *n_los, 
     f->nu = nu / f->D;
 }
 
-/* all eleven lab-frame coefficients (Jet.cpp:15-46, 114-288 restored) */
-typedef struct { double kI, kQ, kU, kV, kF, kC, hQ, eI, eQ, eU, eV; } orc_coef;
+/* The plasma-frame functions of src/utils.cpp at one point: b (vector, or
+ * b[0] for a tangled field), n_los' and nu' in the plasma frame, n the
+ * relativistic electron density there.  Every expression is the one the
+ * lab-frame assembly below used before this split, so the HIP kernels'
+ * bits are unchanged. */
+typedef struct {
+    double ki, ei;            /* k_i, eta_i (vector or random-B overload)            */
+    double kq, eq, kv, ev;    /* k_q, eta_q, k_v, eta_v (vector only)                */
+    double kFc, kCc;          /* k_F_c, k_C_c at n                                   */
+    double kFr, kCr;          /* k_F_r, k_C_r at n                                   */
+    double kF, kC;            /* k_F_r(n) + k_F_c(f n), k_C_r(n) + k_C_c(f n)        */
+} orc_plasma;
 
-static void jet_coefficients(const orc_model *m, const double *p, const double *n_los, double nu,
-                             int full, orc_coef *o) {
-    orc_frame f;
-    jet_frame(m, p, n_los, nu, &f);
+static void plasma_coefficients(const orc_model *m, const double *b, int vec, const double *nl, double fnu,
+                                double n, int full, orc_plasma *u) {
     const orc_math *mth = &m->mth;
-    double D = f.D, n = f.n, fnu = f.nu;
     double nup = nu_p(n);
-    memset(o, 0, sizeof(*o));
-    if (!f.vec) {
+    memset(u, 0, sizeof(*u));
+    if (!vec) {
         /* random B: k_i, eta_i scalar overloads (utils.cpp:31-33,46-51,111-113,125-130) */
-        double bs = f.b[0];
+        double bs = b[0];
         double nub = nu_b_scalar(bs);
         double k0 = K_PI * nup * nup * nu_b_scalar(bs) / (K_C * fnu * fnu);
-        double ki = k0 * M_pow(mth, nub / fnu, m->e_ki) * m->f_ki_s;
+        u->ki = k0 * M_pow(mth, nub / fnu, m->e_ki) * m->f_ki_s;
         double e0 = K_PI * nup * nup * nu_b_scalar(bs) * K_ME / K_C;
-        double ei = e0 * M_pow(mth, nub / fnu, m->e_ei) * m->f_ei_s;
-        o->kI = ki / D;
-        o->eI = ei * D * D;
+        u->ei = e0 * M_pow(mth, nub / fnu, m->e_ei) * m->f_ei_s;
         return; /* tangled field: no net polarisation or Faraday terms */
     }
-    const double *b = f.b, *nl = f.nl;
     double nub = nu_b_vec(b, nl);
-    double k0 = K_PI * nup * nup * nu_b_vec(b, nl) / (K_C * fnu * fnu);
-    double ki = k0 * M_pow(mth, nub / fnu, m->e_ki) * m->f_ki_v;
-    double e0 = K_PI * nup * nup * nu_b_vec(b, nl) * K_ME / K_C;
-    double ei = e0 * M_pow(mth, nub / fnu, m->e_ei) * m->f_ei_v;
-    o->kI = ki / D;
-    o->eI = ei * D * D;
+    double k0 = K_PI * nup * nup * nu_b_vec(b, nl) / (K_C * fnu * fnu);        /* :26-28 */
+    u->ki = k0 * M_pow(mth, nub / fnu, m->e_ki) * m->f_ki_v;                    /* :39-43 */
+    double e0 = K_PI * nup * nup * nu_b_vec(b, nl) * K_ME / K_C;                 /* :107-109 */
+    u->ei = e0 * M_pow(mth, nub / fnu, m->e_ei) * m->f_ei_v;                    /* :119-122 */
     if (!full) return;
-    double kq = m->r_kq * ki;                                   /* utils.cpp:63-65 */
-    double eq = m->r_eq * ei;                                   /* utils.cpp:132-134 */
+    u->kq = m->r_kq * u->ki;                                    /* utils.cpp:63-65 */
+    u->eq = m->r_eq * u->ei;                                    /* utils.cpp:132-134 */
     double cos_theta = dot3(b, nl) / norm3(b);                  /* utils.cpp:72 */
     double nubv = nu_b_value(b);
     double k0v = K_PI * nup * nup * nubv / (K_C * fnu * fnu);
-    double kv = -k0v * cos_theta * M_pow(mth, nub / fnu, m->e_kv) * m->f_kv;     /* :71-75 */
+    u->kv = -k0v * cos_theta * M_pow(mth, nub / fnu, m->e_kv) * m->f_kv;        /* :71-75 */
     double e0v = K_PI * nup * nup * nubv * K_ME / K_C;
-    double ev = -e0v * cos_theta * M_pow(mth, nub / fnu, m->e_ki) * m->f_ev;     /* :140-144 */
-    double kFc = 2. * K_PI * nup * nup * nubv * cos_theta / (K_C * fnu * fnu);  /* :77-80 */
-    double kCc = -K_PI * nup * nup * nub * nub / (K_C * fnu * fnu * fnu);       /* :82-84 */
-    double kF = m->f_kFr * kFc;                                                  /* :86-91 */
-    double kC = m->kCr_pre * (m->kCr_gm - M_pow(mth, nub / fnu, m->e_kCr)) * kCc; /* :93-98 */
+    u->ev = -e0v * cos_theta * M_pow(mth, nub / fnu, m->e_ki) * m->f_ev;        /* :140-144 */
+    u->kFc = 2. * K_PI * nup * nup * nubv * cos_theta / (K_C * fnu * fnu);     /* :77-80 */
+    u->kCc = -K_PI * nup * nup * nub * nub / (K_C * fnu * fnu * fnu);          /* :82-84 */
+    u->kFr = m->f_kFr * u->kFc;                                                 /* :86-91 */
+    u->kCr = m->kCr_pre * (m->kCr_gm - M_pow(mth, nub / fnu, m->e_kCr)) * u->kCc; /* :93-98 */
+    u->kF = u->kFr;
+    u->kC = u->kCr;
     if (m->j->thermal_fraction != 0.0) {
         /* cold thermal electrons (extension): k_F_c, k_C_c at n_th */
         double nth = m->j->thermal_fraction * n;
         double nupt = nu_p(nth);
-        kF = kF + 2. * K_PI * nupt * nupt * nubv * cos_theta / (K_C * fnu * fnu);
-        kC = kC + -K_PI * nupt * nupt * nub * nub / (K_C * fnu * fnu * fnu);
+        u->kF = u->kF + 2. * K_PI * nupt * nupt * nubv * cos_theta / (K_C * fnu * fnu);
+        u->kC = u->kC + -K_PI * nupt * nupt * nub * nub / (K_C * fnu * fnu * fnu);
     }
-    /* Stokes frame (extension): the reference's k_u = eta_u = h_Q = 0 hold in
-     * a frame whose Q axis is perpendicular to the projected B'.  Rotate into
-     * the sky frame built from n' and the jet axis. */
+}
+
+/* all eleven lab-frame coefficients (Jet.cpp:15-46, 114-288 restored) */
+typedef struct { double kI, kQ, kU, kV, kF, kC, hQ, eI, eQ, eU, eV; } orc_coef;
+
+/* Stokes frame (extension): the reference's k_u = eta_u = h_Q = 0 hold in a
+ * frame whose Q axis is perpendicular to the projected B'.  cos 2chi and
+ * sin 2chi of the rotation into the sky frame built from n' and the jet axis. */
+static void sky_frame_angle(const orc_model *m, const double *b, const double *nl, double *c2o, double *s2o) {
     double a[3] = {m->axis[0], m->axis[1], m->axis[2]};
     double an = dot3(a, nl);
     double e1[3] = {a[0] - an * nl[0], a[1] - an * nl[1], a[2] - an * nl[2]};
@@ -683,15 +693,32 @@ static void jet_coefficients(const orc_model *m, const double *p, const double *
     double q2 = bx * bx + by * by;
     double c2 = 1.0, s2 = 0.0;
     if (q2 > 0.0) { c2 = -(bx * bx - by * by) / q2; s2 = -(2.0 * bx * by) / q2; }
-    o->kQ = (kq * c2) / D;
-    o->kU = (kq * s2) / D;
-    o->kV = kv / D;
-    o->kF = kF / D;
-    o->kC = (kC * c2) / D;
-    o->hQ = -(kC * s2) / D;
-    o->eQ = (eq * c2) * D * D;
-    o->eU = (eq * s2) * D * D;
-    o->eV = ev * D * D;
+    *c2o = c2;
+    *s2o = s2;
+}
+
+static void jet_coefficients(const orc_model *m, const double *p, const double *n_los, double nu,
+                             int full, orc_coef *o) {
+    orc_frame f;
+    jet_frame(m, p, n_los, nu, &f);
+    double D = f.D;
+    orc_plasma u;
+    plasma_coefficients(m, f.b, f.vec, f.nl, f.nu, f.n, full, &u);
+    memset(o, 0, sizeof(*o));
+    o->kI = u.ki / D;                /* Jet.cpp:33 */
+    o->eI = u.ei * D * D;            /* Jet.cpp:236 */
+    if (!f.vec || !full) return;
+    double c2, s2;
+    sky_frame_angle(m, f.b, f.nl, &c2, &s2);
+    o->kQ = (u.kq * c2) / D;
+    o->kU = (u.kq * s2) / D;
+    o->kV = u.kv / D;
+    o->kF = u.kF / D;
+    o->kC = (u.kC * c2) / D;
+    o->hQ = -(u.kC * s2) / D;
+    o->eQ = (u.eq * c2) * D * D;
+    o->eU = (u.eq * s2) * D * D;
+    o->eV = u.ev * D * D;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -707,6 +734,14 @@ typedef struct {
     uint32_t *n_rhs;
     int nan_seen;
 } orc_sys;
+
+/* the restored FullStokes system (System.cpp:41-69) */
+static void stokes_rhs(const orc_coef *c, const double *x, double *dxdt) {
+    dxdt[0] = c->eI - c->kI * x[0] - c->kQ * x[1] - c->kU * x[2] - c->kV * x[3];
+    dxdt[1] = c->eQ - c->kI * x[1] - c->kQ * x[0] - c->kF * x[2] - c->hQ * x[3];
+    dxdt[2] = c->eU - c->kI * x[2] - c->kU * x[0] + c->kF * x[1] - c->kC * x[3];
+    dxdt[3] = c->eV - c->kI * x[3] - c->kV * x[0] + c->hQ * x[1] + c->kC * x[2];
+}
 
 static void sys_rhs(orc_sys *S, const double *x, double *dxdt, double t) {
     double p[3];
@@ -725,10 +760,7 @@ static void sys_rhs(orc_sys *S, const double *x, double *dxdt, double t) {
     } else if (S->kind == 1) {
         dxdt[0] = c.eI - c.kI * x[0];
     } else {
-        dxdt[0] = c.eI - c.kI * x[0] - c.kQ * x[1] - c.kU * x[2] - c.kV * x[3];
-        dxdt[1] = c.eQ - c.kI * x[1] - c.kQ * x[0] - c.kF * x[2] - c.hQ * x[3];
-        dxdt[2] = c.eU - c.kI * x[2] - c.kU * x[0] + c.kF * x[1] - c.kC * x[3];
-        dxdt[3] = c.eV - c.kI * x[3] - c.kV * x[0] + c.hQ * x[1] + c.kC * x[2];
+        stokes_rhs(&c, x, dxdt);
     }
     int ns = S->kind == 4 ? 4 : 1;
     for (int i = 0; i < ns; ++i)
@@ -1280,6 +1312,55 @@ int orc_coefficients(const jsx_jet_desc *jet, const orc_grid *grids, int32_t n_g
     jet_coefficients(&m, point, n_los, nu, 1, &c);
     double v[11] = {c.kI, c.kQ, c.kU, c.kV, c.kF, c.kC, c.hQ, c.eI, c.eQ, c.eU, c.eV};
     memcpy(out, v, sizeof v);
+    return 0;
+}
+
+/* the full-Stokes system with constant coefficients, integrated over [0, L]
+ * from S0 by the same controlled Dormand-Prince as the I pass */
+typedef struct { orc_coef c; uint32_t n_rhs; } orc_slab;
+
+static void slab_rhs(void *ctx, const double *x, double *dxdt, double t) {
+    (void)t;
+    orc_slab *s = (orc_slab *)ctx;
+    s->n_rhs++;
+    stokes_rhs(&s->c, x, dxdt);
+}
+
+int orc_stokes_slab(const double *coef, double L, const double *S0, double abs_tol, double rel_tol,
+                    double *out) {
+    orc_slab s;
+    s.c.kI = coef[0]; s.c.kQ = coef[1]; s.c.kU = coef[2]; s.c.kV = coef[3]; s.c.kF = coef[4];
+    s.c.kC = coef[5]; s.c.hQ = coef[6]; s.c.eI = coef[7]; s.c.eQ = coef[8]; s.c.eU = coef[9];
+    s.c.eV = coef[10];
+    s.n_rhs = 0;
+    uint32_t n_steps = 0;
+    orc_math mth = {ORC_MATH_LIBM};
+    orc_dp dp;
+    memset(&dp, 0, sizeof dp);
+    dp.f = slab_rhs; dp.ctx = &s; dp.ns = 4;
+    dp.eps_abs = abs_tol; dp.eps_rel = rel_tol; dp.max_dt = 0.0;
+    dp.n_steps = &n_steps; dp.mth = &mth;
+    double x[4] = {S0[0], S0[1], S0[2], S0[3]};
+    uint32_t fl = dp_integrate_adaptive(&dp, x, L, L / 100.0, 500);
+    memcpy(out, x, sizeof x);
+    return fl ? -1 : (int)s.n_rhs;
+}
+
+int orc_plasma_coefficients(const jsx_jet_desc *jet, const double *b, int32_t vec, const double *n_los_prime,
+                            double nu_prime, double n, int32_t math_mode, double *out) {
+    orc_model m;
+    model_init(&m, jet, NULL, 0, math_mode, 0);
+    orc_plasma u;
+    plasma_coefficients(&m, b, vec, n_los_prime, nu_prime, n, 1, &u);
+    double v[12] = {u.ki, u.ei, u.kq, u.eq, u.kv, u.ev, u.kFc, u.kCc, u.kFr, u.kCr, u.kF, u.kC};
+    memcpy(out, v, sizeof v);
+    return 0;
+}
+
+int orc_sky_angle(const jsx_jet_desc *jet, const double *b, const double *n_los_prime, double *out) {
+    orc_model m;
+    model_init(&m, jet, NULL, 0, ORC_MATH_LIBM, 0);
+    sky_frame_angle(&m, b, n_los_prime, &out[0], &out[1]);
     return 0;
 }
 

@@ -58,6 +58,27 @@ int orc_coefficients(const jsx_jet_desc *jet, const orc_grid *grids, int32_t n_g
                      double nu, const double *point, const double *n_los,
                      int32_t math_mode, double *out);
 
+/* The plasma-frame functions of src/utils.cpp:8-144 at one point, as the
+ * lab-frame coefficients above use them: b' (b[0..2], or b[0] alone when
+ * vec == 0: the random-B overloads), n_los' and nu' in the plasma frame, n the
+ * relativistic density there; s, gamma_min and the thermal fraction come from
+ * `jet`.  out[0..11] = k_i, eta_i, k_q, eta_q, k_v, eta_v, k_F_c, k_C_c,
+ * k_F_r, k_C_r, k_F (k_F_r + k_F_c at the thermal density), k_C (likewise). */
+int orc_plasma_coefficients(const jsx_jet_desc *jet, const double *b, int32_t vec, const double *n_los_prime,
+                            double nu_prime, double n, int32_t math_mode, double *out);
+
+/* The restored full-Stokes system (src/System.cpp:41-69) with constant
+ * coefficients coef[0..10] (kI, kQ, kU, kV, kF, kC, hQ, etaI, etaQ, etaU, etaV)
+ * integrated from S0 over [0, L] by the controlled Dormand-Prince the I pass
+ * uses (first step L/100).  out[0..3] = I, Q, U, V.  Returns the RHS count, or
+ * -1 when a step failed. */
+int orc_stokes_slab(const double *coef, double L, const double *S0, double abs_tol, double rel_tol,
+                    double *out);
+
+/* cos 2chi, sin 2chi of the rotation from the B'-aligned Stokes frame into the
+ * sky frame (e1 along the jet axis projected on the sky, e2 = n' x axis). */
+int orc_sky_angle(const jsx_jet_desc *jet, const double *b, const double *n_los_prime, double *out);
+
 /* Field values at a point: out = v[3], gamma, b'[3] (scalar fields fill b'[0]),
  * n' (plasma frame), D, n_los'[3], nu'. */
 int orc_fields(const jsx_jet_desc *jet, const orc_grid *grids, int32_t n_grids,

@@ -134,6 +134,37 @@ def coefficients(jet, nu, point, n_los, math=MATH_JSX, grids=None):
     return out
 
 
+def plasma_coefficients(jet, b, n_los_prime, nu_prime, n, math=MATH_LIBM):
+    """src/utils.cpp:8-144 in the plasma frame.  b: 3-vector, or a scalar for
+    the random-B overloads.  Returns k_i, eta_i, k_q, eta_q, k_v, eta_v, k_F_c,
+    k_C_c, k_F_r, k_C_r, k_F, k_C (zeros past eta_i for a scalar b)."""
+    out = np.zeros(12)
+    bb = np.atleast_1d(np.asarray(b, float))
+    vec = 1 if bb.size == 3 else 0
+    bb = np.ascontiguousarray(np.resize(bb, 3))
+    lib().orc_plasma_coefficients(C.byref(jet), _p(bb), vec, _p(np.asarray(n_los_prime, float)),
+                                  C.c_double(nu_prime), C.c_double(n), int(math), _p(out))
+    return out
+
+
+def stokes_slab(coef, L, S0=(0.0, 0.0, 0.0, 0.0), abs_tol=1e-14, rel_tol=1e-12):
+    """The full-Stokes system with constant coefficients (kI, kQ, kU, kV, kF,
+    kC, hQ, etaI, etaQ, etaU, etaV) over [0, L] from S0; returns (IQUV, n_rhs)."""
+    out = np.zeros(4)
+    c = np.ascontiguousarray(coef, dtype=np.float64)
+    s0 = np.ascontiguousarray(S0, dtype=np.float64)
+    n = lib().orc_stokes_slab(_p(c), C.c_double(L), _p(s0), C.c_double(abs_tol), C.c_double(rel_tol), _p(out))
+    if n < 0:
+        raise RuntimeError("orc_stokes_slab: step failure")
+    return out, n
+
+
+def sky_angle(jet, b, n_los_prime):
+    out = np.zeros(2)
+    lib().orc_sky_angle(C.byref(jet), _p(np.asarray(b, float)), _p(np.asarray(n_los_prime, float)), _p(out))
+    return out
+
+
 def fields(jet, nu, point, n_los, math=MATH_JSX, grids=None):
     out = np.zeros(14)
     garr, ng, keep = make_grids(grids or [])

@@ -163,16 +163,16 @@ def test_hull_hit_and_image_on_gpu(ctx):
 
 @pytest.mark.gpu
 def test_hull_mixed_precision_image(ctx):
-    """The mixed-precision solver through the hull: identical hit mask and
-    Stokes I within 2e-5 of the FP64 image (analytic fields).  Both runs use
-    relative Stokes control (abs 1e-20; mixed rel 1e-7), as the other
-    mixed-against-FP64 checks do: under the reference's absolute tolerance
-    faint pixels are unconverged at the 1e-3 level in FP64 itself.  The bar is
-    2e-5, not the 1e-5 of the headline images: these rays cross up to ~20 pc
-    of optically thin cylinder (tau ~ 0.003) in ~170 steps, and the local
-    error bound per step (the mixed solver's relative tolerance floor, 1e-7)
-    adds up along the path -- 1.3e-5 on the worst pixel, where FP64 itself at
-    its 1e-8 tolerance is 3.4e-6 from the converged value."""
+    """The mixed-precision solver through the hull: identical hit mask, and
+    Stokes I within 1e-5 of a converged FP64 image (relative tolerance 1e-11)
+    and of the FP64 image at the same tolerance as the mixed run (1e-8).
+    These rays cross up to ~20 pc of optically thin cylinder (tau ~ 0.003),
+    where the Dormand-Prince local errors, each bounded relative to the
+    accumulated intensity, add up along the path (FP64 at 1e-8 is ~5e-6 from
+    the converged image); the mixed solver holds the state to the caller's
+    tolerance and only the increment's share of the error norm to its FP32
+    floor, so at the same tolerance it lands as close as FP64 does.  Both runs
+    use relative Stokes control (abs 1e-20)."""
     # from z = 1 pc on: the conical fields (B ~ 1/r, N ~ 1/r^2) are singular at
     # the origin, which a cylinder from z = 0 would contain
     planes = _cylinder_hull(nz=20, z0_pc=1.0)
@@ -185,13 +185,18 @@ def test_hull_mixed_precision_image(ctx):
         j.z_clip_min = j.z_clip_max = 0.0
         p.stokes_abs_tol = 1e-20
         g64 = ctx.observe(j, im, p)
-        gm = ctx.observe(j, im, models.with_precision(p, abi.PREC_MIXED, rel_tol=1e-7))
+        gconv = ctx.observe(j, im, models.with_precision(p, abi.PREC_FP64, rel_tol=1e-11))
+        gm = ctx.observe(j, im, models.with_precision(p, abi.PREC_MIXED, rel_tol=1e-8))
         hit = (g64["flags"] & abi.PIX_HIT) != 0
         assert hit.sum() > 100
         assert np.array_equal(hit, (gm["flags"] & abi.PIX_HIT) != 0)
-        pos = hit & (g64["I"] > 0)
-        rel = np.abs(gm["I"][pos] - g64["I"][pos]) / g64["I"][pos]
-        print("hull mixed: %d hits, max rel I err %.3g" % (hit.sum(), rel.max()), flush=True)
-        assert rel.max() <= 2e-5, rel.max()
+        pos = hit & (gconv["I"] > 0)
+        rel_conv = np.abs(gm["I"][pos] - gconv["I"][pos]) / gconv["I"][pos]
+        rel_64 = np.abs(gm["I"][pos] - g64["I"][pos]) / g64["I"][pos]
+        fp64_own = np.abs(g64["I"][pos] - gconv["I"][pos]) / gconv["I"][pos]
+        print("hull mixed: %d hits, max rel I err %.3g vs converged, %.3g vs FP64 1e-8 (FP64 1e-8 itself %.3g)"
+              % (hit.sum(), rel_conv.max(), rel_64.max(), fp64_own.max()), flush=True)
+        assert rel_conv.max() <= 1e-5, rel_conv.max()
+        assert rel_64.max() <= 1e-5, rel_64.max()
     finally:
         ctx.release_grid(h)

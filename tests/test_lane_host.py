@@ -174,3 +174,27 @@ def test_split_long_rays_exponential_pieces():
     print("rounds: pieces %s, max rel I %.3g, QUV/peak %.3g" % (hr["n_pieces"], rel_r, quv_r))
     assert rel_r <= 1e-6
     assert quv_r <= 5e-5
+
+
+def test_mixed_thin_path_holds_callers_tolerance():
+    """~20 pc of optically thin cylinder (conical fields, ~40 steps per ray):
+    the local errors add up along the path, so the mixed solver must hold the
+    state to the caller's tolerance rather than its FP32 floor.  At rel 1e-8
+    it is within 1e-5 of the converged FP64 image (rel 1e-11) and of FP64 at
+    1e-8 (itself ~5e-6 from converged); at its old uniform 1e-7 floor it was
+    2.2e-5 off."""
+    j, im, p = models.conical_toroidal_model(n_side=48, pixel_pc=0.05)
+    j.geometry = abi.GEOM_CYLINDER
+    j.geo_param = 0.5 * abi.PC
+    j.z_clip_min, j.z_clip_max = 1.0 * abi.PC, 20.0 * abi.PC
+    p.stokes_abs_tol = 1e-20
+    pix = np.arange(im.n_across * im.n_along)
+    o8 = oracle.observe(j, im, p, pix, math=oracle.MATH_LIBM)
+    o11 = oracle.observe(j, im, models.with_precision(p, abi.PREC_FP64, rel_tol=1e-11), pix, math=oracle.MATH_LIBM)
+    m = LH.observe(j, im, models.with_precision(p, abi.PREC_MIXED, rel_tol=1e-8), pix)
+    hit = (o8["flags"] & abi.PIX_HIT) != 0
+    assert hit.sum() > 500
+    assert np.array_equal(hit, (m["flags"] & abi.PIX_HIT) != 0)
+    pos = hit & (o11["I"] > 0)
+    assert (np.abs(m["I"][pos] - o11["I"][pos]) / o11["I"][pos]).max() <= 1e-5
+    assert (np.abs(m["I"][pos] - o8["I"][pos]) / o8["I"][pos]).max() <= 1e-5
