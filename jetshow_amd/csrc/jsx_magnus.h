@@ -384,10 +384,10 @@ __device__ inline int mg_step(const MCoef &k1, const MCoef &k2, double h, double
             ww = fmax(ww, fabs(w4[i]));
             sP = fmax(sP, fmax(fabs(P[i]), fabs(w4[i])));
         }
-        const double eta = sqrt(g2.e[0] * g2.e[0] + g2.e[1] * g2.e[1] + g2.e[2] * g2.e[2]);
+        const double eta = sqrt(g4.e[0] * g4.e[0] + g4.e[1] * g4.e[1] + g4.e[2] * g4.e[2]);
         const double grow = exp(eta - mu);
         const double e1 = grow * dG / rel;
-        const double eP = grow * (dv + dG * ww);
+        const double eP = mg_phi1(eta - mu) * (dv + dG * ww);
         const double e2 = sP > 0.0 ? eP / (rel * sP) : (eP > 0.0 ? __builtin_inf() : 0.0);
         *err = e1 > e2 ? e1 : e2;
         return (*err == *err) ? 1 : 0;

@@ -967,7 +967,7 @@ __device__ inline void mg_lane_begin(MgLane &L, const LongRay &R, const Piece &p
     L.t = pc.ta;
     L.tb = pc.tb;
     const double len = pc.tb - pc.ta;
-    L.h = pc.dt0 > 0.0625 * len ? pc.dt0 : 0.0625 * len;
+    L.h = pc.dt0 > 0.0 ? pc.dt0 : 0.0625 * len;
     L.p_floor = fabs(R.x[0]);
     L.rel_min = rel_min;
     L.flags = 0; L.n_rhs = 0; L.n_steps = 0; L.tries = 0;

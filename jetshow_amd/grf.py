@@ -11,7 +11,8 @@ def gaussian_random_field(shape, seed, spectral_index=-11.0 / 3.0):
     k1 = np.fft.rfftfreq(shape[1])
     K = np.sqrt(k0[:, None] ** 2 + k1[None, :] ** 2)
     K[0, 0] = np.inf
-    f = np.fft.irfftn(np.fft.rfftn(white) * K ** (spectral_index / 2.0), s=shape)
+    axes = tuple(range(len(shape)))
+    f = np.fft.irfftn(np.fft.rfftn(white, axes=axes) * K ** (spectral_index / 2.0), s=shape, axes=axes)
     f -= f.mean()
     s = f.std()
     return f / s if s > 0 else f

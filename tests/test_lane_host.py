@@ -198,3 +198,30 @@ def test_mixed_thin_path_holds_callers_tolerance():
     pos = hit & (o11["I"] > 0)
     assert (np.abs(m["I"][pos] - o11["I"][pos]) / o11["I"][pos]).max() <= 1e-5
     assert (np.abs(m["I"][pos] - o8["I"][pos]) / o8["I"][pos]).max() <= 1e-5
+
+
+@pytest.mark.parametrize("w_piece", [256.0, 16384.0])
+def test_exponential_pieces_through_cone_apex(w_piece):
+    """A sweep model whose line of sight lies inside the cone (its rays pass
+    the apex, where the power-law coefficients rise by orders of magnitude
+    over a short stretch), split into exponential pieces in budgeted rounds,
+    short and long (w_piece: stiffness per piece).  Three past faults of the
+    exponential step's control show up here as errors of 1e-3 .. 1e25: a
+    first step that could straddle the apex; the growth bound taken from the
+    2nd-order generator while the 4th-order one (with the commutator) was
+    applied; the particular solution's error bounded by the norm of
+    exp(-mu + G) instead of phi1(-mu + G), which is ~1/mu, not e^-mu, on
+    optically thick steps.  Every hit pixel within 1e-5 of the FP64 oracle."""
+    sw = models.abc_sweep_models(n_models=1, sides=(64,), freqs_ghz=(8.1,))
+    j, im, p, _ = sw[0]
+    im.los_angle = 0.5 * j.geo_param
+    q = models.with_precision(p, abi.PREC_FP64, rel_tol=1e-8)
+    q.stokes_abs_tol = 1e-20
+    pix = np.sort(np.random.default_rng(0).choice(im.n_across * im.n_along, 160, replace=False))
+    o = oracle.observe(j, im, q, pix, math=oracle.MATH_JSX)
+    m = LH.observe(j, im, p, pix, defer_rhs=512, method=3, w_piece=w_piece)
+    hit = (o["flags"] & abi.PIX_HIT) != 0
+    assert hit.sum() > 100 and m["n_pieces"].sum() > 100
+    assert np.array_equal(hit, (m["flags"] & abi.PIX_HIT) != 0)
+    rel = np.abs(m["I"][hit] - o["I"][hit]) / o["I"][hit]
+    assert rel.max() <= 1e-5, rel.max()
