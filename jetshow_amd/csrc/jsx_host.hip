@@ -621,23 +621,32 @@ static DOut lower_out(const jsx_outputs &o) {
 }
 
 /* stage launch constants into a device slot on stream st */
-static int stage_bytes(jsx_ctx *ctx, const void *src, size_t n, hipStream_t st, void **dev) {
+/* Launch constants go through a ring of device slots.  A slot is reused only
+ * after the kernel that read it has finished: the caller records slot_ev[k]
+ * on its stream after that launch (stage_done), so a later launch on any
+ * stream waits for the earlier reader, not only for the earlier copy. */
+static int stage_bytes(jsx_ctx *ctx, const void *src, size_t n, hipStream_t st, void **dev, int *slot) {
     int k = ctx->slot_next;
     ctx->slot_next = (k + 1) % jsx_ctx::kSlots;
     HIPCHECK(ctx, hipEventSynchronize(ctx->slot_ev[k]));
     DParamsM *h = ctx->h_params + k;
     std::memcpy((void *)h, src, n);
     HIPCHECK(ctx, hipMemcpyAsync(ctx->d_params + k, h, n, hipMemcpyHostToDevice, st));
-    HIPCHECK(ctx, hipEventRecord(ctx->slot_ev[k], st));
     *dev = (void *)(ctx->d_params + k);
+    *slot = k;
+    return JSX_OK;
+}
+
+static int stage_done(jsx_ctx *ctx, int slot, hipStream_t st) {
+    HIPCHECK(ctx, hipEventRecord(ctx->slot_ev[slot], st));
     return JSX_OK;
 }
 
 static int stage_params(jsx_ctx *ctx, const DModel &m, const DImage &D, const DSolver &S, hipStream_t st,
-                        DParams **dev) {
+                        DParams **dev, int *slot) {
     DParams h;
     h.m = m; h.im = D; h.S = S;
-    return stage_bytes(ctx, &h, sizeof h, st, (void **)dev);
+    return stage_bytes(ctx, &h, sizeof h, st, (void **)dev, slot);
 }
 
 #ifndef JSX_MIXED_TAU_FLOOR
@@ -742,9 +751,22 @@ static void split_target(jsx_ctx *ctx) {
 }
 
 /* one allocation for the split path's tables (kept for the context's life) */
+#define JSX_SPLIT_RAY_CAP 65536
+#define JSX_SPLIT_PIECE_CAP (1 << 20)
+
+/* bytes of the split workspace (allocated once per context, on the first
+ * mixed full-Stokes batched launch) */
+static size_t split_workspace_bytes() {
+    const size_t cap = JSX_SPLIT_RAY_CAP, piece_cap = JSX_SPLIT_PIECE_CAP;
+    auto r = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    return r(8 * sizeof(unsigned) + 2 * sizeof(unsigned long long)) + r(sizeof(LongRay) * cap) +
+           r(sizeof(RaySum) * cap) + r(sizeof(float) * cap * JSX_SPLIT_NSAMP) + r(sizeof(Piece) * piece_cap) +
+           2 * r(sizeof(PieceOut) * piece_cap) + 2 * r(sizeof(int) * piece_cap);
+}
+
 static int split_workspace(jsx_ctx *ctx) {
     if (ctx->split_mem) return JSX_OK;
-    const int cap = 65536, piece_cap = 1 << 20;
+    const int cap = JSX_SPLIT_RAY_CAP, piece_cap = JSX_SPLIT_PIECE_CAP;
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
     size_t o_cnt = take(8 * sizeof(unsigned) + 2 * sizeof(unsigned long long));
@@ -1037,7 +1059,8 @@ static int observe_locked(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_image
             return launch_batch(ctx, st, JSX_PREC_MIXED, params->stokes, one, (params->flags & 0x200u) ? 1 : 0);
         }
         DParamsM *dpm = nullptr;
-        if ((rc = stage_bytes(ctx, &hm, sizeof hm, st, (void **)&dpm))) return rc;
+        int slot = 0;
+        if ((rc = stage_bytes(ctx, &hm, sizeof hm, st, (void **)&dpm, &slot))) return rc;
         {
             int threads = 256;
             int64_t blocks = (npix + threads - 1) / threads;
@@ -1047,7 +1070,7 @@ static int observe_locked(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_image
                 hipLaunchKernelGGL(jsx_km_plain<1>, dim3((unsigned)blocks), dim3(threads), 0, st, dpm, O, npix);
         }
         HIPCHECK(ctx, hipGetLastError());
-        return JSX_OK;
+        return stage_done(ctx, slot, st);
     }
     if (params->flags & JSX_OPT_PERSISTENT) {
         std::vector<BatchItem> one(1);
@@ -1056,7 +1079,8 @@ static int observe_locked(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_image
         return launch_batch(ctx, st, JSX_PREC_FP64, params->stokes, one, (params->flags & 0x200u) ? 1 : 0);
     }
     DParams *dp = nullptr;
-    if ((rc = stage_params(ctx, m, D, S, st, &dp))) return rc;
+    int slot = 0;
+    if ((rc = stage_params(ctx, m, D, S, st, &dp, &slot))) return rc;
     {
         int threads = 256;
         int64_t blocks = (npix + threads - 1) / threads;
@@ -1066,7 +1090,7 @@ static int observe_locked(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_image
             hipLaunchKernelGGL(jsx_k_plain<1>, dim3((unsigned)blocks), dim3(threads), 0, st, dp, O, npix);
     }
     HIPCHECK(ctx, hipGetLastError());
-    return JSX_OK;
+    return stage_done(ctx, slot, st);
 }
 
 extern "C" int jsx_observe(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_image_desc *image,
@@ -1182,12 +1206,17 @@ extern "C" int jsx_tau_stripe(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_i
     if ((rc = lower_image(ctx, *image, D, nullptr))) return rc;
     const int64_t n_out = image->n_along - image->n_along / 2;
     DParams *dp = nullptr;
-    if ((rc = stage_params(ctx, m, D, S, ctx->stream, &dp))) return rc;
+    int slot = 0;
+    if ((rc = stage_params(ctx, m, D, S, ctx->stream, &dp, &slot))) return rc;
     double *d_tau = nullptr;
     HIPCHECK(ctx, hipMalloc(&d_tau, sizeof(double) * (size_t)n_out));
     hipLaunchKernelGGL(jsx_k_tau_stripe, dim3((unsigned)((n_out + 255) / 256)), dim3(256), 0, ctx->stream, dp, d_tau,
                        n_out);
     hipError_t e = hipGetLastError();
+    if (e == hipSuccess && (rc = stage_done(ctx, slot, ctx->stream))) {
+        (void)hipFree(d_tau);
+        return rc;
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e == hipSuccess) e = hipMemcpy(tau_host, d_tau, sizeof(double) * (size_t)n_out, hipMemcpyDeviceToHost);
     (void)hipFree(d_tau);
@@ -1428,13 +1457,21 @@ static int batch_group(jsx_ctx *ctx, const jsx_request *reqs, int32_t i0, int32_
 
 extern "C" int jsx_observe_batch(jsx_ctx *ctx, const jsx_request *reqs, int32_t n_reqs) {
     if (!ctx || (!reqs && n_reqs) || n_reqs < 0) return JSX_EINVAL;
-    const size_t per_pix = 6 * sizeof(double) + 3 * sizeof(uint32_t);
+    /* device bytes per pixel of a group: the nine output planes, plus the
+       triage pass's list, sorted list and two cost buckets (8 + 8 + 1 + 1) */
+    const size_t per_pix = 6 * sizeof(double) + 3 * sizeof(uint32_t) + 18;
+    size_t avail;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        const size_t held = ctx->used + split_workspace_bytes();   /* grids + the split workspace */
+        avail = ctx->budget > held ? ctx->budget - held : 0;
+    }
     for (int32_t i = 0; i < n_reqs; ++i) {
         const jsx_image_desc &im = reqs[i].image;
         if (im.n_across <= 0 || im.n_along <= 0 || (im.n_across % 2))
             return fail(ctx, JSX_EINVAL, "request %d: image needs n_along > 0 and even n_across > 0", i);
         if (!reqs[i].out_host.I) return fail(ctx, JSX_EINVAL, "request %d: out_host.I must be set", i);
-        if ((size_t)(im.n_across * im.n_along) * per_pix > ctx->budget - ctx->used)
+        if ((size_t)(im.n_across * im.n_along) * per_pix > avail)
             return fail(ctx, JSX_EBUDGET, "request %d does not fit the HBM budget", i);
     }
     int32_t i0 = 0;
@@ -1444,7 +1481,7 @@ extern "C" int jsx_observe_batch(jsx_ctx *ctx, const jsx_request *reqs, int32_t 
         while (i1 < n_reqs && reqs[i1].params.precision == reqs[i0].params.precision &&
                reqs[i1].params.stokes == reqs[i0].params.stokes) {
             size_t b = (size_t)(reqs[i1].image.n_across * reqs[i1].image.n_along) * per_pix;
-            if (i1 > i0 && bytes + b > ctx->budget - ctx->used) break;
+            if (i1 > i0 && bytes + b > avail) break;
             bytes += b;
             ++i1;
         }

@@ -80,40 +80,70 @@ def _time(ctx, items, reps):
 
 
 def sweep(ctx, items, space=None, reps=2, budget_s=120.0, log=None):
-    """Time every point of `space` (dict of option -> values) on `items`;
-    returns (best_options, table).  Stops early when budget_s runs out."""
+    """Coordinate descent over `space` (dict of option -> values) on `items`,
+    from the context's current settings: each option in turn is set to each of
+    its values with the others at the best found so far, and the fastest value
+    kept; passes repeat until one changes nothing.  A truncated search still
+    visits every option near the defaults first (a full product visited in
+    sorted order would spend its budget on the first option's corner).
+    Returns (best_options, table, complete): complete is False when budget_s
+    ran out before a pass without change."""
     space = dict(DEFAULT_SPACE if space is None else space)
     keys = sorted(space)
     saved = {k: ctx.get_option(k) for k in keys}
     table = []
+    seen = {}
     t_start = time.perf_counter()
+
+    def run(opts):
+        key = tuple(sorted(opts.items()))
+        if key in seen:
+            return seen[key]
+        for k, v in opts.items():
+            ctx.set_option(k, v)
+        t = _time(ctx, items, reps)
+        seen[key] = t
+        table.append((t, dict(opts)))
+        if log:
+            log("%.4f s  %s" % (t, opts))
+        return t
+
+    complete = False
     try:
-        for combo in itertools.product(*(space[k] for k in keys)):
-            if time.perf_counter() - t_start > budget_s:
+        best = {k: saved[k] for k in keys}
+        best_t = run(best)
+        while time.perf_counter() - t_start <= budget_s:
+            changed = False
+            for k in keys:
+                for v in space[k]:
+                    if time.perf_counter() - t_start > budget_s:
+                        break
+                    trial = dict(best)
+                    trial[k] = v
+                    t = run(trial)
+                    if t < best_t:
+                        best, best_t, changed = trial, t, True
+            if not changed and time.perf_counter() - t_start <= budget_s:
+                complete = True
                 break
-            opts = dict(zip(keys, combo))
-            for k, v in opts.items():
-                ctx.set_option(k, v)
-            t = _time(ctx, items, reps)
-            table.append((t, opts))
-            if log:
-                log("%.4f s  %s" % (t, opts))
     finally:
         for k, v in saved.items():
             ctx.set_option(k, v)
     table.sort(key=lambda r: r[0])
-    return (table[0][1] if table else {}), table
+    return (table[0][1] if table else {}), table, complete
 
 
 def tune(ctx, items, space=None, reps=2, budget_s=120.0, cache=CACHE_PATH, force=False, log=None):
-    """Apply the cached best shape for this device and workload, sweeping
-    first when there is none (or force=True).  Returns the options applied."""
+    """Apply the cached best shape for this device and workload, searching
+    first when there is none, when the cached search was cut short by its
+    budget, or when force=True.  Returns the options applied."""
     key = "%s|%s" % (_device_name(), signature(items))
     data = _load(cache) if cache else {}
-    if force or key not in data:
-        best, table = sweep(ctx, items, space, reps, budget_s, log)
-        data[key] = {"options": best, "seconds": table[0][0] if table else None,
-                     "tried": len(table), "when": time.strftime("%Y-%m-%d %H:%M:%S")}
+    entry = data.get(key)
+    if force or entry is None or not entry.get("complete", False):
+        best, table, complete = sweep(ctx, items, space, reps, budget_s, log)
+        data[key] = {"options": best, "seconds": table[0][0] if table else None, "tried": len(table),
+                     "complete": complete, "when": time.strftime("%Y-%m-%d %H:%M:%S")}
         if cache:
             _save(cache, data)
     best = data[key]["options"]

@@ -262,9 +262,13 @@ def test_triage_list_same_planes(ctx):
 
 def test_server_same_alone_and_in_mix(ctx):
     """Continuous batching: a request gives the same planes alone and when
-    submitted among others of different sizes and frequencies (the server's
-    kernel never splits rays, so equality is exact); every request of the mix
-    completes and reports its latency."""
+    submitted among others of different sizes and frequencies -- its rays run
+    in a batched launch with the other requests', and a long full-Stokes ray
+    may be split into pieces placed by the launch's other long rays, so the
+    two agree to the solver's tolerance (I within 1e-5 relative, Q/U/V within
+    5e-5 of peak I, identical flags for the hit mask), and both agree with the
+    FP64 image to the same bar; every request of the mix completes and reports
+    its latency."""
     from jetshow_amd.host import Server
     j0, im0, p0 = models.paraboloid_helical_model(n_side=32, pixel_pc=0.16)
     p0 = models.with_precision(p0, abi.PREC_MIXED, rel_tol=1e-6)
@@ -272,7 +276,7 @@ def test_server_same_alone_and_in_mix(ctx):
     for k, (n, px, nu) in enumerate([(48, 0.1, 8.1), (16, 0.3, 15.4), (64, 0.05, 22.0), (24, 0.2, 5.0)]):
         j, im, p = models.paraboloid_helical_model(n_side=n, pixel_pc=px, nu_ghz=nu)
         others.append((j, im, models.with_precision(p, abi.PREC_MIXED, rel_tol=1e-6)))
-    with Server(ctx, abi.PREC_MIXED, abi.STOKES_FULL, max_active=3, epoch_budget=2000) as srv:
+    with Server(ctx, abi.PREC_MIXED, abi.STOKES_FULL, max_active=3) as srv:
         alone, _ = srv.wait(srv.submit(j0, im0, p0))
         ids = [srv.submit(*others[0]), srv.submit(*others[1])]
         mid = srv.submit(j0, im0, p0)
@@ -280,13 +284,15 @@ def test_server_same_alone_and_in_mix(ctx):
         mixed, t = srv.wait(mid)
         lat = [srv.wait(i)[1]["latency_s"] for i in ids]
         st = srv.stats()
-    print("server: %d epochs, latencies %s s" % (st["epochs"], ["%.3f" % x for x in lat]), flush=True)
-    for k in ("tau", "I", "Q", "U", "V", "l", "flags", "n_rhs", "n_steps"):
-        assert np.array_equal(alone[k], mixed[k]), k
-    ref = ctx.observe(j0, im0, models.copy_struct(p0))
-    rel, quv, _ = _mixed_errors(ref, mixed)
+    print("server: %d launches, latencies %s s" % (st["epochs"], ["%.3f" % x for x in lat]), flush=True)
+    rel, quv, _ = _mixed_errors(alone, mixed)
     assert rel <= 1e-5 and quv <= 5e-5
+    ref = ctx.observe(j0, im0, models.copy_struct(models.with_precision(p0, abi.PREC_FP64, rel_tol=1e-8)))
+    for got in (alone, mixed):
+        rel, quv, _ = _mixed_errors(ref, got)
+        assert rel <= 1e-5 and quv <= 5e-5
     assert st["epochs"] >= 2
+    assert all(x > 0 for x in lat)
 
 
 def test_split_piece_layouts_agree(ctx):

@@ -39,7 +39,9 @@ def _items():
 
 def test_sweep_picks_fastest_and_restores():
     ctx = FakeCtx()
-    best, table = tune.sweep(ctx, _items(), {"blocks_per_cu": [1, 2, 4], "queue_block": [64, 256]}, reps=2)
+    best, table, complete = tune.sweep(ctx, _items(), {"blocks_per_cu": [1, 2, 4], "queue_block": [64, 256]},
+                                       reps=2)
+    assert complete
     assert best == {"blocks_per_cu": 2, "queue_block": 256}
     assert len(table) == 6 and table[0][1] == best
     # options are restored after the sweep
@@ -61,6 +63,27 @@ def test_tune_caches_and_applies(tmp_path, monkeypatch):
     ctx2 = FakeCtx()
     assert tune.apply_cached(ctx2, _items(), cache=cache) == best
     assert ctx2.calls == 0 and ctx2.opts["blocks_per_cu"] == 2
+
+
+def test_truncated_search_is_not_trusted(tmp_path, monkeypatch):
+    """A search cut short by its budget starts from the current settings
+    (every option near the defaults first), reports itself incomplete, and
+    tune() searches again next time instead of reusing it."""
+    monkeypatch.setattr(tune, "_device_name", lambda: "test-device")
+    cache = str(tmp_path / "tune.json")
+    ctx = FakeCtx()
+    best, table, complete = tune.sweep(ctx, _items(), {"blocks_per_cu": [1, 2, 4], "queue_block": [64, 256]},
+                                       reps=1, budget_s=0.0)
+    assert not complete
+    assert table[-1][1] == {"blocks_per_cu": 4, "queue_block": 64}   # the defaults come first
+    space = {"blocks_per_cu": [2, 4], "queue_block": [64, 256]}
+    tune.tune(ctx, _items(), space, reps=1, cache=cache, budget_s=0.0)
+    key = "test-device|" + tune.signature(_items())
+    assert json.load(open(cache))[key]["complete"] is False
+    calls = ctx.calls
+    best = tune.tune(ctx, _items(), space, reps=1, cache=cache)
+    assert ctx.calls > calls and best == {"blocks_per_cu": 2, "queue_block": 256}
+    assert json.load(open(cache))[key]["complete"] is True
 
 
 def test_signature_ignores_values_but_not_shapes():
