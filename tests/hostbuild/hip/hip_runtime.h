@@ -91,6 +91,11 @@ static inline hipError_t hipEventCreateWithFlags(hipEvent_t *e, int) { *e = null
 static inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
 static inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
 static inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+/* what hipEventQuery reports (set by the host tests to force error paths) */
+#define hipErrorNotReady 600
+#define hipErrorUnknown 999
+static int jsx_hb_event_status = 0;
+static inline hipError_t hipEventQuery(hipEvent_t) { return jsx_hb_event_status; }
 static inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
 static inline hipError_t hipEventCreate(hipEvent_t *e) { *e = nullptr; return hipSuccess; }
 static inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) { *ms = 0.f; return hipSuccess; }

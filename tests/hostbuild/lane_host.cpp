@@ -445,3 +445,39 @@ extern "C" void lh_vis_mixed(const double *x, const double *y, const double *cre
         vis[2 * k + 1] = sgn * B;
     }
 }
+
+/* The request server's scheduler with the stand-in runtime (kernels do not
+ * run; events report jsx_hb_event_status): one request completes normally,
+ * then the launch's completion event reports a HIP error.  rc[0] = status of
+ * the first request, rc[1] = status of the request in the failing launch,
+ * rc[2] = a request submitted afterwards (must fail at once), rc[3] = a
+ * request queued behind the failure's launch. */
+extern "C" int lh_server_error_path(const jsx_jet_desc *jet, const jsx_image_desc *im, const jsx_solver_params *p,
+                                    int32_t *rc) {
+    jsx_ctx *ctx = nullptr;
+    int r = jsx_ctx_create(0, 0, &ctx);
+    if (r) return r;
+    jsx_server *S = nullptr;
+    if ((r = jsx_server_create(ctx, p->precision, p->stokes, 4, 0, 0, &S))) return r;
+    const int64_t n = im->n_across * im->n_along;
+    std::vector<double> I(n);
+    jsx_request q{};
+    q.jet = *jet;
+    q.image = *im;
+    q.params = *p;
+    q.out_host.I = I.data();
+    uint64_t id = 0;
+    jsx_hb_event_status = hipSuccess;
+    jsx_server_submit(S, &q, 0, nullptr, &id);
+    rc[0] = jsx_server_wait(S, id, nullptr);
+    jsx_hb_event_status = hipErrorUnknown;
+    uint64_t id1 = 0, id2 = 0;
+    jsx_server_submit(S, &q, 0, nullptr, &id1);
+    rc[1] = jsx_server_wait(S, id1, nullptr);
+    rc[2] = jsx_server_submit(S, &q, 0, nullptr, &id2);
+    rc[3] = rc[2] == JSX_OK ? jsx_server_wait(S, id2, nullptr) : rc[2];
+    jsx_hb_event_status = hipSuccess;
+    jsx_server_destroy(S);
+    jsx_ctx_destroy(ctx);
+    return JSX_OK;
+}

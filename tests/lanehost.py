@@ -144,3 +144,12 @@ def vis_mixed(x, y, c, u, v, isign=-1):
     lib().lh_vis_mixed(_dp(a[0]), _dp(a[1]), _dp(a[2]), C.c_int64(a[0].size), _dp(a[3]), _dp(a[4]),
                        C.c_int64(a[3].size), C.c_int32(isign), _dp(out))
     return out[0::2] + 1j * out[1::2]
+
+
+def server_error_path(jet, image, params):
+    """The server scheduler on the stand-in runtime with a forced HIP error:
+    statuses of (a normal request, a request whose launch fails, a request
+    submitted after the failure)."""
+    rc = np.zeros(4, dtype=np.int32)
+    lib().lh_server_error_path(C.byref(jet), C.byref(image), C.byref(params), rc.ctypes.data_as(C.POINTER(C.c_int32)))
+    return rc.tolist()

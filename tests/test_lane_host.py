@@ -225,3 +225,14 @@ def test_exponential_pieces_through_cone_apex(w_piece):
     assert np.array_equal(hit, (m["flags"] & abi.PIX_HIT) != 0)
     rel = np.abs(m["I"][hit] - o["I"][hit]) / o["I"][hit]
     assert rel.max() <= 1e-5, rel.max()
+
+
+def test_server_fails_fast_after_hip_error():
+    """A launch whose completion reports a HIP error fails its requests with
+    JSX_EHIP, and later submissions are refused at once instead of queueing
+    to a scheduler that has stopped (they would wait forever)."""
+    j, im, p = models.paraboloid_helical_model(n_side=4, pixel_pc=0.3)
+    p = models.with_precision(p, abi.PREC_MIXED)
+    ok, failed, after, _ = LH.server_error_path(j, im, p)
+    assert ok == abi.JSX_OK
+    assert failed == -3 and after == -3      # JSX_EHIP
