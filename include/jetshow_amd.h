@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define JSX_ABI_VERSION 1
+#define JSX_ABI_VERSION 2
 
 typedef enum jsx_status {
     JSX_OK = 0,
@@ -42,10 +42,13 @@ typedef enum jsx_geometry_kind {
     JSX_GEOM_CONE = 1,        /* src/Cone.cpp:16-118 (double-nappe cone)          */
     JSX_GEOM_PARABOLOID = 2,  /* src/Parabaloid.cpp:11-60                          */
     JSX_GEOM_CYLINDER = 3,    /* src/Cylinder.cpp:9-99 (r^2 fixed; see flags)      */
-    JSX_GEOM_CONVEX_HULL = 4  /* SimulationGeometry (src/Geometry.cpp:92-132) over
+    JSX_GEOM_CONVEX_HULL = 4, /* SimulationGeometry (src/Geometry.cpp:92-132) over
                                  the convex hull of the simulation's points
                                  (src/main.cpp:873-897), as face planes uploaded
                                  with jsx_hull_upload (handle in geo_table)      */
+    JSX_GEOM_PARABOLOID_CONE = 5 /* ParabaloidCone (src/ParabaloidCone.cpp:9-94):
+                                 paraboloid r0 = geo_param below z = geo_z0, the
+                                 cone tangent to it there above                   */
 } jsx_geometry_kind;
 
 typedef enum jsx_bfield_kind {
@@ -59,14 +62,26 @@ typedef enum jsx_bfield_kind {
     JSX_B_GRID_SCALAR = 6,     /* tangled |B| interpolated from grid b         (ext.)  */
     JSX_B_GRID_VECTOR = 7,     /* SimulationBField (-sin fi Bphi, cos fi Bphi, Bp),
                                   Bp from grid b, Bphi from grid b2    BField.cpp:196-207 */
-    JSX_B_CONST_CYLINDER_Z = 8 /* ConstCylinderBFieldZ (0, 0, b0 (|z|/pc)^-e) BField.cpp:76-82 */
+    JSX_B_CONST_CYLINDER_Z = 8, /* ConstCylinderBFieldZ (0, 0, b0 (|z|/pc)^-e) BField.cpp:76-82 */
+    JSX_B_CONST_CYLINDER = 9,  /* ConstCylinderBField (0, 0, b0 (|p|/pc)^-e)   BField.cpp:66-73 */
+    JSX_B_RADIAL_CONICAL = 10, /* RadialConicalBField b0 (|p|/pc)^-e p/|p|     BField.cpp:85-94 */
+    JSX_B_HELICAL_CYLINDER = 11, /* HelicalCylinderBField (b0 tan(pitch y/r),
+                                  -b0 tan(pitch x/r), b0), r = hypot(x, y)     BField.cpp:109-117 */
+    JSX_B_SPIRAL_CONICAL = 12, /* SpiralConicalBField, b_z = b0 / (z/pc)^2     BField.cpp:120-131 */
+    JSX_B_FORCE_FREE_CYLINDRICAL = 13, /* ForceFreeCylindricalBField, I0/I1 of
+                                  b_mu (both transverse components use sin fi,
+                                  as the reference writes them)               BField.cpp:134-146 */
+    JSX_B_COMPOSITE_RANDOM_SCALAR_Z = 14 /* CompositeRandomScalarBFieldZ: b0 (|z|/pc)^-b_exp
+                                  for |z| <= b_z0, ^-b_exp2 beyond             BField.cpp:53-63 */
 } jsx_bfield_kind;
 
 typedef enum jsx_nfield_kind {
     JSX_N_CONST = 1,           /* ConstNField            NField.cpp:17-21 */
     JSX_N_BK = 2,              /* BKNField  n0 (|p|/pc)^-e NField.cpp:24-30 */
     JSX_N_BK_Z = 3,            /* BKNFieldZ n0 (|z|/pc)^-e NField.cpp:33-39 */
-    JSX_N_GRID = 4             /* SimulationNField       NField.cpp:56-59 */
+    JSX_N_GRID = 4,            /* SimulationNField       NField.cpp:56-59 */
+    JSX_N_COMPOSITE_BK = 5     /* CompositeBKNField: BKNField(n0, n_exp) for |z| <= n_z0,
+                                  BKNField(n0, n_exp2) beyond NField.cpp:42-53 */
 } jsx_nfield_kind;
 
 typedef enum jsx_vfield_kind {
@@ -74,7 +89,23 @@ typedef enum jsx_vfield_kind {
     JSX_V_CONST_CENTRAL = 2,   /* ConstCentralVField     VField.cpp:61-72 */
     JSX_V_LINEAR_GAMMA_Z = 3,  /* Gamma linear in z between (z0, g0) and (z1, g1),
                                   clamped outside, v along +z            (ext.) */
-    JSX_V_GRID = 4             /* SimulationVField       VField.cpp:269-274 */
+    JSX_V_GRID = 4,            /* SimulationVField       VField.cpp:269-274 */
+    /* Lorentz factors: gamma0 on the axis / spine, gamma1 at the border / sheath */
+    JSX_V_SHEARED_FLAT = 5,    /* ShearedFlatVField (gamma0, gamma1, v_r)      VField.cpp:32-42 */
+    JSX_V_SHEATH_FLAT = 6,     /* SheathFlatVField (gamma0, gamma1, v_r)       VField.cpp:45-59 */
+    JSX_V_SHEARED_CENTRAL = 7, /* ShearedCentralVField (gamma0, gamma1, v_theta, v_origin)
+                                  VField.cpp:75-86; theta = acos((z + o_z)/r) unless
+                                  JSX_OPT_SHEARED_CENTRAL_REF_BUG              */
+    JSX_V_SHEATH_CENTRAL = 8,  /* SheathCentralVField (gamma0, gamma1, v_theta, v_origin)
+                                  VField.cpp:89-106                            */
+    JSX_V_CONST_PARABOLIC = 9, /* ConstParabolicVField (gamma0, v_Rz0)        VField.cpp:109-129 */
+    JSX_V_ACC_PARABOLIC = 10,  /* AccParabolicVField (gamma0, v_R0, v_Rz0)     VField.cpp:132-157 */
+    JSX_V_SHEARED_ACC_PARABOLIC = 11, /* ShearedAccParabolicVField (gamma0, gamma1,
+                                  v_R0, v_Rz0)                                 VField.cpp:164-204 */
+    JSX_V_ACC_PARABOLIC_CONST_CONE = 12, /* AccParabolicConstConeVField (gamma0,
+                                  v_Rz0, v_z0)                                 VField.cpp:222-240 */
+    JSX_V_SHEARED_ACC_PARABOLIC_CONST_CONE = 13 /* ShearedAccParabolicConstConeVField
+                                  (gamma0, gamma1, v_Rz0, v_z0)                VField.cpp:248-268 */
 } jsx_vfield_kind;
 
 typedef struct jsx_jet_desc {
@@ -105,6 +136,15 @@ typedef struct jsx_jet_desc {
     double s;                     /* electron energy index (reference hard-codes 3.42,
                                      src/Jet.cpp:32,234)                           */
     double gamma_min;             /* k_F_r / k_C_r low-energy cutoff (utils.h:86)  */
+    /* ABI 2: parameters of the further field and geometry classes */
+    double geo_z0;                /* [cm] JSX_GEOM_PARABOLOID_CONE switch height   */
+    double b_exp2, b_z0;          /* composite B: outer exponent, |z| switch [cm]  */
+    double b_mu;                  /* ForceFreeCylindricalBField mu                 */
+    double n_exp2, n_z0;          /* CompositeBKNField: outer exponent, |z| switch [cm] */
+    double v_r;                   /* [cm] sheared / sheath flat V: border radius   */
+    double v_theta;               /* [rad] sheared / sheath central V: angle       */
+    double v_R0, v_Rz0, v_z0;     /* [cm] parabolic V fields: R0, Rz0 (paraboloid
+                                     radius at z = 1 pc), z0 (cone above it)        */
 } jsx_jet_desc;
 
 /* ---- image plane: ImagePlane(pair<ulong,ulong> size, lg_start, lg_stop, los)
@@ -124,6 +164,9 @@ typedef enum jsx_precision { JSX_PREC_FP64 = 0, JSX_PREC_MIXED = 1 } jsx_precisi
 
 /* option bits for jsx_solver_params.flags */
 #define JSX_OPT_CYLINDER_REF_BUG 0x1u  /* c0 = |w|^2 - r/pc as Cylinder.cpp:53 */
+#define JSX_OPT_SHEARED_CENTRAL_REF_BUG 0x2u /* theta = acos(z + o_z / r) as
+                                                VField.cpp:81 writes it (NaN for
+                                                |z| > 1 cm)                    */
 
 typedef struct jsx_solver_params {
     int32_t stokes;               /* jsx_stokes                                   */

@@ -12,14 +12,21 @@ STATUS_NAMES = {
     -7: "JSX_ENOTFOUND", -8: "JSX_ENODEV",
 }
 
-GEOM_CONE, GEOM_PARABOLOID, GEOM_CYLINDER, GEOM_CONVEX_HULL = 1, 2, 3, 4
+ABI_VERSION = 2
+GEOM_CONE, GEOM_PARABOLOID, GEOM_CYLINDER, GEOM_CONVEX_HULL, GEOM_PARABOLOID_CONE = 1, 2, 3, 4, 5
 B_RANDOM_SCALAR, B_RANDOM_SCALAR_Z, B_TOROIDAL, B_TOROIDAL_R, B_HELICAL_R, \
     B_GRID_SCALAR, B_GRID_VECTOR, B_CONST_CYLINDER_Z = 1, 2, 3, 4, 5, 6, 7, 8
-N_CONST, N_BK, N_BK_Z, N_GRID = 1, 2, 3, 4
+B_CONST_CYLINDER, B_RADIAL_CONICAL, B_HELICAL_CYLINDER, B_SPIRAL_CONICAL, \
+    B_FORCE_FREE_CYLINDRICAL, B_COMPOSITE_RANDOM_SCALAR_Z = 9, 10, 11, 12, 13, 14
+N_CONST, N_BK, N_BK_Z, N_GRID, N_COMPOSITE_BK = 1, 2, 3, 4, 5
 V_CONST_FLAT, V_CONST_CENTRAL, V_LINEAR_GAMMA_Z, V_GRID = 1, 2, 3, 4
+V_SHEARED_FLAT, V_SHEATH_FLAT, V_SHEARED_CENTRAL, V_SHEATH_CENTRAL, V_CONST_PARABOLIC, \
+    V_ACC_PARABOLIC, V_SHEARED_ACC_PARABOLIC, V_ACC_PARABOLIC_CONST_CONE, \
+    V_SHEARED_ACC_PARABOLIC_CONST_CONE = 5, 6, 7, 8, 9, 10, 11, 12, 13
 STOKES_I, STOKES_FULL = 1, 4
 PREC_FP64, PREC_MIXED = 0, 1
 OPT_CYLINDER_REF_BUG = 0x1
+OPT_SHEARED_CENTRAL_REF_BUG = 0x2
 
 PIX_HIT = 0x01
 PIX_NSEG2 = 0x02
@@ -50,6 +57,9 @@ class JetDesc(C.Structure):
         ("gamma0", C.c_double), ("gamma1", C.c_double), ("gamma_z0", C.c_double), ("gamma_z1", C.c_double),
         ("v_origin", C.c_double * 3),
         ("s", C.c_double), ("gamma_min", C.c_double),
+        ("geo_z0", C.c_double), ("b_exp2", C.c_double), ("b_z0", C.c_double), ("b_mu", C.c_double),
+        ("n_exp2", C.c_double), ("n_z0", C.c_double), ("v_r", C.c_double), ("v_theta", C.c_double),
+        ("v_R0", C.c_double), ("v_Rz0", C.c_double), ("v_z0", C.c_double),
     ]
 
 

@@ -89,7 +89,7 @@ struct PolFP64 {
         return (float)(fabs(c.kI) + fabs(c.kF) + fabs(c.kC) + fabs(c.hQ) + pol);
     }
     /* FP64 rays are never split: results stay identical to the sequential oracle */
-    __device__ static bool park(const LaneT &, int, const SplitWS &, unsigned) { return false; }
+    __device__ static bool park(const LaneT &, const Params &, int, const SplitWS &, unsigned) { return false; }
 };
 
 /* policy: mixed-precision machine */
@@ -131,9 +131,9 @@ struct PolMixed {
         return split_stiffness(k) * (float)q.c.inv_pc;   /* per pc -> per cm */
     }
     /* full-Stokes rays past the budget go to the time-parallel path */
-    __device__ static bool park(const LaneT &L, int slot, const SplitWS &ws, unsigned defer_rhs) {
+    __device__ static bool park(const LaneT &L, const Params &q, int slot, const SplitWS &ws, unsigned defer_rhs) {
         if constexpr (NS == 4) {
-            if (ws.rays && split_should_park(L, defer_rhs)) return split_park(L, slot, ws);
+            if (ws.rays && q.c.split_ok && split_should_park(L, defer_rhs)) return split_park(L, slot, ws);
         }
         return false;
     }
@@ -390,7 +390,7 @@ __global__ __launch_bounds__(256) void jsx_k_batch(const typename P::Params *__r
                 rhs_acc += P::rhs(L);
                 have = 0;
             } else if ((early || may_park || (drained && held <= park_wave)) &&
-                       P::park(L, req, ws, early ? 0u : defer_rhs)) {
+                       P::park(L, reqs[req], req, ws, early ? 0u : defer_rhs)) {
                 rhs_acc += P::rhs(L);
                 have = 0;
             }

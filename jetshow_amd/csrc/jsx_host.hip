@@ -721,6 +721,12 @@ static void lower_mixed(const jsx_jet_desc &j, const jsx_solver_params &p, const
     c.st_abs = p.stokes_abs_tol;
     c.st_rel = p.stokes_rel_tol > fs ? p.stokes_rel_tol : fs;
     c.st_rel_x = p.stokes_rel_tol > JSX_MIXED_STATE_FLOOR ? p.stokes_rel_tol : JSX_MIXED_STATE_FLOOR;
+    /* the pieces' error control is floored at the FP32 tolerance floor: a
+       request asking for a tighter Stokes tolerance (grid fields, whose
+       kinks the error estimate undershoots, run at 1e-9) is integrated by
+       the sequential lane, whose norm holds the state to the caller's
+       tolerance */
+    c.split_ok = p.stokes_rel_tol >= 0.999 * JSX_MIXED_STOKES_FLOOR_FULL ? 1 : 0;
     c.dt_max_pc = p.dt_max / PC;
 }
 

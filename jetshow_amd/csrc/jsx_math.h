@@ -26,6 +26,17 @@
  *   sin(atan(u)), cos(atan(u)) : the algebraic identities u/sqrt(1+u^2),
  *         1/sqrt(1+u^2) (scaled for |u| > 1), which is what the reference's
  *         ToroidalBField / SimulationBField compute through atan/sin/cos.
+ *   sin, cos, tan : x = k pi/2 + r with pi/2 in three parts (Cody & Waite),
+ *         |r| <= pi/4, Taylor polynomials of sin and cos (to x^27 / x^26),
+ *         quadrant from k; for |x| < 2^20 (the field models call it with
+ *         |x| <= a pitch angle).
+ *   acos : asin's Maclaurin series sum (2k)! / (4^k k!^2 (2k+1)) y^(2k+1) on
+ *         |y| <= 1/2, with acos(u) = pi/2 - asin(u) for |u| <= 1/2 and the
+ *         half-angle forms 2 asin(sqrt((1-u)/2)), pi - 2 asin(sqrt((1+u)/2))
+ *         beyond; NaN outside [-1, 1].
+ *   I0, I1 (modified Bessel functions, parameter constants only) : their
+ *         power series sum (x/2)^(2k+n) / (k! (k+n)!) to the last term that
+ *         changes the sum.
  */
 #ifndef JSX_MATH_H
 #define JSX_MATH_H
@@ -194,6 +205,80 @@ JSX_HD void jsx_sincos_atan(double u, double *s, double *c) {
         *s = (u < 0.0 ? -1.0 : 1.0) / q;
         *c = v / q;
     }
+}
+
+/* sin and cos for |x| < 2^20 (see header) */
+JSX_HD void jsx_sincos(double x, double *s, double *c) {
+    if (x != x) { *s = x; *c = x; return; }
+    const double inv_hpi = 0.63661977236758134308;
+    const double hp1 = 1.5707963267341256e+00;   /* pi/2 = hp1 + hp2 + hp3, hp1 and hp2 with 33 bits */
+    const double hp2 = 6.0771005065061922e-11;
+    const double hp3 = 2.0222662487959506e-21;
+    double k = floor(x * inv_hpi + 0.5);
+    double r = ((x - k * hp1) - k * hp2) - k * hp3;
+    double r2 = r * r;
+    /* sin r = r (1 - r^2/3! + r^4/5! - ...), cos r = 1 - r^2/2! + ... */
+    double ps = 0.0, pc = 0.0;
+    double fs = 1.0, fc = 1.0;
+    double ts[14], tc[14];
+    for (int n = 0; n < 14; ++n) {
+        ts[n] = fs; tc[n] = fc;
+        fs = fs / ((double)(2 * n + 2) * (double)(2 * n + 3));
+        fc = fc / ((double)(2 * n + 1) * (double)(2 * n + 2));
+    }
+    for (int n = 13; n >= 0; --n) {
+        double sg = (n & 1) ? -1.0 : 1.0;
+        ps = fma(ps, r2, sg * ts[n]);
+        pc = fma(pc, r2, sg * tc[n]);
+    }
+    double sr = r * ps, cr = pc;
+    long long q = ((long long)k) & 3;
+    if (q == 0) { *s = sr; *c = cr; }
+    else if (q == 1) { *s = cr; *c = -sr; }
+    else if (q == 2) { *s = -sr; *c = -cr; }
+    else { *s = -cr; *c = sr; }
+}
+
+JSX_HD double jsx_tan(double x) {
+    double s, c;
+    jsx_sincos(x, &s, &c);
+    return s / c;
+}
+
+/* asin(y) for |y| <= 1/2 by its Maclaurin series */
+JSX_HD double jsx_asin_small(double y) {
+    double y2 = y * y, term = y, sum = 0.0;
+    for (int k = 0; k < 40; ++k) {
+        double add = term / (double)(2 * k + 1);
+        double nsum = sum + add;
+        if (nsum == sum) break;
+        sum = nsum;
+        term = term * y2 * ((double)(2 * k + 1) / (double)(2 * k + 2));
+    }
+    return sum;
+}
+
+JSX_HD double jsx_acos(double u) {
+    const double pi = 3.141592653589793238462643383279502884;
+    if (!(u >= -1.0 && u <= 1.0)) return jsx_from_bits(0x7ff8000000000000ULL);
+    if (u >= -0.5 && u <= 0.5) return pi / 2.0 - jsx_asin_small(u);
+    if (u > 0.5) return 2.0 * jsx_asin_small(sqrt((1.0 - u) / 2.0));
+    return pi - 2.0 * jsx_asin_small(sqrt((1.0 + u) / 2.0));
+}
+
+/* modified Bessel functions I0(x), I1(x) by their power series */
+JSX_HD void jsx_bessel_i01(double x, double *i0, double *i1) {
+    double h = 0.5 * x, h2 = h * h;
+    double t0 = 1.0, t1 = h, s0 = 0.0, s1 = 0.0;
+    for (int k = 0; k < 500; ++k) {
+        double n0 = s0 + t0, n1 = s1 + t1;
+        if (n0 == s0 && n1 == s1) break;
+        s0 = n0; s1 = n1;
+        t0 = t0 * h2 / ((double)(k + 1) * (double)(k + 1));
+        t1 = t1 * h2 / ((double)(k + 1) * (double)(k + 2));
+    }
+    *i0 = s0;
+    *i1 = s1;
 }
 
 #endif /* JSX_MATH_H */

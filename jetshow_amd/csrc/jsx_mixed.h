@@ -55,6 +55,7 @@ struct MConsts {
     double kC_y0, kC_y1;                  /* k_C_r's exponent y = y0 + y1 log2(x)          */
     double tau_abs, tau_rel, st_abs, st_rel, dt_max_pc;
     double st_rel_x;                      /* the caller's Stokes tolerance on |x|          */
+    int split_ok;                         /* long rays may go to the time-parallel pieces  */
 };
 
 /* x^e for x > 0 (see header) */

@@ -1459,6 +1459,10 @@ void orc_jsx_math(int32_t fn, const double *x, const double *y, double *out, int
         case 1: out[i] = jsx_log(x[i]); break;
         case 2: out[i] = jsx_pow(x[i], y[i]); break;
         case 3: { double s, c; jsx_sincos_atan(x[i], &s, &c); out[2 * i] = s; out[2 * i + 1] = c; break; }
+        case 4: { double s, c; jsx_sincos(x[i], &s, &c); out[2 * i] = s; out[2 * i + 1] = c; break; }
+        case 5: out[i] = jsx_tan(x[i]); break;
+        case 6: out[i] = jsx_acos(x[i]); break;
+        case 7: { double a, b; jsx_bessel_i01(x[i], &a, &b); out[2 * i] = a; out[2 * i + 1] = b; break; }
         default: out[i] = 0;
         }
     }

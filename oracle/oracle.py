@@ -196,6 +196,7 @@ def dopri5_check(problem, t_end=1.0, dt=0.1, abs_tol=1e-10, rel_tol=1e-10, theta
 def jsx_math(fn, x, y=None):
     x = np.ascontiguousarray(x, dtype=np.float64)
     y = np.ascontiguousarray(y if y is not None else np.zeros_like(x), dtype=np.float64)
-    out = np.zeros(len(x) * (2 if fn == 3 else 1))
+    pair = fn in (3, 4, 7)
+    out = np.zeros(len(x) * (2 if pair else 1))
     lib().orc_jsx_math(int(fn), _p(x), _p(y), _p(out), C.c_int64(len(x)))
-    return out.reshape(-1, 2) if fn == 3 else out
+    return out.reshape(-1, 2) if pair else out

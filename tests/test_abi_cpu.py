@@ -35,7 +35,7 @@ def test_struct_layout_matches_header():
     for i, t in enumerate([abi.JetDesc, abi.ImageDesc, abi.SolverParams, abi.Outputs, abi.GridDesc, abi.Request,
                             abi.GridPayload, abi.RequestTiming]):
         assert L.jsx_sizeof(i) == ctypes.sizeof(t), t
-    assert L.jsx_abi_version() == 1
+    assert L.jsx_abi_version() == abi.ABI_VERSION == 2
 
 
 def test_pixel_sizes_and_cosmology_match_oracle():
