@@ -339,6 +339,30 @@ int jsx_observe_host(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_image_desc
 int jsx_hit_host(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_solver_params *params,
                  const double *rays, int64_t n, int32_t *n_seg, double *segments);
 
+/* Field and geometry values at points: what the class API's virtuals
+ * return -- VField::vf (include/VField.h:13), VectorBField / ScalarBField::bf
+ * and bf_plasma_frame (include/BField.h:38-39,51-52), NField::nf and
+ * nf_plasma_frame (include/NField.h:12-13), Geometry::is_within
+ * (include/Geometry.h:50), Jet::getV / getKI / getEtaI (include/Jet.h:21,38,50)
+ * and the restored Q/U/V/F/C getters (src/Jet.cpp:114-288) -- evaluated by
+ * the kernels' own field code.  points: n x 3 (cm); n_los: n x 3 unit
+ * vectors (lab frame); nu in Hz.  Writes JSX_FIELDS_STRIDE doubles per point
+ * to out (host pointers; synchronous):
+ *   [0..2]   v = vf(point)                [3]      Gamma = getG(v)
+ *   [4..6]   bf(point) (a scalar field: magnitude in [4])
+ *   [7]      1 for a vector field, 0 for a scalar one
+ *   [8..10]  bf_plasma_frame(point, v)    [11]     nf(point)
+ *   [12]     nf_plasma_frame(point, Gamma)
+ *   [13]     D = getD(n_los, v)          [14..16] n_los' = get_n_los_prime
+ *   [17]     nu / D
+ *   [18..28] k_I, k_Q, k_U, k_V, k_F, k_C, h_Q, eta_I, eta_Q, eta_U, eta_V
+ *            in the lab frame (getKI = [18], getEtaI = [25])
+ *   [29]     is_within(point)             [30..31] 0
+ * params may be NULL (only its option flags are read). */
+#define JSX_FIELDS_STRIDE 32
+int jsx_fields_host(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_solver_params *params,
+                    const double *points, const double *n_los, int64_t n, double nu, double *out);
+
 /* Observation::run_stripe + getStripe("tau") (src/Observation.cpp:310-347,
  * src/Image.cpp:31-40): optical depth of the middle row's pixels
  * n_along/2 .. n_along-1 with the fixed-step RK4 integrate_tau

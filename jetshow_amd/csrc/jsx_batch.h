@@ -54,7 +54,7 @@ struct PolFP64 {
     using LaneT = Lane<NS>;
     using Params = DParams;
     template <class Q>
-    __device__ static void begin(LaneT &L, const Q &q, int64_t pix) { lane_begin(L, q.m, q.im, q.S, pix); }
+    __device__ __forceinline__ static void begin(LaneT &L, const Q &q, int64_t pix) { lane_begin(L, q.m, q.im, q.S, pix); }
     template <class Q>
     __device__ __forceinline__ static void rhs_consume(LaneT &L, const Q *q0) {
         double r[NS];
@@ -75,7 +75,7 @@ struct PolFP64 {
         JSX_OPAQUE_S(q);
         lane_consume(L, q->S, q->im, r);
     }
-    __device__ static void write(const LaneT &L, const DOut &o, int64_t pix) { lane_write(L, o, pix); }
+    __device__ __forceinline__ static void write(const LaneT &L, const DOut &o, int64_t pix) { lane_write(L, o, pix); }
     __device__ static bool done(const LaneT &L) { return L.phase == PH_DONE; }
     __device__ static int64_t pix(const LaneT &L) { return L.pix; }
     __device__ static unsigned rhs(const LaneT &L) { return L.n_rhs; }

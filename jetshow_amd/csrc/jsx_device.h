@@ -54,6 +54,16 @@ struct DModel {
     int has_gb, has_gb2, has_gn, has_gv;
     DGrid hull;          /* JSX_GEOM_CONVEX_HULL: face planes [nz][4] = (n, h) in cm */
     int has_hull;
+    /* further reference classes: ParabaloidCone switch height (its cone's
+     * origin, angle and matrix sit in origin / cos_angle / M); composite and
+     * force-free B; composite N; sheared, sheath and parabolic V (for the
+     * const-cone V fields v_origin / v_theta hold the cone part's values) */
+    double geo_z0;
+    double b_negexp2, b_z0, b_i0, b_i1, b_pitch, tan_pitch;
+    double n_negexp2, n_z0;
+    double v_r, v_theta, v_R0, v_Rz0, v_z0;
+    int ext;             /* a kind outside the workloads' set: the mixed path takes
+                            its fields from the FP64 field code */
 };
 
 /* ---------------- 3-vector helpers ---------------- */
@@ -258,9 +268,65 @@ __device__ inline int d_hull_hit(const DModel &m, const double *o, const double 
     return 1;
 }
 
+/* ParabaloidCone::hit (src/ParabaloidCone.cpp:45-94): the paraboloid r0
+ * (geo_param) below z0, the cone tangent to it there above (its origin and
+ * angle in the model's cone fields).  A ray that misses the paraboloid keeps
+ * the cone's result when the cone's first segment enters above z0; one that
+ * hits it enters on the paraboloid below z0, else on the cone, and leaves on
+ * the paraboloid below z0, else on the cone.  The paraboloid lies inside the
+ * cone, so a paraboloid hit without a cone hit (the reference reads an empty
+ * list there) can only come from rounding at a tangency: the paraboloid's own
+ * points are kept. */
+__device__ inline int d_paraboloid_cone_hit(const DModel &m, const double *o, const double *d, DSeg *segs) {
+    DSeg cs[2], ps[1];
+    int nc = d_cone_hit(m, o, d, cs);
+    int np = d_paraboloid_hit(m, o, d, ps);
+    if (nc == 0 && np == 0) return 0;
+    if (np == 0) {
+        if (cs[0].pin[2] > m.geo_z0) {
+            segs[0] = cs[0];
+            if (nc > 1) segs[1] = cs[1];
+            return nc;
+        }
+        return 0;
+    }
+    const DSeg &c0 = nc > 0 ? cs[0] : ps[0];
+    for (int i = 0; i < 3; ++i) {
+        segs[0].pin[i] = ps[0].pin[2] > m.geo_z0 ? c0.pin[i] : ps[0].pin[i];
+        segs[0].pout[i] = ps[0].pout[2] < m.geo_z0 ? ps[0].pout[i] : c0.pout[i];
+    }
+    return 1;
+}
+
+/* Geometry::is_within (Cone.cpp:48-53, Parabaloid.cpp:35-37 with its mixed
+ * units, Cylinder.cpp:29-32, ParabaloidCone.cpp:36-43); the convex hull: on
+ * the inner side of every face plane */
+__device__ inline int d_is_within(const DModel &m, const double *p) {
+    switch (m.geometry) {
+    case JSX_GEOM_CONE: return d_cone_is_within(m, p);
+    case JSX_GEOM_PARABOLOID: return sqrt(p[0] * p[0] + p[1] * p[1]) < m.geo_param * sqrt(p[2]);
+    case JSX_GEOM_CYLINDER: {
+        double w[3] = {p[0] - m.origin[0], p[1] - m.origin[1], p[2] - m.origin[2]}, cr[3];
+        d_cross3(w, m.axis, cr);
+        return d_norm3(cr) < m.geo_param;
+    }
+    case JSX_GEOM_PARABOLOID_CONE:
+        if (p[2] < m.geo_z0) return sqrt(p[0] * p[0] + p[1] * p[1]) < m.geo_param * sqrt(p[2]);
+        return d_cone_is_within(m, p);
+    case JSX_GEOM_CONVEX_HULL:
+        for (int i = 0; i < m.hull.nz; ++i) {
+            const double *q = m.hull.v + 4 * i;
+            if (d_dot3(q, p) > q[3]) return 0;
+        }
+        return 1;
+    default: return 0;
+    }
+}
+
 /* Jet::hit + clip; *clipped set if the clip changed anything.  The two
  * possible segments are handled with constant indices so they stay in
  * registers. */
+template <bool EXT = false>
 __device__ inline int d_model_hit(const DModel &m, const double *o, const double *d, DSeg *segs, int *clipped) {
     int n = 0;
     *clipped = 0;
@@ -268,6 +334,9 @@ __device__ inline int d_model_hit(const DModel &m, const double *o, const double
     else if (m.geometry == JSX_GEOM_PARABOLOID) n = d_paraboloid_hit(m, o, d, segs);
     else if (m.geometry == JSX_GEOM_CYLINDER) n = d_cylinder_hit(m, o, d, segs);
     else if (m.geometry == JSX_GEOM_CONVEX_HULL) n = d_hull_hit(m, o, d, segs);
+    else if constexpr (EXT) {
+        if (m.geometry == JSX_GEOM_PARABOLOID_CONE) n = d_paraboloid_cone_hit(m, o, d, segs);
+    }
     if (m.clip && n > 0) {
         DSeg a = segs[0], b = segs[1];
         int ra = d_clip_seg(m, a);
@@ -338,6 +407,110 @@ __device__ inline double d_pow_shared(double x, double lh, double ll, double y) 
 }
 
 /* ---------------- fields ---------------- */
+/* flow along +z with Lorentz factor g (VField.cpp:28,40,57) */
+__device__ inline void d_v_axial(double g, double *v) {
+    v[0] = 0; v[1] = 0; v[2] = JSX_CL * sqrt(1. - 1. / (g * g));
+}
+
+/* radial flow away from -o, r = |p + o| (VField.cpp:68-71,85-86,105-106) */
+__device__ inline void d_v_radial(const double *p, const double *o, double r, double g, double *v) {
+    double v_r = JSX_CL * sqrt(1. - 1. / (g * g));
+    v[0] = v_r * (p[0] + o[0]) / r;
+    v[1] = v_r * (p[1] + o[1]) / r;
+    v[2] = v_r * (p[2] + o[2]) / r;
+}
+
+/* ShearedCentralVField::vf (VField.cpp:78-87): the reference writes
+ * acos(z + o_z / r); JSX_OPT_SHEARED_CENTRAL_REF_BUG keeps that, the default
+ * takes the polar angle acos((z + o_z) / r) */
+__device__ inline void d_v_sheared_central(const DModel &m, const double *p, double *v) {
+    double q[3] = {p[0] + m.v_origin[0], p[1] + m.v_origin[1], p[2] + m.v_origin[2]};
+    double r = d_norm3(q);
+    double theta = (m.opt & JSX_OPT_SHEARED_CENTRAL_REF_BUG) ? jsx_acos(p[2] + m.v_origin[2] / r)
+                                                            : jsx_acos((p[2] + m.v_origin[2]) / r);
+    double g = m.gamma0 - (m.gamma0 - m.gamma1) * theta / m.v_theta;
+    d_v_radial(p, m.v_origin, r, g, v);
+}
+
+/* the flow of the parabolic fields (VField.cpp:118-132): along the
+ * paraboloid through p, r = hypot(x, y), R the radius of the enclosing
+ * paraboloid at z */
+__device__ inline void d_v_parabolic(const DModel &m, const double *p, double r, double R, double g, double *v) {
+    double v_r = JSX_CL * sqrt(1. - 1. / (g * g));
+    double r0 = m.v_Rz0 * r / R;
+    double sa, ca;
+    jsx_sincos_atan(2 * JSX_PC * r / r0 / r0, &sa, &ca);
+    if (r == 0) {
+        v[0] = 0; v[1] = 0; v[2] = v_r;
+    } else {
+        v[0] = v_r * ca * p[0] / r;
+        v[1] = v_r * ca * p[1] / r;
+        v[2] = v_r * sa;
+    }
+}
+
+/* AccParabolicVField (VField.cpp:138-161) and ShearedAccParabolicVField
+ * (VField.cpp:171-206); R0 = v_R0 */
+__device__ inline void d_v_acc_parabolic(const DModel &m, const double *p, int sheared, double *v) {
+    double r = sqrt(p[0] * p[0] + p[1] * p[1]);
+    double R = sqrt(m.v_Rz0 * m.v_Rz0 * p[2] / JSX_PC);
+    double g0 = m.gamma0;
+    if (sheared) {
+        double R0_border = sqrt(m.v_Rz0 * m.v_Rz0 * m.v_R0 / JSX_PC);
+        double rR0 = R0_border * r / R;
+        g0 = m.gamma0 - (m.gamma0 - m.gamma1) * jsx_pow(rR0 / R0_border, 0.25);
+        if (g0 < 1) g0 = 1;
+    }
+    double g = 1 + (g0 - 1) / sqrt(m.v_R0) * sqrt(p[2]);
+    d_v_parabolic(m, p, r, R, g, v);
+}
+
+/* the V fields beyond the workloads' set (VField.cpp:32-59,75-268) */
+__device__ inline void d_vfield_ext(const DModel &m, const double *p, double *v) {
+    switch (m.vfield) {
+    case JSX_V_SHEARED_FLAT: {
+        double r = sqrt(p[0] * p[0] + p[1] * p[1]);
+        d_v_axial(m.gamma0 - (m.gamma0 - m.gamma1) * r / m.v_r, v);
+        return;
+    }
+    case JSX_V_SHEATH_FLAT: {
+        double r = sqrt(p[0] * p[0] + p[1] * p[1]);
+        d_v_axial(r < m.v_r ? m.gamma0 : m.gamma1, v);
+        return;
+    }
+    case JSX_V_SHEARED_CENTRAL: d_v_sheared_central(m, p, v); return;
+    case JSX_V_SHEATH_CENTRAL: {
+        double q[3] = {p[0] + m.v_origin[0], p[1] + m.v_origin[1], p[2] + m.v_origin[2]};
+        double r = d_norm3(q);
+        double theta = jsx_acos(p[2] / r);
+        d_v_radial(p, m.v_origin, r, theta < m.v_theta ? m.gamma0 : m.gamma1, v);
+        return;
+    }
+    case JSX_V_CONST_PARABOLIC: {
+        double r = sqrt(p[0] * p[0] + p[1] * p[1]);
+        d_v_parabolic(m, p, r, sqrt(m.v_Rz0 * m.v_Rz0 * p[2] / JSX_PC), m.gamma0, v);
+        return;
+    }
+    case JSX_V_ACC_PARABOLIC: d_v_acc_parabolic(m, p, 0, v); return;
+    case JSX_V_SHEARED_ACC_PARABOLIC: d_v_acc_parabolic(m, p, 1, v); return;
+    case JSX_V_ACC_PARABOLIC_CONST_CONE:
+        if (p[2] > m.v_z0) {
+            double q[3] = {p[0] + m.v_origin[0], p[1] + m.v_origin[1], p[2] + m.v_origin[2]};
+            d_v_radial(p, m.v_origin, d_norm3(q), m.gamma0, v);
+        } else {
+            d_v_acc_parabolic(m, p, 0, v);
+        }
+        return;
+    case JSX_V_SHEARED_ACC_PARABOLIC_CONST_CONE:
+        if (p[2] > m.v_z0) d_v_sheared_central(m, p, v);
+        else d_v_acc_parabolic(m, p, 1, v);
+        return;
+    default:
+        v[0] = v[1] = v[2] = 0;
+    }
+}
+
+template <bool EXT = false>
 __device__ inline void d_vfield(const DModel &m, const double *p, double *v) {
     switch (m.vfield) {
     case JSX_V_CONST_FLAT:
@@ -366,7 +539,8 @@ __device__ inline void d_vfield(const DModel &m, const double *p, double *v) {
         return;
     }
     default:
-        v[0] = v[1] = v[2] = 0;
+        if constexpr (EXT) d_vfield_ext(m, p, v);
+        else v[0] = v[1] = v[2] = 0;
     }
 }
 
@@ -375,51 +549,86 @@ __device__ inline double d_getG(const double *v) {
     return sqrt(1. / (1. - d_sqn3(beta)));
 }
 
-/* frame quantities of Jet::getKI (src/Jet.cpp:25-31) */
-struct DFrame {
-    double gamma, D, n, nu;
-    double nl[3];
-    double b[3];
-    int vec;
-};
+/* fields with a power law in |p| / pc share log(|p| / pc) */
+__device__ __forceinline__ int d_need_r(const DModel &m) {
+    return m.bfield == JSX_B_RANDOM_SCALAR || m.bfield == JSX_B_TOROIDAL_R || m.bfield == JSX_B_HELICAL_R ||
+           m.bfield == JSX_B_CONST_CYLINDER || m.bfield == JSX_B_RADIAL_CONICAL || m.nfield == JSX_N_BK ||
+           m.nfield == JSX_N_COMPOSITE_BK;
+}
+__device__ __forceinline__ void d_radius_log(const double *p, double *rr, double *lh, double *ll) {
+    double r = d_norm3(p) / JSX_PC, h = 0.0, l = 0.0;
+    if (r > 0.0 && r < 1.79769313486231570815e+308) h = jsx_log_dd(r, &l);
+    *rr = r; *lh = h; *ll = l;
+}
 
-__device__ inline void d_frame(const DModel &m, const double *p, const double *n_los, double nu, DFrame &f) {
-    double v[3];
-    d_vfield(m, p, v);
-    double gamma = d_getG(v);
-    f.gamma = gamma;
-    /* fields with a power law in |p| / pc share log(|p| / pc) */
-    double rr = 0.0, lr_h = 0.0, lr_l = 0.0;
-    int need_r = (m.bfield == JSX_B_RANDOM_SCALAR || m.bfield == JSX_B_TOROIDAL_R || m.bfield == JSX_B_HELICAL_R ||
-                  m.nfield == JSX_N_BK);
-    if (need_r) {
-        rr = d_norm3(p) / JSX_PC;
-        if (rr > 0.0 && rr < 1.79769313486231570815e+308) lr_h = jsx_log_dd(rr, &lr_l);
+/* the further reference B fields (BField.cpp:53-146) */
+__device__ inline int d_bfield_ext(const DModel &m, const double *p, double rr, double lr_h, double lr_l, double *bl) {
+    switch (m.bfield) {
+    case JSX_B_COMPOSITE_RANDOM_SCALAR_Z: {  /* BField.cpp:56-64 */
+        double r = fabs(p[2]);
+        bl[0] = m.b0 * jsx_pow(r / JSX_PC, r > m.b_z0 ? m.b_negexp2 : m.b_negexp);
+        return 0;
     }
-    /* B field, plasma frame (BField.cpp:35-146,196-224) */
-    double bl[3];
-    f.vec = 1;
+    case JSX_B_CONST_CYLINDER:                /* BField.cpp:70-73 */
+        bl[0] = 0.0; bl[1] = 0.0; bl[2] = m.b0 * d_pow_shared(rr, lr_h, lr_l, m.b_negexp);
+        return 1;
+    case JSX_B_RADIAL_CONICAL: {              /* BField.cpp:88-93 */
+        double r = d_norm3(p);
+        double bb = m.b0 * d_pow_shared(rr, lr_h, lr_l, m.b_negexp);
+        bl[0] = bb * p[0] / r; bl[1] = bb * p[1] / r; bl[2] = bb * p[2] / r;
+        return 1;
+    }
+    case JSX_B_HELICAL_CYLINDER: {            /* BField.cpp:112-117 */
+        double r = sqrt(p[0] * p[0] + p[1] * p[1]);
+        bl[0] = m.b0 * jsx_tan(m.b_pitch * p[1] / r);
+        bl[1] = -m.b0 * jsx_tan(m.b_pitch * p[0] / r);
+        bl[2] = m.b0;
+        return 1;
+    }
+    case JSX_B_SPIRAL_CONICAL: {              /* BField.cpp:123-131 */
+        double z = p[2], x = p[0], y = p[1];
+        double b_z = m.b0 / (z * z / (JSX_PC * JSX_PC));
+        bl[0] = b_z * (x / z + y * m.tan_pitch / JSX_PC);
+        bl[1] = b_z * (y / z - x * m.tan_pitch / JSX_PC);
+        bl[2] = b_z;
+        return 1;
+    }
+    case JSX_B_FORCE_FREE_CYLINDRICAL: {      /* BField.cpp:137-146 */
+        double sf, cf;
+        jsx_sincos_atan(p[1] / p[0], &sf, &cf);
+        bl[0] = -m.b0 * m.b_i1 * sf;
+        bl[1] = m.b0 * m.b_i1 * sf;
+        bl[2] = m.b0 * m.b_i0;
+        return 1;
+    }
+    default:
+        bl[0] = 0;
+        return 0;
+    }
+}
+
+/* the B field's bf(point) (BField.cpp:35-146,196-207): returns 1 with the
+ * vector in bl[0..2], or 0 with a tangled field's magnitude in bl[0] */
+template <bool EXT = false>
+__device__ __forceinline__ int d_bfield(const DModel &m, const double *p, double rr, double lr_h, double lr_l, double *bl) {
     switch (m.bfield) {
     case JSX_B_RANDOM_SCALAR:
-        f.b[0] = m.b0 * d_pow_shared(rr, lr_h, lr_l, m.b_negexp);
-        f.vec = 0;
-        break;
+        bl[0] = m.b0 * d_pow_shared(rr, lr_h, lr_l, m.b_negexp);
+        return 0;
     case JSX_B_RANDOM_SCALAR_Z: {
         double r = fabs(p[2]);
-        f.b[0] = m.b0 * jsx_pow(r / JSX_PC, m.b_negexp);
-        f.vec = 0;
-        break;
+        bl[0] = m.b0 * jsx_pow(r / JSX_PC, m.b_negexp);
+        return 0;
     }
     case JSX_B_GRID_SCALAR:
-        f.b[0] = m.has_gb ? d_grid_interp(m.gb, p) : 0.0;
-        f.vec = 0;
-        break;
+        bl[0] = m.has_gb ? d_grid_interp(m.gb, p) : 0.0;
+        return 0;
     case JSX_B_TOROIDAL: {
         double sf, cf;
         jsx_sincos_atan(p[1] / p[0], &sf, &cf);
         double bb = m.b0 * jsx_pow(p[2] / JSX_PC, m.b_negexp);
         bl[0] = -sf * bb; bl[1] = cf * bb; bl[2] = 0;
-        break;
+        return 1;
     }
     case JSX_B_TOROIDAL_R:
     case JSX_B_HELICAL_R: {
@@ -432,45 +641,75 @@ __device__ inline void d_frame(const DModel &m, const double *p, const double *n
             double bphi = bb * m.sin_pitch;
             bl[0] = bphi * ex; bl[1] = bphi * ey; bl[2] = bb * m.cos_pitch;
         }
-        break;
+        return 1;
     }
     case JSX_B_CONST_CYLINDER_Z:
         bl[0] = 0.0; bl[1] = 0.0; bl[2] = m.b0 * jsx_pow(fabs(p[2]) / JSX_PC, m.b_negexp);
-        break;
+        return 1;
     case JSX_B_GRID_VECTOR: {
         double sf, cf;
         jsx_sincos_atan(p[1] / p[0], &sf, &cf);
         double vp = m.has_gb ? d_grid_interp(m.gb, p) : 0.0;
         double vf = m.has_gb2 ? d_grid_interp(m.gb2, p) : 0.0;
         bl[0] = -sf * vf; bl[1] = cf * vf; bl[2] = vp;
-        break;
+        return 1;
     }
     default:
-        f.b[0] = 0;
-        f.vec = 0;
+        if constexpr (EXT) return d_bfield_ext(m, p, rr, lr_h, lr_l, bl);
+        bl[0] = 0;
+        return 0;
     }
-    if (f.vec) {
-        if (m.b_plasma) {
-            f.b[0] = bl[0]; f.b[1] = bl[1]; f.b[2] = bl[2];
-        } else { /* getBprime (utils.cpp:164-168) */
-            double s1 = gamma / ((1. + gamma) * (JSX_CL * JSX_CL));
-            double vb = d_dot3(v, bl);
-            for (int i = 0; i < 3; ++i) f.b[i] = bl[i] / gamma + (s1 * v[i]) * vb;
+}
+
+/* the N field's nf(point) (NField.cpp:17-59) */
+template <bool EXT = false>
+__device__ __forceinline__ double d_nfield(const DModel &m, const double *p, double rr, double lr_h, double lr_l) {
+    switch (m.nfield) {
+    case JSX_N_CONST: return m.n0;
+    case JSX_N_BK: return m.n0 * d_pow_shared(rr, lr_h, lr_l, m.n_negexp);
+    case JSX_N_BK_Z: return m.n0 * jsx_pow(fabs(p[2]) / JSX_PC, m.n_negexp);
+    case JSX_N_GRID: return m.has_gn ? d_grid_interp(m.gn, p) : 0.0;
+    default:
+        if constexpr (EXT) {
+            if (m.nfield == JSX_N_COMPOSITE_BK)   /* NField.cpp:45-53 */
+                return m.n0 * d_pow_shared(rr, lr_h, lr_l, fabs(p[2]) > m.n_z0 ? m.n_negexp2 : m.n_negexp);
         }
+        return 0;
+    }
+}
+
+/* frame quantities of Jet::getKI (src/Jet.cpp:25-31) */
+struct DFrame {
+    double gamma, D, n, nu;
+    double nl[3];
+    double b[3];
+    int vec;
+};
+
+template <bool EXT = false>
+__device__ inline void d_frame(const DModel &m, const double *p, const double *n_los, double nu, DFrame &f) {
+    double v[3];
+    d_vfield<EXT>(m, p, v);
+    double gamma = d_getG(v);
+    f.gamma = gamma;
+    double rr = 0.0, lr_h = 0.0, lr_l = 0.0;
+    if (d_need_r(m)) d_radius_log(p, &rr, &lr_h, &lr_l);
+    /* B field, plasma frame (BField.cpp:35-146,196-224) */
+    double bl[3] = {0.0, 0.0, 0.0};
+    f.vec = d_bfield<EXT>(m, p, rr, lr_h, lr_l, bl);
+    if (f.vec && !m.b_plasma) { /* getBprime (utils.cpp:164-168) */
+        double s1 = gamma / ((1. + gamma) * (JSX_CL * JSX_CL));
+        double vb = d_dot3(v, bl);
+        for (int i = 0; i < 3; ++i) f.b[i] = bl[i] / gamma + (s1 * v[i]) * vb;
+    } else {
+        f.b[0] = bl[0]; f.b[1] = bl[1]; f.b[2] = bl[2];
     }
     /* getD (utils.cpp:152-155) */
     double beta[3] = {v[0] / JSX_CL, v[1] / JSX_CL, v[2] / JSX_CL};
     double D = 1. / (gamma * (1. - d_dot3(beta, n_los)));
     f.D = D;
     /* NField (NField.cpp:7-59) */
-    double n;
-    switch (m.nfield) {
-    case JSX_N_CONST: n = m.n0; break;
-    case JSX_N_BK: n = m.n0 * d_pow_shared(rr, lr_h, lr_l, m.n_negexp); break;
-    case JSX_N_BK_Z: n = m.n0 * jsx_pow(fabs(p[2]) / JSX_PC, m.n_negexp); break;
-    case JSX_N_GRID: n = m.has_gn ? d_grid_interp(m.gn, p) : 0.0; break;
-    default: n = 0;
-    }
+    double n = d_nfield<EXT>(m, p, rr, lr_h, lr_l);
     f.n = m.n_plasma ? n : n / gamma;
     /* get_n_los_prime (utils.cpp:157-162) */
     double fct = (D + 1.) * (gamma / (gamma + 1.));
@@ -481,10 +720,10 @@ __device__ inline void d_frame(const DModel &m, const double *p, const double *n
 /* lab-frame transfer coefficients (sky Stokes frame for Q/U) */
 struct DCoef { double kI, kQ, kU, kV, kF, kC, hQ, eI, eQ, eU, eV; };
 
-template <int FULL>
+template <int FULL, bool EXT = false>
 __device__ inline void d_coefficients(const DModel &m, const double *p, const double *n_los, double nu, DCoef &o) {
     DFrame f;
-    d_frame(m, p, n_los, nu, f);
+    d_frame<EXT>(m, p, n_los, nu, f);
     double D = f.D, n = f.n, fnu = f.nu;
     double nup = sqrt(n * JSX_QE * JSX_QE / (JSX_PI * JSX_ME));
     if (!f.vec) {

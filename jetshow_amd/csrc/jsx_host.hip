@@ -457,14 +457,52 @@ static int lower_grid(jsx_ctx *ctx, int32_t h, DGrid &g, int &has) {
     return JSX_OK;
 }
 
+/* parameter-only constants of the further reference classes, evaluated with
+ * the C library as their constructors and bf()/vf() evaluate them */
+static void lower_model_ext(const jsx_jet_desc &j, DModel &m) {
+    const double PC = 3.0856775814671913e+18;
+    m.ext = j.geometry == JSX_GEOM_PARABOLOID_CONE || j.bfield >= JSX_B_CONST_CYLINDER ||
+            j.nfield >= JSX_N_COMPOSITE_BK || j.vfield >= JSX_V_SHEARED_FLAT;
+    if (j.geometry == JSX_GEOM_PARABOLOID_CONE) {
+        /* ParabaloidCone.cpp:12-16: the cone below the paraboloid's tangent at z0 */
+        const double r0 = j.geo_param, z0 = j.geo_z0;
+        const double shift = -(r0 * sqrt(z0 / PC) / (0.5 * r0 * pow(z0 * PC, -0.5)) - z0);
+        const double angle = atan(0.5 * r0 * pow(z0 * PC, -0.5));
+        m.origin[0] = m.origin[0] + 0.0;
+        m.origin[1] = m.origin[1] + 0.0;
+        m.origin[2] = m.origin[2] + shift;
+        m.cos_angle = cos(angle);
+        const double cc = cos(angle) * cos(angle);
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) m.M[r][c] = m.axis[r] * m.axis[c] - cc * (r == c ? 1.0 : 0.0);
+        m.geo_z0 = z0;
+    }
+    m.b_negexp2 = -j.b_exp2; m.b_z0 = j.b_z0;
+    m.b_pitch = j.b_pitch; m.tan_pitch = tan(j.b_pitch);
+    if (j.bfield == JSX_B_FORCE_FREE_CYLINDRICAL) jsx_bessel_i01(j.b_mu, &m.b_i0, &m.b_i1);
+    m.n_negexp2 = -j.n_exp2; m.n_z0 = j.n_z0;
+    m.v_r = j.v_r; m.v_theta = j.v_theta; m.v_R0 = j.v_R0; m.v_Rz0 = j.v_Rz0; m.v_z0 = j.v_z0;
+    if (j.vfield == JSX_V_ACC_PARABOLIC_CONST_CONE || j.vfield == JSX_V_SHEARED_ACC_PARABOLIC_CONST_CONE) {
+        /* VField.cpp:222-257: the paraboloid part has R0 = z0; the cone part is
+           central from (0, 0, -oz) with the paraboloid's tangent angle */
+        const double Rz0 = j.v_Rz0, z0 = j.v_z0;
+        m.v_R0 = z0;
+        m.v_origin[0] = 0.0; m.v_origin[1] = 0.0;
+        m.v_origin[2] = Rz0 * sqrt(z0 / PC) / (0.5 * Rz0 * pow(z0 * PC, -0.5)) - z0;
+        if (j.vfield == JSX_V_SHEARED_ACC_PARABOLIC_CONST_CONE) m.v_theta = atan(sqrt(Rz0 * Rz0 * z0 / PC) / z0);
+    }
+}
+
 static int lower_model(jsx_ctx *ctx, const jsx_jet_desc &j, unsigned opt, DModel &m) {
     std::memset(&m, 0, sizeof m);
-    if (j.geometry < JSX_GEOM_CONE || j.geometry > JSX_GEOM_CONVEX_HULL)
+    if (j.geometry < JSX_GEOM_CONE || j.geometry > JSX_GEOM_PARABOLOID_CONE)
         return fail(ctx, JSX_EUNSUPPORTED, "geometry kind %d", j.geometry);
-    if (j.bfield < JSX_B_RANDOM_SCALAR || j.bfield > JSX_B_CONST_CYLINDER_Z)
+    if (j.bfield < JSX_B_RANDOM_SCALAR || j.bfield > JSX_B_COMPOSITE_RANDOM_SCALAR_Z)
         return fail(ctx, JSX_EUNSUPPORTED, "bfield kind %d", j.bfield);
-    if (j.nfield < JSX_N_CONST || j.nfield > JSX_N_GRID) return fail(ctx, JSX_EUNSUPPORTED, "nfield kind %d", j.nfield);
-    if (j.vfield < JSX_V_CONST_FLAT || j.vfield > JSX_V_GRID) return fail(ctx, JSX_EUNSUPPORTED, "vfield kind %d", j.vfield);
+    if (j.nfield < JSX_N_CONST || j.nfield > JSX_N_COMPOSITE_BK)
+        return fail(ctx, JSX_EUNSUPPORTED, "nfield kind %d", j.nfield);
+    if (j.vfield < JSX_V_CONST_FLAT || j.vfield > JSX_V_SHEARED_ACC_PARABOLIC_CONST_CONE)
+        return fail(ctx, JSX_EUNSUPPORTED, "vfield kind %d", j.vfield);
     m.geometry = j.geometry; m.bfield = j.bfield; m.nfield = j.nfield; m.vfield = j.vfield;
     m.b_plasma = j.b_in_plasma_frame; m.n_plasma = j.n_in_plasma_frame;
     m.opt = (int)opt;
@@ -501,6 +539,7 @@ static int lower_model(jsx_ctx *ctx, const jsx_jet_desc &j, unsigned opt, DModel
     m.e_ei = (s - 1.) / 2.;
     m.e_kv = (s + 1.) / 2.;
     m.e_kCr = (s - 2.) / 2.;
+    lower_model_ext(j, m);
     int rc;
     if (j.geometry == JSX_GEOM_CONVEX_HULL) {
         auto it = ctx->grids.find(j.geo_table);
@@ -672,7 +711,7 @@ static void lower_mixed(const jsx_jet_desc &j, const jsx_solver_params &p, const
     const double ME = 9.10938356e-28, QE = 4.80320467299766e-10, CL = 29979245800.0;
     const double CB = QE / (2. * PI * ME * CL), CP = QE * QE / (PI * ME), nu = p.nu, s = j.s;
     const bool vec = !(j.bfield == JSX_B_RANDOM_SCALAR || j.bfield == JSX_B_RANDOM_SCALAR_Z ||
-                       j.bfield == JSX_B_GRID_SCALAR);
+                       j.bfield == JSX_B_GRID_SCALAR || j.bfield == JSX_B_COMPOSITE_RANDOM_SCALAR_Z);
     const double fk = vec ? m.f_ki_v : m.f_ki_s, fe = vec ? m.f_ei_v : m.f_ei_s;
     const double g = CB / nu;
     c.A_k = (float)(PI * CP * CB * fk * pow(g, s / 2.) / (CL * nu * nu) * PC);
@@ -819,8 +858,62 @@ static int split_workspace(jsx_ctx *ctx) {
 /* Upload per-request constants and launch one persistent kernel over all
  * rays.  All items share precision and Stokes mode.  Synchronises the stream
  * before reusing the context's batch buffers. */
+/* The plain form: one thread per pixel, one pass per image.  Models with a
+ * field or geometry kind outside the workloads' set run in the kernels built
+ * with that code (the common kernels leave it out: it costs them registers). */
+template <int NS, bool EXT>
+static void plain_kernel(hipStream_t st, int precision, const void *dp, const DOut &O, int64_t npix) {
+    const int threads = 256;
+    const dim3 grid((unsigned)((npix + threads - 1) / threads));
+    if (precision == JSX_PREC_MIXED)
+        hipLaunchKernelGGL((jsx_km_plain<NS, EXT>), grid, dim3(threads), 0, st, (const DParamsM *)dp, O, npix);
+    else
+        hipLaunchKernelGGL((jsx_k_plain<NS, EXT>), grid, dim3(threads), 0, st, (const DParams *)dp, O, npix);
+}
+
+static int launch_plain(jsx_ctx *ctx, hipStream_t st, int precision, bool full, const DModel &m, const DImage &D,
+                        const DSolver &S, const MConsts *c, const DOut &O, int64_t npix) {
+    if (npix == 0) return JSX_OK;
+    void *dp = nullptr;
+    int slot = 0, rc;
+    if (precision == JSX_PREC_MIXED) {
+        DParamsM hm;
+        hm.m = m; hm.im = D; hm.S = S; hm.c = *c;
+        if ((rc = stage_bytes(ctx, &hm, sizeof hm, st, &dp, &slot))) return rc;
+    } else {
+        DParams *p = nullptr;
+        if ((rc = stage_params(ctx, m, D, S, st, &p, &slot))) return rc;
+        dp = p;
+    }
+    if (m.ext) {
+        if (full) plain_kernel<4, true>(st, precision, dp, O, npix);
+        else plain_kernel<1, true>(st, precision, dp, O, npix);
+    } else {
+        if (full) plain_kernel<4, false>(st, precision, dp, O, npix);
+        else plain_kernel<1, false>(st, precision, dp, O, npix);
+    }
+    HIPCHECK(ctx, hipGetLastError());
+    return stage_done(ctx, slot, st);
+}
+
 static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes, const std::vector<BatchItem> &items,
                         int order) {
+    if (!items.empty()) {
+        /* models with the further reference kinds run through the plain kernels,
+           on the same stream, before the batched launch of the others */
+        bool any_ext = false;
+        for (const BatchItem &it : items) any_ext = any_ext || it.m.ext;
+        if (any_ext) {
+            std::vector<BatchItem> common;
+            for (const BatchItem &it : items) {
+                if (!it.m.ext) { common.push_back(it); continue; }
+                int rc = launch_plain(ctx, st, precision, stokes == JSX_STOKES_FULL, it.m, it.im, it.S, &it.c, it.out,
+                                      it.npix);
+                if (rc) return rc;
+            }
+            return launch_batch(ctx, st, precision, stokes, common, order);
+        }
+    }
     size_t n = items.size();
     if (!n) return JSX_OK;
     size_t esz = precision == JSX_PREC_MIXED ? sizeof(DParamsM) : sizeof(DParams);
@@ -1064,19 +1157,7 @@ static int observe_locked(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_image
             one[0].m = m; one[0].im = D; one[0].S = S; one[0].c = hm.c; one[0].out = O; one[0].npix = npix;
             return launch_batch(ctx, st, JSX_PREC_MIXED, params->stokes, one, (params->flags & 0x200u) ? 1 : 0);
         }
-        DParamsM *dpm = nullptr;
-        int slot = 0;
-        if ((rc = stage_bytes(ctx, &hm, sizeof hm, st, (void **)&dpm, &slot))) return rc;
-        {
-            int threads = 256;
-            int64_t blocks = (npix + threads - 1) / threads;
-            if (full)
-                hipLaunchKernelGGL(jsx_km_plain<4>, dim3((unsigned)blocks), dim3(threads), 0, st, dpm, O, npix);
-            else
-                hipLaunchKernelGGL(jsx_km_plain<1>, dim3((unsigned)blocks), dim3(threads), 0, st, dpm, O, npix);
-        }
-        HIPCHECK(ctx, hipGetLastError());
-        return stage_done(ctx, slot, st);
+        return launch_plain(ctx, st, JSX_PREC_MIXED, full, m, D, S, &hm.c, O, npix);
     }
     if (params->flags & JSX_OPT_PERSISTENT) {
         std::vector<BatchItem> one(1);
@@ -1084,19 +1165,7 @@ static int observe_locked(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_image
         one[0].m = m; one[0].im = D; one[0].S = S; one[0].out = O; one[0].npix = npix;
         return launch_batch(ctx, st, JSX_PREC_FP64, params->stokes, one, (params->flags & 0x200u) ? 1 : 0);
     }
-    DParams *dp = nullptr;
-    int slot = 0;
-    if ((rc = stage_params(ctx, m, D, S, st, &dp, &slot))) return rc;
-    {
-        int threads = 256;
-        int64_t blocks = (npix + threads - 1) / threads;
-        if (full)
-            hipLaunchKernelGGL(jsx_k_plain<4>, dim3((unsigned)blocks), dim3(threads), 0, st, dp, O, npix);
-        else
-            hipLaunchKernelGGL(jsx_k_plain<1>, dim3((unsigned)blocks), dim3(threads), 0, st, dp, O, npix);
-    }
-    HIPCHECK(ctx, hipGetLastError());
-    return stage_done(ctx, slot, st);
+    return launch_plain(ctx, st, JSX_PREC_FP64, full, m, D, S, nullptr, O, npix);
 }
 
 extern "C" int jsx_observe(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_image_desc *image,
@@ -1194,6 +1263,34 @@ extern "C" int jsx_hit_host(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_sol
     return JSX_OK;
 }
 
+extern "C" int jsx_fields_host(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_solver_params *params,
+                               const double *points, const double *n_los, int64_t n, double nu, double *out) {
+    if (!ctx || !jet || !points || !n_los || !out || n <= 0) return fail(ctx, JSX_EINVAL, "bad arguments");
+    if (!(nu > 0)) return fail(ctx, JSX_EINVAL, "nu must be > 0");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    DModel m;
+    int rc = lower_model(ctx, *jet, params ? params->flags : 0, m);
+    if (rc) return rc;
+    double *d_in = nullptr, *d_out = nullptr;
+    const size_t b_in = sizeof(double) * 6 * (size_t)n, b_out = sizeof(double) * JSX_FIELDS_STRIDE * (size_t)n;
+    hipError_t e = hipMalloc(&d_in, b_in);
+    if (e == hipSuccess) e = hipMalloc(&d_out, b_out);
+    if (e == hipSuccess) e = hipMemcpy(d_in, points, b_in / 2, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_in + 3 * n, n_los, b_in / 2, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(jsx_k_fields, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, m, d_in,
+                           d_in + 3 * n, n, nu, d_out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(out, d_out, b_out, hipMemcpyDeviceToHost);
+    (void)hipFree(d_in);
+    (void)hipFree(d_out);
+    if (e != hipSuccess) return fail(ctx, JSX_EHIP, "jsx_fields_host: %s", hipGetErrorString(e));
+    return JSX_OK;
+}
+
 /* ---------------- tau stripe ---------------- */
 extern "C" int jsx_tau_stripe(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_image_desc *image,
                               const jsx_solver_params *params, double *tau_host) {
@@ -1216,8 +1313,12 @@ extern "C" int jsx_tau_stripe(jsx_ctx *ctx, const jsx_jet_desc *jet, const jsx_i
     if ((rc = stage_params(ctx, m, D, S, ctx->stream, &dp, &slot))) return rc;
     double *d_tau = nullptr;
     HIPCHECK(ctx, hipMalloc(&d_tau, sizeof(double) * (size_t)n_out));
-    hipLaunchKernelGGL(jsx_k_tau_stripe, dim3((unsigned)((n_out + 255) / 256)), dim3(256), 0, ctx->stream, dp, d_tau,
-                       n_out);
+    if (m.ext)
+        hipLaunchKernelGGL(jsx_k_tau_stripe<true>, dim3((unsigned)((n_out + 255) / 256)), dim3(256), 0, ctx->stream, dp,
+                           d_tau, n_out);
+    else
+        hipLaunchKernelGGL(jsx_k_tau_stripe<false>, dim3((unsigned)((n_out + 255) / 256)), dim3(256), 0, ctx->stream,
+                           dp, d_tau, n_out);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess && (rc = stage_done(ctx, slot, ctx->stream))) {
         (void)hipFree(d_tau);

@@ -52,17 +52,18 @@ __device__ inline int64_t d_ray_to_pixel(const DImage &im, int64_t id, int order
     return id;
 }
 
-template <int NS>
+/* EXT: the model uses a field or geometry kind outside the workloads' set */
+template <int NS, bool EXT>
 __global__ __launch_bounds__(256) void jsx_k_plain(const DParams *__restrict__ P, DOut out, int64_t npix) {
     int64_t pix = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (pix >= npix) return;
     Lane<NS> L;
     const JSX_CONST DParams *q = d_launder(P);
-    lane_begin(L, q->m, q->im, q->S, pix);
+    lane_begin<NS, EXT>(L, q->m, q->im, q->S, pix);
     while (L.phase != PH_DONE) {
         q = d_launder(P);
         double r[NS];
-        lane_rhs(L, q->m, q->im, q->S, r);
+        lane_rhs<NS, EXT>(L, q->m, q->im, q->S, r);
         q = d_launder(P);
         lane_consume(L, q->S, q->im, r);
     }
@@ -75,6 +76,7 @@ __global__ __launch_bounds__(256) void jsx_k_plain(const DParams *__restrict__ P
  * is taken while (step+1) dt - L <= DBL_EPSILON; times are step * dt), stopping
  * at the first state with tau >= tau_max, whose value is added; later
  * segments are dropped.  One thread per stripe pixel. */
+template <bool EXT>
 __global__ __launch_bounds__(256) void jsx_k_tau_stripe(const DParams *__restrict__ P, double *tau, int64_t n_out) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n_out) return;
@@ -85,7 +87,7 @@ __global__ __launch_bounds__(256) void jsx_k_tau_stripe(const DParams *__restric
     double dir[3] = {c->im.dir[0], c->im.dir[1], c->im.dir[2]};
     DSeg segs[2];
     int clipped = 0;
-    const int ns = d_model_hit(c->m, o, dir, segs, &clipped);
+    const int ns = d_model_hit<EXT>(c->m, o, dir, segs, &clipped);
     const double n_los[3] = {-1. * dir[0], -1. * dir[1], -1. * dir[2]};
     double bg = 0.0;
     for (int si = 0; si < ns; ++si) {
@@ -107,7 +109,7 @@ __global__ __launch_bounds__(256) void jsx_k_tau_stripe(const DParams *__restric
                 for (int i = 0; i < 3; ++i) p[i] = pin[i] + ts[s] * dir[i];
                 DCoef cf;
                 c = d_launder(P);
-                d_coefficients<0>(c->m, p, n_los, c->S.nu, cf);
+                d_coefficients<0, EXT>(c->m, p, n_los, c->S.nu, cf);
                 k[s] = cf.kI;
             }
             x = x + dt * (1.0 / 6.0) * k[0] + dt * (1.0 / 3.0) * k[1] + dt * (1.0 / 3.0) * k[2] +
@@ -131,7 +133,7 @@ __global__ void jsx_k_hit(DModel m, const double *rays, int64_t n, int32_t *n_se
     d_normalize3(d);
     DSeg s[2];
     int clipped;
-    int ns = d_model_hit(m, o, d, s, &clipped);
+    int ns = d_model_hit<true>(m, o, d, s, &clipped);
     n_seg[q] = ns;
     for (int k = 0; k < 3; ++k) {
         segs[12 * q + k] = ns > 0 ? s[0].pin[k] : 0.0;
@@ -139,6 +141,35 @@ __global__ void jsx_k_hit(DModel m, const double *rays, int64_t n, int32_t *n_se
         segs[12 * q + 6 + k] = ns > 1 ? s[1].pin[k] : 0.0;
         segs[12 * q + 9 + k] = ns > 1 ? s[1].pout[k] : 0.0;
     }
+}
+
+/* Field and geometry values at explicit points (the layout of
+ * jsx_fields_host): every kind, FP64 */
+__global__ void jsx_k_fields(DModel m, const double *pts, const double *nls, int64_t n, double nu, double *out) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const double p[3] = {pts[3 * q], pts[3 * q + 1], pts[3 * q + 2]};
+    const double nl[3] = {nls[3 * q], nls[3 * q + 1], nls[3 * q + 2]};
+    double *o = out + (int64_t)JSX_FIELDS_STRIDE * q;
+    double v[3];
+    d_vfield<true>(m, p, v);
+    const double gamma = d_getG(v);
+    double rr = 0.0, lh = 0.0, ll = 0.0, bl[3] = {0.0, 0.0, 0.0};
+    if (d_need_r(m)) d_radius_log(p, &rr, &lh, &ll);
+    const int vec = d_bfield<true>(m, p, rr, lh, ll, bl);
+    const double n_lab = d_nfield<true>(m, p, rr, lh, ll);
+    DFrame f;
+    d_frame<true>(m, p, nl, nu, f);
+    DCoef c;
+    d_coefficients<1, true>(m, p, nl, nu, c);
+    const double row[JSX_FIELDS_STRIDE] = {
+        v[0], v[1], v[2], gamma,
+        bl[0], vec ? bl[1] : 0.0, vec ? bl[2] : 0.0, (double)vec,
+        f.b[0], vec ? f.b[1] : 0.0, vec ? f.b[2] : 0.0, n_lab, f.n,
+        f.D, f.nl[0], f.nl[1], f.nl[2], f.nu,
+        c.kI, c.kQ, c.kU, c.kV, c.kF, c.kC, c.hQ, c.eI, c.eQ, c.eU, c.eV,
+        (double)d_is_within(m, p), 0.0, 0.0};
+    for (int k = 0; k < JSX_FIELDS_STRIDE; ++k) o[k] = row[k];
 }
 
 #endif /* JSX_KERNELS_H */

@@ -245,17 +245,19 @@ JSX_HD double jsx_tan(double x) {
     return s / c;
 }
 
-/* asin(y) for |y| <= 1/2 by its Maclaurin series */
+/* asin(y) for |y| <= 1/2 by its Maclaurin series y sum_k c_k y^2k,
+ * c_k = (2k)! / (4^k (k!)^2 (2k + 1)), 30 terms (c_29 (1/4)^29 < 1e-19)
+ * evaluated by Horner's rule from the smallest term */
 JSX_HD double jsx_asin_small(double y) {
-    double y2 = y * y, term = y, sum = 0.0;
-    for (int k = 0; k < 40; ++k) {
-        double add = term / (double)(2 * k + 1);
-        double nsum = sum + add;
-        if (nsum == sum) break;
-        sum = nsum;
-        term = term * y2 * ((double)(2 * k + 1) / (double)(2 * k + 2));
+    double c[30];
+    double a = 1.0;                      /* (2k)! / (4^k (k!)^2) */
+    for (int k = 0; k < 30; ++k) {
+        c[k] = a / (double)(2 * k + 1);
+        a = a * ((double)(2 * k + 1) / (double)(2 * k + 2));
     }
-    return sum;
+    double y2 = y * y, p = c[29];
+    for (int k = 28; k >= 0; --k) p = p * y2 + c[k];
+    return y * p;
 }
 
 JSX_HD double jsx_acos(double u) {

@@ -78,13 +78,13 @@ __device__ inline float m_grid(const DGrid &G, const double *p_cm) { return (flo
 struct MCoef { float kI, kQ, kU, kV, kF, kC, hQ, eI, eQ, eU, eV; };
 
 /* p_pc: point in parsec (FP64); p_cm needed only for grid lookups */
-template <int FULL>
+template <int FULL, bool EXT = false>
 __device__ inline void m_coefficients(const DModel &m, const MConsts &c, const double *p_pc, MCoef &o) {
     JSX_MIXED_FMA
     float px = (float)p_pc[0], py = (float)p_pc[1], pz = (float)p_pc[2];
     double p_cm[3];
     int need_cm = (m.bfield == JSX_B_GRID_SCALAR || m.bfield == JSX_B_GRID_VECTOR || m.nfield == JSX_N_GRID ||
-                   m.vfield == JSX_V_GRID);
+                   m.vfield == JSX_V_GRID || (EXT && m.ext));
     if (need_cm)
         for (int i = 0; i < 3; ++i) p_cm[i] = p_pc[i] * JSX_PC;
     /* ---- bulk velocity: direction v_hat, Gamma, beta, 1 - beta ---- */
@@ -102,6 +102,13 @@ __device__ inline void m_coefficients(const DModel &m, const MConsts &c, const d
             gam = (float)g;
         } else if (m.vfield == JSX_V_GRID) {
             gam = m.has_gv ? m_grid(m.gv, p_cm) : 1.0f;
+        } else if (EXT && m.vfield >= JSX_V_SHEARED_FLAT) {
+            /* the further reference V fields: velocity from the FP64 field code */
+            double v[3];
+            d_vfield_ext(m, p_cm, v);
+            double vv = d_norm3(v);
+            gam = (float)d_getG(v);
+            if (vv > 0.0) { vh[0] = (float)(v[0] / vv); vh[1] = (float)(v[1] / vv); vh[2] = (float)(v[2] / vv); }
         } else { /* ConstCentral: radial from -v_origin */
             gam = c.gam_flat;
             float q[3] = {px + (float)(m.v_origin[0] * c.inv_pc), py + (float)(m.v_origin[1] * c.inv_pc),
@@ -116,7 +123,7 @@ __device__ inline void m_coefficients(const DModel &m, const MConsts &c, const d
     }
     const float *n = c.n_los;
     float mu, one_mu;
-    if (m.vfield == JSX_V_CONST_CENTRAL) {
+    if (m.vfield == JSX_V_CONST_CENTRAL || (EXT && m.vfield >= JSX_V_SHEARED_FLAT)) {
         float d0 = vh[0] - n[0], d1 = vh[1] - n[1], d2 = vh[2] - n[2];
         one_mu = 0.5f * (d0 * d0 + d1 * d1 + d2 * d2);
         mu = 1.0f - one_mu;
@@ -143,7 +150,13 @@ __device__ inline void m_coefficients(const DModel &m, const MConsts &c, const d
     case JSX_N_BK: nd = c.n0 * (m.n_negexp == -2.0 ? irr * irr : (m.n_negexp == -1.0 ? irr : m_powf(rr, m.n_negexp))); break;
     case JSX_N_BK_Z: { float az = fabsf(pz); nd = c.n0 * (m.n_negexp == -2.0 ? m_rcp(az * az) : m_powf(az, m.n_negexp)); break; }
     case JSX_N_GRID: nd = m.has_gn ? m_grid(m.gn, p_cm) : 0.f; break;
-    default: nd = 0.f;
+    default:
+        nd = 0.f;
+        if constexpr (EXT) {   /* the further reference N fields, from the FP64 field code */
+            double r64, lh, ll;
+            d_radius_log(p_cm, &r64, &lh, &ll);
+            nd = (float)d_nfield<true>(m, p_cm, r64, lh, ll);
+        }
     }
     if (!m.n_plasma) nd = nd * m_rcp(gam);
     /* ---- magnetic field (plasma frame) ---- */
@@ -190,7 +203,15 @@ __device__ inline void m_coefficients(const DModel &m, const MConsts &c, const d
         b[0] = -sf * vf; b[1] = cf * vf; b[2] = vp;
         break;
     }
-    default: bs = 0.f; vec = 0;
+    default:
+        bs = 0.f; vec = 0;
+        if constexpr (EXT) {   /* the further reference B fields, from the FP64 field code */
+            double r64 = 0.0, lh = 0.0, ll = 0.0, bl[3];
+            if (d_need_r(m)) d_radius_log(p_cm, &r64, &lh, &ll);
+            vec = d_bfield<true>(m, p_cm, r64, lh, ll, bl);
+            if (vec) { b[0] = (float)bl[0]; b[1] = (float)bl[1]; b[2] = (float)bl[2]; }
+            else bs = (float)bl[0];
+        }
     }
     if (vec && !m.b_plasma) {
         float ig = m_rcp(gam);
@@ -465,7 +486,7 @@ __device__ void ml_stokes_seg_begin(MLane<NS> &L, const DSolver &S, const MConst
     ml_stokes_loop(L, S, c);
 }
 
-template <int NS>
+template <int NS, bool EXT = false>
 __device__ inline void ml_begin(MLane<NS> &L, const DModel &m, const DImage &im, const DSolver &S, const MConsts &c,
                                 int64_t pix) {
     L.pix = pix;
@@ -476,7 +497,7 @@ __device__ inline void ml_begin(MLane<NS> &L, const DModel &m, const DImage &im,
     d_pixel_origin(im, pix, o);
     DSeg segs[2];
     int clipped = 0;
-    int ns = d_model_hit(m, o, im.dir, segs, &clipped);
+    int ns = d_model_hit<EXT>(m, o, im.dir, segs, &clipped);
     if (clipped) L.flags |= JSX_PIX_CLIPPED;
     L.nseg = ns;
     if (ns == 0) { L.phase = PH_DONE; return; }
@@ -503,13 +524,13 @@ __device__ inline void ml_point_pc(const MLane<NS> &L, const DImage &im, const M
     }
 }
 
-template <int NS>
+template <int NS, bool EXT = false>
 __device__ inline void ml_rhs(const MLane<NS> &L, const DModel &m, const DImage &im, const MConsts &c, float *r) {
     JSX_MIXED_FMA
     double p[3];
     ml_point_pc(L, im, c, p);
     MCoef k;
-    m_coefficients<NS == 4 ? 1 : 0>(m, c, p, k);
+    m_coefficients<NS == 4 ? 1 : 0, EXT>(m, c, p, k);
     if (L.phase == PH_TAU) {
         r[0] = k.kI;
         for (int i = 1; i < NS; ++i) r[i] = 0.f;
@@ -696,17 +717,18 @@ __device__ __forceinline__ const JSX_CONST DParamsM *d_launder_m(const DParamsM 
     return q;
 }
 
-template <int NS>
+/* EXT: the model uses a field or geometry kind outside the workloads' set */
+template <int NS, bool EXT>
 __global__ __launch_bounds__(256) void jsx_km_plain(const DParamsM *__restrict__ P, DOut out, int64_t npix) {
     int64_t pix = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (pix >= npix) return;
     MLane<NS> L;
     const JSX_CONST DParamsM *q = d_launder_m(P);
-    ml_begin(L, q->m, q->im, q->S, q->c, pix);
+    ml_begin<NS, EXT>(L, q->m, q->im, q->S, q->c, pix);
     while (L.phase != PH_DONE) {
         float r[NS];
         q = d_launder_m(P);
-        ml_rhs(L, q->m, q->im, q->c, r);
+        ml_rhs<NS, EXT>(L, q->m, q->im, q->c, r);
         q = d_launder_m(P);
         ml_consume(L, q->S, q->im, q->c, r);
     }

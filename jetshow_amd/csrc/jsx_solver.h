@@ -331,7 +331,7 @@ __device__ inline void lane_tau_seg_end(Lane<NS> &L, const DSolver &S, int found
 }
 
 /* begin a ray: generate it, hit the geometry, set up the tau pass */
-template <int NS>
+template <int NS, bool EXT = false>
 __device__ inline void lane_begin(Lane<NS> &L, const DModel &m, const DImage &im, const DSolver &S, int64_t pix) {
     L.pix = pix;
     L.flags = 0; L.n_rhs = 0; L.n_steps = 0;
@@ -342,7 +342,7 @@ __device__ inline void lane_begin(Lane<NS> &L, const DModel &m, const DImage &im
     d_pixel_origin(im, pix, o);
     DSeg segs[2];
     int clipped = 0;
-    int ns = d_model_hit(m, o, im.dir, segs, &clipped);
+    int ns = d_model_hit<EXT>(m, o, im.dir, segs, &clipped);
     if (clipped) L.flags |= JSX_PIX_CLIPPED;
     L.nseg = ns;
     if (ns == 0) { L.phase = PH_DONE; return; }
@@ -481,13 +481,13 @@ __device__ inline void lane_consume(Lane<NS> &L, const DSolver &S, const DImage 
 }
 
 /* evaluate the pending right-hand side for the lane */
-template <int NS>
+template <int NS, bool EXT = false>
 __device__ inline void lane_rhs(const Lane<NS> &L, const DModel &m, const DImage &im, const DSolver &S, double *r) {
     double p[3];
     lane_point(L, im, p);
     double n_los[3] = {-1. * im.dir[0], -1. * im.dir[1], -1. * im.dir[2]};
     DCoef c;
-    d_coefficients<NS == 4 ? 1 : 0>(m, p, n_los, S.nu, c);
+    d_coefficients<NS == 4 ? 1 : 0, EXT>(m, p, n_los, S.nu, c);
     if (L.phase == PH_TAU) {
         r[0] = c.kI;
         for (int i = 1; i < NS; ++i) r[i] = 0.0;

@@ -58,12 +58,101 @@ int32_t Context::upload_grid(const std::vector<double> &values, int nz, int nr, 
     return h;
 }
 
-/* ---- geometry ---- */
-std::list<Intersection> BaseGeometry::hit(Ray &ray) const {
+std::vector<double> Context::fields(const jsx_jet_desc &d, const std::vector<double> &points,
+                                    const std::vector<double> &n_los, double nu, uint32_t flags) {
+    if (points.size() % 3 || points.size() != n_los.size()) throw Error(JSX_EINVAL, "points and n_los: n x 3 each");
+    const int64_t n = (int64_t)(points.size() / 3);
+    std::vector<double> out((size_t)n * JSX_FIELDS_STRIDE);
+    if (n == 0) return out;
+    jsx_solver_params p;
+    std::memset(&p, 0, sizeof p);
+    p.flags = flags;
+    check(jsx_fields_host(ctx_, &d, &p, points.data(), n_los.data(), n, nu, out.data()));
+    return out;
+}
+
+/* a descriptor with every kind set to a plain default; a class lowers itself
+ * over it to be evaluated on its own */
+static jsx_jet_desc default_desc() {
     jsx_jet_desc d;
     std::memset(&d, 0, sizeof d);
-    d.bfield = JSX_B_RANDOM_SCALAR; d.nfield = JSX_N_CONST; d.vfield = JSX_V_CONST_FLAT;
-    d.gamma0 = 2.0; d.s = 2.5; d.gamma_min = 100.0;
+    d.geometry = JSX_GEOM_CONE; d.geo_axis[2] = 1.0; d.geo_param = 0.1; d.geo_big_scale = 100.0;
+    d.bfield = JSX_B_RANDOM_SCALAR; d.b0 = 1.0;
+    d.nfield = JSX_N_CONST; d.n0 = 1.0;
+    d.vfield = JSX_V_CONST_FLAT; d.gamma0 = 2.0;
+    d.s = 2.5; d.gamma_min = 100.0;
+    return d;
+}
+
+/* the JSX_FIELDS_STRIDE values at one point (unit LOS along +z, 1 GHz when
+ * only a field's own value is wanted) */
+static std::vector<double> one_point(const jsx_jet_desc &d, const Vector3d &p, const Vector3d &n_los, double nu,
+                                     uint32_t flags = 0) {
+    return Context::instance().fields(d, {p[0], p[1], p[2]}, {n_los[0], n_los[1], n_los[2]}, nu, flags);
+}
+static std::vector<double> one_point(const jsx_jet_desc &d, const Vector3d &p) {
+    return one_point(d, p, Vector3d(0, 0, 1), 1e9);
+}
+
+/* ---- frame transforms (src/utils.cpp:147-168) ---- */
+static double dot3(const Vector3d &a, const Vector3d &b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+double getG(const Vector3d &v) {
+    Vector3d beta(v[0] / c, v[1] / c, v[2] / c);
+    return std::sqrt(1. / (1. - dot3(beta, beta)));
+}
+double getD(const Vector3d &n_los, const Vector3d &v) {
+    Vector3d beta(v[0] / c, v[1] / c, v[2] / c);
+    return 1. / (getG(v) * (1. - dot3(beta, n_los)));
+}
+Vector3d get_n_los_prime(const Vector3d &n_los, const Vector3d &v) {
+    double df = getD(n_los, v), gamma = getG(v);
+    Vector3d beta(v[0] / c, v[1] / c, v[2] / c);
+    double f = (df + 1.) * (gamma / (gamma + 1.));
+    return Vector3d(df * n_los[0] - f * beta[0], df * n_los[1] - f * beta[1], df * n_los[2] - f * beta[2]);
+}
+Vector3d getBprime(const Vector3d &b, const Vector3d &v) {
+    double gamma = getG(v);
+    double s1 = gamma / ((1. + gamma) * (c * c));
+    double vb = dot3(v, b);
+    return Vector3d(b[0] / gamma + (s1 * v[0]) * vb, b[1] / gamma + (s1 * v[1]) * vb, b[2] / gamma + (s1 * v[2]) * vb);
+}
+
+/* ---- field values (the class's bf / nf / vf on the GPU) ---- */
+double ScalarBField::bf(const Vector3d &point) const {
+    jsx_jet_desc d = default_desc();
+    lower(d);
+    return one_point(d, point)[4];
+}
+Vector3d VectorBField::bf(const Vector3d &point) const {
+    jsx_jet_desc d = default_desc();
+    lower(d);
+    auto r = one_point(d, point);
+    return Vector3d(r[4], r[5], r[6]);
+}
+Vector3d VectorBField::bf_plasma_frame(const Vector3d &point, Vector3d &v) const {
+    Vector3d b = bf(point);
+    return in_plasma_frame_ ? b : getBprime(b, v);
+}
+double NField::nf(const Vector3d &point) const {
+    jsx_jet_desc d = default_desc();
+    lower(d);
+    return one_point(d, point)[11];
+}
+Vector3d VField::vf(const Vector3d &point) const {
+    jsx_jet_desc d = default_desc();
+    lower(d);
+    auto r = one_point(d, point);
+    return Vector3d(r[0], r[1], r[2]);
+}
+bool Geometry::is_within(Vector3d &point) const {
+    jsx_jet_desc d = default_desc();
+    lower(d);
+    return one_point(d, point)[29] != 0.0;
+}
+
+/* ---- geometry ---- */
+std::list<Intersection> BaseGeometry::hit(Ray &ray) const {
+    jsx_jet_desc d = default_desc();
     lower(d);
     double r[6] = {ray.origin()[0], ray.origin()[1], ray.origin()[2],
                    ray.direction()[0], ray.direction()[1], ray.direction()[2]};
@@ -102,6 +191,15 @@ void Cylinder::lower(jsx_jet_desc &d) const {
     d.z_clip_min = zmin_; d.z_clip_max = zmax_;
 }
 
+ParabaloidCone::ParabaloidCone(const Vector3d &origin, const Vector3d &direction, const double &r0, const double &z0,
+                               const double &scale)
+    : origin_(origin), direction_(direction), r0_(r0), z0_(z0), big_scale_(scale) {}
+void ParabaloidCone::lower(jsx_jet_desc &d) const {
+    d.geometry = JSX_GEOM_PARABOLOID_CONE;
+    set3(d.geo_origin, origin_); set3(d.geo_axis, direction_);
+    d.geo_param = r0_; d.geo_z0 = z0_; d.geo_big_scale = big_scale_;
+}
+
 void SimulationGeometry::lower(jsx_jet_desc &d) const {
     d.geometry = JSX_GEOM_CONVEX_HULL;
     d.geo_table = h_;
@@ -116,11 +214,24 @@ void ToroidalRBField::lower(jsx_jet_desc &d) const { d.bfield = JSX_B_TOROIDAL_R
 void HelicalRBField::lower(jsx_jet_desc &d) const {
     d.bfield = JSX_B_HELICAL_R; d.b0 = b_0_; d.b_exp = n_b_; d.b_pitch = pitch_; d.b_in_plasma_frame = in_plasma_frame_;
 }
+void CompositeRandomScalarBFieldZ::lower(jsx_jet_desc &d) const {
+    d.bfield = JSX_B_COMPOSITE_RANDOM_SCALAR_Z; d.b0 = b_0_; d.b_exp = m_in_; d.b_exp2 = m_out_; d.b_z0 = z0_;
+    d.b_in_plasma_frame = 1;
+}
+void ConstCylinderBField::lower(jsx_jet_desc &d) const { d.bfield = JSX_B_CONST_CYLINDER; d.b0 = b_0_; d.b_exp = n_b_; d.b_in_plasma_frame = in_plasma_frame_; }
+void RadialConicalBField::lower(jsx_jet_desc &d) const { d.bfield = JSX_B_RADIAL_CONICAL; d.b0 = b_0_; d.b_exp = n_b_; d.b_in_plasma_frame = in_plasma_frame_; }
+void HelicalCylinderBField::lower(jsx_jet_desc &d) const { d.bfield = JSX_B_HELICAL_CYLINDER; d.b0 = b_0_; d.b_pitch = pitch_; d.b_in_plasma_frame = in_plasma_frame_; }
+void SpiralConicalBField::lower(jsx_jet_desc &d) const { d.bfield = JSX_B_SPIRAL_CONICAL; d.b0 = b_0_; d.b_pitch = pitch_; d.b_in_plasma_frame = in_plasma_frame_; }
+void ForceFreeCylindricalBField::lower(jsx_jet_desc &d) const { d.bfield = JSX_B_FORCE_FREE_CYLINDRICAL; d.b0 = b_0_; d.b_mu = mu_; d.b_in_plasma_frame = in_plasma_frame_; }
 void SimulationBField::lower(jsx_jet_desc &d) const { d.bfield = JSX_B_GRID_VECTOR; d.grid_b = gp_; d.grid_b2 = gf_; d.b_in_plasma_frame = in_plasma_frame_; }
 void SimulationScalarBField::lower(jsx_jet_desc &d) const { d.bfield = JSX_B_GRID_SCALAR; d.grid_b = g_; d.b_in_plasma_frame = 1; }
 void ConstNField::lower(jsx_jet_desc &d) const { d.nfield = JSX_N_CONST; d.n0 = n_; d.n_in_plasma_frame = in_plasma_frame_; }
 void BKNField::lower(jsx_jet_desc &d) const { d.nfield = JSX_N_BK; d.n0 = n_0_; d.n_exp = n_n_; d.n_in_plasma_frame = in_plasma_frame_; }
 void BKNFieldZ::lower(jsx_jet_desc &d) const { d.nfield = JSX_N_BK_Z; d.n0 = n_0_; d.n_exp = n_n_; d.n_in_plasma_frame = in_plasma_frame_; }
+void CompositeBKNField::lower(jsx_jet_desc &d) const {
+    d.nfield = JSX_N_COMPOSITE_BK; d.n0 = n_0_; d.n_exp = n_in_; d.n_exp2 = n_out_; d.n_z0 = z0_;
+    d.n_in_plasma_frame = in_plasma_frame_;
+}
 void SimulationNField::lower(jsx_jet_desc &d) const { d.nfield = JSX_N_GRID; d.grid_n = g_; d.n_in_plasma_frame = in_plasma_frame_; }
 void ConstFlatVField::lower(jsx_jet_desc &d) const { d.vfield = JSX_V_CONST_FLAT; d.gamma0 = gamma_; }
 void ConstCentralVField::lower(jsx_jet_desc &d) const { d.vfield = JSX_V_CONST_CENTRAL; d.gamma0 = gamma_; set3(d.v_origin, origin_); }
@@ -128,6 +239,25 @@ void LinearGammaZVField::lower(jsx_jet_desc &d) const {
     d.vfield = JSX_V_LINEAR_GAMMA_Z; d.gamma0 = g0_; d.gamma1 = g1_; d.gamma_z0 = z0_; d.gamma_z1 = z1_;
 }
 void SimulationVField::lower(jsx_jet_desc &d) const { d.vfield = JSX_V_GRID; d.grid_v = g_; }
+void ShearedFlatVField::lower(jsx_jet_desc &d) const { d.vfield = JSX_V_SHEARED_FLAT; d.gamma0 = ga_; d.gamma1 = gb_; d.v_r = r_; }
+void SheathFlatVField::lower(jsx_jet_desc &d) const { d.vfield = JSX_V_SHEATH_FLAT; d.gamma0 = ga_; d.gamma1 = gb_; d.v_r = r_; }
+void ShearedCentralVField::lower(jsx_jet_desc &d) const {
+    d.vfield = JSX_V_SHEARED_CENTRAL; d.gamma0 = ga_; d.gamma1 = gb_; d.v_theta = theta_; set3(d.v_origin, origin_);
+}
+void SheathCentralVField::lower(jsx_jet_desc &d) const {
+    d.vfield = JSX_V_SHEATH_CENTRAL; d.gamma0 = ga_; d.gamma1 = gb_; d.v_theta = theta_; set3(d.v_origin, origin_);
+}
+void ConstParabolicVField::lower(jsx_jet_desc &d) const { d.vfield = JSX_V_CONST_PARABOLIC; d.gamma0 = g_; d.v_Rz0 = Rz0_; }
+void AccParabolicVField::lower(jsx_jet_desc &d) const { d.vfield = JSX_V_ACC_PARABOLIC; d.gamma0 = g_; d.v_R0 = R0_; d.v_Rz0 = Rz0_; }
+void ShearedAccParabolicVField::lower(jsx_jet_desc &d) const {
+    d.vfield = JSX_V_SHEARED_ACC_PARABOLIC; d.gamma0 = ga_; d.gamma1 = gb_; d.v_R0 = R0_; d.v_Rz0 = Rz0_;
+}
+void AccParabolicConstConeVField::lower(jsx_jet_desc &d) const {
+    d.vfield = JSX_V_ACC_PARABOLIC_CONST_CONE; d.gamma0 = g_; d.v_Rz0 = Rz0_; d.v_z0 = z0_;
+}
+void ShearedAccParabolicConstConeVField::lower(jsx_jet_desc &d) const {
+    d.vfield = JSX_V_SHEARED_ACC_PARABOLIC_CONST_CONE; d.gamma0 = ga_; d.gamma1 = gb_; d.v_Rz0 = Rz0_; d.v_z0 = z0_;
+}
 
 /* ---- Jet ---- */
 Jet::Jet(BaseGeometry *geo, VField *vfield, ScalarBField *bField, NField *nField)
@@ -136,6 +266,27 @@ Jet::Jet(BaseGeometry *geo, VField *vfield, VectorBField *bField, NField *nField
     : geometry_(geo), vfield_(vfield), vbfield_(bField), nfield_(nField) {}
 
 std::list<Intersection> Jet::hit(Ray &ray) { return geometry_->hit(ray); }
+
+/* Jet::getKI, getEtaI, getV and the restored getters: the lab-frame values
+ * the kernels compute at the point (layout of jsx_fields_host) */
+static double jet_value(const Jet &j, Vector3d &point, Vector3d &n_los, double nu, int k) {
+    return one_point(j.descriptor(), point, n_los, nu, j.flags)[k];
+}
+double Jet::getKI(Vector3d &p, Vector3d &n, double nu) { return jet_value(*this, p, n, nu, 18); }
+double Jet::getKQ(Vector3d &p, Vector3d &n, double nu) { return jet_value(*this, p, n, nu, 19); }
+double Jet::getKU(Vector3d &p, Vector3d &n, double nu) { return jet_value(*this, p, n, nu, 20); }
+double Jet::getKV(Vector3d &p, Vector3d &n, double nu) { return jet_value(*this, p, n, nu, 21); }
+double Jet::getKF(Vector3d &p, Vector3d &n, double nu) { return jet_value(*this, p, n, nu, 22); }
+double Jet::getKC(Vector3d &p, Vector3d &n, double nu) { return jet_value(*this, p, n, nu, 23); }
+double Jet::gethQ(Vector3d &p, Vector3d &n, double nu) { return jet_value(*this, p, n, nu, 24); }
+double Jet::getEtaI(Vector3d &p, Vector3d &n, double nu) { return jet_value(*this, p, n, nu, 25); }
+double Jet::getEtaQ(Vector3d &p, Vector3d &n, double nu) { return jet_value(*this, p, n, nu, 26); }
+double Jet::getEtaU(Vector3d &p, Vector3d &n, double nu) { return jet_value(*this, p, n, nu, 27); }
+double Jet::getEtaV(Vector3d &p, Vector3d &n, double nu) { return jet_value(*this, p, n, nu, 28); }
+const Vector3d Jet::getV(const Vector3d &point) {
+    auto r = one_point(descriptor(), point);
+    return Vector3d(r[0], r[1], r[2]);
+}
 
 jsx_jet_desc Jet::descriptor() const {
     jsx_jet_desc d;

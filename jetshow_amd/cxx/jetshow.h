@@ -2,6 +2,11 @@
  * jetshow.h -- the reference's C++ class API for the per-pixel path, backed by
  * the MI355X C ABI (include/jetshow_amd.h).
  *
+ * Field values at single points (bf, nf, vf, is_within, Jet::getKI / getEtaI
+ * / getV) are evaluated by the kernels' own field code through
+ * jsx_fields_host, one launch per call: they are for inspection, not for
+ * loops over many points (Context::fields takes arrays).
+ *
  * Class names, constructor arguments and method names follow the reference
  * (include/Geometry.h, Cone.h, Parabaloid.h, Cylinder.h, BField.h, NField.h,
  * VField.h, Jet.h, ImagePlane.h, Observation.h, Ray.h, Intersection.h) so code
@@ -98,6 +103,9 @@ class Geometry : public BaseGeometry {
 public:
     virtual const double big_scale() const = 0;
     virtual const Vector3d &origin() const = 0;
+    /* include/Geometry.h:50 (Cone.cpp:48-53, Parabaloid.cpp:35-37,
+     * Cylinder.cpp:29-32, ParabaloidCone.cpp:36-43), on the GPU */
+    virtual bool is_within(Vector3d &point) const;
 };
 
 class Cone : public Geometry {
@@ -145,6 +153,23 @@ private:
     double r_, zmin_ = 0, zmax_ = 0;
 };
 
+/* ParabaloidCone (include/ParabaloidCone.h, src/ParabaloidCone.cpp:12-94):
+ * the paraboloid r0 below z0, the cone tangent to it there above */
+class ParabaloidCone : public Geometry {
+public:
+    ParabaloidCone(const Vector3d &origin, const Vector3d &direction, const double &r0, const double &z0,
+                   const double &scale);
+    const Vector3d &origin() const override { return origin_; }
+    const Vector3d &direction() const { return direction_; }
+    double z0() const { return z0_; }
+    const double big_scale() const override { return big_scale_; }
+    void lower(jsx_jet_desc &d) const override;
+
+private:
+    Vector3d origin_, direction_;
+    double r0_, z0_, big_scale_;
+};
+
 /* ---- fields (include/BField.h, NField.h, VField.h) ---- */
 /* SimulationGeometry (include/Geometry.h:58-66, src/Geometry.cpp:92-132): the
  * convex hull of the simulation points, as face planes uploaded with
@@ -157,16 +182,23 @@ private:
     int32_t h_;
 };
 
+/* include/BField.h:49-53: a tangled field, given in the plasma frame */
 class ScalarBField {
 public:
     virtual ~ScalarBField() = default;
     virtual void lower(jsx_jet_desc &d) const = 0;
+    virtual double bf(const Vector3d &point) const;
+    double bf_plasma_frame(const Vector3d &point, Vector3d &v) const { (void)v; return bf(point); }
 };
+/* include/BField.h:36-44 */
 class VectorBField {
 public:
     explicit VectorBField(bool in_plasma_frame) : in_plasma_frame_(in_plasma_frame) {}
     virtual ~VectorBField() = default;
     virtual void lower(jsx_jet_desc &d) const = 0;
+    virtual Vector3d bf(const Vector3d &point) const;
+    /* BField.cpp:212-219: unchanged if given in the plasma frame, else getBprime(b, v) */
+    Vector3d bf_plasma_frame(const Vector3d &point, Vector3d &v) const;
 
 protected:
     bool in_plasma_frame_;
@@ -185,6 +217,56 @@ public:
     void lower(jsx_jet_desc &d) const override;
 private:
     double b_0_, m_b_;
+};
+/* CompositeRandomScalarBFieldZ (BField.h:76-86): exponent m_b_inner for
+ * |z| <= z0, m_b_outer beyond */
+class CompositeRandomScalarBFieldZ : public ScalarBField {
+public:
+    CompositeRandomScalarBFieldZ(double b_0, double m_b_inner, double m_b_outer, double z0)
+        : b_0_(b_0), m_in_(m_b_inner), m_out_(m_b_outer), z0_(z0) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    double b_0_, m_in_, m_out_, z0_;
+};
+class ConstCylinderBField : public VectorBField {
+public:
+    ConstCylinderBField(double b_0, double n_b, bool in_plasma_frame)
+        : VectorBField(in_plasma_frame), b_0_(b_0), n_b_(n_b) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    double b_0_, n_b_;
+};
+class RadialConicalBField : public VectorBField {
+public:
+    RadialConicalBField(double b_0, double n_b, bool in_plasma_frame)
+        : VectorBField(in_plasma_frame), b_0_(b_0), n_b_(n_b) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    double b_0_, n_b_;
+};
+class HelicalCylinderBField : public VectorBField {
+public:
+    HelicalCylinderBField(double b_0, double pitch_angle, bool in_plasma_frame)
+        : VectorBField(in_plasma_frame), b_0_(b_0), pitch_(pitch_angle) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    double b_0_, pitch_;
+};
+class SpiralConicalBField : public VectorBField {
+public:
+    SpiralConicalBField(double b_0, double pitch_angle, bool in_plasma_frame)
+        : VectorBField(in_plasma_frame), b_0_(b_0), pitch_(pitch_angle) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    double b_0_, pitch_;
+};
+class ForceFreeCylindricalBField : public VectorBField {
+public:
+    ForceFreeCylindricalBField(double b_0, double mu, bool in_plasma_frame)
+        : VectorBField(in_plasma_frame), b_0_(b_0), mu_(mu) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    double b_0_, mu_;
 };
 class ToroidalBField : public VectorBField {
 public:
@@ -234,11 +316,18 @@ private:
     int32_t g_;
 };
 
+/* include/NField.h:10-18 */
 class NField {
 public:
     explicit NField(bool in_plasma_frame) : in_plasma_frame_(in_plasma_frame) {}
     virtual ~NField() = default;
     virtual void lower(jsx_jet_desc &d) const = 0;
+    virtual double nf(const Vector3d &point) const;
+    /* NField.cpp:7-14 */
+    double nf_plasma_frame(const Vector3d &point, double &gamma) const {
+        double n = nf(point);
+        return in_plasma_frame_ ? n : n / gamma;
+    }
 protected:
     bool in_plasma_frame_;
 };
@@ -263,6 +352,15 @@ public:
 private:
     double n_0_, n_n_;
 };
+/* CompositeBKNField (NField.h:50-60): exponent n_n_inner for |z| <= z0, n_n_outer beyond */
+class CompositeBKNField : public NField {
+public:
+    CompositeBKNField(double n_0, double n_n_inner, double n_n_outer, double z0, bool in_plasma_frame)
+        : NField(in_plasma_frame), n_0_(n_0), n_in_(n_n_inner), n_out_(n_n_outer), z0_(z0) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    double n_0_, n_in_, n_out_, z0_;
+};
 class SimulationNField : public NField {
 public:
     SimulationNField(int32_t grid, bool in_plasma_frame) : NField(in_plasma_frame), g_(grid) {}
@@ -271,10 +369,12 @@ private:
     int32_t g_;
 };
 
+/* include/VField.h:11-14 */
 class VField {
 public:
     virtual ~VField() = default;
     virtual void lower(jsx_jet_desc &d) const = 0;
+    virtual Vector3d vf(const Vector3d &point) const;
 };
 class ConstFlatVField : public VField {
 public:
@@ -290,6 +390,81 @@ public:
 private:
     double gamma_;
     Vector3d origin_;
+};
+/* include/VField.h:27-176 */
+class ShearedFlatVField : public VField {
+public:
+    ShearedFlatVField(double gamma_axis, double gamma_border, double r) : ga_(gamma_axis), gb_(gamma_border), r_(r) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    double ga_, gb_, r_;
+};
+class SheathFlatVField : public VField {
+public:
+    SheathFlatVField(double gamma_spine, double gamma_sheath, double r_sheath)
+        : ga_(gamma_spine), gb_(gamma_sheath), r_(r_sheath) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    double ga_, gb_, r_;
+};
+/* the reference's theta = acos(z + o_z / r) (VField.cpp:83) is NaN off the
+ * origin; the polar angle acos((z + o_z) / r) is used unless the solver
+ * flag JSX_OPT_SHEARED_CENTRAL_REF_BUG asks for the reference's form */
+class ShearedCentralVField : public VField {
+public:
+    ShearedCentralVField(double gamma_axis, double gamma_border, double theta, Vector3d origin = Vector3d(0, 0, 0))
+        : ga_(gamma_axis), gb_(gamma_border), theta_(theta), origin_(origin) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    double ga_, gb_, theta_;
+    Vector3d origin_;
+};
+class SheathCentralVField : public VField {
+public:
+    SheathCentralVField(double gamma_spine, double gamma_sheath, double theta_sheath,
+                        Vector3d origin = Vector3d(0, 0, 0))
+        : ga_(gamma_spine), gb_(gamma_sheath), theta_(theta_sheath), origin_(origin) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    double ga_, gb_, theta_;
+    Vector3d origin_;
+};
+class ConstParabolicVField : public VField {
+public:
+    ConstParabolicVField(double gamma, double Rz0) : g_(gamma), Rz0_(Rz0) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    double g_, Rz0_;
+};
+class AccParabolicVField : public VField {
+public:
+    AccParabolicVField(double gamma0, double R0, double Rz0) : g_(gamma0), R0_(R0), Rz0_(Rz0) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    double g_, R0_, Rz0_;
+};
+class ShearedAccParabolicVField : public VField {
+public:
+    ShearedAccParabolicVField(double gamma_axis0, double gamma_border0, double R0, double Rz0)
+        : ga_(gamma_axis0), gb_(gamma_border0), R0_(R0), Rz0_(Rz0) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    double ga_, gb_, R0_, Rz0_;
+};
+class AccParabolicConstConeVField : public VField {
+public:
+    AccParabolicConstConeVField(double gamma0, double Rz0, double z0) : g_(gamma0), Rz0_(Rz0), z0_(z0) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    double g_, Rz0_, z0_;
+};
+class ShearedAccParabolicConstConeVField : public VField {
+public:
+    ShearedAccParabolicConstConeVField(double gamma_axis0, double gamma_border0, double Rz0, double z0)
+        : ga_(gamma_axis0), gb_(gamma_border0), Rz0_(Rz0), z0_(z0) {}
+    void lower(jsx_jet_desc &d) const override;
+private:
+    double ga_, gb_, Rz0_, z0_;
 };
 class LinearGammaZVField : public VField {
 public:
@@ -312,6 +487,25 @@ public:
     Jet(BaseGeometry *geo, VField *vfield, ScalarBField *bField, NField *nField);
     Jet(BaseGeometry *geo, VField *vfield, VectorBField *bField, NField *nField);
     std::list<Intersection> hit(Ray &ray);
+    /* include/Jet.h:21,38,50 (src/Jet.cpp:15-46,218-237,290-296): lab-frame
+     * k_I, eta_I and bulk velocity at a point; and the restored getters of
+     * the commented full-Stokes code (src/Jet.cpp:114-214,239-288) in the sky
+     * Stokes frame */
+    double getKI(Vector3d &point, Vector3d &n_los, double nu);
+    double getEtaI(Vector3d &point, Vector3d &n_los, double nu);
+    const Vector3d getV(const Vector3d &point);
+    double getKQ(Vector3d &point, Vector3d &n_los, double nu);
+    double getKU(Vector3d &point, Vector3d &n_los, double nu);
+    double getKV(Vector3d &point, Vector3d &n_los, double nu);
+    double getKF(Vector3d &point, Vector3d &n_los, double nu);
+    double getKC(Vector3d &point, Vector3d &n_los, double nu);
+    double gethQ(Vector3d &point, Vector3d &n_los, double nu);
+    double getEtaQ(Vector3d &point, Vector3d &n_los, double nu);
+    double getEtaU(Vector3d &point, Vector3d &n_los, double nu);
+    double getEtaV(Vector3d &point, Vector3d &n_los, double nu);
+    /* solver option bits used when the getters evaluate (e.g.
+     * JSX_OPT_SHEARED_CENTRAL_REF_BUG); Observation takes its own */
+    uint32_t flags = 0;
     /* emission parameters the reference hard-codes (s = 3.42, Jet.cpp:32) */
     void set_s(double s) { s_ = s; }
     void set_gamma_min(double g) { gamma_min_ = g; }
@@ -353,6 +547,9 @@ public:
     Context(const Context &) = delete;
     Context &operator=(const Context &) = delete;
     jsx_ctx *get() const { return ctx_; }
+    /* jsx_fields_host: JSX_FIELDS_STRIDE values per point */
+    std::vector<double> fields(const jsx_jet_desc &d, const std::vector<double> &points,
+                               const std::vector<double> &n_los, double nu, uint32_t flags = 0);
     int32_t upload_grid(const std::vector<double> &values, int nz, int nr, double z0, double dz, double r0,
                         double dr, double nan_value);
     /* planes: n_planes x (n_x, n_y, n_z, h) in cm, inside where n . p <= h */
@@ -391,6 +588,12 @@ private:
     Jet *jet;
     ImagePlane *imagePlane;
 };
+
+/* frame transforms of include/utils.h (src/utils.cpp:147-168) */
+double getG(const Vector3d &v);
+double getD(const Vector3d &n_los, const Vector3d &v);
+Vector3d get_n_los_prime(const Vector3d &n_los, const Vector3d &v);
+Vector3d getBprime(const Vector3d &b, const Vector3d &v);
 
 /* mas_to_pc (src/utils.cpp:205-208) */
 double mas_to_pc(double z);

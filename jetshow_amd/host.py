@@ -217,6 +217,23 @@ class Context:
         return ns, seg
 
 
+    FIELDS_STRIDE = 32
+
+    def fields(self, jet, points, n_los, nu, params=None):
+        """Field, frame and coefficient values at points [n, 3] (cm) with unit
+        LOS vectors n_los [n, 3]: [n, 32] in the layout of jsx_fields_host
+        (v, Gamma, bf, vec, bf_plasma_frame, nf, nf_plasma_frame, D, n_los',
+        nu', the 11 lab-frame coefficients, is_within)."""
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        nl = np.ascontiguousarray(n_los, dtype=np.float64).reshape(-1, 3)
+        if nl.shape != pts.shape:
+            raise ValueError("points and n_los must have the same shape")
+        out = np.zeros((pts.shape[0], self.FIELDS_STRIDE))
+        self._check(lib().jsx_fields_host(self._h, C.byref(jet), C.byref(params) if params is not None else None,
+                                          _ptr(pts), _ptr(nl), C.c_int64(pts.shape[0]), C.c_double(nu), _ptr(out)))
+        return out
+
+
 class Server:
     """Continuous-batching request server on a context (jsx_server_*).
 

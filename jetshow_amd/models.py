@@ -228,3 +228,93 @@ def grid_jet_model(handles, n_side=1024, pixel_pc=0.002, los=0.1, redshift=0.1, 
     p = abi.default_params(nu_ghz * 1e9 * (1.0 + redshift), stokes=stokes, precision=precision)
     p.stokes_abs_tol, p.stokes_rel_tol = 1e-20, 1e-8 if precision == abi.PREC_FP64 else 1e-9
     return j, im, p
+
+
+def reference_class_models(n_side=24, pixel_pc=0.05, los=0.1, redshift=0.1, nu_ghz=15.4,
+                           stokes=abi.STOKES_FULL, precision=abi.PREC_FP64):
+    """One small model per further reference class (the B, N, V fields and the
+    ParabaloidCone geometry of include/BField.h:56-164, NField.h:20-60,
+    VField.h:17-176, ParabaloidCone.h), each in a jet where it shapes the
+    image: a cone of half-angle 0.03 rad (apex 0, +z, clipped to 0..12 pc), a
+    paraboloid r = 0.1 pc (z/pc)^1/2, or a ParabaloidCone of r0 = 0.1 pc
+    switching to its tangent cone at z0 = 2 pc.  Unless the class under test
+    replaces them: helical B 0.1 G (|p|/pc)^-1 with pitch 45 deg given in the
+    lab frame, N = 500 (|p|/pc)^-2 in the plasma frame, ConstFlat Gamma = 5,
+    s = 2.5.  LOS 0.1 rad, uniform pixels, relative Stokes control (abs 1e-20,
+    rel 1e-8).  Returns a list of (name, JetDesc, ImageDesc, SolverParams)."""
+    def base(geometry="cone"):
+        j = _jet()
+        j.geo_big_scale = 100.0 * PC
+        if geometry == "cone":
+            j.geometry = abi.GEOM_CONE
+            j.geo_param = 0.03
+            j.z_clip_min, j.z_clip_max = 0.0, 12.0 * PC
+        elif geometry == "paraboloid":
+            j.geometry = abi.GEOM_PARABOLOID
+            j.geo_param = 0.1 * PC
+        else:
+            j.geometry = abi.GEOM_PARABOLOID_CONE
+            j.geo_param = 0.1 * PC
+            j.geo_z0 = 2.0 * PC
+        j.bfield = abi.B_HELICAL_R
+        j.b_in_plasma_frame = 0
+        j.b0, j.b_exp, j.b_pitch = 0.1, 1.0, math.pi / 4.0
+        j.nfield = abi.N_BK
+        j.n_in_plasma_frame = 1
+        j.n0, j.n_exp = 500.0, 2.0
+        j.vfield = abi.V_CONST_FLAT
+        j.gamma0 = 5.0
+        j.s = 2.5
+        return j
+
+    out = []
+
+    def add(name, j):
+        im = abi.ImageDesc()
+        im.n_across = im.n_along = n_side
+        im.lg_pixel_size_start_cm = im.lg_pixel_size_stop_cm = math.log10(pixel_pc * PC)
+        im.los_angle = los
+        p = abi.default_params(nu_ghz * 1e9 * (1.0 + redshift), stokes=stokes, precision=precision)
+        p.stokes_abs_tol, p.stokes_rel_tol = 1e-20, 1e-8
+        out.append((name, j, im, p))
+
+    j = base("paraboloid_cone")
+    add("ParabaloidCone", j)
+    j = base(); j.bfield = abi.B_CONST_CYLINDER; j.b0, j.b_exp = 0.1, 1.0
+    add("ConstCylinderBField", j)
+    j = base(); j.bfield = abi.B_RADIAL_CONICAL; j.b0, j.b_exp = 0.1, 1.0
+    add("RadialConicalBField", j)
+    j = base(); j.bfield = abi.B_HELICAL_CYLINDER; j.b0, j.b_pitch = 0.05, 0.5
+    add("HelicalCylinderBField", j)
+    j = base(); j.bfield = abi.B_SPIRAL_CONICAL; j.b0, j.b_pitch = 0.5, 0.3
+    add("SpiralConicalBField", j)
+    j = base(); j.bfield = abi.B_FORCE_FREE_CYLINDRICAL; j.b0, j.b_mu = 0.05, 1.5
+    add("ForceFreeCylindricalBField", j)
+    j = base(); j.bfield = abi.B_COMPOSITE_RANDOM_SCALAR_Z; j.b_in_plasma_frame = 1
+    j.b0, j.b_exp, j.b_exp2, j.b_z0 = 0.1, 1.0, 2.0, 3.0 * PC
+    add("CompositeRandomScalarBFieldZ", j)
+    j = base(); j.nfield = abi.N_COMPOSITE_BK; j.n_exp2, j.n_z0 = 3.0, 3.0 * PC
+    add("CompositeBKNField", j)
+    j = base(); j.vfield = abi.V_SHEARED_FLAT; j.gamma0, j.gamma1, j.v_r = 10.0, 3.0, 1.0 * PC
+    add("ShearedFlatVField", j)
+    j = base(); j.vfield = abi.V_SHEATH_FLAT; j.gamma0, j.gamma1, j.v_r = 10.0, 3.0, 0.1 * PC
+    add("SheathFlatVField", j)
+    j = base(); j.vfield = abi.V_SHEARED_CENTRAL; j.gamma0, j.gamma1, j.v_theta = 10.0, 3.0, 0.05
+    add("ShearedCentralVField", j)
+    j = base(); j.vfield = abi.V_SHEATH_CENTRAL; j.gamma0, j.gamma1, j.v_theta = 10.0, 3.0, 0.015
+    add("SheathCentralVField", j)
+    j = base("paraboloid"); j.vfield = abi.V_CONST_PARABOLIC; j.gamma0, j.v_Rz0 = 5.0, 0.1 * PC
+    add("ConstParabolicVField", j)
+    j = base("paraboloid"); j.vfield = abi.V_ACC_PARABOLIC
+    j.gamma0, j.v_R0, j.v_Rz0 = 10.0, 5.0 * PC, 0.1 * PC
+    add("AccParabolicVField", j)
+    j = base("paraboloid"); j.vfield = abi.V_SHEARED_ACC_PARABOLIC
+    j.gamma0, j.gamma1, j.v_R0, j.v_Rz0 = 10.0, 3.0, 5.0 * PC, 0.1 * PC
+    add("ShearedAccParabolicVField", j)
+    j = base("paraboloid_cone"); j.vfield = abi.V_ACC_PARABOLIC_CONST_CONE
+    j.gamma0, j.v_Rz0, j.v_z0 = 10.0, 0.1 * PC, 2.0 * PC
+    add("AccParabolicConstConeVField", j)
+    j = base("paraboloid_cone"); j.vfield = abi.V_SHEARED_ACC_PARABOLIC_CONST_CONE
+    j.gamma0, j.gamma1, j.v_Rz0, j.v_z0 = 10.0, 3.0, 0.1 * PC, 2.0 * PC
+    add("ShearedAccParabolicConstConeVField", j)
+    return out

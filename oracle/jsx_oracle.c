@@ -63,6 +63,12 @@ static void M_sincos_atan(const orc_math *m, double u, double *s, double *c) {
         *c = cos(fi);
     }
 }
+static double M_tan(const orc_math *m, double x) {
+    return m->mode == ORC_MATH_JSX ? jsx_tan(x) : tan(x);
+}
+static double M_acos(const orc_math *m, double x) {
+    return m->mode == ORC_MATH_JSX ? jsx_acos(x) : acos(x);
+}
 static double M_hypot(const orc_math *m, double x, double y) {
     return m->mode == ORC_MATH_JSX ? sqrt(x * x + y * y) : hypot(x, y);
 }
@@ -111,6 +117,12 @@ typedef struct {
     double s;
     double f_ki_v, f_ki_s, f_ei_v, f_ei_s, r_kq, r_eq, f_kv, f_ev, f_kFr, kCr_pre, kCr_gm;
     double e_ki, e_ei, e_kv, e_kCr;
+    /* further reference classes */
+    double origin[3];       /* Cone origin: geo_origin, or ParabaloidCone's cone (ParabaloidCone.cpp:14) */
+    double tan_pitch;       /* SpiralConicalBField tan(pitch) (BField.cpp:128-129)        */
+    double b_i0, b_i1;      /* ForceFreeCylindricalBField I0(mu), I1(mu) (BField.cpp:141-142) */
+    double v_origin[3];     /* central V fields' origin (the const-cone fields' cone part) */
+    double v_theta, v_R0;   /* sheared/sheath central angle; parabolic R0                 */
 } orc_model;
 
 static const orc_grid *grid_of(const orc_model *m, int h) {
@@ -156,6 +168,35 @@ static void model_init(orc_model *m, const jsx_jet_desc *j, const orc_grid *grid
     m->e_ei = (s - 1.) / 2.;
     m->e_kv = (s + 1.) / 2.;
     m->e_kCr = (s - 2.) / 2.;
+    for (int i = 0; i < 3; ++i) { m->origin[i] = j->geo_origin[i]; m->v_origin[i] = j->v_origin[i]; }
+    if (j->geometry == JSX_GEOM_PARABOLOID_CONE) {
+        /* ParabaloidCone.cpp:12-16: cone_(origin + (0, 0, -(r0 sqrt(z0/pc) / (0.5 r0 (z0 pc)^-0.5) - z0)),
+           direction, atan(0.5 r0 (z0 pc)^-0.5), big_scale) */
+        double r0 = j->geo_param, z0 = j->geo_z0;
+        double angle = atan(0.5 * r0 * pow(z0 * K_PC, -0.5));
+        m->origin[0] = j->geo_origin[0] + 0.0;
+        m->origin[1] = j->geo_origin[1] + 0.0;
+        m->origin[2] = j->geo_origin[2] + -(r0 * sqrt(z0 / K_PC) / (0.5 * r0 * pow(z0 * K_PC, -0.5)) - z0);
+        m->cos_angle = cos(angle);
+        m->cc = cos(angle) * cos(angle);
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c)
+                m->M[r][c] = m->axis[r] * m->axis[c] - m->cc * (r == c ? 1.0 : 0.0);
+    }
+    m->tan_pitch = tan(j->b_pitch);
+    if (j->bfield == JSX_B_FORCE_FREE_CYLINDRICAL) jsx_bessel_i01(j->b_mu, &m->b_i0, &m->b_i1);
+    m->v_theta = j->v_theta;
+    m->v_R0 = j->v_R0;
+    if (j->vfield == JSX_V_ACC_PARABOLIC_CONST_CONE || j->vfield == JSX_V_SHEARED_ACC_PARABOLIC_CONST_CONE) {
+        /* VField.cpp:222-251: parav(.., z0, Rz0) has R0 = z0; conev is central from
+           origin (0, 0, Rz0 sqrt(z0/pc) / (0.5 Rz0 (z0 pc)^-0.5) - z0) */
+        double Rz0 = j->v_Rz0, z0 = j->v_z0;
+        m->v_R0 = z0;
+        m->v_origin[0] = 0.0;
+        m->v_origin[1] = 0.0;
+        m->v_origin[2] = Rz0 * sqrt(z0 / K_PC) / (0.5 * Rz0 * pow(z0 * K_PC, -0.5)) - z0;
+        if (j->vfield == JSX_V_SHEARED_ACC_PARABOLIC_CONST_CONE) m->v_theta = atan(sqrt(Rz0 * Rz0 * z0 / K_PC) / z0);
+    }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -167,7 +208,7 @@ typedef struct { double pin[3], pout[3]; } orc_seg;
 /* Intersection(ray, geo) full infinite path (Intersection.cpp:41-48) */
 static void full_infinite(const orc_model *m, const double *o, const double *d, orc_seg *s) {
     double diff[3], ref[3];
-    sub3(o, m->j->geo_origin, diff);
+    sub3(o, m->origin, diff);
     double dd = dot3(diff, d);
     for (int i = 0; i < 3; ++i) ref[i] = diff[i] - dd * d[i];
     for (int i = 0; i < 3; ++i) {
@@ -179,7 +220,7 @@ static void full_infinite(const orc_model *m, const double *o, const double *d, 
 /* Cone::is_within (Cone.cpp:48-53) */
 static int cone_is_within(const orc_model *m, const double *p) {
     double diff[3];
-    sub3(p, m->j->geo_origin, diff);
+    sub3(p, m->origin, diff);
     normalize3(diff);
     double cosp = dot3(m->axis, diff);
     return cosp > m->cos_angle || cosp < -m->cos_angle;
@@ -206,7 +247,7 @@ static double quad_form(const double M[3][3], const double *d, const double *e) 
 /* Cone::hit (Cone.cpp:56-118) */
 static int cone_hit(const orc_model *m, const double *o, const double *d, orc_seg *segs) {
     double delta[3];
-    sub3(o, m->j->geo_origin, delta);
+    sub3(o, m->origin, delta);
     double c2 = quad_form(m->M, d, d);
     double c1 = quad_form(m->M, d, delta);
     double c0 = quad_form(m->M, delta, delta);
@@ -216,7 +257,7 @@ static int cone_hit(const orc_model *m, const double *o, const double *d, orc_se
         double t = -c1 / c2;
         double p[3], dp[3];
         ray_point(o, d, t, p);
-        sub3(p, m->j->geo_origin, dp);
+        sub3(p, m->origin, dp);
         /* isMuchSmallerThan(1e-6, 1e-6): squaredNorm <= (1e-6 * 1e-6)^2 */
         if (sqn3(dp) <= (1E-06 * 1E-06) * (1E-06 * 1E-06) && c1 == 0.0) {
             full_infinite(m, o, d, &segs[0]);
@@ -342,6 +383,63 @@ static int cylinder_hit(const orc_model *m, const double *o, const double *d, or
     return 0; /* c0 > 0 along the border, or the reference's unreturned branches */
 }
 
+/* ParabaloidCone::hit (ParabaloidCone.cpp:45-94).  Both parts are hit;
+ * without a paraboloid hit the cone's list stands when its first segment
+ * enters above z0 (:57-68); with one, the segment enters on the paraboloid
+ * unless that entry lies above z0 (then on the cone) and leaves on the
+ * paraboloid if that exit lies below z0 (else on the cone) (:71-92).  The
+ * reference reads the front of an empty cone list if the paraboloid is hit
+ * and the cone is not; the paraboloid lies inside the cone, so that happens
+ * only through rounding at a tangency, and the paraboloid's points stand. */
+static int paraboloid_cone_hit(const orc_model *m, const double *o, const double *d, orc_seg *segs) {
+    orc_seg cone[2], para[1];
+    int nc = cone_hit(m, o, d, cone);
+    int np = paraboloid_hit(m, o, d, para);
+    double z0 = m->j->geo_z0;
+    if (nc == 0 && np == 0) return 0;
+    if (np == 0) {
+        if (!(cone[0].pin[2] > z0)) return 0;
+        for (int k = 0; k < nc; ++k) segs[k] = cone[k];
+        return nc;
+    }
+    const orc_seg *c = nc > 0 ? &cone[0] : &para[0];
+    orc_seg out;
+    if (para[0].pin[2] > z0) memcpy(out.pin, c->pin, sizeof out.pin);
+    else memcpy(out.pin, para[0].pin, sizeof out.pin);
+    if (para[0].pout[2] < z0) memcpy(out.pout, para[0].pout, sizeof out.pout);
+    else memcpy(out.pout, c->pout, sizeof out.pout);
+    segs[0] = out;
+    return 1;
+}
+
+/* Geometry::is_within: Cone.cpp:48-53, Parabaloid.cpp:35-37 (hypot of x, y
+ * against r0 sqrt(z), units as written), Cylinder.cpp:29-32,
+ * ParabaloidCone.cpp:36-43; the convex hull: inside every face plane */
+static int geometry_is_within(const orc_model *m, const double *p) {
+    const jsx_jet_desc *j = m->j;
+    switch (j->geometry) {
+    case JSX_GEOM_CONE: return cone_is_within(m, p);
+    case JSX_GEOM_PARABOLOID: return M_hypot(&m->mth, p[0], p[1]) < j->geo_param * sqrt(p[2]);
+    case JSX_GEOM_CYLINDER: {
+        double w[3], cr[3];
+        sub3(p, j->geo_origin, w);
+        cross3(w, m->axis, cr);
+        return norm3(cr) < j->geo_param;
+    }
+    case JSX_GEOM_PARABOLOID_CONE:
+        if (p[2] < j->geo_z0) return M_hypot(&m->mth, p[0], p[1]) < j->geo_param * sqrt(p[2]);
+        return cone_is_within(m, p);
+    case JSX_GEOM_CONVEX_HULL: {
+        const orc_grid *g = grid_of(m, j->geo_table);
+        if (!g) return 0;
+        for (int i = 0; i < g->desc.nz; ++i)
+            if (dot3(g->values + 4 * i, p) > g->values[4 * i + 3]) return 0;
+        return 1;
+    }
+    default: return 0;
+    }
+}
+
 /* z clip of a segment to [zmin, zmax] (extension for finite-length jets) */
 static int clip_seg(const orc_model *m, orc_seg *s) {
     double zmin = m->j->z_clip_min, zmax = m->j->z_clip_max;
@@ -375,6 +473,7 @@ static int model_hit(const orc_model *m, const double *o, const double *d, orc_s
     case JSX_GEOM_PARABOLOID: n = paraboloid_hit(m, o, d, segs); break;
     case JSX_GEOM_CYLINDER: n = cylinder_hit(m, o, d, segs); break;
     case JSX_GEOM_CONVEX_HULL: n = hull_hit(m, o, d, segs); break;
+    case JSX_GEOM_PARABOLOID_CONE: n = paraboloid_cone_hit(m, o, d, segs); break;
     default: return 0;
     }
     if (m->j->z_clip_max > m->j->z_clip_min) {
@@ -447,6 +546,62 @@ double orc_grid_interp(const orc_grid *grid, const double *point_cm, int32_t mat
 /* ------------------------------------------------------------------------ */
 /* fields: src/BField.cpp, NField.cpp, VField.cpp                            */
 
+/* radial flow away from -v_origin, r = |p + v_origin| (VField.cpp:68-71) */
+static void central_v(const orc_model *m, const double *p, double r, double gamma, double *v) {
+    double v_r = K_C * sqrt(1. - 1. / (gamma * gamma));
+    v[0] = v_r * (p[0] + m->v_origin[0]) / r;
+    v[1] = v_r * (p[1] + m->v_origin[1]) / r;
+    v[2] = v_r * (p[2] + m->v_origin[2]) / r;
+}
+
+/* ShearedCentralVField::vf (VField.cpp:78-87).  The reference writes
+ * acos(z + origin_z / r), NaN for any |z| > 1 cm; that form is kept under
+ * JSX_OPT_SHEARED_CENTRAL_REF_BUG, the default takes acos((z + origin_z) / r). */
+static void sheared_central_v(const orc_model *m, const double *p, double *v) {
+    const jsx_jet_desc *j = m->j;
+    double q[3] = {p[0] + m->v_origin[0], p[1] + m->v_origin[1], p[2] + m->v_origin[2]};
+    double r = norm3(q);
+    double arg = (m->opt & JSX_OPT_SHEARED_CENTRAL_REF_BUG) ? p[2] + m->v_origin[2] / r : (p[2] + m->v_origin[2]) / r;
+    double theta = M_acos(&m->mth, arg);
+    double gamma = j->gamma0 - (j->gamma0 - j->gamma1) * theta / m->v_theta;
+    central_v(m, p, r, gamma, v);
+}
+
+/* flow along the paraboloid through p (VField.cpp:118-132): r = hypot(x, y),
+ * R = radius of the enclosing paraboloid at z, r0 = its radius at 1 pc */
+static void parabolic_v(const orc_model *m, const double *p, double r, double R, double gamma, double *v) {
+    double v_r = K_C * sqrt(1. - 1. / (gamma * gamma));
+    double r0 = m->j->v_Rz0 * r / R;
+    double sa, ca;
+    M_sincos_atan(&m->mth, 2 * K_PC * r / r0 / r0, &sa, &ca);
+    if (r == 0) {
+        v[0] = 0; v[1] = 0; v[2] = v_r;
+    } else {
+        v[0] = v_r * ca * p[0] / r;
+        v[1] = v_r * ca * p[1] / r;
+        v[2] = v_r * sa;
+    }
+}
+
+/* AccParabolicVField (VField.cpp:138-161), gamma = 1 + (gamma0 - 1) sqrt(z / R0);
+ * ShearedAccParabolicVField (VField.cpp:171-206), gamma0 sheared across the
+ * paraboloid as (r/R at z = R0)^(1/4) and clamped at 1 */
+static void acc_parabolic_v(const orc_model *m, const double *p, int sheared, double *v) {
+    const jsx_jet_desc *j = m->j;
+    double Rz0 = j->v_Rz0, R0 = m->v_R0;
+    double r = M_hypot(&m->mth, p[0], p[1]);
+    double R = sqrt(Rz0 * Rz0 * p[2] / K_PC);
+    double gamma0 = j->gamma0;
+    if (sheared) {
+        double R0_border = sqrt(Rz0 * Rz0 * R0 / K_PC);
+        double rR0 = R0_border * r / R;
+        gamma0 = j->gamma0 - (j->gamma0 - j->gamma1) * M_pow(&m->mth, rR0 / R0_border, 0.25);
+        if (gamma0 < 1) gamma0 = 1;
+    }
+    double gamma = 1 + (gamma0 - 1) / sqrt(R0) * sqrt(p[2]);
+    parabolic_v(m, p, r, R, gamma, v);
+}
+
 /* VField::vf */
 static void vfield(const orc_model *m, const double *p, double *v) {
     const jsx_jet_desc *j = m->j;
@@ -478,6 +633,53 @@ static void vfield(const orc_model *m, const double *p, double *v) {
         v[0] = 0; v[1] = 0; v[2] = K_C * sqrt(1. - 1. / (gam * gam));
         return;
     }
+    case JSX_V_SHEARED_FLAT: { /* ShearedFlatVField (VField.cpp:35-41) */
+        double r = sqrt(p[0] * p[0] + p[1] * p[1]);
+        double gamma = j->gamma0 - (j->gamma0 - j->gamma1) * r / j->v_r;
+        v[0] = 0; v[1] = 0; v[2] = K_C * sqrt(1. - 1. / (gamma * gamma));
+        return;
+    }
+    case JSX_V_SHEATH_FLAT: { /* SheathFlatVField (VField.cpp:47-58) */
+        double r = sqrt(p[0] * p[0] + p[1] * p[1]);
+        double gamma = r < j->v_r ? j->gamma0 : j->gamma1;
+        v[0] = 0; v[1] = 0; v[2] = K_C * sqrt(1. - 1. / (gamma * gamma));
+        return;
+    }
+    case JSX_V_SHEARED_CENTRAL:
+        sheared_central_v(m, p, v);
+        return;
+    case JSX_V_SHEATH_CENTRAL: { /* SheathCentralVField (VField.cpp:93-108): theta from z / r */
+        double q[3] = {p[0] + m->v_origin[0], p[1] + m->v_origin[1], p[2] + m->v_origin[2]};
+        double r = norm3(q);
+        double theta = M_acos(&m->mth, p[2] / r);
+        double gamma = theta < m->v_theta ? j->gamma0 : j->gamma1;
+        central_v(m, p, r, gamma, v);
+        return;
+    }
+    case JSX_V_CONST_PARABOLIC: { /* ConstParabolicVField (VField.cpp:114-133) */
+        double r = M_hypot(&m->mth, p[0], p[1]);
+        double R = sqrt(j->v_Rz0 * j->v_Rz0 * p[2] / K_PC);
+        parabolic_v(m, p, r, R, j->gamma0, v);
+        return;
+    }
+    case JSX_V_ACC_PARABOLIC:
+        acc_parabolic_v(m, p, 0, v);
+        return;
+    case JSX_V_SHEARED_ACC_PARABOLIC:
+        acc_parabolic_v(m, p, 1, v);
+        return;
+    case JSX_V_ACC_PARABOLIC_CONST_CONE: /* VField.cpp:231-238 */
+        if (p[2] > j->v_z0) {
+            double q[3] = {p[0] + m->v_origin[0], p[1] + m->v_origin[1], p[2] + m->v_origin[2]};
+            central_v(m, p, norm3(q), j->gamma0, v);
+        } else {
+            acc_parabolic_v(m, p, 0, v);
+        }
+        return;
+    case JSX_V_SHEARED_ACC_PARABOLIC_CONST_CONE: /* VField.cpp:259-266 */
+        if (p[2] > j->v_z0) sheared_central_v(m, p, v);
+        else acc_parabolic_v(m, p, 1, v);
+        return;
     default:
         v[0] = v[1] = v[2] = 0;
     }
@@ -509,12 +711,12 @@ static void getBprime(const double *b, const double *v, double *out) {
     for (int i = 0; i < 3; ++i) out[i] = b[i] / gamma + (s1 * v[i]) * vb;
 }
 
-/* B field in the plasma frame. Returns 1 for a vector field (b[3]), 0 for a
- * scalar (tangled) field (b[0]).  BField.cpp:35-146,196-224 */
-static int bfield_plasma(const orc_model *m, const double *p, const double *v, double *b) {
+/* the B field's bf(point): returns 1 for a vector field (bl[3]), 0 for a
+ * scalar (tangled) field (bl[0]).  BField.cpp:35-146,196-207 */
+static int bfield_lab(const orc_model *m, const double *p, double *bl) {
     const jsx_jet_desc *j = m->j;
     const orc_math *mth = &m->mth;
-    double bl[3];
+    double *b = bl;   /* a tangled field's magnitude goes to bl[0] */
     switch (j->bfield) {
     case JSX_B_RANDOM_SCALAR: { /* BField.cpp:38-41 */
         double r = norm3(p);
@@ -558,6 +760,46 @@ static int bfield_plasma(const orc_model *m, const double *p, const double *v, d
         bl[0] = 0.0; bl[1] = 0.0; bl[2] = j->b0 * M_pow(mth, r / K_PC, -j->b_exp);
         break;
     }
+    case JSX_B_COMPOSITE_RANDOM_SCALAR_Z: { /* CompositeRandomScalarBFieldZ (BField.cpp:56-64) */
+        double r = fabs(p[2]);
+        b[0] = j->b0 * M_pow(mth, r / K_PC, -(r > j->b_z0 ? j->b_exp2 : j->b_exp));
+        return 0;
+    }
+    case JSX_B_CONST_CYLINDER: { /* ConstCylinderBField (BField.cpp:70-73) */
+        double r = norm3(p);
+        bl[0] = 0.0; bl[1] = 0.0; bl[2] = j->b0 * M_pow(mth, r / K_PC, -j->b_exp);
+        break;
+    }
+    case JSX_B_RADIAL_CONICAL: { /* RadialConicalBField (BField.cpp:88-93) */
+        double r = norm3(p);
+        double bb = j->b0 * M_pow(mth, r / K_PC, -j->b_exp);
+        bl[0] = bb * p[0] / r; bl[1] = bb * p[1] / r; bl[2] = bb * p[2] / r;
+        break;
+    }
+    case JSX_B_HELICAL_CYLINDER: { /* HelicalCylinderBField (BField.cpp:112-117) */
+        double r = sqrt(p[0] * p[0] + p[1] * p[1]);
+        bl[0] = j->b0 * M_tan(mth, j->b_pitch * p[1] / r);
+        bl[1] = -j->b0 * M_tan(mth, j->b_pitch * p[0] / r);
+        bl[2] = j->b0;
+        break;
+    }
+    case JSX_B_SPIRAL_CONICAL: { /* SpiralConicalBField (BField.cpp:123-131) */
+        double x = p[0], y = p[1], z = p[2];
+        double b_z = j->b0 / (z * z / (K_PC * K_PC));
+        bl[0] = b_z * (x / z + y * m->tan_pitch / K_PC);
+        bl[1] = b_z * (y / z - x * m->tan_pitch / K_PC);
+        bl[2] = b_z;
+        break;
+    }
+    case JSX_B_FORCE_FREE_CYLINDRICAL: { /* ForceFreeCylindricalBField (BField.cpp:137-146):
+                                            both transverse components use sin(atan(y/x)) */
+        double sf, cf;
+        M_sincos_atan(mth, p[1] / p[0], &sf, &cf);
+        bl[0] = -j->b0 * m->b_i1 * sf;
+        bl[1] = j->b0 * m->b_i1 * sf;
+        bl[2] = j->b0 * m->b_i0;
+        break;
+    }
     case JSX_B_GRID_VECTOR: { /* SimulationBField (BField.cpp:199-207) */
         double sf, cf;
         M_sincos_atan(mth, p[1] / p[0], &sf, &cf);
@@ -572,8 +814,15 @@ static int bfield_plasma(const orc_model *m, const double *p, const double *v, d
         b[0] = 0;
         return 0;
     }
-    /* VectorBField::bf_plasma_frame (BField.cpp:212-219) */
-    if (j->b_in_plasma_frame) { b[0] = bl[0]; b[1] = bl[1]; b[2] = bl[2]; }
+    return 1;
+}
+
+/* bf_plasma_frame (BField.cpp:212-224): a vector field given in the lab frame
+ * goes through getBprime with the local v; a tangled one is unchanged */
+static int bfield_plasma(const orc_model *m, const double *p, const double *v, double *b) {
+    double bl[3];
+    if (!bfield_lab(m, p, bl)) { b[0] = bl[0]; return 0; }
+    if (m->j->b_in_plasma_frame) { b[0] = bl[0]; b[1] = bl[1]; b[2] = bl[2]; }
     else getBprime(bl, v, b);
     return 1;
 }
@@ -586,6 +835,11 @@ static double nfield(const orc_model *m, const double *p) {
     case JSX_N_BK: { double r = norm3(p); return j->n0 * M_pow(&m->mth, r / K_PC, -j->n_exp); }
     case JSX_N_BK_Z: { double z = fabs(p[2]); return j->n0 * M_pow(&m->mth, z / K_PC, -j->n_exp); }
     case JSX_N_GRID: { const orc_grid *g = grid_of(m, j->grid_n); return g ? grid_interp(g, p, &m->mth) : 0.0; }
+    case JSX_N_COMPOSITE_BK: { /* CompositeBKNField (NField.cpp:45-53): BKNField inner / outer by |z| */
+        double r = norm3(p);
+        double e = fabs(p[2]) > j->n_z0 ? j->n_exp2 : j->n_exp;
+        return j->n0 * M_pow(&m->mth, r / K_PC, -e);
+    }
     default: return 0;
     }
 }
@@ -1373,6 +1627,33 @@ int orc_fields(const jsx_jet_desc *jet, const orc_grid *grids, int32_t n_grids, 
     double o[14] = {f.v[0], f.v[1], f.v[2], f.gamma, f.b[0], f.vec ? f.b[1] : 0, f.vec ? f.b[2] : 0,
                     f.n, f.D, f.nl[0], f.nl[1], f.nl[2], f.nu, (double)f.vec};
     memcpy(out, o, sizeof o);
+    return 0;
+}
+
+/* field and geometry values at points in the layout of jsx_fields_host
+ * (JSX_FIELDS_STRIDE doubles per point) */
+int orc_point_values(const jsx_jet_desc *jet, const orc_grid *grids, int32_t n_grids, uint32_t opt, double nu,
+                     const double *points, const double *n_los, int64_t n, int32_t math_mode, double *out) {
+    orc_model m;
+    model_init(&m, jet, grids, n_grids, math_mode, opt);
+    for (int64_t q = 0; q < n; ++q) {
+        const double *p = points + 3 * q, *nl = n_los + 3 * q;
+        double *o = out + (int64_t)JSX_FIELDS_STRIDE * q;
+        memset(o, 0, sizeof(double) * JSX_FIELDS_STRIDE);
+        double v[3], bl[3] = {0, 0, 0};
+        vfield(&m, p, v);
+        int vec = bfield_lab(&m, p, bl);
+        orc_frame f;
+        jet_frame(&m, p, nl, nu, &f);
+        orc_coef c;
+        jet_coefficients(&m, p, nl, nu, 1, &c);
+        double row[30] = {v[0], v[1], v[2], getG(v), bl[0], vec ? bl[1] : 0, vec ? bl[2] : 0, (double)vec,
+                          f.b[0], vec ? f.b[1] : 0, vec ? f.b[2] : 0, nfield(&m, p), f.n,
+                          f.D, f.nl[0], f.nl[1], f.nl[2], f.nu,
+                          c.kI, c.kQ, c.kU, c.kV, c.kF, c.kC, c.hQ, c.eI, c.eQ, c.eU, c.eV,
+                          (double)geometry_is_within(&m, p)};
+        memcpy(o, row, sizeof row);
+    }
     return 0;
 }
 

@@ -81,6 +81,9 @@ int orc_sky_angle(const jsx_jet_desc *jet, const double *b, const double *n_los_
 
 /* Field values at a point: out = v[3], gamma, b'[3] (scalar fields fill b'[0]),
  * n' (plasma frame), D, n_los'[3], nu'. */
+/* values at points in the layout of jsx_fields_host (JSX_FIELDS_STRIDE per point) */
+int orc_point_values(const jsx_jet_desc *jet, const orc_grid *grids, int32_t n_grids, uint32_t opt, double nu,
+                     const double *points, const double *n_los, int64_t n, int32_t math_mode, double *out);
 int orc_fields(const jsx_jet_desc *jet, const orc_grid *grids, int32_t n_grids,
                double nu, const double *point, const double *n_los,
                int32_t math_mode, double *out);

@@ -173,6 +173,22 @@ def fields(jet, nu, point, n_los, math=MATH_JSX, grids=None):
     return out
 
 
+FIELDS_STRIDE = 32
+
+
+def point_values(jet, nu, points, n_los, opt=0, math=MATH_JSX, grids=None):
+    """Field, frame, coefficient and is_within values at points [n, 3] (cm)
+    with unit LOS vectors n_los [n, 3], in the layout of jsx_fields_host."""
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    nl = np.ascontiguousarray(n_los, dtype=np.float64).reshape(-1, 3)
+    n = pts.shape[0]
+    out = np.zeros((n, FIELDS_STRIDE))
+    garr, ng, keep = make_grids(grids or [])
+    lib().orc_point_values(C.byref(jet), garr, ng, C.c_uint32(opt), C.c_double(nu), _p(pts), _p(nl), C.c_int64(n),
+                           int(math), _p(out))
+    return out
+
+
 def grid_interp(desc, values, point_cm, math=MATH_JSX):
     garr, ng, keep = make_grids([(desc, values)])
     return lib().orc_grid_interp(C.byref(garr[0]), _p(np.asarray(point_cm, float)), int(math))

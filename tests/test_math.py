@@ -64,3 +64,38 @@ def test_sincos_atan():
     s_exact = np.where(big, np.sign(ul), ul / q)
     assert (np.abs(sc[:, 1] - c_exact) <= 5e-16 * c_exact).all()
     assert (np.abs(sc[:, 0] - s_exact) <= 5e-16 * np.abs(s_exact)).all()
+
+
+def test_sincos_tan():
+    rng = np.random.default_rng(4)
+    x = np.concatenate([rng.uniform(-10, 10, 100000), rng.uniform(-1e5, 1e5, 50000), [0.0, -0.0, 1e-300]])
+    sc = oracle.jsx_math(4, x)
+    # the Cody-Waite reduction with a 3-part pi/2 keeps both within 2 ulp of the
+    # magnitude of the result's scale (1) over |x| < 2^20
+    assert np.abs(sc[:, 0] - np.sin(x)).max() <= 2.3e-16
+    assert np.abs(sc[:, 1] - np.cos(x)).max() <= 2.3e-16
+    small = np.abs(x) < 1.0
+    assert (ulp_err(sc[small, 0][np.abs(x[small]) > 1e-30], np.sin(x[small][np.abs(x[small]) > 1e-30])) <= 2).all()
+    t = oracle.jsx_math(5, x[np.abs(x) < 1.5])
+    ref = np.tan(x[np.abs(x) < 1.5])
+    assert (np.abs(t - ref) <= 5e-16 * np.maximum(1.0, np.abs(ref))).all()
+    assert np.isnan(oracle.jsx_math(4, [np.nan])).all()
+
+
+def test_acos():
+    rng = np.random.default_rng(5)
+    u = np.concatenate([rng.uniform(-1, 1, 200000), [-1.0, 1.0, 0.0, 0.5, -0.5, 1 - 1e-16, -1 + 1e-16]])
+    got = oracle.jsx_math(6, u)
+    ref = np.arccos(u)
+    assert (np.abs(got - ref) <= 3 * np.spacing(np.maximum(ref, 1e-300)) + 1e-300).all()
+    assert np.isnan(oracle.jsx_math(6, [1.5, -1.0000001, np.nan])).all()
+
+
+def test_bessel_i0_i1():
+    from scipy import special
+    x = np.concatenate([np.linspace(0.0, 50.0, 5001), [1e-8, 1.5, 2.4048]])
+    got = oracle.jsx_math(7, x)
+    assert (np.abs(got[:, 0] - special.i0(x)) <= 1e-14 * special.i0(x)).all()
+    i1 = special.i1(x)
+    assert (np.abs(got[:, 1] - i1) <= 1e-14 * np.maximum(i1, 1e-300)).all()
+    assert got[0, 0] == 1.0 and got[0, 1] == 0.0
