@@ -430,6 +430,22 @@ int jsx_visibilities_device(jsx_ctx *ctx, const double *x, const double *y, cons
                             const double *c_im, int64_t n_src, const double *u, const double *v, int64_t n_vis,
                             int32_t isign, int32_t precision, double *vis, void *stream);
 
+/* The same without allocating: no reference counterpart.  For callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The chunk partial sums go to a
+ * fixed device array of JSX_VIS_SCRATCH_MAX double2 entries (64 MiB) that each
+ * GPU holds until the process ends; the call issues kernel launches only (no
+ * hipMalloc, hipFree or event).  A launch whose partial sums would exceed it
+ * (n_chunks x n_vis, n_chunks chosen as by jsx_visibilities_device) fails
+ * with JSX_EINVAL before anything runs.  Results equal
+ * jsx_visibilities_device's bit for bit.  Every call on one GPU shares the
+ * array: calls that overlap on one GPU (other streams, other contexts, other
+ * threads) race. */
+#define JSX_VIS_SCRATCH_MAX (4 * 1024 * 1024)
+int jsx_visibilities_device_noalloc(jsx_ctx *ctx, const double *x, const double *y, const double *c_re,
+                                    const double *c_im, int64_t n_src, const double *u, const double *v,
+                                    int64_t n_vis, int32_t isign, int32_t precision, double *vis, void *stream);
+
 /* ---- batched and served requests (no reference counterpart beyond
  *      get_i_image called in a loop by JetModelZoom, jetmodel.py:92-99) ----- */
 typedef struct jsx_request {

@@ -114,6 +114,7 @@ struct jsx_ctx {
     double2 *d_vis_part = nullptr;
     size_t vis_part_cap = 0;
     hipEvent_t vis_ev = nullptr;
+    double2 *vis_scratch = nullptr;   /* this device's address of the file-scope `scratch` */
     int vis_trig = 1;                   /* mixed visibilities: 0 polynomial sine/cosine, 1 hardware */
     /* optional event timing of the batched launches (option "timing") */
     int timing = 0;
@@ -122,6 +123,27 @@ struct jsx_ctx {
     int ev_n = 0;               /* launches recorded since timing was (re)enabled */
     static const int kEvMax = 64;
 };
+
+/* Partial sums of jsx_visibilities_device_noalloc: a file-scope device array
+ * (each GPU the library runs on holds its own copy until the process ends).
+ * In host code `scratch` names a host-side stand-in, not device memory: its
+ * device address is taken once per device with hipGetSymbolAddress (at
+ * context creation) and only that address is handed to kernels. */
+static __device__ __align__(256) double2 scratch[JSX_VIS_SCRATCH_MAX];
+static std::mutex g_scratch_mu;
+static double2 *g_scratch_ptr[64];      /* device address of `scratch`, per device id */
+
+/* with the device current: resolve (once) and return its address of `scratch` */
+static double2 *scratch_on_device(int device) {
+    if (device < 0 || device >= 64) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        g_scratch_ptr[device] = (double2 *)ptr;
+    }
+    return g_scratch_ptr[device];
+}
 
 static int fail(jsx_ctx *ctx, int code, const char *fmt, ...) {
     if (ctx) {
@@ -169,6 +191,11 @@ extern "C" int jsx_ctx_create(int device, size_t hbm_budget, jsx_ctx **out) {
         return JSX_ENOMEM;
     }
     for (int i = 0; i < jsx_ctx::kSlots; ++i) (void)hipEventCreateWithFlags(&ctx->slot_ev[i], hipEventDisableTiming);
+    ctx->vis_scratch = scratch_on_device(device);
+    if (!ctx->vis_scratch) {
+        jsx_ctx_destroy(ctx);
+        return JSX_EHIP;
+    }
     *out = ctx;
     return JSX_OK;
 }
@@ -1349,9 +1376,13 @@ static int vis_check(jsx_ctx *ctx, const double *x, const double *y, const doubl
     return JSX_OK;
 }
 
+/* fixed: NULL -- partial sums in the context's buffer (grown with hipMalloc,
+ * ordered across streams by an event); else the caller's fixed buffer of
+ * fixed_cap entries, used as is (no allocation, no event) */
 static int vis_launch(jsx_ctx *ctx, const double *x, const double *y, const double *cre, const double *cim,
                       int64_t n_src, const double *u, const double *v, int64_t n_vis, int32_t isign,
-                      int32_t precision, double *vis, hipStream_t st) {
+                      int32_t precision, double *vis, hipStream_t st, double2 *fixed = nullptr,
+                      size_t fixed_cap = 0) {
     if (n_vis == 0) return JSX_OK;
     double2 *out = (double2 *)vis;
     if (n_src == 0) {
@@ -1370,7 +1401,15 @@ static int vis_launch(jsx_ctx *ctx, const double *x, const double *y, const doub
     const int64_t chunk = (n_src + n_chunks - 1) / n_chunks;
     n_chunks = (n_src + chunk - 1) / chunk;
     double2 *dst = out;
-    if (n_chunks > 1) {
+    if (n_chunks > 1 && fixed) {
+        const size_t need = (size_t)n_chunks * (size_t)n_vis;
+        if (need > fixed_cap)
+            return fail(ctx, JSX_EINVAL,
+                        "%lld visibilities of %lld pixels need %zu partial sums; the no-allocation entry point "
+                        "holds at most %zu (use jsx_visibilities_device)",
+                        (long long)n_vis, (long long)n_src, need, fixed_cap);
+        dst = fixed;
+    } else if (n_chunks > 1) {
         if (!ctx->vis_ev) HIPCHECK(ctx, hipEventCreateWithFlags(&ctx->vis_ev, hipEventDisableTiming));
         const size_t need = (size_t)n_chunks * (size_t)n_vis;
         if (need > ctx->vis_part_cap) {
@@ -1409,9 +1448,21 @@ static int vis_launch(jsx_ctx *ctx, const double *x, const double *y, const doub
         hipLaunchKernelGGL(jsx_k_vis_reduce, dim3((unsigned)((n_vis + 255) / 256)), dim3(256), 0, st, dst, out, n_vis,
                            (int)n_chunks);
         HIPCHECK(ctx, hipGetLastError());
-        HIPCHECK(ctx, hipEventRecord(ctx->vis_ev, st));
+        if (!fixed) HIPCHECK(ctx, hipEventRecord(ctx->vis_ev, st));
     }
     return JSX_OK;
+}
+
+extern "C" int jsx_visibilities_device_noalloc(jsx_ctx *ctx, const double *x, const double *y, const double *c_re,
+                                               const double *c_im, int64_t n_src, const double *u, const double *v,
+                                               int64_t n_vis, int32_t isign, int32_t precision, double *vis,
+                                               void *stream) {
+    int rc = vis_check(ctx, x, y, c_re, n_src, u, v, n_vis, precision, vis);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    return vis_launch(ctx, x, y, c_re, c_im, n_src, u, v, n_vis, isign, precision, vis,
+                      stream ? (hipStream_t)stream : ctx->stream, ctx->vis_scratch, (size_t)JSX_VIS_SCRATCH_MAX);
 }
 
 extern "C" int jsx_visibilities_device(jsx_ctx *ctx, const double *x, const double *y, const double *c_re,

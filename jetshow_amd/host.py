@@ -198,6 +198,16 @@ class Context:
                                                   vp(u), vp(v), C.c_int64(n_vis), C.c_int32(isign),
                                                   C.c_int32(precision), vp(vis), vp(stream)))
 
+    def visibilities_device_noalloc(self, x, y, c_re, c_im, n_src, u, v, n_vis, vis, isign=-1,
+                                    precision=abi.PREC_FP64, stream=None):
+        """jsx_visibilities_device_noalloc: the same as visibilities_device,
+        with the partial sums in the library's fixed per-device array (no
+        device allocation; calls overlapping on one GPU race)."""
+        vp = lambda a: C.c_void_p(a) if a else None
+        self._check(lib().jsx_visibilities_device_noalloc(
+            self._h, vp(x), vp(y), vp(c_re), vp(c_im), C.c_int64(n_src), vp(u), vp(v), C.c_int64(n_vis),
+            C.c_int32(isign), C.c_int32(precision), vp(vis), vp(stream)))
+
     def tau_stripe(self, jet, image, params):
         """Observation::run_stripe + getStripe("tau"): optical depth of the
         middle row's pixels n_along/2 .. n_along-1 (fixed-step RK4, params.n

@@ -75,6 +75,10 @@ struct double2 { double x, y; };
 static inline double2 make_double2(double a, double b) { return double2{a, b}; }
 
 static inline hipError_t hipMalloc(void *p, size_t n) { *(void **)p = calloc(1, n ? n : 1); return hipSuccess; }
+/* file-scope device arrays: ordinary host arrays here, their own address */
+#define __align__(n) __attribute__((aligned(n)))
+#define HIP_SYMBOL(x) (&(x))
+static inline hipError_t hipGetSymbolAddress(void **p, const void *sym) { *p = (void *)sym; return hipSuccess; }
 static inline hipError_t hipHostMalloc(void *p, size_t n, int) { return hipMalloc(p, n); }
 static inline hipError_t hipFree(void *p) { free(p); return hipSuccess; }
 static inline hipError_t hipHostFree(void *p) { free(p); return hipSuccess; }
