@@ -950,8 +950,19 @@ __global__ __launch_bounds__(256) void jsx_k_split_piece20(const DParamsM *__res
 }
 
 /* ---- 5c. pieces by exponential propagators (one lane per piece) ---- */
+/* A piece's propagator and particular solution (20 doubles) live outside
+ * the registers: in the kernel, in LDS, component-major across the 256 lanes
+ * of the workgroup (U[i] of lane l at [i][l]: consecutive lanes, consecutive
+ * addresses).  Held in registers they kept the step's coefficient and
+ * exponential code at one wavefront per SIMD (256 VGPRs + ~70 AGPRs); the
+ * step reads them only to form the error scale and to apply the update. */
+#define JSX_MG_USTRIDE 256
+struct MgU {
+    double *p;                    /* component 0 of this lane's 20 */
+    __device__ double &operator[](int i) const { return p[i * JSX_MG_USTRIDE]; }
+};
 struct MgLane {
-    double U[20];                 /* Phi column-major (U[4 j + i]), then P (U[16 + i]) */
+    MgU U;                        /* Phi column-major (U[4 j + i]), then P (U[16 + i]) */
     double t, tb, h, p_floor, rel_min;
     double o[3];                  /* origin of the piece's pass [cm] */
     uint32_t flags, n_rhs, n_steps, tries;
@@ -996,19 +1007,18 @@ __device__ inline void mg_lane_step(MgLane &L, const DModel &m, const DImage &im
     L.n_rhs += 2;
     const double rel = c.st_rel > L.rel_min ? c.st_rel : L.rel_min;
     double E[16], w[4], err;
-    const int ok = mg_step(A0, A1, h, rel, L.p_floor, L.U + 16, E, w, &err, L.est);
+    const double P[4] = {L.U[16], L.U[17], L.U[18], L.U[19]};
+    const int ok = mg_step(A0, A1, h, rel, L.p_floor, P, E, w, &err, L.est);
     if (ok && err <= 1.0) {
-        double nU[20];
 #pragma unroll
-        for (int j = 0; j < 5; ++j)
+        for (int j = 0; j < 5; ++j) {
+            const double u[4] = {L.U[4 * j], L.U[4 * j + 1], L.U[4 * j + 2], L.U[4 * j + 3]};
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const double *u = L.U + 4 * j;
                 double acc = E[i] * u[0] + E[4 + i] * u[1] + E[8 + i] * u[2] + E[12 + i] * u[3];
-                nU[4 * j + i] = j == 4 ? acc + w[i] : acc;
+                L.U[4 * j + i] = j == 4 ? acc + w[i] : acc;
             }
-#pragma unroll
-        for (int i = 0; i < 20; ++i) L.U[i] = nU[i];
+        }
         L.t = last ? L.tb : L.t + h;
         L.n_steps++;
         L.tries = 0;
@@ -1077,14 +1087,17 @@ __device__ inline int mg_spawn_parts(const MgLane &L, unsigned budget) {
     return m < (double)JSX_MG_SPAWN ? (int)m : JSX_MG_SPAWN;
 }
 
-__global__ __launch_bounds__(256) void jsx_k_split_magnus(const DParamsM *__restrict__ reqs, SplitWS ws,
-                                                          unsigned budget) {
+/* two wavefronts per SIMD (at most 256 registers per lane) */
+__global__ __launch_bounds__(256, 2) void jsx_k_split_magnus(const DParamsM *__restrict__ reqs, SplitWS ws,
+                                                             unsigned budget) {
     const int lane = __lane_id();
     const unsigned npieces = *ws.round_end;
     unsigned nrays = *ws.n_long;
     unsigned *head = ws.mg_head;
     if (nrays > (unsigned)ws.cap) nrays = ws.cap;
     MgLane L;
+    __shared__ double sU[20 * JSX_MG_USTRIDE];
+    L.U.p = &sU[threadIdx.x];
     int have = 0, exhausted = 0, slot = 0, pid = 0, spawn_ok = 0;
     unsigned long long rhs_acc = 0;
     for (;;) {

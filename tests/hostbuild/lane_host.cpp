@@ -126,6 +126,8 @@ void run_piece(const DParamsM &q, const LongRay &R, const Piece &pc, PieceOut &p
 /* one piece by exponential propagators (the GPU's jsx_k_split_magnus lane) */
 void run_piece_magnus(const DParamsM &q, const LongRay &R, const Piece &pc, PieceOut &po) {
     MgLane L;
+    static double ustore[20 * JSX_MG_USTRIDE];
+    L.U.p = ustore;
     mg_lane_begin(L, R, pc, JSX_SPLIT_REL_MIN);
     while (!L.done) mg_lane_step(L, q.m, q.im, q.c, q.S);
     for (int i = 0; i < 4; ++i) po.P[i] = L.U[16 + i];
@@ -145,6 +147,8 @@ void run_pieces_magnus_rounds(const DParamsM &q, const LongRay &R, SplitWS &ws, 
         for (unsigned id = head; id < end; ++id) {
             if (ws.pieces[id].ray < 0) continue;
             MgLane L;
+            static double ustore[20 * JSX_MG_USTRIDE];
+            L.U.p = ustore;
             mg_lane_begin(L, R, ws.pieces[id], JSX_SPLIT_REL_MIN);
             int spawn_ok = b != 0u;
             while (!L.done) {
