@@ -1,7 +1,7 @@
 """Image sizing from a tau stripe (the step before the full run).
 
-`analyze_tau_stripe` restates the decision rule of example_simple.py:105-137
-on an optical-depth stripe (Observation::run_stripe, here on the GPU through
+`analyze_tau_stripe` applies the decision rule of example_simple.py:105-135
+to an optical-depth stripe (Observation::run_stripe, here on the GPU through
 jsx_tau_stripe), and `find_image_params` the iteration of
 example_simple.py:172-237 that grows or shrinks the pixel size and the number
 of pixels until the tau = 1 surface and the optically thin end sit where the
@@ -17,31 +17,35 @@ from . import abi
 
 def analyze_tau_stripe(tau, log10_tau_small=-4.0, border_tau1=(0.001, 0.3), border_tau_min=(0.5, 1),
                        min_pixels_to_tau1=20):
-    """Decision flags for a tau stripe (example_simple.py:105-137)."""
-    tau = np.asarray(tau, dtype=np.float64)
-    length = len(tau)
+    """Pixel size and count decisions from a tau stripe (the rule of
+    example_simple.py:105-135; same arguments and returned keys).
+
+    The stripe runs from the image centre outwards along the jet.  Two
+    markers locate the jet on it: m1, the pixel whose log10 tau is nearest 0
+    (the tau = 1 surface), and ms, the pixel nearest log10_tau_small (where
+    the jet has turned thin).  With n pixels and f1 = m1 / n:
+
+    * the pixels are too coarse when f1 <= border_tau1[0] or when fewer than
+      min_pixels_to_tau1 pixels lead up to the tau = 1 surface, and too fine
+      when f1 lies beyond both borders of border_tau1;
+    * the image is too short when ms reaches n border_tau_min[1] - 1 (the
+      thin end is not on it), too long when ms falls below
+      n border_tau_min[0].
+    """
+    n = len(tau)
     with np.errstate(divide="ignore"):
-        lt = np.log10(tau)
-    idx_tau1 = int(np.argmin(np.abs(lt - 0.0)))
-    idx_tau_ = int(np.argmin(np.abs(lt - log10_tau_small)))
-    increase_pixel_size = decrease_pixels_size = False
-    increase_number_of_pixels = decrease_number_of_pixels = False
-    if float(idx_tau1) / length > border_tau1[0]:
-        if float(idx_tau1) / length > border_tau1[1]:
-            increase_pixel_size = True
-    else:
-        decrease_pixels_size = True
-    if idx_tau_ < length * border_tau_min[1] - 1:
-        if idx_tau_ < border_tau_min[0] * length:
-            decrease_number_of_pixels = True
-    else:
-        increase_number_of_pixels = True
-    if idx_tau1 < min_pixels_to_tau1:
-        decrease_pixels_size = True
-    return {"increase_pixel_size": increase_pixel_size,
-            "decrease_pixels_size": decrease_pixels_size,
-            "increase_number_of_pixels": increase_number_of_pixels,
-            "decrease_number_of_pixels": decrease_number_of_pixels}
+        lg = np.log10(np.asarray(tau, dtype=np.float64))
+    m1 = int(np.argmin(np.abs(lg)))
+    ms = int(np.argmin(np.abs(lg - log10_tau_small)))
+    f1 = float(m1) / n
+    too_coarse = not f1 > border_tau1[0] or m1 < min_pixels_to_tau1
+    too_fine = f1 > border_tau1[0] and f1 > border_tau1[1]
+    too_short = not ms < n * border_tau_min[1] - 1
+    too_long = not too_short and ms < border_tau_min[0] * n
+    return {"increase_pixel_size": too_fine,
+            "decrease_pixels_size": too_coarse,
+            "increase_number_of_pixels": too_short,
+            "decrease_number_of_pixels": too_long}
 
 
 def find_image_params(ctx, jet, params, n_pixels, pixel_size_cm, los_angle, max_iter=40, stripe=None):

@@ -109,6 +109,32 @@ def test_analyze_tau_stripe_rule():
     assert d["increase_number_of_pixels"]
 
 
+def _marked(n, m1, ms):
+    """a stripe whose tau = 1 marker sits at m1 and tau = 1e-4 marker at ms"""
+    lg = np.full(n, 50.0)
+    lg[m1] = 0.0
+    lg[ms] = -4.0
+    return 10.0 ** lg
+
+
+@pytest.mark.parametrize("m1, coarse, fine", [(1, True, False), (2, True, False), (19, True, False),
+                                              (20, False, False), (300, False, False), (301, False, True)])
+def test_analyze_tau_stripe_pixel_size_borders(m1, coarse, fine):
+    """n = 1000: the tau = 1 marker must lie beyond 0.1% of the stripe and at
+    least 20 pixels out (else the pixels are too coarse) and within 30% (else
+    too fine); the borders belong to the accepted side"""
+    d = stripe.analyze_tau_stripe(_marked(1000, m1, 700))
+    assert d["decrease_pixels_size"] == coarse and d["increase_pixel_size"] == fine
+
+
+@pytest.mark.parametrize("ms, short, long_", [(499, False, True), (500, False, False), (998, False, False),
+                                              (999, True, False)])
+def test_analyze_tau_stripe_pixel_count_borders(ms, short, long_):
+    """n = 1000: the thin-end marker must lie in [50%, n - 1) of the stripe"""
+    d = stripe.analyze_tau_stripe(_marked(1000, 100, ms))
+    assert d["increase_number_of_pixels"] == short and d["decrease_number_of_pixels"] == long_
+
+
 @pytest.mark.gpu
 def test_stripe_gpu_matches_oracle(ctx):
     from tests import workloads as W
