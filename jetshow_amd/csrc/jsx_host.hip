@@ -40,6 +40,22 @@ struct ImageRec {
     size_t bytes = 0;
 };
 
+/* what the batched launch's kernel sequence depends on (compared bytewise) */
+struct BatchGraphKey {
+    int precision, stokes, order, nreq;
+    int64_t total;
+    int blocks, tri_blocks;
+    unsigned park_lanes;
+    const void *reqs, *off, *outs, *queues, *live, *sorted, *bucket, *sbucket;
+    int64_t queue_block;
+    int steal;
+    unsigned defer_rhs;
+    int park_wave, park_bucket, piece_layout;
+    unsigned piece_budget;
+    int n_cu, split;
+    SplitWS ws;
+};
+
 struct jsx_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -118,6 +134,12 @@ struct jsx_ctx {
     int vis_trig = 1;                   /* mixed visibilities: 0 polynomial sine/cosine, 1 hardware */
     /* optional event timing of the batched launches (option "timing") */
     int timing = 0;
+    /* the batched launch as a HIP graph (option "graph", default on) */
+    int graph = 1;
+    hipGraphExec_t bg_exec = nullptr;
+    BatchGraphKey bg_key;
+    bool bg_seen = false;
+    int64_t graph_captures = 0, graph_launches = 0;
     struct EvSet { hipEvent_t e[7]; };
     std::vector<EvSet> evs;     /* created on first use, reused */
     int ev_n = 0;               /* launches recorded since timing was (re)enabled */
@@ -218,6 +240,10 @@ extern "C" int jsx_ctx_destroy(jsx_ctx *ctx) {
         (void)hipHostFree(ctx->h_stage[k]);
         if (ctx->stage_ev[k]) (void)hipEventDestroy(ctx->stage_ev[k]);
     }
+    if (ctx->bg_exec) {
+        if (ctx->batch_ev) (void)hipEventSynchronize(ctx->batch_ev);
+        (void)hipGraphExecDestroy(ctx->bg_exec);
+    }
     if (ctx->batch_ev) (void)hipEventDestroy(ctx->batch_ev);
     (void)hipFree(ctx->d_live); (void)hipFree(ctx->d_sorted); (void)hipFree(ctx->d_bucket);
     (void)hipFree(ctx->d_sbucket);
@@ -305,6 +331,8 @@ extern "C" int jsx_ctx_set_option(jsx_ctx *ctx, const char *name, double v) {
     } else if (k == "timing") {
         ctx->timing = v != 0.0;
         ctx->ev_n = 0;
+    } else if (k == "graph") {
+        ctx->graph = v != 0.0;
     } else {
         return fail(ctx, JSX_EINVAL, "unknown option '%s'", name);
     }
@@ -333,6 +361,9 @@ extern "C" int jsx_ctx_get_option(jsx_ctx *ctx, const char *name, double *v) {
     else if (k == "mg_rel") *v = ctx->mg_rel;
     else if (k == "mg_err") *v = ctx->mg_err;
     else if (k == "park_lanes") *v = ctx->park_lanes;
+    else if (k == "graph") *v = ctx->graph;
+    else if (k == "stat_graph_captures") *v = (double)ctx->graph_captures;
+    else if (k == "stat_graph_launches") *v = (double)ctx->graph_launches;
     else if (k == "park_wave") *v = ctx->park_wave;
     else if (k == "stat_parked" || k == "stat_pieces") {
         unsigned c[2] = {0, 0};
@@ -1006,7 +1037,6 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
     HIPCHECK(ctx, hipMemcpyAsync(ctx->d_off, off, sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, st));
     HIPCHECK(ctx, hipMemcpyAsync(ctx->d_outs, outs, sizeof(DOut) * n, hipMemcpyHostToDevice, st));
     HIPCHECK(ctx, hipEventRecord(ctx->stage_ev[sk], st));
-    hipLaunchKernelGGL(jsx_k_reset_queues, dim3(1), dim3(64), 0, st, ctx->d_queues);
     int blocks = ctx->n_cu * ctx->persist_blocks_per_cu;
     int64_t total = off[n];   /* == total_pix */
     int nreq = (int)n;
@@ -1027,7 +1057,6 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
         int rc = split_workspace(ctx);
         if (rc) return rc;
         ws = ctx->split;
-        hipLaunchKernelGGL(jsx_k_split_reset, dim3(1), dim3(64), 0, st, ws);
     }
     /* triage list: one int64 per ray of the launch */
     int64_t *live = nullptr, *sorted = nullptr;
@@ -1060,6 +1089,13 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
      * wavefront per SIMD left the long rays of a many-image sweep running
      * sequentially for longer: 11% slower on 512 images, 2% on one) */
     const unsigned park_lanes = ctx->park_lanes ? ctx->park_lanes : (unsigned)ctx->n_cu * 4u * 64u;
+    /* the launch sequence: it depends on the request table only through
+       device buffers, so while the key below is unchanged it is replayed as
+       one HIP graph captured at its first use (no memset nodes: the queues
+       and the split counters are reset by kernels) */
+    auto enqueue = [&]() {
+    hipLaunchKernelGGL(jsx_k_reset_queues, dim3(1), dim3(64), 0, st, ctx->d_queues);
+    if (split) hipLaunchKernelGGL(jsx_k_split_reset, dim3(1), dim3(64), 0, st, ws);
     mark(5);
     if (precision == JSX_PREC_MIXED) {
         const DParamsM *R = (const DParamsM *)ctx->d_reqs;
@@ -1149,6 +1185,53 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
         mark(6);
         mark(3);
         mark(4);
+    }
+    };
+    BatchGraphKey key;
+    std::memset(&key, 0, sizeof key);
+    key.precision = precision; key.stokes = stokes; key.order = order; key.nreq = nreq; key.total = total;
+    key.blocks = blocks; key.tri_blocks = tri_blocks; key.park_lanes = park_lanes;
+    key.reqs = ctx->d_reqs; key.off = ctx->d_off; key.outs = ctx->d_outs; key.queues = ctx->d_queues;
+    key.live = live; key.sorted = sorted; key.bucket = bucket; key.sbucket = ctx->d_sbucket;
+    key.queue_block = ctx->queue_block; key.steal = ctx->steal; key.defer_rhs = ctx->defer_rhs;
+    key.park_wave = ctx->park_wave; key.park_bucket = ctx->park_bucket; key.piece_layout = ctx->piece_layout;
+    key.piece_budget = ctx->piece_budget; key.n_cu = ctx->n_cu; key.split = split;
+    if (split) key.ws = ws;
+    hipStreamCaptureStatus cap_st = hipStreamCaptureStatusNone;
+    const bool use_graph = ctx->graph && !ev && hipStreamIsCapturing(st, &cap_st) == hipSuccess &&
+                           cap_st == hipStreamCaptureStatusNone;
+    /* a shape seen for the first time runs as it is (a server's launches
+       change shape all the time); the second launch of a shape in a row is
+       captured, later ones replay the graph */
+    const bool same = ctx->bg_seen && std::memcmp(&key, &ctx->bg_key, sizeof key) == 0;
+    if (!same && ctx->bg_exec) {
+        /* the last replay may still run: wait for it before destroying */
+        if (ctx->batch_ev) HIPCHECK(ctx, hipEventSynchronize(ctx->batch_ev));
+        (void)hipGraphExecDestroy(ctx->bg_exec);
+        ctx->bg_exec = nullptr;
+    }
+    if (use_graph) {
+        ctx->bg_key = key;
+        ctx->bg_seen = true;
+    }
+    if (!use_graph || !same) {
+        enqueue();
+    } else {
+        if (!ctx->bg_exec) {
+            hipGraph_t g = nullptr;
+            HIPCHECK(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            enqueue();
+            hipError_t e = hipStreamEndCapture(st, &g);
+            if (e == hipSuccess) e = hipGraphInstantiate(&ctx->bg_exec, g, nullptr, nullptr, 0);
+            if (g) (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                ctx->bg_exec = nullptr;
+                return fail(ctx, JSX_EHIP, "batched launch graph: %s", hipGetErrorString(e));
+            }
+            ctx->graph_captures++;
+        }
+        HIPCHECK(ctx, hipGraphLaunch(ctx->bg_exec, st));
+        ctx->graph_launches++;
     }
     HIPCHECK(ctx, hipGetLastError());
     if (!ctx->batch_ev) HIPCHECK(ctx, hipEventCreateWithFlags(&ctx->batch_ev, hipEventDisableTiming));

@@ -106,6 +106,9 @@ static inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) 
 static inline hipError_t hipDeviceSynchronize() { return hipSuccess; }
 static inline hipError_t hipGetLastError() { return hipSuccess; }
 static inline const char *hipGetErrorString(hipError_t) { return "host build"; }
+typedef enum { hipStreamCaptureStatusNone = 0, hipStreamCaptureStatusActive = 1 } hipStreamCaptureStatus;
+/* no graphs here: the capture query fails, so the library launches eagerly */
+static inline hipError_t hipStreamIsCapturing(hipStream_t, hipStreamCaptureStatus *) { return hipErrorNoDevice; }
 static inline hipError_t hipStreamBeginCapture(hipStream_t, int) { return hipErrorNoDevice; }
 static inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *) { return hipErrorNoDevice; }
 static inline hipError_t hipGraphInstantiate(hipGraphExec_t *, hipGraph_t, void *, void *, size_t) { return hipErrorNoDevice; }
