@@ -75,6 +75,9 @@ def parse():
     ap.add_argument("--duration", type=float, default=10.0, help="c4: seconds of arrivals")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"], help="c2: solver precision")
     ap.add_argument("--models", type=int, default=256, help="c3: models in the sweep (x2 frequencies)")
+    ap.add_argument("--tune", type=float, default=0.0, metavar="SECONDS",
+                    help="c3: autotune the launch shape (tune.sweep) on the first 128 models for this many seconds "
+                         "before timing")
     ap.add_argument("--set", action="append", default=[], metavar="OPTION=VALUE",
                     help="solver option applied to the context before the run (jsx_ctx_set_option), repeatable")
     ap.add_argument("--traffic", action="store_true",
@@ -586,6 +589,14 @@ def sweep_bench(a):
         if rc:
             raise RuntimeError(lib.jsx_last_error(ctx.handle).decode())
 
+    tuned = None
+    if a.tune > 0:
+        from jetshow_amd import tune
+        sub = [(j, im, p) for j, im, p, _ in reqs_py[:256]]
+        _log("autotuning on %d images for %.0f s" % (len(sub), a.tune))
+        tuned = tune.tune(ctx, sub, reps=1, budget_s=a.tune, cache=None, force=True, log=_log)
+        _log("tuned: %s" % tuned)
+
     for i in range(a.warmup):
         _log("sweep warmup %d/%d" % (i + 1, a.warmup))
         run()
@@ -605,14 +616,17 @@ def sweep_bench(a):
     nrhs = buf["n_rhs"].cpu().numpy().astype(np.int64)
     ms = float(lat.mean())
     table, dom = kernel_rooflines(kern)
-    # FP64 error on the images of the first 8 models
-    host_planes = {k: v.cpu().numpy() for k, v in buf.items()}
+    # FP64 error on the images of the first 8 models (only their planes come to the host)
     offs = np.concatenate([[0], np.cumsum(sizes)])
     sample = [i for i, r in enumerate(reqs_py) if r[3] < 8]
-    err = fp64_error_sample(ctx, [reqs_py[i][:3] for i in sample],
-                            [{k: host_planes[k][offs[i]:offs[i + 1]].view(np.uint32) if k in ("flags", "n_rhs", "n_steps")
-                              else host_planes[k][offs[i]:offs[i + 1]] for k in ("I", "Q", "U", "V", "flags")}
-                             for i in sample], "sweep")
+    mixed = []
+    for i in sample:
+        d = {}
+        for k in ("I", "Q", "U", "V", "flags"):
+            v = buf[k][int(offs[i]):int(offs[i + 1])].cpu().numpy()
+            d[k] = v.view(np.uint32) if k == "flags" else v
+        mixed.append(d)
+    err = fp64_error_sample(ctx, [reqs_py[i][:3] for i in sample], mixed, "sweep")
     cpu = None
     if not a.no_cpu:
         cpu = sweep_cpu_baseline(reqs_py, a.cpu_seconds)
@@ -638,6 +652,9 @@ def sweep_bench(a):
     }
     if a.options:
         out["config"]["options"] = a.options
+    if tuned is not None:
+        out["config"]["tuned"] = {"options": tuned, "on": "first 128 models x 2 frequencies",
+                                  "seconds": a.tune}
     line = json.dumps(out)
     print(line, flush=True)
     if a.json_out:

@@ -167,6 +167,10 @@ typedef enum jsx_precision { JSX_PREC_FP64 = 0, JSX_PREC_MIXED = 1 } jsx_precisi
 #define JSX_OPT_SHEARED_CENTRAL_REF_BUG 0x2u /* theta = acos(z + o_z / r) as
                                                 VField.cpp:81 writes it (NaN for
                                                 |z| > 1 cm)                    */
+/* mixed precision only: run the Stokes pass over the whole path to tau_max,
+ * as Observation.cpp:225-250 does, instead of from the optical-depth horizon
+ * (JSX_PIX_HORIZON) */
+#define JSX_OPT_FULL_DEPTH 0x4u
 
 typedef struct jsx_solver_params {
     int32_t stokes;               /* jsx_stokes                                   */
@@ -196,6 +200,12 @@ typedef struct jsx_solver_params {
 #define JSX_PIX_NAN            0x20u /* a NaN coefficient or state was produced    */
 #define JSX_PIX_CLIPPED        0x40u /* z clip removed part or all of the path     */
 #define JSX_PIX_FAIL_BISECT    0x80u /* tau bisection hit its iteration cap        */
+#define JSX_PIX_HORIZON        0x100u /* mixed: Stokes pass began at the optical-depth
+                                        horizon (tau = 250 from the observer), not at
+                                        the tau_max point: what lies deeper reaches
+                                        the observer attenuated by e^-250 (e^-50 for
+                                        the slowest polarized mode); tau and l still
+                                        run to tau_max                            */
 
 /* Output planes, [n_across][n_along] row-major (Image::getImage order,
  * src/Image.cpp:16-28).  Pointers are device pointers for jsx_observe and
@@ -400,9 +410,32 @@ int jsx_read_field_text(const char *path, double *zrv, int64_t cap, int64_t *n_p
  * r values uniformly spaced (1e-9), every node present once.  Fills `desc`
  * (and, when values != NULL, values[i * nr + j] at (z0 + i dz, r0 + j dr), cap
  * >= nz * nr) for jsx_grid_upload.  JSX_EUNSUPPORTED: the points are not such a
- * grid (scattered Delaunay data: resample it onto a regular grid first). */
+ * grid (jsx_grid_upload_points takes any point set). */
 int jsx_grid_from_points(const double *zrv, int64_t n_points, double nan_value, jsx_grid_desc *desc,
                          double *values, int64_t cap);
+
+/* create_triangulation (src/SimulationInterpolater.cpp:105-117) and the
+ * SimulationInterpolater over it (:24-102) for any point set: rows (z [cm],
+ * r [cm], value) as jsx_read_field_text returns them.  Points that form a
+ * complete regular grid are laid out as by jsx_grid_from_points; any other
+ * set is Delaunay-triangulated in (z/pc, r/pc) (a point repeated at the same
+ * coordinates keeps its first value) and uploaded with a bucket index.
+ * Lookups: the value interpolated linearly in the triangle holding the point
+ * (the lowest-numbered one, for a point on a shared edge), nan_value outside
+ * the points' convex hull or for a NaN result.  The handle is used like
+ * jsx_grid_upload's (JetDesc grid_b / grid_b2 / grid_n / grid_v) and freed by
+ * jsx_grid_release.  JSX_EUNSUPPORTED: fewer than three distinct points, or
+ * all on one line. */
+int jsx_grid_upload_points(jsx_ctx *ctx, const double *zrv, int64_t n_points, double nan_value,
+                           int32_t *handle_out);
+
+/* The Delaunay triangulation jsx_grid_upload_points builds, for inspection:
+ * zr = (z, r) pairs in any unit; *n_tri triangles of three point indices,
+ * counter-clockwise, each starting at its smallest index, sorted (cap 0:
+ * count only; JSX_EINVAL if cap is too small).  No reference counterpart
+ * (CGAL's Delaunay_triangulation_2 in create_triangulation, which the
+ * reference does not expose). */
+int jsx_triangulate(const double *zr, int64_t n_points, int32_t *tri, int64_t cap, int64_t *n_tri);
 
 /* ---- visibilities of an image (the step after the path in the fitting loop) ----
  * Replaces FINUFFT_NUNU.forward (fourier.py:39-94: finufft nufft2d3, isign -1,

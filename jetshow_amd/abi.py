@@ -27,6 +27,7 @@ STOKES_I, STOKES_FULL = 1, 4
 PREC_FP64, PREC_MIXED = 0, 1
 OPT_CYLINDER_REF_BUG = 0x1
 OPT_SHEARED_CENTRAL_REF_BUG = 0x2
+OPT_FULL_DEPTH = 0x4
 
 PIX_HIT = 0x01
 PIX_NSEG2 = 0x02
@@ -36,6 +37,7 @@ PIX_FAIL_MAXTRIALS = 0x10
 PIX_NAN = 0x20
 PIX_CLIPPED = 0x40
 PIX_FAIL_BISECT = 0x80
+PIX_HORIZON = 0x100
 
 # constants of include/utils.h:17-36 in the reference
 PC = 3.0856775814671913e+18

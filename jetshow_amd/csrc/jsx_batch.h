@@ -101,7 +101,7 @@ struct PolMixed {
     __device__ static void begin(LaneT &L, const Q &q, int64_t pix) { ml_begin(L, q.m, q.im, q.S, q.c, pix); }
     template <class Q>
     __device__ static void rhs_consume(LaneT &L, const Q *q0) {
-        float r[NS];
+        float r[ml_nr<NS>()];
         const Q *q = q0;
         JSX_OPAQUE_V(q);
         ml_rhs(L, q->m, q->im, q->c, r);
@@ -111,7 +111,7 @@ struct PolMixed {
     }
     template <class Q>
     __device__ static void rhs_consume_uniform(LaneT &L, const Q *q0) {
-        float r[NS];
+        float r[ml_nr<NS>()];
         const JSX_CONST Q *q = (const JSX_CONST Q *)q0;
         JSX_OPAQUE_S(q);
         ml_rhs(L, q->m, q->im, q->c, r);

@@ -32,9 +32,16 @@
 #define JSX_DBL_EPS 2.220446049250313080847e-16
 
 struct DGrid {
-    const double *v;      /* device values [nz][nr] */
-    int nz, nr;
-    double z0, dz, r0, dr, nan_value, zlast, rlast;
+    const double *v;      /* device values [nz][nr]; triangulated: per node */
+    int nz, nr;           /* nz == 0: a triangulated point set */
+    double z0, dz, r0, dr, nan_value, zlast, rlast;   /* triangulated: the bucket grid */
+    /* triangulated point set: node coordinates [pc], counter-clockwise
+       triangles (3 node indices each), and per bucket (nbz x nbr over the
+       nodes' bounding box, bucket (i, j) at i * nbr + j) the triangles that
+       reach into it, in increasing order: blist[boff[b] .. boff[b + 1]) */
+    const double *pz, *pr;
+    const int32_t *tri, *boff, *blist;
+    int nbz, nbr;
 };
 
 /* POD model: descriptor + host-evaluated parameter constants */
@@ -365,9 +372,54 @@ __device__ inline double d_tri_area(double px, double py, double qx, double qy, 
 
 /* SimulationInterpolater::interpolated_value on a regular (z, r_p) grid
  * (src/SimulationInterpolater.cpp:43-102): fixed (i,j)-(i+1,j+1) diagonal. */
+/* SimulationInterpolater::interpolated_value (src/SimulationInterpolater.cpp:
+ * 33-102) on a triangulated point set: the lowest-numbered triangle of the
+ * query's bucket whose barycentric weights are all >= -1e-12 (a point on a
+ * shared edge, where either triangle gives the value up to rounding, takes
+ * the lower number), weighted as Triangle_coordinates_2 forms the weights;
+ * outside every triangle (the reference's infinite face): nan_value, as for
+ * a NaN result. */
+__device__ inline double d_tri_interp(const DGrid &G, double z, double rp) {
+    if (!(z >= G.z0 && z <= G.zlast && rp >= G.r0 && rp <= G.rlast)) return G.nan_value;
+    int i = (int)floor((z - G.z0) / G.dz);
+    int j = (int)floor((rp - G.r0) / G.dr);
+    if (i > G.nbz - 1) i = G.nbz - 1;
+    if (j > G.nbr - 1) j = G.nbr - 1;
+    if (i < 0) i = 0;
+    if (j < 0) j = 0;
+    const int b = i * G.nbr + j;
+    const int k1 = G.boff[b + 1];
+    for (int k = G.boff[b]; k < k1; ++k) {
+        const int32_t *t = G.tri + 3 * (int64_t)G.blist[k];
+        const int a0 = t[0], a1 = t[1], a2 = t[2];
+        const double v0x = G.pz[a0], v0y = G.pr[a0], v1x = G.pz[a1], v1y = G.pr[a1];
+        const double v2x = G.pz[a2], v2y = G.pr[a2];
+        double total = d_tri_area(v0x, v0y, v1x, v1y, v2x, v2y);
+        double inv = 1.0 / total;
+        double b0 = d_tri_area(v1x, v1y, v2x, v2y, z, rp) * inv;
+        double b1 = d_tri_area(v2x, v2y, v0x, v0y, z, rp) * inv;
+        double b2 = 1.0 - b0 - b1;
+        if (b0 >= -1e-12 && b1 >= -1e-12 && b2 >= -1e-12) {
+            double val = 0;
+            val += b0 * G.v[a0];
+            val += b1 * G.v[a1];
+            val += b2 * G.v[a2];
+            if (val != val) val = G.nan_value;
+            return val;
+        }
+    }
+    return G.nan_value;
+}
+
+/* EXT: the model may hold triangulated point sets (only the kernels built
+ * for the further kinds carry their lookup) */
+template <bool EXT = false>
 __device__ inline double d_grid_interp(const DGrid &G, const double *p) {
     double x = p[0] / JSX_PC, y = p[1] / JSX_PC, z = p[2] / JSX_PC;
     double rp = sqrt(x * x + y * y);
+    if constexpr (EXT) {
+        if (G.nz == 0) return d_tri_interp(G, z, rp);
+    }
     if (!(z >= G.z0 && z <= G.zlast && rp >= G.r0 && rp <= G.rlast)) return G.nan_value;
     int i = (int)floor((z - G.z0) / G.dz);
     int j = (int)floor((rp - G.r0) / G.dr);
@@ -534,7 +586,7 @@ __device__ inline void d_vfield(const DModel &m, const double *p, double *v) {
         return;
     }
     case JSX_V_GRID: {
-        double gam = m.has_gv ? d_grid_interp(m.gv, p) : 1.0;
+        double gam = m.has_gv ? d_grid_interp<EXT>(m.gv, p) : 1.0;
         v[0] = 0; v[1] = 0; v[2] = JSX_CL * sqrt(1. - 1. / (gam * gam));
         return;
     }
@@ -621,7 +673,7 @@ __device__ __forceinline__ int d_bfield(const DModel &m, const double *p, double
         return 0;
     }
     case JSX_B_GRID_SCALAR:
-        bl[0] = m.has_gb ? d_grid_interp(m.gb, p) : 0.0;
+        bl[0] = m.has_gb ? d_grid_interp<EXT>(m.gb, p) : 0.0;
         return 0;
     case JSX_B_TOROIDAL: {
         double sf, cf;
@@ -649,8 +701,8 @@ __device__ __forceinline__ int d_bfield(const DModel &m, const double *p, double
     case JSX_B_GRID_VECTOR: {
         double sf, cf;
         jsx_sincos_atan(p[1] / p[0], &sf, &cf);
-        double vp = m.has_gb ? d_grid_interp(m.gb, p) : 0.0;
-        double vf = m.has_gb2 ? d_grid_interp(m.gb2, p) : 0.0;
+        double vp = m.has_gb ? d_grid_interp<EXT>(m.gb, p) : 0.0;
+        double vf = m.has_gb2 ? d_grid_interp<EXT>(m.gb2, p) : 0.0;
         bl[0] = -sf * vf; bl[1] = cf * vf; bl[2] = vp;
         return 1;
     }
@@ -668,7 +720,7 @@ __device__ __forceinline__ double d_nfield(const DModel &m, const double *p, dou
     case JSX_N_CONST: return m.n0;
     case JSX_N_BK: return m.n0 * d_pow_shared(rr, lr_h, lr_l, m.n_negexp);
     case JSX_N_BK_Z: return m.n0 * jsx_pow(fabs(p[2]) / JSX_PC, m.n_negexp);
-    case JSX_N_GRID: return m.has_gn ? d_grid_interp(m.gn, p) : 0.0;
+    case JSX_N_GRID: return m.has_gn ? d_grid_interp<EXT>(m.gn, p) : 0.0;
     default:
         if constexpr (EXT) {
             if (m.nfield == JSX_N_COMPOSITE_BK)   /* NField.cpp:45-53 */

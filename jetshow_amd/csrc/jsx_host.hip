@@ -22,6 +22,7 @@
 #include "jsx_mixed.h"
 #include "jsx_batch.h"
 #include "jsx_fourier.h"
+#include "jsx_delaunay.h"
 
 
 #define JSX_OPT_PERSISTENT 0x100u   /* internal: persistent queue kernel        */
@@ -31,6 +32,12 @@ struct GridRec {
     double *dev;
     size_t bytes;
     int hull = 0;        /* face planes of a convex hull (jsx_hull_upload) */
+    /* a triangulated point set (jsx_grid_upload_points): dev holds the node
+       values, z, r [pc], then the int32 triangles, bucket offsets and lists */
+    int tri = 0;
+    int64_t n_nodes = 0, n_tri = 0, n_list = 0;
+    int nbz = 0, nbr = 0;
+    double bz0 = 0, bdz = 1, br0 = 0, bdr = 1, bzl = 0, brl = 0;
 };
 
 struct ImageRec {
@@ -464,6 +471,89 @@ extern "C" int jsx_grid_upload(jsx_ctx *ctx, const jsx_grid_desc *desc, const do
     return JSX_OK;
 }
 
+extern "C" int jsx_triangulate(const double *zr, int64_t n_points, int32_t *tri, int64_t cap, int64_t *n_tri) {
+    if (!zr || !n_tri || n_points < 0 || n_points > 0x7fffffff || (cap > 0 && !tri)) return JSX_EINVAL;
+    std::vector<double> z((size_t)n_points), r((size_t)n_points);
+    for (int64_t i = 0; i < n_points; ++i) {
+        z[i] = zr[2 * i];
+        r[i] = zr[2 * i + 1];
+        if (!std::isfinite(z[i]) || !std::isfinite(r[i])) return JSX_EINVAL;
+    }
+    std::vector<int32_t> T;
+    if (!jsx_del::triangulate(z, r, T)) {
+        *n_tri = 0;
+        return JSX_EUNSUPPORTED;
+    }
+    *n_tri = (int64_t)T.size() / 3;
+    if (cap == 0) return JSX_OK;
+    if (cap < *n_tri) return JSX_EINVAL;
+    std::memcpy(tri, T.data(), sizeof(int32_t) * T.size());
+    return JSX_OK;
+}
+
+extern "C" int jsx_grid_upload_points(jsx_ctx *ctx, const double *zrv, int64_t n_points, double nan_value,
+                                      int32_t *handle_out) {
+    if (!ctx || !zrv || !handle_out) return fail(ctx, JSX_EINVAL, "null argument");
+    if (n_points < 3 || n_points > 0x7fffffff) return fail(ctx, JSX_EINVAL, "need 3 to 2^31 - 1 points");
+    /* nodes that form a complete regular grid keep the regular layout */
+    jsx_grid_desc d;
+    int rc = jsx_grid_from_points(zrv, n_points, nan_value, &d, nullptr, 0);
+    if (rc == JSX_OK) {
+        std::vector<double> vals((size_t)d.nz * (size_t)d.nr);
+        if ((rc = jsx_grid_from_points(zrv, n_points, nan_value, &d, vals.data(), (int64_t)vals.size())))
+            return fail(ctx, rc, "regular grid points");
+        return jsx_grid_upload(ctx, &d, vals.data(), handle_out);
+    }
+    if (rc != JSX_EUNSUPPORTED) return fail(ctx, rc, "bad field points");
+    /* create_triangulation (src/SimulationInterpolater.cpp:105-117):
+       coordinates divided by pc */
+    const double PC = 3.0856775814671913e+18;
+    const size_t n = (size_t)n_points;
+    std::vector<double> z(n), r(n), v(n);
+    for (size_t i = 0; i < n; ++i) {
+        z[i] = zrv[3 * i] / PC;
+        r[i] = zrv[3 * i + 1] / PC;
+        v[i] = zrv[3 * i + 2];
+        if (!std::isfinite(z[i]) || !std::isfinite(r[i])) return fail(ctx, JSX_EINVAL, "non-finite point");
+    }
+    std::vector<int32_t> T;
+    if (!jsx_del::triangulate(z, r, T))
+        return fail(ctx, JSX_EUNSUPPORTED, "the points span no triangle (fewer than three, or all on one line)");
+    jsx_del::Buckets B;
+    jsx_del::build_buckets(z, r, T, B);
+    GridRec g;
+    std::memset(&g.desc, 0, sizeof g.desc);
+    g.desc.nan_value = nan_value;
+    g.tri = 1;
+    g.n_nodes = (int64_t)n;
+    g.n_tri = (int64_t)T.size() / 3;
+    g.n_list = (int64_t)B.list.size();
+    g.nbz = B.nbz; g.nbr = B.nbr;
+    g.bz0 = B.z0; g.bdz = B.dz; g.br0 = B.r0; g.bdr = B.dr; g.bzl = B.zlast; g.brl = B.rlast;
+    const size_t nd = 3 * n, ni = T.size() + B.off.size() + B.list.size();
+    g.bytes = sizeof(double) * nd + sizeof(int32_t) * ni;
+    std::vector<char> blob(g.bytes);
+    double *dp = (double *)blob.data();
+    std::memcpy(dp, v.data(), sizeof(double) * n);
+    std::memcpy(dp + n, z.data(), sizeof(double) * n);
+    std::memcpy(dp + 2 * n, r.data(), sizeof(double) * n);
+    int32_t *ip = (int32_t *)(dp + nd);
+    std::memcpy(ip, T.data(), sizeof(int32_t) * T.size());
+    std::memcpy(ip + T.size(), B.off.data(), sizeof(int32_t) * B.off.size());
+    std::memcpy(ip + T.size() + B.off.size(), B.list.data(), sizeof(int32_t) * B.list.size());
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (ctx->used + g.bytes > ctx->budget)
+        return fail(ctx, JSX_EBUDGET, "triangulated grid of %zu bytes exceeds HBM budget", g.bytes);
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    HIPCHECK(ctx, hipMalloc(&g.dev, g.bytes));
+    HIPCHECK(ctx, hipMemcpy(g.dev, blob.data(), g.bytes, hipMemcpyHostToDevice));
+    ctx->used += g.bytes;
+    const int32_t h = ctx->next_grid++;
+    ctx->grids[h] = g;
+    *handle_out = h;
+    return JSX_OK;
+}
+
 extern "C" int jsx_hull_upload(jsx_ctx *ctx, const double *planes, int32_t n_planes, int32_t *handle_out) {
     if (!ctx || !planes || !handle_out) return fail(ctx, JSX_EINVAL, "null argument");
     if (n_planes < 4) return fail(ctx, JSX_EINVAL, "a bounded convex hull needs at least 4 planes");
@@ -506,6 +596,26 @@ static int lower_grid(jsx_ctx *ctx, int32_t h, DGrid &g, int &has) {
     auto it = ctx->grids.find(h);
     if (it == ctx->grids.end()) return fail(ctx, JSX_ENOTFOUND, "unknown grid handle %d", h);
     const jsx_grid_desc &d = it->second.desc;
+    g.pz = g.pr = nullptr;
+    g.tri = g.boff = g.blist = nullptr;
+    g.nbz = g.nbr = 0;
+    if (it->second.tri) {
+        const GridRec &R = it->second;
+        const size_t n = (size_t)R.n_nodes;
+        g.v = R.dev;
+        g.pz = R.dev + n;
+        g.pr = R.dev + 2 * n;
+        g.tri = (const int32_t *)(R.dev + 3 * n);
+        g.boff = g.tri + 3 * R.n_tri;
+        g.blist = g.boff + ((size_t)R.nbz * (size_t)R.nbr + 1);
+        g.nz = 0; g.nr = 0;
+        g.nbz = R.nbz; g.nbr = R.nbr;
+        g.z0 = R.bz0; g.dz = R.bdz; g.r0 = R.br0; g.dr = R.bdr;
+        g.zlast = R.bzl; g.rlast = R.brl;
+        g.nan_value = d.nan_value;
+        has = 1;
+        return JSX_OK;
+    }
     g.v = it->second.dev;
     g.nz = d.nz; g.nr = d.nr;
     g.z0 = d.z0; g.dz = d.dz; g.r0 = d.r0; g.dr = d.dr; g.nan_value = d.nan_value;
@@ -611,6 +721,11 @@ static int lower_model(jsx_ctx *ctx, const jsx_jet_desc &j, unsigned opt, DModel
     }
     if (j.nfield == JSX_N_GRID && (rc = lower_grid(ctx, j.grid_n, m.gn, m.has_gn))) return rc;
     if (j.vfield == JSX_V_GRID && (rc = lower_grid(ctx, j.grid_v, m.gv, m.has_gv))) return rc;
+    /* triangulated point sets are looked up by the kernels built for the
+       further kinds */
+    if ((m.has_gb && m.gb.nz == 0) || (m.has_gb2 && m.gb2.nz == 0) || (m.has_gn && m.gn.nz == 0) ||
+        (m.has_gv && m.gv.nz == 0))
+        m.ext = 1;
     return JSX_OK;
 }
 
@@ -760,6 +875,13 @@ static int stage_params(jsx_ctx *ctx, const DModel &m, const DImage &D, const DS
 #ifndef JSX_MIXED_STOKES_FLOOR_I
 #define JSX_MIXED_STOKES_FLOOR_I 1e-7
 #endif
+/* optical depth from the observer at which the mixed Stokes pass starts
+   (JSX_PIX_HORIZON): emission from deeper reaches the observer attenuated by
+   e^-tau through Stokes I and by at least e^-(1 - p) tau through a polarized
+   mode, p = |k_pol| / k_I <= 0.8 for the power-law electrons here */
+#ifndef JSX_MIXED_TAU_HORIZON
+#define JSX_MIXED_TAU_HORIZON 250.0
+#endif
 
 /* FP32 constants of the mixed path (see jsx_mixed.h for the formulas) */
 static void lower_mixed(const jsx_jet_desc &j, const jsx_solver_params &p, const DModel &m, const DImage &D,
@@ -825,6 +947,7 @@ static void lower_mixed(const jsx_jet_desc &j, const jsx_solver_params &p, const
        tolerance */
     c.split_ok = p.stokes_rel_tol >= 0.999 * JSX_MIXED_STOKES_FLOOR_FULL ? 1 : 0;
     c.dt_max_pc = p.dt_max / PC;
+    c.tau_hz = (p.flags & JSX_OPT_FULL_DEPTH) || !(JSX_MIXED_TAU_HORIZON < p.tau_max) ? 0.0 : JSX_MIXED_TAU_HORIZON;
 }
 
 /* ---------------- persistent batch launch ---------------- */

@@ -42,6 +42,10 @@
 #define JSX_MIXED_FMA
 #endif
 
+/* lane-internal flag bit, cleared before the flags are written: the optical-
+ * depth horizon lies in the second segment of a two-segment path */
+#define JSX_PIX_HZ_SEG1 0x80000000u
+
 /* host-derived FP32 constants of the mixed path */
 struct MConsts {
     float A_k, A_e, A_kv, A_ev, A_F, A_C;  /* per-parsec coefficient prefactors           */
@@ -55,6 +59,7 @@ struct MConsts {
     double kC_y0, kC_y1;                  /* k_C_r's exponent y = y0 + y1 log2(x)          */
     double tau_abs, tau_rel, st_abs, st_rel, dt_max_pc;
     double st_rel_x;                      /* the caller's Stokes tolerance on |x|          */
+    double tau_hz;                        /* optical-depth horizon of the Stokes pass, 0 off */
     int split_ok;                         /* long rays may go to the time-parallel pieces  */
 };
 
@@ -73,7 +78,8 @@ __device__ __forceinline__ float m_powf(float x, double e) {
 __device__ __forceinline__ float m_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 /* grid lookup in FP32 after the FP64 index computation */
-__device__ inline float m_grid(const DGrid &G, const double *p_cm) { return (float)d_grid_interp(G, p_cm); }
+template <bool EXT = false>
+__device__ inline float m_grid(const DGrid &G, const double *p_cm) { return (float)d_grid_interp<EXT>(G, p_cm); }
 
 struct MCoef { float kI, kQ, kU, kV, kF, kC, hQ, eI, eQ, eU, eV; };
 
@@ -101,7 +107,7 @@ __device__ inline void m_coefficients(const DModel &m, const MConsts &c, const d
             else g = m.gamma0 + (m.gamma1 - m.gamma0) * ((z - m.gz0) / (m.gz1 - m.gz0));
             gam = (float)g;
         } else if (m.vfield == JSX_V_GRID) {
-            gam = m.has_gv ? m_grid(m.gv, p_cm) : 1.0f;
+            gam = m.has_gv ? m_grid<EXT>(m.gv, p_cm) : 1.0f;
         } else if (EXT && m.vfield >= JSX_V_SHEARED_FLAT) {
             /* the further reference V fields: velocity from the FP64 field code */
             double v[3];
@@ -149,7 +155,7 @@ __device__ inline void m_coefficients(const DModel &m, const MConsts &c, const d
     case JSX_N_CONST: nd = c.n0; break;
     case JSX_N_BK: nd = c.n0 * (m.n_negexp == -2.0 ? irr * irr : (m.n_negexp == -1.0 ? irr : m_powf(rr, m.n_negexp))); break;
     case JSX_N_BK_Z: { float az = fabsf(pz); nd = c.n0 * (m.n_negexp == -2.0 ? m_rcp(az * az) : m_powf(az, m.n_negexp)); break; }
-    case JSX_N_GRID: nd = m.has_gn ? m_grid(m.gn, p_cm) : 0.f; break;
+    case JSX_N_GRID: nd = m.has_gn ? m_grid<EXT>(m.gn, p_cm) : 0.f; break;
     default:
         nd = 0.f;
         if constexpr (EXT) {   /* the further reference N fields, from the FP64 field code */
@@ -170,7 +176,7 @@ __device__ inline void m_coefficients(const DModel &m, const MConsts &c, const d
         float az = fabsf(pz);
         bs = c.b0 * (m.b_negexp == -1.0 ? m_rcp(az) : m_powf(az, m.b_negexp)); vec = 0; break;
     }
-    case JSX_B_GRID_SCALAR: bs = m.has_gb ? m_grid(m.gb, p_cm) : 0.f; vec = 0; break;
+    case JSX_B_GRID_SCALAR: bs = m.has_gb ? m_grid<EXT>(m.gb, p_cm) : 0.f; vec = 0; break;
     case JSX_B_TOROIDAL: {
         /* (-sin fi, cos fi) with fi = atan(y/x): sign of x folds into the azimuth */
         float u = py * m_rcp(px);
@@ -198,8 +204,8 @@ __device__ inline void m_coefficients(const DModel &m, const MConsts &c, const d
         float u = py * m_rcp(px);
         float q = __builtin_amdgcn_rsqf(1.0f + u * u);
         float sf = u * q, cf = q;
-        float vp = m.has_gb ? m_grid(m.gb, p_cm) : 0.f;
-        float vf = m.has_gb2 ? m_grid(m.gb2, p_cm) : 0.f;
+        float vp = m.has_gb ? m_grid<EXT>(m.gb, p_cm) : 0.f;
+        float vf = m.has_gb2 ? m_grid<EXT>(m.gb2, p_cm) : 0.f;
         b[0] = -sf * vf; b[1] = cf * vf; b[2] = vp;
         break;
     }
@@ -301,7 +307,13 @@ struct MLane {
     double x_old, k_old;
     double tau_bg, thick;
     uint32_t flags, n_rhs, n_steps;
+    float mag, magp;                            /* rounding scales of the I and Q/U/V rows of the RHS at
+                                                   the step's start (ml_rhs) */
 };
+
+/* floats ml_rhs writes: the NS derivatives and the rounding scale of the I
+ * row (and of the Q/U/V rows) */
+template <int NS> constexpr int ml_nr() { return NS == 4 ? 6 : 2; }
 
 /* compensated accumulation: (hi, lo) += inc with Knuth's two-sum */
 __device__ __forceinline__ void m_kahan_add(float &hi, float &lo, float inc) {
@@ -438,6 +450,8 @@ __device__ inline void ml_tau_seg_end(MLane<NS> &L, const DSolver &S, const MCon
     L.thick = L.thick + L.L * JSX_PC;
     if (found || L.seg + 1 >= L.nseg) {
         if (found) L.nseg = L.seg + 1;
+        /* the Stokes pass ends (starts, going outwards) at the horizon */
+        if (L.flags & JSX_PIX_HORIZON) L.nseg = (L.flags & JSX_PIX_HZ_SEG1) ? 2 : 1;
         ml_stokes_begin(L, S, c);
         return;
     }
@@ -479,7 +493,9 @@ template <int NS>
 __device__ void ml_stokes_seg_begin(MLane<NS> &L, const DSolver &S, const MConsts &c) {
     L.L = ml_seg_length(L) * c.inv_pc;
     double auto_n = S.n;
-    if (L.tau_bg > S.tau_n_min) auto_n = d_steps_schedule(S, L.tau_bg);
+    /* the first step from the optical depth the pass covers */
+    const double tau_pass = (L.flags & JSX_PIX_HORIZON) ? c.tau_hz : L.tau_bg;
+    if (tau_pass > S.tau_n_min) auto_n = d_steps_schedule(S, tau_pass);
     L.dt = L.L / auto_n;
     L.t = 0.0;
     L.first = 1;
@@ -491,7 +507,7 @@ __device__ inline void ml_begin(MLane<NS> &L, const DModel &m, const DImage &im,
                                 int64_t pix) {
     L.pix = pix;
     L.flags = 0; L.n_rhs = 0; L.n_steps = 0;
-    L.tau_bg = 0.0; L.thick = 0.0; L.seg = 0;
+    L.tau_bg = 0.0; L.thick = 0.0; L.seg = 0; L.mag = 0.f; L.magp = 0.f;
     for (int i = 0; i < NS; ++i) { L.x[i] = 0.f; L.xlo[i] = 0.f; }
     double o[3];
     d_pixel_origin(im, pix, o);
@@ -524,6 +540,12 @@ __device__ inline void ml_point_pc(const MLane<NS> &L, const DImage &im, const M
     }
 }
 
+/* the derivative at the stage point: r[0..NS-1]; then the size of the terms
+ * the I row is the sum of (|e_I| + sum_j |K_Ij x_j|), and for full Stokes the
+ * largest such size of the Q, U, V rows: the scales of their FP32 rounding.
+ * Where emission and absorption balance (the Stokes vector near the source
+ * function inside an optically thick region) a derivative is small while its
+ * rounding is not (see ml_step_error).  r holds ml_nr<NS>() floats. */
 template <int NS, bool EXT = false>
 __device__ inline void ml_rhs(const MLane<NS> &L, const DModel &m, const DImage &im, const MConsts &c, float *r) {
     JSX_MIXED_FMA
@@ -533,16 +555,28 @@ __device__ inline void ml_rhs(const MLane<NS> &L, const DModel &m, const DImage 
     m_coefficients<NS == 4 ? 1 : 0, EXT>(m, c, p, k);
     if (L.phase == PH_TAU) {
         r[0] = k.kI;
-        for (int i = 1; i < NS; ++i) r[i] = 0.f;
+        for (int i = 1; i < ml_nr<NS>(); ++i) r[i] = 0.f;
         return;
     }
     const float *x = L.xt;
-    if (NS == 1) { r[0] = k.eI - k.kI * x[0]; return; }
+    if (NS == 1) {
+        r[0] = k.eI - k.kI * x[0];
+        r[1] = fabsf(k.eI) + fabsf(k.kI * x[0]);
+        return;
+    }
     r[0] = k.eI - k.kI * x[0] - k.kQ * x[1] - k.kU * x[2] - k.kV * x[NS > 3 ? 3 : 0];
     if (NS == 4) {
         r[1] = k.eQ - k.kI * x[1] - k.kQ * x[0] - k.kF * x[2] - k.hQ * x[3];
         r[2] = k.eU - k.kI * x[2] - k.kU * x[0] + k.kF * x[1] - k.kC * x[3];
         r[3] = k.eV - k.kI * x[3] - k.kV * x[0] + k.hQ * x[1] + k.kC * x[2];
+        const float a0 = fabsf(x[0]), a1 = fabsf(x[1]), a2 = fabsf(x[2]), a3 = fabsf(x[3]);
+        const float kI = fabsf(k.kI), kQ = fabsf(k.kQ), kU = fabsf(k.kU), kV = fabsf(k.kV);
+        const float kF = fabsf(k.kF), kC = fabsf(k.kC), hQ = fabsf(k.hQ);
+        r[4] = fabsf(k.eI) + kI * a0 + kQ * a1 + kU * a2 + kV * a3;
+        const float m1 = fabsf(k.eQ) + kI * a1 + kQ * a0 + kF * a2 + hQ * a3;
+        const float m2 = fabsf(k.eU) + kI * a2 + kU * a0 + kF * a1 + kC * a3;
+        const float m3 = fabsf(k.eV) + kI * a3 + kV * a0 + hQ * a1 + kC * a2;
+        r[5] = fmaxf(m1, fmaxf(m2, m3));
     }
 }
 
@@ -559,9 +593,19 @@ __device__ inline void ml_rhs(const MLane<NS> &L, const DModel &m, const DImage 
  * its error matters relative to the ray's intensity (floor |I| where the ray
  * was split).  Without the floors both would keep resolving decays and
  * build-ups to 1e-6 of their own small size, at a fraction of the step the
- * whole ray takes there.  `rel_min` raises the relative tolerance (pieces). */
+ * whole ray takes there.  `rel_min` raises the relative tolerance (pieces).
+ * `mag` and `magp` (0 for pieces) are the sizes of the terms the stage
+ * derivatives of the I and of the Q/U/V rows are sums of (ml_rhs): the FP32
+ * floor applies to dt * max(|dx/dt|, mag), since the stages' rounding scales
+ * with the terms, not with their sum.  Inside an
+ * optically thick region the intensity sits at the source function, the
+ * derivative is a small difference of emission and absorption and, floored
+ * at dt |dx/dt| alone, the rounding held each step to a few hundredths of an
+ * optical depth (590k derivative evaluations for a ray through tau 1e4, where
+ * FP64 takes 41k). */
 template <int NS>
-__device__ inline double ml_step_error(const MLane<NS> &L, const MConsts &c, double floor = 0.0, double rel_min = 0.0) {
+__device__ inline double ml_step_error(const MLane<NS> &L, const MConsts &c, double floor = 0.0, double rel_min = 0.0,
+                                       float mag = 0.f, float magp = 0.f) {
     double dt = L.dt, err = 0.0;
     int nl = L.phase == PH_TAU ? 1 : NS;
     /* odeint's norm |xerr| / (abs + rel (|x| + dt |dx/dt|)) with the relative
@@ -584,6 +628,11 @@ __device__ inline double ml_step_error(const MLane<NS> &L, const MConsts &c, dou
             pk = sk[i] > pk ? sk[i] : pk;
         }
     }
+    if (L.phase != PH_TAU) {
+        const double dm = dt * (double)mag, dmp = dt * (double)magp;
+        sk[0] = sk[0] > dm ? sk[0] : dm;
+        pk = pk > dmp ? pk : dmp;
+    }
     for (int i = 0; i < nl; ++i) {
         double s = i > 0 ? erx * px + er * pk : erx * sx[i] + er * sk[i];
         double sf = er * floor;
@@ -597,8 +646,16 @@ __device__ inline double ml_step_error(const MLane<NS> &L, const MConsts &c, dou
 /* error of the finished step: one lane per ray */
 struct MlNoReduce {
     template <int NS>
-    __device__ double operator()(const MLane<NS> &L, const MConsts &c) const { return ml_step_error(L, c); }
+    __device__ double operator()(const MLane<NS> &L, const MConsts &c) const {
+        return ml_step_error(L, c, 0.0, 0.0, L.mag, L.magp);
+    }
 };
+
+/* whether a step-error functor reads the lane's RHS rounding scales (whole
+ * rays: their callers pass ml_rhs's ml_nr<NS>() floats to ml_consume; the
+ * pieces of a split ray pass NS) */
+template <class Red> constexpr bool ml_red_mag() { return false; }
+template <> constexpr bool ml_red_mag<MlNoReduce>() { return true; }
 
 /* step-size controller on the finished step's error norm (odeint's
  * controlled_runge_kutta with the default error checker): reject and shrink
@@ -624,6 +681,7 @@ __device__ __forceinline__ int ml_control(MLane<NS> &L, const DSolver &S, double
 template <int NS, class Red = MlNoReduce>
 __device__ inline void ml_consume(MLane<NS> &L, const DSolver &S, const DImage &im, const MConsts &c, const float *r,
                                   Red red = Red()) {
+    constexpr bool MAG = ml_red_mag<Red>();
     L.n_rhs++;
     int ns_live = L.phase == PH_TAU ? 1 : NS;
     for (int i = 0; i < ns_live; ++i)
@@ -631,6 +689,10 @@ __device__ inline void ml_consume(MLane<NS> &L, const DSolver &S, const DImage &
     switch (L.sub) {
     case SUB_INIT:
         for (int i = 0; i < NS; ++i) L.k1[i] = r[i];
+        if constexpr (MAG) {
+            L.mag = r[NS];
+            if constexpr (NS == 4) L.magp = r[NS + 1];
+        }
         if (L.phase == PH_TAU) L.deriv_ok = 1;
         L.tries = 0;
         ml_try_begin(L, S, c);
@@ -643,9 +705,23 @@ __device__ inline void ml_consume(MLane<NS> &L, const DSolver &S, const DImage &
     default: break;
     }
     for (int i = 0; i < NS; ++i) L.k7[i] = r[i];
+    /* the step's rounding scale: the larger of its two ends' */
+    float mag_start = 0.f, magp_start = 0.f;
+    if constexpr (MAG) {
+        mag_start = L.mag;
+        L.mag = fmaxf(mag_start, r[NS]);
+        if constexpr (NS == 4) {
+            magp_start = L.magp;
+            L.magp = fmaxf(magp_start, r[NS + 1]);
+        }
+    }
     double err = red(L, c);
     const double h = L.dt;   /* the step just taken; the controller rescales L.dt */
     const int ok = ml_control(L, S, L.phase == PH_TAU ? c.dt_max_pc : 0.0, err);
+    if constexpr (MAG) {
+        L.mag = ok ? r[NS] : mag_start;
+        if constexpr (NS == 4) L.magp = ok ? r[NS + 1] : magp_start;
+    }
     if (L.tries++ >= S.max_fail) { ml_fail(L); return; }
     if (!ok) { ml_try_begin(L, S, c); return; }
     /* accepted: Kahan-compensated state update with the increment x_new - x */
@@ -666,16 +742,32 @@ __device__ inline void ml_consume(MLane<NS> &L, const DSolver &S, const DImage &
                 else t0 = t_m;
             }
             t_m = 0.5 * (t0 + t1);
-            double po[3], ci[3] = {ml_cin(L, 0), ml_cin(L, 1), ml_cin(L, 2)};
-            d_ray_point(ci, im.d2, t_m * JSX_PC, po);
-            for (int i = 0; i < 3; ++i) {
-                if (L.seg) L.s1out[i] = po[i];
-                else L.s0out[i] = po[i];
-                L.cout[i] = po[i];
+            /* a horizon placed before this step lies nearer the observer:
+               the Stokes pass keeps it */
+            if (!(L.flags & JSX_PIX_HORIZON)) {
+                double po[3], ci[3] = {ml_cin(L, 0), ml_cin(L, 1), ml_cin(L, 2)};
+                d_ray_point(ci, im.d2, t_m * JSX_PC, po);
+                for (int i = 0; i < 3; ++i) {
+                    if (L.seg) L.s1out[i] = po[i];
+                    else L.s0out[i] = po[i];
+                    L.cout[i] = po[i];
+                }
             }
             L.flags |= JSX_PIX_TAU_CROSSED;
             ml_tau_seg_end(L, S, c, 1);
             return;
+        }
+        /* optical-depth horizon: the first step end at tau >= tau_hz from the
+           observer ends the Stokes pass's path (the tau pass runs on, so tau
+           and l stay those of the whole path) */
+        if (c.tau_hz > 0.0 && !(L.flags & JSX_PIX_HORIZON) && L.tau_bg + xv >= c.tau_hz) {
+            double po[3], ci[3] = {ml_cin(L, 0), ml_cin(L, 1), ml_cin(L, 2)};
+            d_ray_point(ci, im.d2, L.t * JSX_PC, po);
+            for (int i = 0; i < 3; ++i) {
+                if (L.seg) L.s1out[i] = po[i];
+                else L.s0out[i] = po[i];
+            }
+            L.flags |= JSX_PIX_HORIZON | (L.seg ? JSX_PIX_HZ_SEG1 : 0u);
         }
         ml_tau_range_next(L, S, c);
         return;
@@ -689,7 +781,7 @@ __device__ inline void ml_consume(MLane<NS> &L, const DSolver &S, const DImage &
 
 template <int NS>
 __device__ inline void ml_write(const MLane<NS> &L, const DOut &o, int64_t q) {
-    uint32_t fl = L.flags;
+    uint32_t fl = L.flags & ~JSX_PIX_HZ_SEG1;
     double x0 = ml_xval(L, 0);
     if ((fl & JSX_PIX_HIT) && x0 != x0) fl |= JSX_PIX_NAN;
     if (o.tau) o.tau[q] = L.tau_bg;
@@ -726,7 +818,7 @@ __global__ __launch_bounds__(256) void jsx_km_plain(const DParamsM *__restrict__
     const JSX_CONST DParamsM *q = d_launder_m(P);
     ml_begin<NS, EXT>(L, q->m, q->im, q->S, q->c, pix);
     while (L.phase != PH_DONE) {
-        float r[NS];
+        float r[ml_nr<NS>()];
         q = d_launder_m(P);
         ml_rhs<NS, EXT>(L, q->m, q->im, q->c, r);
         q = d_launder_m(P);

@@ -1299,7 +1299,7 @@ __global__ __launch_bounds__(256) void jsx_k_split_combine(const DOut *__restric
     if (O.U) O.U[q] = S[2];
     if (O.V) O.V[q] = S[3];
     if (O.l) O.l[q] = R.thick;
-    if (O.flags) O.flags[q] = fl;
+    if (O.flags) O.flags[q] = fl & ~JSX_PIX_HZ_SEG1;
     if (O.n_rhs) O.n_rhs[q] = nr;
     if (O.n_steps) O.n_steps[q] = nst;
 }

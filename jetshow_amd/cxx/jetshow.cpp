@@ -48,6 +48,15 @@ int32_t Context::upload_hull(const std::vector<double> &planes) {
     return h;
 }
 
+int32_t Context::upload_points(const std::vector<double> &zrv, double nan_value) {
+    if (zrv.size() % 3) throw Error(JSX_EINVAL, "nodes come in threes (z, r, value)");
+    int32_t h = 0;
+    const int rc = jsx_grid_upload_points(ctx_, zrv.data(), (int64_t)(zrv.size() / 3), nan_value, &h);
+    if (rc == JSX_EUNSUPPORTED) throw Unsupported("the nodes span no triangle (fewer than three, or all on one line)");
+    check(rc);
+    return h;
+}
+
 int32_t Context::upload_grid(const std::vector<double> &values, int nz, int nr, double z0, double dz, double r0,
                              double dr, double nan_value) {
     if ((size_t)nz * (size_t)nr != values.size()) throw Error(JSX_EINVAL, "grid size mismatch");
@@ -444,14 +453,8 @@ int32_t create_grid(const std::string &fn, Context &ctx, double nan_value) {
     if (rc) throw Error(rc, "cannot read " + fn);
     std::vector<double> pts((size_t)(3 * n));
     if (n && (rc = jsx_read_field_text(fn.c_str(), pts.data(), n, &n))) throw Error(rc, "cannot read " + fn);
-    jsx_grid_desc g;
-    rc = jsx_grid_from_points(pts.data(), n, nan_value, &g, nullptr, 0);
-    if (rc == JSX_EUNSUPPORTED) throw Unsupported(fn + ": the points are not a complete regular (z, r) grid");
-    if (rc) throw Error(rc, fn + ": bad field points");
-    std::vector<double> vals((size_t)g.nz * (size_t)g.nr);
-    if ((rc = jsx_grid_from_points(pts.data(), n, nan_value, &g, vals.data(), (int64_t)vals.size())))
-        throw Error(rc, fn + ": bad field points");
-    return ctx.upload_grid(vals, g.nz, g.nr, g.z0, g.dz, g.r0, g.dr, nan_value);
+    pts.resize((size_t)(3 * n));
+    return ctx.upload_points(pts, nan_value);
 }
 
 }  // namespace jetshow

@@ -552,6 +552,8 @@ public:
                                const std::vector<double> &n_los, double nu, uint32_t flags = 0);
     int32_t upload_grid(const std::vector<double> &values, int nz, int nr, double z0, double dz, double r0,
                         double dr, double nan_value);
+    /* nodes: n x (z [cm], r [cm], value) at any positions (jsx_grid_upload_points) */
+    int32_t upload_points(const std::vector<double> &zrv, double nan_value);
     /* planes: n_planes x (n_x, n_y, n_z, h) in cm, inside where n . p <= h */
     int32_t upload_hull(const std::vector<double> &planes);
     static Context &instance();   /* process-wide default context on device 0 */
@@ -606,8 +608,9 @@ std::ostream &write_2dvector(std::ostream &os, std::vector<std::vector<double>> 
 std::ostream &write_vector(std::ostream &os, std::vector<double> &v, double scale = 1.0);
 void read_from_txt(std::string fntxt, std::vector<std::vector<double>> &properties);
 /* create_triangulation (src/SimulationInterpolater.cpp:105-117) for a text
- * file whose points form a complete regular (z, r) grid: the grid in HBM, its
- * handle for the Simulation*Field classes (Unsupported for scattered points) */
+ * file of field nodes at any positions: the nodes in HBM (a complete regular
+ * (z, r) grid in the regular layout, any other set Delaunay-triangulated),
+ * their handle for the Simulation*Field classes */
 int32_t create_grid(const std::string &fn, Context &ctx, double nan_value = 0.0);
 
 }  // namespace jetshow

@@ -24,6 +24,7 @@ ABI_SYMBOLS = [
     "jsx_server_create", "jsx_server_submit", "jsx_server_poll", "jsx_server_wait", "jsx_server_stats",
     "jsx_server_destroy", "jsx_tau_stripe", "jsx_visibilities", "jsx_visibilities_device",
     "jsx_write_map", "jsx_write_stripe", "jsx_read_field_text", "jsx_grid_from_points", "jsx_hull_upload",
+    "jsx_grid_upload_points", "jsx_triangulate",
 ]
 
 
@@ -127,6 +128,17 @@ class Context:
         g.z0, g.dz, g.r0, g.dr, g.nan_value = z0, dz, r0, dr, nan_value
         h = C.c_int32()
         self._check(lib().jsx_grid_upload(self._h, C.byref(g), _ptr(v), C.byref(h)))
+        return h.value
+
+    def upload_points(self, points, nan_value=0.0):
+        """Field nodes [n, 3] = (z [cm], r [cm], value) at any positions
+        (create_triangulation + SimulationInterpolater): a complete regular
+        grid keeps the regular layout, any other set is Delaunay-triangulated
+        on the host and uploaded with its bucket index; returns the handle."""
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        h = C.c_int32()
+        self._check(lib().jsx_grid_upload_points(self._h, _ptr(pts), C.c_int64(pts.shape[0]), C.c_double(nan_value),
+                                                 C.byref(h)))
         return h.value
 
     def upload_hull(self, planes):

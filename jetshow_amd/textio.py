@@ -97,7 +97,21 @@ def grid_from_points(points, nan_value=0.0):
     return vals, (g.z0, g.dz, g.r0, g.dr)
 
 
+def triangulate(z, r):
+    """The Delaunay triangles [m, 3] (counter-clockwise point indices) that
+    jsx_grid_upload_points builds for the points (z, r)."""
+    zr = np.ascontiguousarray(np.stack([np.asarray(z, dtype=np.float64), np.asarray(r, dtype=np.float64)], 1))
+    n = C.c_int64()
+    L = host.lib()
+    _check(L.jsx_triangulate(zr.ctypes.data_as(C.c_void_p), C.c_int64(zr.shape[0]), None, C.c_int64(0), C.byref(n)),
+           "the points span no triangle")
+    tri = np.zeros((n.value, 3), dtype=np.int32)
+    _check(L.jsx_triangulate(zr.ctypes.data_as(C.c_void_p), C.c_int64(zr.shape[0]), tri.ctypes.data_as(C.c_void_p),
+                             C.c_int64(n.value), C.byref(n)), "the points span no triangle")
+    return tri
+
+
 def load_field_grid(ctx, path, nan_value=0.0):
-    """Upload a simulation field text file to HBM; returns the grid handle."""
-    vals, geo = grid_from_points(read_field_text(path), nan_value)
-    return ctx.upload_grid(vals, *geo, nan_value=nan_value)
+    """Upload a simulation field text file (nodes at any positions) to HBM;
+    returns the grid handle."""
+    return ctx.upload_points(read_field_text(path), nan_value)

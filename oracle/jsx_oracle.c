@@ -499,10 +499,46 @@ static double tri_area(double px, double py, double qx, double qy, double rx, do
     return ((qx - px) * (ry - py) - (rx - px) * (qy - py)) / 2;
 }
 
+/* SimulationInterpolater::interpolated_value (SimulationInterpolater.cpp:
+ * 33-102) on a triangulated point set: the triangle holding (z, rp) --
+ * barycentric weights all >= -1e-12, the lowest-numbered such triangle when a
+ * point lies on an edge two share -- and Triangle_coordinates_2's weights;
+ * nan_value outside every triangle (the infinite face) or for a NaN. */
+static double tri_interp(const orc_grid *g, double z, double rp) {
+    if (!(z >= g->bz0 && z <= g->bzl && rp >= g->br0 && rp <= g->brl)) return g->desc.nan_value;
+    int i = (int)floor((z - g->bz0) / g->bdz);
+    int j = (int)floor((rp - g->br0) / g->bdr);
+    if (i > g->nbz - 1) i = g->nbz - 1;
+    if (j > g->nbr - 1) j = g->nbr - 1;
+    if (i < 0) i = 0;
+    if (j < 0) j = 0;
+    const int b = i * g->nbr + j;
+    for (int k = g->boff[b]; k < g->boff[b + 1]; ++k) {
+        const int32_t *t = g->tri + 3 * (int64_t)g->blist[k];
+        double ax = g->pz[t[0]], ay = g->pr[t[0]], bx = g->pz[t[1]], by = g->pr[t[1]];
+        double cx = g->pz[t[2]], cy = g->pr[t[2]];
+        double total = tri_area(ax, ay, bx, by, cx, cy);
+        double inv = 1.0 / total;
+        double w0 = tri_area(bx, by, cx, cy, z, rp) * inv;
+        double w1 = tri_area(cx, cy, ax, ay, z, rp) * inv;
+        double w2 = 1.0 - w0 - w1;
+        if (w0 >= -1e-12 && w1 >= -1e-12 && w2 >= -1e-12) {
+            double val = 0;
+            val += w0 * g->values[t[0]];
+            val += w1 * g->values[t[1]];
+            val += w2 * g->values[t[2]];
+            if (val != val) val = g->desc.nan_value;
+            return val;
+        }
+    }
+    return g->desc.nan_value;
+}
+
 static double grid_interp(const orc_grid *g, const double *p, const orc_math *mth) {
     const jsx_grid_desc *G = &g->desc;
     double x = p[0] / K_PC, y = p[1] / K_PC, z = p[2] / K_PC;
     double rp = M_hypot(mth, x, y);
+    if (G->nz == 0) return tri_interp(g, z, rp);
     double zlast = G->z0 + (double)(G->nz - 1) * G->dz;
     double rlast = G->r0 + (double)(G->nr - 1) * G->dr;
     if (!(z >= G->z0 && z <= zlast && rp >= G->r0 && rp <= rlast)) return G->nan_value;

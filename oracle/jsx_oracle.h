@@ -27,7 +27,17 @@ extern "C" {
 
 typedef struct orc_grid {
     jsx_grid_desc desc;
-    const double *values;
+    const double *values;       /* regular: [nz][nr]; triangulated: per node */
+    /* triangulated point set (desc.nz == 0): node coordinates [pc],
+       counter-clockwise triangles, and a bucket index over the box
+       [bz0, bzl] x [br0, brl] (nbz x nbr buckets of bdz x bdr; bucket (i, j)
+       lists, in increasing order, every triangle that may hold a point of
+       it: blist[boff[i * nbr + j] .. boff[i * nbr + j + 1])) */
+    const double *pz, *pr;
+    const int32_t *tri;
+    const int32_t *boff, *blist;
+    int32_t nbz, nbr;
+    double bz0, bdz, br0, bdr, bzl, brl;
 } orc_grid;
 
 /* Observation::run restricted to the listed pixel indices (row-major

@@ -45,23 +45,85 @@ def _grid_type():
     from jetshow_amd import abi
 
     class OrcGrid(C.Structure):
-        _fields_ = [("desc", abi.GridDesc), ("values", C.c_void_p)]
+        _fields_ = [("desc", abi.GridDesc), ("values", C.c_void_p), ("pz", C.c_void_p), ("pr", C.c_void_p),
+                    ("tri", C.c_void_p), ("boff", C.c_void_p), ("blist", C.c_void_p), ("nbz", C.c_int32),
+                    ("nbr", C.c_int32), ("bz0", C.c_double), ("bdz", C.c_double), ("br0", C.c_double),
+                    ("bdr", C.c_double), ("bzl", C.c_double), ("brl", C.c_double)]
     return OrcGrid
+
+
+class TriGrid:
+    """A triangulated point set for the oracle: node coordinates z, r
+    [pc], node values, counter-clockwise triangles [m, 3] and nan_value, with
+    a bucket index built here (its own layout: ~3 triangles per bucket; the
+    lookup's rule -- lowest-numbered triangle holding the point -- does not
+    depend on the layout)."""
+
+    def __init__(self, z, r, values, tri, nan_value=0.0):
+        self.z = np.ascontiguousarray(z, dtype=np.float64)
+        self.r = np.ascontiguousarray(r, dtype=np.float64)
+        self.values = np.ascontiguousarray(values, dtype=np.float64)
+        self.tri = np.ascontiguousarray(tri, dtype=np.int32).reshape(-1, 3)
+        self.nan_value = float(nan_value)
+        T = self.tri
+        tz, tr = self.z[T], self.r[T]
+        self.bz0, self.bzl = float(tz.min()), float(tz.max())
+        self.br0, self.brl = float(tr.min()), float(tr.max())
+        wz, wr = max(self.bzl - self.bz0, 1e-300), max(self.brl - self.br0, 1e-300)
+        nb = max(1.0, len(T) / 3.0)
+        nbz = int(min(max(round(np.sqrt(nb * wz / wr)), 1), 1 << 15))
+        nbr = int(min(max(round(nb / nbz), 1), 1 << 15))
+        self.nbz, self.nbr = nbz, nbr
+        self.bdz, self.bdr = wz / nbz, wr / nbr
+        lo_z, hi_z, lo_r, hi_r = tz.min(1), tz.max(1), tr.min(1), tr.max(1)
+        mz = 1e-9 * (hi_z - lo_z) + 1e-12 * wz
+        mr = 1e-9 * (hi_r - lo_r) + 1e-12 * wr
+        i0 = np.clip(np.floor((lo_z - mz - self.bz0) / self.bdz), 0, nbz - 1).astype(np.int64)
+        i1 = np.clip(np.floor((hi_z + mz - self.bz0) / self.bdz), 0, nbz - 1).astype(np.int64)
+        j0 = np.clip(np.floor((lo_r - mr - self.br0) / self.bdr), 0, nbr - 1).astype(np.int64)
+        j1 = np.clip(np.floor((hi_r + mr - self.br0) / self.bdr), 0, nbr - 1).astype(np.int64)
+        ni, nj = i1 - i0 + 1, j1 - j0 + 1
+        cnt = ni * nj
+        tid = np.repeat(np.arange(len(T), dtype=np.int64), cnt)
+        k = np.arange(cnt.sum()) - np.repeat(np.cumsum(cnt) - cnt, cnt)
+        bi = np.repeat(i0, cnt) + k // np.repeat(nj, cnt)
+        bj = np.repeat(j0, cnt) + k % np.repeat(nj, cnt)
+        bucket = bi * nbr + bj
+        order = np.lexsort((tid, bucket))
+        self.blist = np.ascontiguousarray(tid[order], dtype=np.int32)
+        self.boff = np.zeros(nbz * nbr + 1, dtype=np.int32)
+        np.add.at(self.boff, bucket + 1, 1)
+        self.boff = np.ascontiguousarray(np.cumsum(self.boff), dtype=np.int32)
 
 
 _ORC_GRID = None
 
 
 def make_grids(grids):
-    """grids: list of (abi.GridDesc, values ndarray) -> (ctypes array, keepalive)."""
+    """grids: list of (abi.GridDesc, values ndarray) or TriGrid -> (ctypes array, keepalive)."""
     global _ORC_GRID
     if _ORC_GRID is None:
         _ORC_GRID = _grid_type()
     if not grids:
         return None, 0, []
+    from jetshow_amd import abi
     arr = (_ORC_GRID * len(grids))()
     keep = []
-    for i, (d, v) in enumerate(grids):
+    for i, g in enumerate(grids):
+        if isinstance(g, TriGrid):
+            d = abi.GridDesc()
+            d.nz = d.nr = 0
+            d.nan_value = g.nan_value
+            keep.append(g)
+            arr[i].desc = d
+            arr[i].values = g.values.ctypes.data
+            arr[i].pz, arr[i].pr = g.z.ctypes.data, g.r.ctypes.data
+            arr[i].tri, arr[i].boff, arr[i].blist = g.tri.ctypes.data, g.boff.ctypes.data, g.blist.ctypes.data
+            arr[i].nbz, arr[i].nbr = g.nbz, g.nbr
+            arr[i].bz0, arr[i].bdz, arr[i].br0, arr[i].bdr, arr[i].bzl, arr[i].brl = (
+                g.bz0, g.bdz, g.br0, g.bdr, g.bzl, g.brl)
+            continue
+        d, v = g
         v = np.ascontiguousarray(v, dtype=np.float64)
         keep.append(v)
         arr[i].desc = d
@@ -192,6 +254,14 @@ def point_values(jet, nu, points, n_los, opt=0, math=MATH_JSX, grids=None):
 def grid_interp(desc, values, point_cm, math=MATH_JSX):
     garr, ng, keep = make_grids([(desc, values)])
     return lib().orc_grid_interp(C.byref(garr[0]), _p(np.asarray(point_cm, float)), int(math))
+
+
+def tri_grid_interp(grid, points_cm, math=MATH_JSX):
+    """SimulationInterpolater::interpolated_value of a TriGrid at points [n, 3] (cm)."""
+    garr, ng, keep = make_grids([grid])
+    pts = np.ascontiguousarray(points_cm, dtype=np.float64).reshape(-1, 3)
+    L = lib()
+    return np.array([L.orc_grid_interp(C.byref(garr[0]), _p(q), int(math)) for q in pts])
 
 
 def mas_to_pc(z):

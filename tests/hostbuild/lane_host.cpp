@@ -63,7 +63,7 @@ void run_mixed(const DParamsM &q, int64_t pix, Out1 &o) {
     MLane<NS> L;
     ml_begin(L, q.m, q.im, q.S, q.c, pix);
     while (L.phase != PH_DONE) {
-        float r[NS];
+        float r[ml_nr<NS>()];
         ml_rhs(L, q.m, q.im, q.c, r);
         ml_consume(L, q.S, q.im, q.c, r);
     }
@@ -188,7 +188,7 @@ void run_mixed_split(const DParamsM &q, int64_t pix, unsigned defer_rhs, float w
     ml_begin(L, q.m, q.im, q.S, q.c, pix);
     *n_pieces = 0;
     while (L.phase != PH_DONE) {
-        float r[4];
+        float r[ml_nr<4>()];
         ml_rhs(L, q.m, q.im, q.c, r);
         ml_consume(L, q.S, q.im, q.c, r);
         if (L.phase != PH_DONE && split_should_park(L, defer_rhs)) break;

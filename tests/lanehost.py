@@ -28,7 +28,7 @@ def build(force=False):
         if all(os.path.getmtime(s) <= t for s in _sources()):
             return LIB
     cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math", "-w",
-           "-I", os.path.join(HERE, "hostbuild"), "-I", os.path.join(ROOT, "include"), SRC, "-o", LIB]
+           "-I", os.path.join(HERE, "hostbuild"), "-I", os.path.join(ROOT, "include"), SRC, os.path.join(ROOT, "jetshow_amd", "csrc", "jsx_textio.cpp"), "-o", LIB]
     subprocess.run(cmd, check=True)
     return LIB
 
