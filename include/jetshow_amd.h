@@ -201,11 +201,12 @@ typedef struct jsx_solver_params {
 #define JSX_PIX_CLIPPED        0x40u /* z clip removed part or all of the path     */
 #define JSX_PIX_FAIL_BISECT    0x80u /* tau bisection hit its iteration cap        */
 #define JSX_PIX_HORIZON        0x100u /* mixed: Stokes pass began at the optical-depth
-                                        horizon (tau = 250 from the observer), not at
-                                        the tau_max point: what lies deeper reaches
-                                        the observer attenuated by e^-250 (e^-50 for
-                                        the slowest polarized mode); tau and l still
-                                        run to tau_max                            */
+                                        horizon, not at the tau_max point: past
+                                        tau = 50 (Stokes I), past int (k_I - |eta|)
+                                        ds = 50 (full Stokes, eta = (k_Q, k_U, k_V):
+                                        the transfer operator's norm decay), so what
+                                        lies deeper reaches the observer attenuated
+                                        by e^-50; tau and l still run to tau_max  */
 
 /* Output planes, [n_across][n_along] row-major (Image::getImage order,
  * src/Image.cpp:16-28).  Pointers are device pointers for jsx_observe and

@@ -200,13 +200,21 @@ __global__ __launch_bounds__(256) void jsx_k_triage(const typename P::Params *__
     __syncthreads();
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t base_id = (int64_t)blockIdx.x * blockDim.x; base_id < total; base_id += stride) {
-        const int64_t id = base_id + threadIdx.x;
+        /* threads walk the pixels in memory order, so that the planes a
+           wavefront writes for the rays that need no integration are
+           contiguous; the live list holds ray ids (the launch's order) */
+        const int64_t t = base_id + threadIdx.x;
+        int64_t id = t;
         int alive = 0, bk = 0;
-        if (id < total) {
+        if (t < total) {
             LaneT L;
-            const int req = d_find_request(ray_off, n_req, id);
+            const int req = d_find_request(ray_off, n_req, t);
             const typename P::Params *qp = reqs + req;
-            const int64_t pix = d_ray_to_pixel(qp->im, id - ray_off[req], order);
+            const int64_t pix = t - ray_off[req];
+            if (order == 1) {
+                const int64_t row = pix / qp->im.nl, col = pix - row * qp->im.nl;
+                id = ray_off[req] + col * qp->im.na + row;
+            }
             P::begin(L, *qp, pix);
             if (P::done(L)) {
                 P::write(L, outs[req], pix);

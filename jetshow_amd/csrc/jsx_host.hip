@@ -875,12 +875,13 @@ static int stage_params(jsx_ctx *ctx, const DModel &m, const DImage &D, const DS
 #ifndef JSX_MIXED_STOKES_FLOOR_I
 #define JSX_MIXED_STOKES_FLOOR_I 1e-7
 #endif
-/* optical depth from the observer at which the mixed Stokes pass starts
-   (JSX_PIX_HORIZON): emission from deeper reaches the observer attenuated by
-   e^-tau through Stokes I and by at least e^-(1 - p) tau through a polarized
-   mode, p = |k_pol| / k_I <= 0.8 for the power-law electrons here */
+/* decay exponent from the observer at which the mixed Stokes pass starts
+   (JSX_PIX_HORIZON): Stokes I past tau = 50; full Stokes past
+   int (k_I - |eta|) ds = 50, the bound on the transfer operator's norm decay
+   (dichroism eta = (k_Q, k_U, k_V)).  Emission from deeper reaches the
+   observer attenuated by at least e^-50 = 2e-22. */
 #ifndef JSX_MIXED_TAU_HORIZON
-#define JSX_MIXED_TAU_HORIZON 250.0
+#define JSX_MIXED_TAU_HORIZON 50.0
 #endif
 
 /* FP32 constants of the mixed path (see jsx_mixed.h for the formulas) */

@@ -494,7 +494,7 @@ __device__ void ml_stokes_seg_begin(MLane<NS> &L, const DSolver &S, const MConst
     L.L = ml_seg_length(L) * c.inv_pc;
     double auto_n = S.n;
     /* the first step from the optical depth the pass covers */
-    const double tau_pass = (L.flags & JSX_PIX_HORIZON) ? c.tau_hz : L.tau_bg;
+    const double tau_pass = (L.flags & JSX_PIX_HORIZON) && c.tau_hz < L.tau_bg ? c.tau_hz : L.tau_bg;
     if (tau_pass > S.tau_n_min) auto_n = d_steps_schedule(S, tau_pass);
     L.dt = L.L / auto_n;
     L.t = 0.0;
@@ -556,6 +556,10 @@ __device__ inline void ml_rhs(const MLane<NS> &L, const DModel &m, const DImage 
     if (L.phase == PH_TAU) {
         r[0] = k.kI;
         for (int i = 1; i < ml_nr<NS>(); ++i) r[i] = 0.f;
+        /* full Stokes: the decay rate bound of the transfer operator's norm,
+           k_I - |eta| (eta = (k_Q, k_U, k_V); rotation does not change the
+           norm), integrated beside tau for the optical-depth horizon */
+        if constexpr (NS == 4) r[1] = k.kI - sqrtf(k.kQ * k.kQ + k.kU * k.kU + k.kV * k.kV);
         return;
     }
     const float *x = L.xt;
@@ -594,10 +598,10 @@ __device__ inline void ml_rhs(const MLane<NS> &L, const DModel &m, const DImage 
  * was split).  Without the floors both would keep resolving decays and
  * build-ups to 1e-6 of their own small size, at a fraction of the step the
  * whole ray takes there.  `rel_min` raises the relative tolerance (pieces).
- * `mag` and `magp` (0 for pieces) are the sizes of the terms the stage
- * derivatives of the I and of the Q/U/V rows are sums of (ml_rhs): the FP32
- * floor applies to dt * max(|dx/dt|, mag), since the stages' rounding scales
- * with the terms, not with their sum.  Inside an
+ * `mag` and `magp` (0 for pieces) are the sizes of the terms the
+ * derivatives of the I and of the Q/U/V rows are sums of at the step's start
+ * (ml_rhs): the FP32 floor applies to dt * max(|dx/dt|, mag), since the
+ * stages' rounding scales with the terms, not with their sum.  Inside an
  * optically thick region the intensity sits at the source function, the
  * derivative is a small difference of emission and absorption and, floored
  * at dt |dx/dt| alone, the rounding held each step to a few hundredths of an
@@ -705,22 +709,17 @@ __device__ inline void ml_consume(MLane<NS> &L, const DSolver &S, const DImage &
     default: break;
     }
     for (int i = 0; i < NS; ++i) L.k7[i] = r[i];
-    /* the step's rounding scale: the larger of its two ends' */
-    float mag_start = 0.f, magp_start = 0.f;
-    if constexpr (MAG) {
-        mag_start = L.mag;
-        L.mag = fmaxf(mag_start, r[NS]);
-        if constexpr (NS == 4) {
-            magp_start = L.magp;
-            L.magp = fmaxf(magp_start, r[NS + 1]);
-        }
-    }
+    /* the step's rounding scale is that at its start (as the floor's
+       dt |dx/dt| takes k1): the end's, where a field's kink makes the
+       derivative jump, would loosen the control exactly there */
     double err = red(L, c);
     const double h = L.dt;   /* the step just taken; the controller rescales L.dt */
     const int ok = ml_control(L, S, L.phase == PH_TAU ? c.dt_max_pc : 0.0, err);
     if constexpr (MAG) {
-        L.mag = ok ? r[NS] : mag_start;
-        if constexpr (NS == 4) L.magp = ok ? r[NS + 1] : magp_start;
+        if (ok) {
+            L.mag = r[NS];
+            if constexpr (NS == 4) L.magp = r[NS + 1];
+        }
     }
     if (L.tries++ >= S.max_fail) { ml_fail(L); return; }
     if (!ok) { ml_try_begin(L, S, c); return; }
@@ -730,6 +729,12 @@ __device__ inline void ml_consume(MLane<NS> &L, const DSolver &S, const DImage &
         L.k_old = (double)L.k1[0];
         m_kahan_add(L.x[0], L.xlo[0], ml_increment(L, 0, h));
         L.k1[0] = L.k7[0];
+        if constexpr (NS == 4) {
+            /* component 1: the decay exponent along the whole path so far
+               (not reset between segments) */
+            m_kahan_add(L.x[1], L.xlo[1], ml_increment(L, 1, h));
+            L.k1[1] = L.k7[1];
+        }
         double xv = ml_xval(L, 0);
         if (xv >= S.tau_max) {
             double t0 = L.t_old, t1 = L.t, t_m, x_m = xv;
@@ -757,10 +762,16 @@ __device__ inline void ml_consume(MLane<NS> &L, const DSolver &S, const DImage &
             ml_tau_seg_end(L, S, c, 1);
             return;
         }
-        /* optical-depth horizon: the first step end at tau >= tau_hz from the
-           observer ends the Stokes pass's path (the tau pass runs on, so tau
-           and l stay those of the whole path) */
-        if (c.tau_hz > 0.0 && !(L.flags & JSX_PIX_HORIZON) && L.tau_bg + xv >= c.tau_hz) {
+        /* optical-depth horizon: the first step end past which everything
+           deeper reaches the observer attenuated by e^-tau_hz ends the
+           Stokes pass's path -- Stokes I: tau itself; full Stokes: the
+           integral of k_I - |eta|, the bound on the transfer operator's
+           norm decay, which a growing polarized mode (|eta| > k_I, as the
+           reference's coefficients allow near a field along the line of
+           sight) holds back.  The tau pass runs on, so tau and l stay those
+           of the whole path. */
+        const double decay = NS == 4 ? ml_xval(L, NS > 1 ? 1 : 0) : L.tau_bg + xv;
+        if (c.tau_hz > 0.0 && !(L.flags & JSX_PIX_HORIZON) && decay >= c.tau_hz) {
             double po[3], ci[3] = {ml_cin(L, 0), ml_cin(L, 1), ml_cin(L, 2)};
             d_ray_point(ci, im.d2, L.t * JSX_PC, po);
             for (int i = 0; i < 3; ++i) {
