@@ -50,6 +50,9 @@ PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 # known size (profiles/c1_hbm_traffic_counters.md); reported as roofline
 # "traffic" when that kernel dominates the default headline run.
 TRAFFIC_BYTES_C1 = {"jsx_k_split_magnus": 143.8e6, "jsx_k_batch<PolMixed<4>>": 24.1e6}
+# the same for the c2 workload's FP64 batch kernel, per launch of both images
+# (profiles/c2_grid_fields_fp64_hbm_traffic_counters.md)
+TRAFFIC_BYTES_C2_FP64 = 82.4e9
 
 
 def parse():
@@ -770,7 +773,11 @@ def grid_bench(a):
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "fp64_error_sample": err,
         "roofline": {"bound": "valu", "pipe": "valu", "kernel": "jsx_k_batch", "achieved": ach, "peak": peak,
-                     "unit": "TFLOP/s", "frac": ach / peak, "traffic": None,
+                     "unit": "TFLOP/s", "frac": ach / peak,
+                     "traffic": TRAFFIC_BYTES_C2_FP64 if prec == abi.PREC_FP64 and not a.options else None,
+                     "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE of the default c2 run)",
+                     "hbm_GBps": (TRAFFIC_BYTES_C2_FP64 / (kern["stat_batch_ms"] / 1e3) / 1e9
+                                  if prec == abi.PREC_FP64 and not a.options and kern["stat_batch_ms"] else None),
                      "grid_lookup_GBps_algorithmic": grid_gbs},
         "cpu_baseline": cpu,
     }

@@ -192,7 +192,7 @@ __global__ __launch_bounds__(256) void jsx_k_triage(const typename P::Params *__
                                                     const int64_t *__restrict__ ray_off,
                                                     const DOut *__restrict__ outs, int n_req, int64_t total,
                                                     int order, BatchQueues *qs, int64_t *__restrict__ live,
-                                                    unsigned char *__restrict__ bucket) {
+                                                    unsigned char *__restrict__ bucket, int pixel_walk) {
     using LaneT = typename P::LaneT;
     __shared__ unsigned s_hist[JSX_LPT_BUCKETS];
     const int lane = __lane_id();
@@ -200,9 +200,10 @@ __global__ __launch_bounds__(256) void jsx_k_triage(const typename P::Params *__
     __syncthreads();
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t base_id = (int64_t)blockIdx.x * blockDim.x; base_id < total; base_id += stride) {
-        /* threads walk the pixels in memory order, so that the planes a
-           wavefront writes for the rays that need no integration are
-           contiguous; the live list holds ray ids (the launch's order) */
+        /* pixel_walk: threads walk the pixels in memory order, so that the
+           planes a wavefront writes for the rays that need no integration
+           are contiguous; else they walk the ray ids (the launch's order).
+           The live list holds ray ids either way. */
         const int64_t t = base_id + threadIdx.x;
         int64_t id = t;
         int alive = 0, bk = 0;
@@ -210,10 +211,15 @@ __global__ __launch_bounds__(256) void jsx_k_triage(const typename P::Params *__
             LaneT L;
             const int req = d_find_request(ray_off, n_req, t);
             const typename P::Params *qp = reqs + req;
-            const int64_t pix = t - ray_off[req];
-            if (order == 1) {
-                const int64_t row = pix / qp->im.nl, col = pix - row * qp->im.nl;
-                id = ray_off[req] + col * qp->im.na + row;
+            int64_t pix;
+            if (pixel_walk) {
+                pix = t - ray_off[req];
+                if (order == 1) {
+                    const int64_t row = pix / qp->im.nl, col = pix - row * qp->im.nl;
+                    id = ray_off[req] + col * qp->im.na + row;
+                }
+            } else {
+                pix = d_ray_to_pixel(qp->im, t - ray_off[req], order);
             }
             P::begin(L, *qp, pix);
             if (P::done(L)) {

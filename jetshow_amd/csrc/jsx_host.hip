@@ -59,7 +59,7 @@ struct BatchGraphKey {
     unsigned defer_rhs;
     int park_wave, park_bucket, piece_layout;
     unsigned piece_budget;
-    int n_cu, split;
+    int n_cu, split, triage_walk;
     SplitWS ws;
 };
 
@@ -143,6 +143,8 @@ struct jsx_ctx {
     int timing = 0;
     /* the batched launch as a HIP graph (option "graph", default on) */
     int graph = 1;
+    /* triage threads walk pixels in memory order (1) or ray ids (0) */
+    int triage_walk = 1;
     hipGraphExec_t bg_exec = nullptr;
     BatchGraphKey bg_key;
     bool bg_seen = false;
@@ -340,6 +342,8 @@ extern "C" int jsx_ctx_set_option(jsx_ctx *ctx, const char *name, double v) {
         ctx->ev_n = 0;
     } else if (k == "graph") {
         ctx->graph = v != 0.0;
+    } else if (k == "triage_walk") {
+        ctx->triage_walk = v != 0.0;
     } else {
         return fail(ctx, JSX_EINVAL, "unknown option '%s'", name);
     }
@@ -369,6 +373,7 @@ extern "C" int jsx_ctx_get_option(jsx_ctx *ctx, const char *name, double *v) {
     else if (k == "mg_err") *v = ctx->mg_err;
     else if (k == "park_lanes") *v = ctx->park_lanes;
     else if (k == "graph") *v = ctx->graph;
+    else if (k == "triage_walk") *v = ctx->triage_walk;
     else if (k == "stat_graph_captures") *v = (double)ctx->graph_captures;
     else if (k == "stat_graph_launches") *v = (double)ctx->graph_launches;
     else if (k == "park_wave") *v = ctx->park_wave;
@@ -880,6 +885,11 @@ static int stage_params(jsx_ctx *ctx, const DModel &m, const DImage &D, const DS
    int (k_I - |eta|) ds = 50, the bound on the transfer operator's norm decay
    (dichroism eta = (k_Q, k_U, k_V)).  Emission from deeper reaches the
    observer attenuated by at least e^-50 = 2e-22. */
+/* longest mixed Stokes-pass step through grid fields, in cells of the
+   finest grid spacing (requests tighter than the FP32 floor) */
+#ifndef JSX_MIXED_GRID_STEP
+#define JSX_MIXED_GRID_STEP 4.0
+#endif
 #ifndef JSX_MIXED_TAU_HORIZON
 #define JSX_MIXED_TAU_HORIZON 50.0
 #endif
@@ -949,6 +959,34 @@ static void lower_mixed(const jsx_jet_desc &j, const jsx_solver_params &p, const
     c.split_ok = p.stokes_rel_tol >= 0.999 * JSX_MIXED_STOKES_FLOOR_FULL ? 1 : 0;
     c.dt_max_pc = p.dt_max / PC;
     c.tau_hz = (p.flags & JSX_OPT_FULL_DEPTH) || !(JSX_MIXED_TAU_HORIZON < p.tau_max) ? 0.0 : JSX_MIXED_TAU_HORIZON;
+    /* the terms' rounding floor matters where the state tolerance is tighter
+       than the FP32 floor: there the rounding of emission and absorption,
+       ~1e-8 dt k_I I per step, exceeds rel |I| in an optically thick region
+       once dt k_I > 0.1 rel / 1e-8; at rel >= 1e-7 stability bounds dt k_I
+       first (3.3), so those requests skip the 16 operations per RHS */
+    c.mag_floor = c.st_rel_x < JSX_MIXED_STOKES_FLOOR_I ? 1 : 0;
+    /* fields interpolated on a regular grid are linear inside each triangle
+       and kinked on its edges, where the embedded error estimate does not
+       see the error; the FP32 floor on the step's increment (1e-7, above a
+       tight caller's tolerance) then let a step across several cells through
+       with its whole kink error -- 3e-4 of a thin ray's intensity from one
+       first step of 0.08 pc through cells of 0.004 pc.  A request tighter than
+       the floor therefore steps at most JSX_MIXED_GRID_STEP cells of the
+       finest grid spacing: 3330 pixels of a 256^2 image at rel 1e-9 against
+       FP64 at 1e-11, 512 x 128 grid: max 4.7e-6 (no cap: 3.1e-4); 2048 x 512:
+       1.5e-6 (no cap: 4.5e-5) for 16% more evaluations; 8 cells: 1.0e-5 */
+    c.st_dt_max_pc = 0.0;
+    if (c.mag_floor) {
+        double cell = 0.0;
+        const DGrid *gs[4] = {m.has_gb ? &m.gb : nullptr, m.has_gb2 ? &m.gb2 : nullptr, m.has_gn ? &m.gn : nullptr,
+                              m.has_gv ? &m.gv : nullptr};
+        for (const DGrid *g : gs) {
+            if (!g || g->nz == 0) continue;
+            const double d = g->dz < g->dr ? g->dz : g->dr;
+            cell = (cell == 0.0 || d < cell) ? d : cell;
+        }
+        if (cell > 0.0) c.st_dt_max_pc = JSX_MIXED_GRID_STEP * cell;
+    }
 }
 
 /* ---------------- persistent batch launch ---------------- */
@@ -1226,10 +1264,10 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
         if (live) {
             if (full)
                 hipLaunchKernelGGL(jsx_k_triage<PolMixed<4>>, dim3(tri_blocks), dim3(256), 0, st, R, ctx->d_off,
-                                   ctx->d_outs, nreq, total, order, ctx->d_queues, live, bucket);
+                                   ctx->d_outs, nreq, total, order, ctx->d_queues, live, bucket, ctx->triage_walk);
             else
                 hipLaunchKernelGGL(jsx_k_triage<PolMixed<1>>, dim3(tri_blocks), dim3(256), 0, st, R, ctx->d_off,
-                                   ctx->d_outs, nreq, total, order, ctx->d_queues, live, bucket);
+                                   ctx->d_outs, nreq, total, order, ctx->d_queues, live, bucket, ctx->triage_walk);
         }
         if (sorted) {
             hipLaunchKernelGGL(jsx_k_lpt_scatter, dim3(tri_blocks), dim3(256), 0, st, ctx->d_queues, live, bucket, sorted,
@@ -1284,10 +1322,10 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
         if (live) {
             if (full)
                 hipLaunchKernelGGL(jsx_k_triage<PolFP64<4>>, dim3(tri_blocks), dim3(256), 0, st, R, ctx->d_off,
-                                   ctx->d_outs, nreq, total, order, ctx->d_queues, live, bucket);
+                                   ctx->d_outs, nreq, total, order, ctx->d_queues, live, bucket, ctx->triage_walk);
             else
                 hipLaunchKernelGGL(jsx_k_triage<PolFP64<1>>, dim3(tri_blocks), dim3(256), 0, st, R, ctx->d_off,
-                                   ctx->d_outs, nreq, total, order, ctx->d_queues, live, bucket);
+                                   ctx->d_outs, nreq, total, order, ctx->d_queues, live, bucket, ctx->triage_walk);
         }
         if (sorted) {
             hipLaunchKernelGGL(jsx_k_lpt_scatter, dim3(tri_blocks), dim3(256), 0, st, ctx->d_queues, live, bucket, sorted,
@@ -1320,6 +1358,7 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
     key.queue_block = ctx->queue_block; key.steal = ctx->steal; key.defer_rhs = ctx->defer_rhs;
     key.park_wave = ctx->park_wave; key.park_bucket = ctx->park_bucket; key.piece_layout = ctx->piece_layout;
     key.piece_budget = ctx->piece_budget; key.n_cu = ctx->n_cu; key.split = split;
+    key.triage_walk = ctx->triage_walk;
     if (split) key.ws = ws;
     hipStreamCaptureStatus cap_st = hipStreamCaptureStatusNone;
     const bool use_graph = ctx->graph && !ev && hipStreamIsCapturing(st, &cap_st) == hipSuccess &&

@@ -60,6 +60,8 @@ struct MConsts {
     double tau_abs, tau_rel, st_abs, st_rel, dt_max_pc;
     double st_rel_x;                      /* the caller's Stokes tolerance on |x|          */
     double tau_hz;                        /* optical-depth horizon of the Stokes pass, 0 off */
+    int mag_floor;                        /* floor the error scale at the RHS terms' rounding */
+    double st_dt_max_pc;                  /* longest Stokes-pass step [pc] (grid fields), 0 none */
     int split_ok;                         /* long rays may go to the time-parallel pieces  */
 };
 
@@ -497,6 +499,7 @@ __device__ void ml_stokes_seg_begin(MLane<NS> &L, const DSolver &S, const MConst
     const double tau_pass = (L.flags & JSX_PIX_HORIZON) && c.tau_hz < L.tau_bg ? c.tau_hz : L.tau_bg;
     if (tau_pass > S.tau_n_min) auto_n = d_steps_schedule(S, tau_pass);
     L.dt = L.L / auto_n;
+    if (c.st_dt_max_pc > 0.0 && L.dt > c.st_dt_max_pc) L.dt = c.st_dt_max_pc;
     L.t = 0.0;
     L.first = 1;
     ml_stokes_loop(L, S, c);
@@ -565,7 +568,7 @@ __device__ inline void ml_rhs(const MLane<NS> &L, const DModel &m, const DImage 
     const float *x = L.xt;
     if (NS == 1) {
         r[0] = k.eI - k.kI * x[0];
-        r[1] = fabsf(k.eI) + fabsf(k.kI * x[0]);
+        r[1] = c.mag_floor ? fabsf(k.eI) + fabsf(k.kI * x[0]) : 0.f;
         return;
     }
     r[0] = k.eI - k.kI * x[0] - k.kQ * x[1] - k.kU * x[2] - k.kV * x[NS > 3 ? 3 : 0];
@@ -573,6 +576,8 @@ __device__ inline void ml_rhs(const MLane<NS> &L, const DModel &m, const DImage 
         r[1] = k.eQ - k.kI * x[1] - k.kQ * x[0] - k.kF * x[2] - k.hQ * x[3];
         r[2] = k.eU - k.kI * x[2] - k.kU * x[0] + k.kF * x[1] - k.kC * x[3];
         r[3] = k.eV - k.kI * x[3] - k.kV * x[0] + k.hQ * x[1] + k.kC * x[2];
+        r[4] = r[5] = 0.f;
+        if (!c.mag_floor) return;
         const float a0 = fabsf(x[0]), a1 = fabsf(x[1]), a2 = fabsf(x[2]), a3 = fabsf(x[3]);
         const float kI = fabsf(k.kI), kQ = fabsf(k.kQ), kU = fabsf(k.kU), kV = fabsf(k.kV);
         const float kF = fabsf(k.kF), kC = fabsf(k.kC), hQ = fabsf(k.hQ);
@@ -714,7 +719,7 @@ __device__ inline void ml_consume(MLane<NS> &L, const DSolver &S, const DImage &
        derivative jump, would loosen the control exactly there */
     double err = red(L, c);
     const double h = L.dt;   /* the step just taken; the controller rescales L.dt */
-    const int ok = ml_control(L, S, L.phase == PH_TAU ? c.dt_max_pc : 0.0, err);
+    const int ok = ml_control(L, S, L.phase == PH_TAU ? c.dt_max_pc : c.st_dt_max_pc, err);
     if constexpr (MAG) {
         if (ok) {
             L.mag = r[NS];

@@ -376,8 +376,9 @@ extern "C" int lh_observe(const jsx_jet_desc *jet, const jsx_image_desc *im, con
 /* The mixed path's 11 coefficients (MCoef order kI kQ kU kV kF kC hQ eI eQ eU eV)
  * at points given in cm, through the same lowering as the kernels. */
 extern "C" int lh_coefficients_mixed(const jsx_jet_desc *jet, const jsx_image_desc *im, const jsx_solver_params *p,
-                                     const double *pts_cm, int64_t n, double *out) {
-    HostCtx H(0, nullptr, nullptr);
+                                     const double *pts_cm, int64_t n, double *out, int32_t ngrids,
+                                     const jsx_grid_desc *gd, const double *const *gv) {
+    HostCtx H(ngrids, gd, gv);
     DParams d;
     DParamsM dm;
     int rc = prepare(H, jet, im, p, d, dm);

@@ -51,7 +51,8 @@ def lib():
         _lib.lh_coefficients_mixed.restype = C.c_int
         _lib.lh_coefficients_mixed.argtypes = [C.POINTER(abi.JetDesc), C.POINTER(abi.ImageDesc),
                                                C.POINTER(abi.SolverParams), C.POINTER(C.c_double), C.c_int64,
-                                               C.POINTER(C.c_double)]
+                                               C.POINTER(C.c_double), C.c_int32, C.POINTER(abi.GridDesc),
+                                               C.POINTER(C.POINTER(C.c_double))]
     return _lib
 
 
@@ -92,10 +93,19 @@ def observe(jet, image, params, pixels, grids=None, defer_rhs=0, w_piece=256.0, 
     return res
 
 
-def coefficients_mixed(jet, image, params, pts_cm):
+def coefficients_mixed(jet, image, params, pts_cm, grids=None):
+    """grids = [(GridDesc, values)] in handle order 1..n"""
     pts = np.ascontiguousarray(pts_cm, dtype=np.float64).reshape(-1, 3)
     out = np.zeros((len(pts), 11))
-    rc = lib().lh_coefficients_mixed(C.byref(jet), C.byref(image), C.byref(params), _dp(pts), len(pts), _dp(out))
+    grids = grids or []
+    gd = (abi.GridDesc * max(1, len(grids)))()
+    vals = [np.ascontiguousarray(v, dtype=np.float64) for _, v in grids]
+    gv = (C.POINTER(C.c_double) * max(1, len(grids)))()
+    for i, (d, _) in enumerate(grids):
+        gd[i] = d
+        gv[i] = _dp(vals[i])
+    rc = lib().lh_coefficients_mixed(C.byref(jet), C.byref(image), C.byref(params), _dp(pts), len(pts), _dp(out),
+                                     len(grids), gd, gv)
     if rc:
         raise RuntimeError("lh_coefficients_mixed failed: %d" % rc)
     return out
