@@ -23,6 +23,7 @@
 #include "jsx_batch.h"
 #include "jsx_fourier.h"
 #include "jsx_delaunay.h"
+#include "jsx_walk.h"
 
 
 #define JSX_OPT_PERSISTENT 0x100u   /* internal: persistent queue kernel        */
@@ -145,6 +146,18 @@ struct jsx_ctx {
     int graph = 1;
     /* triage threads walk pixels in memory order (1) or ray ids (0) */
     int triage_walk = 1;
+    /* mixed requests on regular-grid fields integrated cell by cell
+       (option "grid_walk", default on) and the walk kernel's ray counter */
+    int grid_walk = 1;
+    unsigned long long *d_walk_ctr = nullptr;
+    /* the walk launch's request table, offsets and outputs (device, and a
+       pinned staging copy); walk_ev marks the last launch that read them */
+    DParamsM *d_walk_reqs = nullptr;
+    int64_t *d_walk_off = nullptr;
+    DOut *d_walk_outs = nullptr;
+    char *h_walk = nullptr;
+    size_t walk_cap = 0;
+    hipEvent_t walk_ev = nullptr;
     hipGraphExec_t bg_exec = nullptr;
     BatchGraphKey bg_key;
     bool bg_seen = false;
@@ -254,6 +267,13 @@ extern "C" int jsx_ctx_destroy(jsx_ctx *ctx) {
         (void)hipGraphExecDestroy(ctx->bg_exec);
     }
     if (ctx->batch_ev) (void)hipEventDestroy(ctx->batch_ev);
+    if (ctx->walk_ev) {
+        (void)hipEventSynchronize(ctx->walk_ev);
+        (void)hipEventDestroy(ctx->walk_ev);
+    }
+    (void)hipFree(ctx->d_walk_ctr);
+    (void)hipFree(ctx->d_walk_reqs); (void)hipFree(ctx->d_walk_off); (void)hipFree(ctx->d_walk_outs);
+    if (ctx->h_walk) (void)hipHostFree(ctx->h_walk);
     (void)hipFree(ctx->d_live); (void)hipFree(ctx->d_sorted); (void)hipFree(ctx->d_bucket);
     (void)hipFree(ctx->d_sbucket);
     (void)hipFree(ctx->split_mem);
@@ -344,6 +364,8 @@ extern "C" int jsx_ctx_set_option(jsx_ctx *ctx, const char *name, double v) {
         ctx->graph = v != 0.0;
     } else if (k == "triage_walk") {
         ctx->triage_walk = v != 0.0;
+    } else if (k == "grid_walk") {
+        ctx->grid_walk = v != 0.0;
     } else {
         return fail(ctx, JSX_EINVAL, "unknown option '%s'", name);
     }
@@ -374,6 +396,7 @@ extern "C" int jsx_ctx_get_option(jsx_ctx *ctx, const char *name, double *v) {
     else if (k == "park_lanes") *v = ctx->park_lanes;
     else if (k == "graph") *v = ctx->graph;
     else if (k == "triage_walk") *v = ctx->triage_walk;
+    else if (k == "grid_walk") *v = ctx->grid_walk;
     else if (k == "stat_graph_captures") *v = (double)ctx->graph_captures;
     else if (k == "stat_graph_launches") *v = (double)ctx->graph_launches;
     else if (k == "park_wave") *v = ctx->park_wave;
@@ -1091,9 +1114,81 @@ static void plain_kernel(hipStream_t st, int precision, const void *dp, const DO
         hipLaunchKernelGGL((jsx_k_plain<NS, EXT>), grid, dim3(threads), 0, st, (const DParams *)dp, O, npix);
 }
 
+/* a mixed request whose B (random-direction magnitude), N and Gamma come
+   from regular grids of one geometry: integrated cell by cell (jsx_walk.h) */
+static bool walk_eligible(const jsx_ctx *ctx, int precision, const DModel &m) {
+    if (!ctx->grid_walk || precision != JSX_PREC_MIXED || m.ext) return false;
+    if (m.bfield != JSX_B_GRID_SCALAR || m.nfield != JSX_N_GRID || m.vfield != JSX_V_GRID) return false;
+    if (!m.has_gb || !m.has_gn || !m.has_gv) return false;
+    const DGrid *g[3] = {&m.gb, &m.gn, &m.gv};
+    for (const DGrid *x : g)
+        if (x->nz < 2 || x->nr < 2 || x->nz != m.gb.nz || x->nr != m.gb.nr || x->z0 != m.gb.z0 || x->dz != m.gb.dz ||
+            x->r0 != m.gb.r0 || x->dr != m.gb.dr)
+            return false;
+    return true;
+}
+
+/* every walk-eligible request of a launch in one walk launch (rays of all
+   of them in one pool, so a request's rays fill the device together with
+   the others'); the tables are reused once the last walk launch that read
+   them has finished */
+static int launch_walk(jsx_ctx *ctx, hipStream_t st, bool full, const std::vector<const BatchItem *> &items) {
+    const size_t n = items.size();
+    if (n == 0) return JSX_OK;
+    if (!ctx->d_walk_ctr) HIPCHECK(ctx, hipMalloc(&ctx->d_walk_ctr, sizeof(unsigned long long)));
+    if (!ctx->walk_ev) HIPCHECK(ctx, hipEventCreateWithFlags(&ctx->walk_ev, hipEventDisableTiming));
+    HIPCHECK(ctx, hipEventSynchronize(ctx->walk_ev));
+    if (n > ctx->walk_cap) {
+        (void)hipFree(ctx->d_walk_reqs); (void)hipFree(ctx->d_walk_off); (void)hipFree(ctx->d_walk_outs);
+        if (ctx->h_walk) (void)hipHostFree(ctx->h_walk);
+        ctx->d_walk_reqs = nullptr; ctx->d_walk_off = nullptr; ctx->d_walk_outs = nullptr; ctx->h_walk = nullptr;
+        ctx->walk_cap = 0;
+        const size_t cap = n < 64 ? 64 : n;
+        HIPCHECK(ctx, hipMalloc(&ctx->d_walk_reqs, sizeof(DParamsM) * cap));
+        HIPCHECK(ctx, hipMalloc(&ctx->d_walk_off, sizeof(int64_t) * (cap + 1)));
+        HIPCHECK(ctx, hipMalloc(&ctx->d_walk_outs, sizeof(DOut) * cap));
+        HIPCHECK(ctx, hipHostMalloc(&ctx->h_walk, (sizeof(DParamsM) + sizeof(int64_t) + sizeof(DOut)) * cap + 64,
+                                    hipHostMallocDefault));
+        ctx->walk_cap = cap;
+    }
+    DParamsM *hr = (DParamsM *)ctx->h_walk;
+    int64_t *ho = (int64_t *)(hr + ctx->walk_cap);
+    DOut *hd = (DOut *)(ho + ctx->walk_cap + 1);
+    int64_t total = 0;
+    for (size_t k = 0; k < n; ++k) {
+        const BatchItem &it = *items[k];
+        hr[k].m = it.m; hr[k].im = it.im; hr[k].S = it.S; hr[k].c = it.c;
+        ho[k] = total;
+        hd[k] = it.out;
+        total += it.npix;
+    }
+    ho[n] = total;
+    if (total == 0) return JSX_OK;
+    HIPCHECK(ctx, hipMemcpyAsync(ctx->d_walk_reqs, hr, sizeof(DParamsM) * n, hipMemcpyHostToDevice, st));
+    HIPCHECK(ctx, hipMemcpyAsync(ctx->d_walk_off, ho, sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, st));
+    HIPCHECK(ctx, hipMemcpyAsync(ctx->d_walk_outs, hd, sizeof(DOut) * n, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(jsx_k_walk_reset, dim3(1), dim3(64), 0, st, ctx->d_walk_ctr);
+    const dim3 grid(ctx->n_cu * 8), block(256);
+    if (full)
+        hipLaunchKernelGGL(jsx_k_walk<4>, grid, block, 0, st, ctx->d_walk_reqs, ctx->d_walk_off, ctx->d_walk_outs, (int)n,
+                           total, ctx->d_walk_ctr);
+    else
+        hipLaunchKernelGGL(jsx_k_walk<1>, grid, block, 0, st, ctx->d_walk_reqs, ctx->d_walk_off, ctx->d_walk_outs, (int)n,
+                           total, ctx->d_walk_ctr);
+    HIPCHECK(ctx, hipGetLastError());
+    HIPCHECK(ctx, hipEventRecord(ctx->walk_ev, st));
+    return JSX_OK;
+}
+
 static int launch_plain(jsx_ctx *ctx, hipStream_t st, int precision, bool full, const DModel &m, const DImage &D,
                         const DSolver &S, const MConsts *c, const DOut &O, int64_t npix) {
     if (npix == 0) return JSX_OK;
+    if (walk_eligible(ctx, precision, m)) {
+        BatchItem it;
+        it.m = m; it.im = D; it.S = S; it.c = *c; it.out = O; it.npix = npix;
+        std::vector<const BatchItem *> one{&it};
+        return launch_walk(ctx, st, full, one);
+    }
     void *dp = nullptr;
     int slot = 0, rc;
     if (precision == JSX_PREC_MIXED) {
@@ -1122,15 +1217,19 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
         /* models with the further reference kinds run through the plain kernels,
            on the same stream, before the batched launch of the others */
         bool any_ext = false;
-        for (const BatchItem &it : items) any_ext = any_ext || it.m.ext;
+        for (const BatchItem &it : items) any_ext = any_ext || it.m.ext || walk_eligible(ctx, precision, it.m);
         if (any_ext) {
             std::vector<BatchItem> common;
+            std::vector<const BatchItem *> walk;
             for (const BatchItem &it : items) {
+                if (walk_eligible(ctx, precision, it.m)) { walk.push_back(&it); continue; }
                 if (!it.m.ext) { common.push_back(it); continue; }
                 int rc = launch_plain(ctx, st, precision, stokes == JSX_STOKES_FULL, it.m, it.im, it.S, &it.c, it.out,
                                       it.npix);
                 if (rc) return rc;
             }
+            int rc = launch_walk(ctx, st, stokes == JSX_STOKES_FULL, walk);
+            if (rc) return rc;
             return launch_batch(ctx, st, precision, stokes, common, order);
         }
     }

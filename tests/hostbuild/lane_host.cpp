@@ -348,8 +348,9 @@ int prepare(HostCtx &H, const jsx_jet_desc *jet, const jsx_image_desc *im, const
 extern "C" int lh_observe(const jsx_jet_desc *jet, const jsx_image_desc *im, const jsx_solver_params *p,
                           const int64_t *pix, int64_t n, int32_t ngrids, const jsx_grid_desc *gd,
                           const double *const *gv, unsigned defer_rhs, float w_piece, int32_t method,
-                          const jsx_outputs *out, int32_t *n_pieces) {
+                          const jsx_outputs *out, int32_t *n_pieces, int32_t grid_walk) {
     HostCtx H(ngrids, gd, gv);
+    H.ctx->grid_walk = grid_walk;
     DParams d;
     DParamsM dm;
     int rc = prepare(H, jet, im, p, d, dm);
@@ -359,7 +360,13 @@ extern "C" int lh_observe(const jsx_jet_desc *jet, const jsx_image_desc *im, con
     for (int64_t i = 0; i < n; ++i) {
         Out1 o{};
         int np = 0;
-        if (p->precision == JSX_PREC_MIXED) {
+        if (p->precision == JSX_PREC_MIXED && walk_eligible(H.ctx, JSX_PREC_MIXED, dm.m)) {
+            WalkOut w;
+            if (full) walk_ray<1>(dm.m, dm.im, dm.S, dm.c, pix[i], w);
+            else walk_ray<0>(dm.m, dm.im, dm.S, dm.c, pix[i], w);
+            o.tau = w.tau; o.I = w.I; o.Q = o.U = o.V = 0.0; o.l = w.l;
+            o.flags = w.flags; o.n_rhs = w.n_rhs; o.n_steps = w.n_steps;
+        } else if (p->precision == JSX_PREC_MIXED) {
             if (full && defer_rhs) run_mixed_split(dm, pix[i], defer_rhs, w_piece, method, o, &np);
             else if (full) run_mixed<4>(dm, pix[i], o);
             else run_mixed<1>(dm, pix[i], o);

@@ -41,7 +41,7 @@ def lib():
         _lib.lh_observe.argtypes = [C.POINTER(abi.JetDesc), C.POINTER(abi.ImageDesc), C.POINTER(abi.SolverParams),
                                     C.POINTER(C.c_int64), C.c_int64, C.c_int32, C.POINTER(abi.GridDesc),
                                     C.POINTER(C.POINTER(C.c_double)), C.c_uint, C.c_float, C.c_int32,
-                                    C.POINTER(abi.Outputs), C.POINTER(C.c_int32)]
+                                    C.POINTER(abi.Outputs), C.POINTER(C.c_int32), C.c_int32]
         _lib.lh_magnus_expphi.restype = C.c_int
         _lib.lh_magnus_expphi.argtypes = [C.c_double] + [C.POINTER(C.c_double)] * 5
         _lib.lh_split_costs.restype = C.c_int64
@@ -60,7 +60,7 @@ def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-def observe(jet, image, params, pixels, grids=None, defer_rhs=0, w_piece=256.0, method=0):
+def observe(jet, image, params, pixels, grids=None, defer_rhs=0, w_piece=256.0, method=0, grid_walk=True):
     """Lane-by-lane run of the listed pixels; grids = [(GridDesc, values)] in
     handle order 1..n.  defer_rhs > 0 splits rays past that many right-hand
     sides into pieces of w_piece stiffness, integrated by Dormand-Prince
@@ -86,7 +86,7 @@ def observe(jet, image, params, pixels, grids=None, defer_rhs=0, w_piece=256.0, 
         gv[i] = _dp(vals[i])
     rc = lib().lh_observe(C.byref(jet), C.byref(image), C.byref(params), pix.ctypes.data_as(C.POINTER(C.c_int64)),
                           n, len(grids), gd, gv, defer_rhs, w_piece, method, C.byref(o),
-                          npcs.ctypes.data_as(C.POINTER(C.c_int32)))
+                          npcs.ctypes.data_as(C.POINTER(C.c_int32)), 1 if grid_walk else 0)
     if rc:
         raise RuntimeError("lh_observe failed: %d" % rc)
     res["n_pieces"] = npcs

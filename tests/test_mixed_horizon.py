@@ -31,10 +31,11 @@ def test_thick_grid_rays_start_at_the_horizon():
     pix = np.array([513, 449, 482, 544, 256, 300], dtype=np.int64)   # tau 1e4, 1e4, 7e3, 1e4, 444; 0.04
     q64 = models.with_precision(p, abi.PREC_FP64, rel_tol=1e-11)
     o = oracle.observe(j, im, q64, pix, math=oracle.MATH_LIBM, grids=table, threads=8)
-    a = LH.observe(j, im, p, pix, grids=table)
+    # the Dormand-Prince lane (grid models go cell by cell unless told not to)
+    a = LH.observe(j, im, p, pix, grids=table, grid_walk=False)
     full = models.copy_struct(p)
     full.flags |= abi.OPT_FULL_DEPTH
-    b = LH.observe(j, im, full, pix, grids=table)
+    b = LH.observe(j, im, full, pix, grids=table, grid_walk=False)
     thick = o["tau"] > 50.0
     assert thick.sum() == 5
     assert ((a["flags"][thick] & abi.PIX_HORIZON) != 0).all()
@@ -69,3 +70,29 @@ def test_growing_polarized_mode_keeps_the_whole_path():
     assert o["tau"][0] > 1e3
     assert (a["flags"][0] & abi.PIX_HORIZON) == 0
     assert abs(a["I"][0] / o["I"][0] - 1.0) <= 1e-5
+
+
+def test_grid_rays_cell_by_cell():
+    """the cell-by-cell walk of grid fields: thin and thick rays of both grid
+    resolutions within 1e-5 of converged FP64 (rel 1e-11), tau, l and flags
+    as the lanes give them, a fraction of the lane's evaluations"""
+    j, im, p, table = _grid_model()
+    pix = np.array([513, 449, 482, 544, 256, 300, 600, 280, 405], dtype=np.int64)
+    q64 = models.with_precision(p, abi.PREC_FP64, rel_tol=1e-11)
+    q64.tau_abs_tol, q64.tau_rel_tol = 1e-20, 1e-11      # tau itself converged too
+    o = oracle.observe(j, im, q64, pix, math=oracle.MATH_LIBM, grids=table, threads=8)
+    w = LH.observe(j, im, p, pix, grids=table)
+    lane = LH.observe(j, im, p, pix, grids=table, grid_walk=False)
+    assert np.abs(w["I"] / o["I"] - 1.0).max() <= 1e-5
+    crossed = (o["flags"] & abi.PIX_TAU_CROSSED) != 0
+    assert crossed.sum() == 3 and (w["tau"][crossed] >= p.tau_max).all()
+    assert np.abs(w["tau"][~crossed] / o["tau"][~crossed] - 1.0).max() <= 1e-5
+    assert np.allclose(w["l"], lane["l"], rtol=1e-12, atol=0)
+    assert np.array_equal(w["flags"], lane["flags"])
+    assert w["n_rhs"].sum() < 0.5 * lane["n_rhs"].sum()
+    # full Stokes: Q, U, V zero (random-direction B), I as for Stokes I
+    pf = models.copy_struct(p)
+    pf.stokes = abi.STOKES_FULL
+    wf = LH.observe(j, im, pf, pix, grids=table)
+    assert np.array_equal(wf["I"], w["I"])
+    assert not wf["Q"].any() and not wf["U"].any() and not wf["V"].any()
