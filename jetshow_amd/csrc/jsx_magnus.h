@@ -226,12 +226,15 @@ __device__ inline int mg_coef_t(double mu, const MgGen &g, double *c, double *d)
             chm1 = em * mg_chm1(a2);
             shm1 = em * mg_shm1(a2);
         } else {
-            const double ep = exp(a - mu), en = exp(-a - mu);
+            /* e^(a - mu) and e^(-a - mu) from one exponential */
+            const double ea = exp(a);
+            const double ep = em * ea, en = em / ea;
             chm1 = 0.5 * (ep + en) - em;
             shm1 = 0.5 * (ep - en) / a - em;
         }
         const double ch = em + chm1, sh = em + shm1;     /* e^-mu cosh a, e^-mu sinh(a)/a */
-        const double hs = sin(0.5 * b), hc = cos(0.5 * b);
+        double hs, hc;
+        sincos(0.5 * b, &hs, &hc);
         const double sb = 2.0 * hs * hc;                 /* sin b */
         const double omc = 2.0 * hs * hs;                /* 1 - cos b */
         const double cb = 1.0 - omc;
@@ -385,9 +388,11 @@ __device__ inline int mg_step(const MCoef &k1, const MCoef &k2, double h, double
             sP = fmax(sP, fmax(fabs(P[i]), fabs(w4[i])));
         }
         const double eta = sqrt(g4.e[0] * g4.e[0] + g4.e[1] * g4.e[1] + g4.e[2] * g4.e[2]);
-        const double grow = exp(eta - mu);
+        /* e^z and phi1(z) = (e^z - 1) / z from one expm1 */
+        const double zg = eta - mu, em1 = expm1(zg);
+        const double grow = em1 + 1.0;
         const double e1 = grow * dG / rel;
-        const double eP = mg_phi1(eta - mu) * (dv + dG * ww);
+        const double eP = (zg != 0.0 ? em1 / zg : 1.0) * (dv + dG * ww);
         const double e2 = sP > 0.0 ? eP / (rel * sP) : (eP > 0.0 ? __builtin_inf() : 0.0);
         *err = e1 > e2 ? e1 : e2;
         return (*err == *err) ? 1 : 0;
