@@ -61,3 +61,25 @@ def test_steps_schedule():
     k = 1000 / (math.log(100.0) + b) ** 2
     for tau in (0.5, 3.0, 42.0):
         assert oracle.steps_schedule(tau, 100, 1000) == math.ceil(k * (math.log(tau) + b) ** 2 + 100)
+
+
+def test_math_modes_converge_to_the_same_image():
+    """The oracle's two math modes (its own polynomial pow/atan2/sincos, which
+    the kernels share, and the C library's) differ by last-ulp amounts; run to
+    convergence (relative tolerance 1e-12 on Stokes and tau) they must give the
+    same images to 1e-9 on every hit pixel with the same flags: a shared-math
+    function off by more than rounding would fail this."""
+    from jetshow_amd import abi, models
+    cases = [models.conical_toroidal_model(), models.paraboloid_helical_model(n_side=16, pixel_pc=0.3)]
+    for j, im, p in cases:
+        p.stokes_abs_tol, p.stokes_rel_tol = 1e-30, 1e-12
+        p.tau_abs_tol, p.tau_rel_tol = 1e-30, 1e-12
+        a = oracle.observe(j, im, p, None, math=oracle.MATH_JSX, threads=8)
+        b = oracle.observe(j, im, p, None, math=oracle.MATH_LIBM, threads=8)
+        assert np.array_equal(a["flags"], b["flags"])
+        hit = (b["flags"] & abi.PIX_HIT) != 0
+        assert hit.sum() >= 40
+        for k in ("tau", "I", "Q", "U", "V"):
+            scale = np.abs(b[k]) if k in ("tau", "I") else np.maximum(np.abs(b[k]), np.abs(b["I"]))
+            rel = np.abs(a[k] - b[k])[hit] / np.maximum(scale[hit], 1e-300)
+            assert rel.max() <= 1e-9, k
