@@ -59,7 +59,7 @@ struct BatchGraphKey {
     int steal;
     unsigned defer_rhs;
     int park_wave, park_bucket, piece_layout;
-    unsigned piece_budget;
+    unsigned piece_budget, mg_part;
     int n_cu, split, triage_walk;
     SplitWS ws;
 };
@@ -120,6 +120,10 @@ struct jsx_ctx {
     /* exponential pieces: right-hand sides a piece may use in each of the
        first rounds before handing the rest of its interval on (0: one round) */
     unsigned piece_budget = 64;
+    /* exponential pieces in one launch: a piece that has used this many
+       right-hand sides hands the rest it estimates at more than two such parts
+       on to the same launch (0: the budgeted rounds above) */
+    unsigned mg_part = 48;
     /* time-parallel path for long full-Stokes rays (mixed precision) */
     unsigned defer_rhs = 512;           /* 0 disables parking                  */
     float split_w_min = 256.0f;
@@ -343,6 +347,9 @@ extern "C" int jsx_ctx_set_option(jsx_ctx *ctx, const char *name, double v) {
     } else if (k == "piece_budget") {
         if (!(v >= 0 && v < 1e9)) return fail(ctx, JSX_EINVAL, "piece_budget out of range");
         ctx->piece_budget = (unsigned)v;
+    } else if (k == "mg_part") {
+        if (!(v >= 0 && v < 1e6)) return fail(ctx, JSX_EINVAL, "mg_part out of range");
+        ctx->mg_part = (unsigned)v;
     } else if (k == "triage") {
         if (!(v == 0.0 || v == 1.0 || v == 2.0)) return fail(ctx, JSX_EINVAL, "triage must be 0, 1 or 2");
         ctx->triage = (int)v;
@@ -389,6 +396,7 @@ extern "C" int jsx_ctx_get_option(jsx_ctx *ctx, const char *name, double *v) {
     else if (k == "park_bucket") *v = ctx->park_bucket;
     else if (k == "piece_target") *v = ctx->piece_target;
     else if (k == "piece_budget") *v = ctx->piece_budget;
+    else if (k == "mg_part") *v = ctx->mg_part;
     else if (k == "mg_cost") *v = ctx->mg_cost;
     else if (k == "mg_profile") *v = ctx->mg_profile;
     else if (k == "mg_rel") *v = ctx->mg_rel;
@@ -404,11 +412,20 @@ extern "C" int jsx_ctx_get_option(jsx_ctx *ctx, const char *name, double *v) {
         unsigned c[2] = {0, 0};
         if (ctx->split_mem) {
             HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
-            HIPCHECK(ctx, hipMemcpy(c, ctx->split.n_long, 2 * sizeof(unsigned), hipMemcpyDeviceToHost));
+            HIPCHECK(ctx, hipMemcpy(&c[0], ctx->split.n_long, sizeof(unsigned), hipMemcpyDeviceToHost));
+            HIPCHECK(ctx, hipMemcpy(&c[1], ctx->split.n_pieces, sizeof(unsigned), hipMemcpyDeviceToHost));
         }
         unsigned parked = c[0] < (unsigned)ctx->split.cap ? c[0] : (unsigned)ctx->split.cap;
         unsigned pieces = c[1] < (unsigned)ctx->split.piece_cap ? c[1] : (unsigned)ctx->split.piece_cap;
         *v = k == "stat_parked" ? (double)parked : (double)pieces;
+    } else if (k == "stat_mg_trips") {
+        /* wavefront loop trips of the last launch's exponential pieces */
+        unsigned long long c = 0;
+        if (ctx->split_mem) {
+            HIPCHECK(ctx, hipDeviceSynchronize());
+            HIPCHECK(ctx, hipMemcpy(&c, ctx->split.mg_stat, sizeof c, hipMemcpyDeviceToHost));
+        }
+        *v = (double)c;
     } else if (k == "stat_parked_exp") {
         unsigned c = 0;
         if (ctx->split_mem) {
@@ -1047,9 +1064,9 @@ static void split_target(jsx_ctx *ctx) {
 static size_t split_workspace_bytes() {
     const size_t cap = JSX_SPLIT_RAY_CAP, piece_cap = JSX_SPLIT_PIECE_CAP;
     auto r = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    return r(8 * sizeof(unsigned) + 2 * sizeof(unsigned long long)) + r(sizeof(LongRay) * cap) +
+    return r(128 * sizeof(unsigned) + 3 * sizeof(unsigned long long)) + r(sizeof(LongRay) * cap) +
            r(sizeof(RaySum) * cap) + r(sizeof(float) * cap * JSX_SPLIT_NSAMP) + r(sizeof(Piece) * piece_cap) +
-           2 * r(sizeof(PieceOut) * piece_cap) + 2 * r(sizeof(int) * piece_cap);
+           2 * r(sizeof(PieceOut) * piece_cap) + 3 * r(sizeof(int) * piece_cap);
 }
 
 static int split_workspace(jsx_ctx *ctx) {
@@ -1057,7 +1074,7 @@ static int split_workspace(jsx_ctx *ctx) {
     const int cap = JSX_SPLIT_RAY_CAP, piece_cap = JSX_SPLIT_PIECE_CAP;
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
-    size_t o_cnt = take(8 * sizeof(unsigned) + 2 * sizeof(unsigned long long));
+    size_t o_cnt = take(128 * sizeof(unsigned) + 3 * sizeof(unsigned long long));
     size_t o_rays = take(sizeof(LongRay) * cap);
     size_t o_rsum = take(sizeof(RaySum) * cap);
     size_t o_prof = take(sizeof(float) * (size_t)cap * JSX_SPLIT_NSAMP);
@@ -1066,18 +1083,27 @@ static int split_workspace(jsx_ctx *ctx) {
     size_t o_pout2 = take(sizeof(PieceOut) * piece_cap);
     size_t o_na = take(sizeof(int) * piece_cap);
     size_t o_nb = take(sizeof(int) * piece_cap);
+    size_t o_ready = take(sizeof(unsigned) * piece_cap);
     HIPCHECK(ctx, hipMalloc(&ctx->split_mem, off));
+    /* the launch epoch and the slot marks start at zero (the first launch's
+       reset makes the epoch 1) */
+    HIPCHECK(ctx, hipMemset(ctx->split_mem, 0, off));
     char *b = (char *)ctx->split_mem;
     SplitWS &w = ctx->split;
+    /* the three counters the exponential launch polls sit on lines of their own */
     w.n_long = (unsigned *)(b + o_cnt);
-    w.n_pieces = w.n_long + 1;
     w.piece_head = w.n_long + 2;
     w.round_end = w.n_long + 3;
     w.dp_end = w.n_long + 4;
-    w.mg_head = w.n_long + 5;
     w.n_mg = w.n_long + 6;
-    w.rhs_pieces = (unsigned long long *)(b + o_cnt + 8 * sizeof(unsigned));
+    w.mg_epoch = w.n_long + 8;
+    w.n_pieces = w.n_long + 32;
+    w.mg_head = w.n_long + 64;
+    w.mg_pending = w.n_long + 96;
+    w.mg_ready = (unsigned *)(b + o_ready);
+    w.rhs_pieces = (unsigned long long *)(b + o_cnt + 128 * sizeof(unsigned));
     w.rhs_mg = w.rhs_pieces + 1;
+    w.mg_stat = w.rhs_pieces + 2;
     w.rays = (LongRay *)(b + o_rays);
     w.rsum = (RaySum *)(b + o_rsum);
     w.cap = cap;
@@ -1396,16 +1422,22 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
                 hipLaunchKernelGGL(jsx_k_split_piece20, dim3(ctx->n_cu * 4), dim3(256), 0, st, R, ws);
             mark(6);
             if (ctx->piece_layout >= 2) {
-                const int rounds = ctx->piece_budget ? 4 : 1;
-                for (int r = 0; r < rounds; ++r) {
-                    if (r) hipLaunchKernelGGL(jsx_k_split_round, dim3(1), dim3(64), 0, st, ws);
-                    hipLaunchKernelGGL(jsx_k_split_magnus, dim3(ctx->n_cu * 8), dim3(256), 0, st, R, ws,
-                                       r + 1 < rounds ? ctx->piece_budget : 0u);
+                if (ctx->mg_part) {
+                    /* one launch, two workgroups per CU (its residency) */
+                    hipLaunchKernelGGL(jsx_k_split_magnus, dim3(ctx->n_cu * 2), dim3(256), 0, st, R, ws, 0u,
+                                       ctx->mg_part);
+                } else {
+                    const int rounds = ctx->piece_budget ? 4 : 1;
+                    for (int r = 0; r < rounds; ++r) {
+                        if (r) hipLaunchKernelGGL(jsx_k_split_round, dim3(1), dim3(64), 0, st, ws);
+                        hipLaunchKernelGGL(jsx_k_split_magnus, dim3(ctx->n_cu * 8), dim3(256), 0, st, R, ws,
+                                           r + 1 < rounds ? ctx->piece_budget : 0u, 0u);
+                    }
                 }
             }
             mark(3);
-            /* four chain-jumping rounds when the exponential rounds made long chains */
-            const int jump = ctx->piece_layout >= 2 && ctx->piece_budget ? 4 : 0;
+            /* four chain-jumping rounds when the exponential pieces split into chains */
+            const int jump = ctx->piece_layout >= 2 && (ctx->piece_budget || ctx->mg_part) ? 4 : 0;
             for (int r = 0; r < jump; ++r)
                 hipLaunchKernelGGL(jsx_k_split_jump, dim3(ctx->n_cu * 8), dim3(256), 0, st, ws, r);
             hipLaunchKernelGGL(jsx_k_split_combine, dim3((cap + 255) / 256), dim3(256), 0, st, ctx->d_outs, ws,
@@ -1456,7 +1488,7 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
     key.live = live; key.sorted = sorted; key.bucket = bucket; key.sbucket = ctx->d_sbucket;
     key.queue_block = ctx->queue_block; key.steal = ctx->steal; key.defer_rhs = ctx->defer_rhs;
     key.park_wave = ctx->park_wave; key.park_bucket = ctx->park_bucket; key.piece_layout = ctx->piece_layout;
-    key.piece_budget = ctx->piece_budget; key.n_cu = ctx->n_cu; key.split = split;
+    key.piece_budget = ctx->piece_budget; key.mg_part = ctx->mg_part; key.n_cu = ctx->n_cu; key.split = split;
     key.triage_walk = ctx->triage_walk;
     if (split) key.ws = ws;
     hipStreamCaptureStatus cap_st = hipStreamCaptureStatusNone;

@@ -163,7 +163,7 @@ static bool serve_make_room(jsx_server *S, size_t need) {
 /* the launch options a lane's context takes from the server's context */
 static const char *const kServeOptions[] = {"blocks_per_cu", "queue_block", "steal", "defer_rhs", "split_w_min",
                                             "split_floor_max", "piece_layout", "piece_target", "mg_rel", "mg_err",
-                                            "mg_profile", "mg_cost", "piece_budget", "triage", "lpt", "park_bucket",
+                                            "mg_profile", "mg_cost", "piece_budget", "mg_part", "triage", "lpt", "park_bucket",
                                             "park_wave", "park_lanes", "triage_walk"};
 
 static int serve_copy_options(jsx_ctx *dst, jsx_ctx *src) {

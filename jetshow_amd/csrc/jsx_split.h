@@ -77,6 +77,8 @@ struct Piece {
     int next, pad;                /* the ray's following piece (-1: last) */
 };
 
+static_assert(sizeof(Piece) == 40, "Piece is read as five 8-byte words");
+
 struct PieceOut {
     double P[4];
     double Phi[16];               /* column-major: Phi[4 * j + i] = row i of column j */
@@ -119,6 +121,11 @@ struct SplitWS {
                                      own step estimate                               */
     float mg_rel;                 /* smallest relative tolerance of exponential steps */
     int mg_err;                   /* their error estimate: 0 first order, 1 second exponential */
+    unsigned *mg_pending;         /* single-launch mode: exponential pieces reserved
+                                     and not yet finished                            */
+    unsigned *mg_epoch;           /* launch number (jsx_k_split_reset counts it)     */
+    unsigned *mg_ready;           /* piece_cap marks: slot written in launch mg_epoch */
+    unsigned long long *mg_stat;  /* exponential launches: wavefront loop trips */
     float w_min;                  /* smallest stiffness per piece worth a split     */
     float floor_max;              /* cap of the column error-scale estimate (1: off) */
 };
@@ -470,6 +477,7 @@ __global__ __launch_bounds__(1024) void jsx_k_split_alloc(SplitWS ws) {
         *ws.n_pieces = np;
         *ws.mg_head = de;
         *ws.round_end = np;
+        *ws.mg_pending = np - de;
         *ws.n_mg = (unsigned)s_nmg;
     }
 }
@@ -950,47 +958,55 @@ __global__ __launch_bounds__(256) void jsx_k_split_piece20(const DParamsM *__res
 }
 
 /* ---- 5c. pieces by exponential propagators (one lane per piece) ---- */
-/* A piece's propagator and particular solution (20 doubles) live outside
- * the registers: in the kernel, in LDS, component-major across the 256 lanes
- * of the workgroup (U[i] of lane l at [i][l]: consecutive lanes, consecutive
- * addresses).  Held in registers they kept the step's coefficient and
+/* A piece's propagator and particular solution (20 doubles), with its pass
+ * origin, end and error-scale floor (5 more), live outside the registers: in
+ * the kernel, in LDS, component-major across the 256 lanes of the workgroup
+ * (U[i] of lane l at [i][l]: consecutive lanes, consecutive addresses; 50 KB
+ * per workgroup).  Held in registers they kept the step's coefficient and
  * exponential code at one wavefront per SIMD (256 VGPRs + ~70 AGPRs); the
- * step reads them only to form the error scale and to apply the update. */
+ * step reads them once per step. */
 #define JSX_MG_USTRIDE 256
+/* per-lane doubles in LDS: Phi (16), P (4), the pass origin (3), the
+ * piece's end and the particular solution's error-scale floor */
+#define JSX_MG_O 20
+#define JSX_MG_TB 23
+#define JSX_MG_PF 24
+#define JSX_MG_NU 25
 struct MgU {
     double *p;                    /* component 0 of this lane's 20 */
     __device__ double &operator[](int i) const { return p[i * JSX_MG_USTRIDE]; }
 };
 struct MgLane {
-    MgU U;                        /* Phi column-major (U[4 j + i]), then P (U[16 + i]) */
-    double t, tb, h, p_floor, rel_min;
-    double o[3];                  /* origin of the piece's pass [cm] */
+    MgU U;                        /* Phi column-major (U[4 j + i]), then P (U[16 + i]),
+                                     the pass origin [cm] (U[JSX_MG_O + i]), the
+                                     piece's end (U[JSX_MG_TB]), p_floor (U[JSX_MG_PF]) */
+    double t, h;
     uint32_t flags, n_rhs, n_steps, tries;
     int done;
-    int est;                      /* error estimate of mg_step (0 first order, 1 second exponential) */
 };
 
-__device__ inline void mg_lane_begin(MgLane &L, const LongRay &R, const Piece &pc, double rel_min, int est = 0) {
-    L.est = est;
+__device__ inline void mg_lane_begin(MgLane &L, const LongRay &R, const Piece &pc) {
 #pragma unroll
     for (int i = 0; i < 20; ++i) L.U[i] = (i < 16 && (i >> 2) == (i & 3)) ? 1.0 : 0.0;
-    for (int i = 0; i < 3; ++i) L.o[i] = pc.interval ? R.s0out[i] : R.cout[i];
+    for (int i = 0; i < 3; ++i) L.U[JSX_MG_O + i] = pc.interval ? R.s0out[i] : R.cout[i];
     L.t = pc.ta;
-    L.tb = pc.tb;
+    L.U[JSX_MG_TB] = pc.tb;
     const double len = pc.tb - pc.ta;
     L.h = pc.dt0 > 0.0 ? pc.dt0 : 0.0625 * len;
-    L.p_floor = fabs(R.x[0]);
-    L.rel_min = rel_min;
+    L.U[JSX_MG_PF] = fabs(R.x[0]);
     L.flags = 0; L.n_rhs = 0; L.n_steps = 0; L.tries = 0;
     L.done = !(pc.tb > pc.ta);
 }
 
-/* one attempted step: accept (propagator updated, t advanced) or shrink h */
-__device__ inline void mg_lane_step(MgLane &L, const DModel &m, const DImage &im, const MConsts &c, const DSolver &S) {
+/* one attempted step: accept (propagator updated, t advanced) or shrink h;
+ * rel_min floors the relative step tolerance, est picks mg_step's error
+ * estimate (0 first order, 1 second exponential) */
+__device__ inline void mg_lane_step(MgLane &L, const DModel &m, const DImage &im, const MConsts &c, const DSolver &S,
+                                    double rel_min, int est) {
     JSX_MIXED_FMA
     double h = L.h;
     int last = 0;
-    if (!(L.t + h < L.tb)) { h = L.tb - L.t; last = 1; }
+    if (!(L.t + h < L.U[JSX_MG_TB])) { h = L.U[JSX_MG_TB] - L.t; last = 1; }
     /* the two Gauss points (kept as two named values: an indexed pair would
        live in scratch memory) */
     MCoef A0, A1;
@@ -998,17 +1014,17 @@ __device__ inline void mg_lane_step(MgLane &L, const DModel &m, const DImage &im
         const double t0 = L.t + 0.21132486540518713 * h, t1 = L.t + 0.78867513459481287 * h;
         double p0[3], p1[3];
         for (int i = 0; i < 3; ++i) {
-            p0[i] = L.o[i] * c.inv_pc - t0 * im.dir[i];
-            p1[i] = L.o[i] * c.inv_pc - t1 * im.dir[i];
+            p0[i] = L.U[JSX_MG_O + i] * c.inv_pc - t0 * im.dir[i];
+            p1[i] = L.U[JSX_MG_O + i] * c.inv_pc - t1 * im.dir[i];
         }
         m_coefficients<1>(m, c, p0, A0);
         m_coefficients<1>(m, c, p1, A1);
     }
     L.n_rhs += 2;
-    const double rel = c.st_rel > L.rel_min ? c.st_rel : L.rel_min;
+    const double rel = c.st_rel > rel_min ? c.st_rel : rel_min;
     double E[16], w[4], err;
     const double P[4] = {L.U[16], L.U[17], L.U[18], L.U[19]};
-    const int ok = mg_step(A0, A1, h, rel, L.p_floor, P, E, w, &err, L.est);
+    const int ok = mg_step(A0, A1, h, rel, L.U[JSX_MG_PF], P, E, w, &err, est);
     if (ok && err <= 1.0) {
 #pragma unroll
         for (int j = 0; j < 5; ++j) {
@@ -1019,7 +1035,7 @@ __device__ inline void mg_lane_step(MgLane &L, const DModel &m, const DImage &im
                 L.U[4 * j + i] = j == 4 ? acc + w[i] : acc;
             }
         }
-        L.t = last ? L.tb : L.t + h;
+        L.t = last ? L.U[JSX_MG_TB] : L.t + h;
         L.n_steps++;
         L.tries = 0;
         float fac = 2.0f;
@@ -1053,20 +1069,43 @@ __device__ inline void mg_lane_step(MgLane &L, const DModel &m, const DImage &im
  * the rest; the rounds bound the longest piece a lane holds. */
 #define JSX_MG_SPAWN 8
 
+/* a piece another wavefront of the running launch wrote: its five 8-byte
+ * words read at the device's coherence point (no cache invalidation) */
+__device__ inline Piece mg_piece_load(const Piece *p) {
+    const unsigned long long *w = (const unsigned long long *)p;
+    unsigned long long v[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) v[i] = __hip_atomic_load(w + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    Piece c;
+    __builtin_memcpy(&c, v, sizeof c);
+    return c;
+}
+
+/* and the writer's side: five 8-byte stores at the coherence point */
+__device__ inline void mg_piece_store(Piece *p, const Piece &c) {
+    unsigned long long v[5];
+    __builtin_memcpy(v, &c, sizeof c);
+    unsigned long long *w = (unsigned long long *)p;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) __hip_atomic_store(w + i, v[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 /* write the continuation pieces at slots base.. and chain them after pid */
-__device__ inline void mg_spawn_at(const SplitWS &ws, int pid, const MgLane &L, unsigned base, int m) {
+__device__ inline void mg_spawn_at(const SplitWS &ws, int pid, const MgLane &L, unsigned base, int m,
+                                    bool coherent = false) {
     const Piece self = ws.pieces[pid];
-    const double len = (L.tb - L.t) / (double)m;
+    const double len = (L.U[JSX_MG_TB] - L.t) / (double)m;
     for (int i = 0; i < m; ++i) {
         Piece c;
         c.ray = self.ray;
         c.interval = self.interval;
         c.ta = L.t + (double)i * len;
-        c.tb = i + 1 == m ? L.tb : L.t + (double)(i + 1) * len;
+        c.tb = i + 1 == m ? L.U[JSX_MG_TB] : L.t + (double)(i + 1) * len;
         c.dt0 = L.h;
         c.next = i + 1 == m ? self.next : (int)base + i + 1;
         c.pad = 0;
-        ws.pieces[base + i] = c;
+        if (coherent) mg_piece_store(ws.pieces + base + i, c);
+        else ws.pieces[base + i] = c;
         ws.pout[base + i].flags = 0u;
     }
     ws.pieces[pid].next = (int)base;
@@ -1074,34 +1113,69 @@ __device__ inline void mg_spawn_at(const SplitWS &ws, int pid, const MgLane &L, 
 
 /* slots reserved past the table's end by a failed reservation: the part
  * inside the table is marked empty (no ray), so a later round skips it */
-__device__ inline void mg_spawn_void(const SplitWS &ws, unsigned base, int m) {
-    for (unsigned i = base; i < base + (unsigned)m && i < (unsigned)ws.piece_cap; ++i) ws.pieces[i].ray = -1;
+__device__ inline void mg_spawn_void(const SplitWS &ws, unsigned base, int m, bool coherent = false) {
+    for (unsigned i = base; i < base + (unsigned)m && i < (unsigned)ws.piece_cap; ++i) {
+        if (coherent) __hip_atomic_store(&ws.pieces[i].ray, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else ws.pieces[i].ray = -1;
+    }
 }
 
 /* continuation parts a piece asks for: the right-hand sides it still needs at
  * its current step size, in parts of about `budget` (0 or 1: go on alone) */
 __device__ inline int mg_spawn_parts(const MgLane &L, unsigned budget) {
-    const double est = 2.0 * (L.tb - L.t) / L.h;
+    const double est = 2.0 * (L.U[JSX_MG_TB] - L.t) / L.h;
     if (!(est > 2.0 * (double)budget)) return 0;
     const double m = ceil(est / (double)budget);
     return m < (double)JSX_MG_SPAWN ? (int)m : JSX_MG_SPAWN;
 }
 
-/* two wavefronts per SIMD (at most 256 registers per lane) */
+/* Two wavefronts per SIMD (at most 256 registers per lane).
+ *
+ * Rounds mode (part == 0): the launch integrates the pieces [mg_head,
+ * round_end); with a budget, a piece past it hands the rest it estimates at
+ * more than two budgets to the next launch (mg_spawn_parts).
+ *
+ * Single-launch mode (part > 0): the same rule with `part` right-hand sides
+ * in place of the budget, but the parts run in the same launch, so the
+ * launch has one tail instead of one per round.  An idle lane takes a
+ * ticket, the next slot number of the piece table.  A slot below round_end
+ * holds a planned piece; a slot past it becomes a piece when a running lane
+ * reserves it (n_pieces), stores it at the coherence point (mg_piece_store)
+ * and, after vmcnt(0), marks it with this launch's epoch; the ticket's
+ * holder loads the mark, then the piece (mg_piece_load).  No cache is
+ * flushed or invalidated on either side.  mg_pending counts slots reserved
+ * and not finished: a wavefront adds the slots it reserves before marking
+ * them, and takes off the pieces it finished in the same atomic or later, so
+ * the count never drops below the slots still to finish; mg_pending = 0 means
+ * no piece runs and none can appear, and a lane whose ticket is unfilled then,
+ * or lies past the table, is done.  A wait is only ever on a lane that is
+ * running (a spawner writes and marks its slots in the step that reserves
+ * them), so the launch drains whatever its residency.  Wavefronts with no
+ * piece running poll the shared counters every fourth trip and sleep in
+ * between: polling them in every trip made them the launch's bottleneck. */
 __global__ __launch_bounds__(256, 2) void jsx_k_split_magnus(const DParamsM *__restrict__ reqs, SplitWS ws,
-                                                             unsigned budget) {
+                                                             unsigned budget, unsigned part) {
     const int lane = __lane_id();
     const unsigned npieces = *ws.round_end;
     unsigned nrays = *ws.n_long;
     unsigned *head = ws.mg_head;
+    const unsigned cap = (unsigned)ws.piece_cap;
+    const unsigned epoch = *ws.mg_epoch;
     if (nrays > (unsigned)ws.cap) nrays = ws.cap;
     MgLane L;
-    __shared__ double sU[20 * JSX_MG_USTRIDE];
+    __shared__ double sU[JSX_MG_NU * JSX_MG_USTRIDE];
     L.U.p = &sU[threadIdx.x];
-    int have = 0, exhausted = 0, slot = 0, pid = 0, spawn_ok = 0;
+    int have = 0, exhausted = 0, slot = 0, spawn_ok = 0;
+    int holding = 0;              /* a ticket not yet filled */
+    unsigned ticket = 0;          /* the slot taken: the piece's, once filled */
     unsigned long long rhs_acc = 0;
+    /* single-launch mode: pieces this wavefront finished and has not yet
+       taken off mg_pending (taking them off late only delays the end) */
+    int dec = 0;
+    unsigned it = 0;
     for (;;) {
-        int want = !have && !exhausted;
+        ++it;
+        int want = !have && !exhausted && !holding;
         unsigned long long mask = __ballot(want);
         if (mask) {
             int cnt = __popcll(mask);
@@ -1110,38 +1184,87 @@ __global__ __launch_bounds__(256, 2) void jsx_k_split_magnus(const DParamsM *__r
             if (lane == leader) base = atomicAdd(head, (unsigned)cnt);
             base = __shfl(base, leader);
             unsigned long long below = lane ? (mask & ((1ull << lane) - 1ull)) : 0ull;
-            unsigned id = base + (unsigned)__popcll(below);
             if (want) {
-                if (id >= npieces) {
+                ticket = base + (unsigned)__popcll(below);
+                holding = 1;
+            }
+        }
+        if (__any(holding)) {
+            int fill = 0, fresh = 0;
+            /* a wavefront with no piece running polls every fourth time */
+            const int idle = !__any(have == 1);
+            const int look = !idle || (it & 3u) == 0u;
+            if (holding) {
+                if (part == 0u) {
+                    if (ticket < npieces) fill = 1;
+                    else exhausted = 1;
+                } else if (ticket >= cap) {
                     exhausted = 1;
-                } else if ((unsigned)ws.pieces[id].ray < nrays) {
-                    pid = (int)id;
-                    const Piece pc = ws.pieces[pid];
+                } else if (ticket < npieces) {
+                    fill = 1;
+                } else if (look) {
+                    fill = fresh = __hip_atomic_load(ws.mg_ready + ticket, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT) == epoch;
+                }
+                if (exhausted) holding = 0;
+            }
+            int empty = 0;
+            if (fill) {
+                holding = 0;
+                const Piece pc = fresh ? mg_piece_load(ws.pieces + ticket) : ws.pieces[ticket];
+                if ((unsigned)pc.ray < nrays) {
                     const LongRay &R = ws.rays[pc.ray];
                     slot = R.slot;
-                    mg_lane_begin(L, R, pc, (double)ws.mg_rel, ws.mg_err);
+                    mg_lane_begin(L, R, pc);
                     have = L.done ? 2 : 1;
-                    spawn_ok = budget != 0u;
+                    spawn_ok = budget != 0u || part != 0u;
+                } else {
+                    empty = part != 0u;
+                }
+            }
+            dec += __popcll(__ballot(empty));
+            /* single-launch mode, a wavefront with no piece running: its own
+               count goes in, then a ticket whose slot is not written is done
+               if nothing runs anywhere */
+            if (part && !__any(have == 1)) {
+                if (dec && lane == 0) atomicSub(ws.mg_pending, (unsigned)dec);
+                dec = 0;
+                if (look && __any(holding)) {
+                    const unsigned pend = __hip_atomic_load(ws.mg_pending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (holding && __builtin_amdgcn_readfirstlane(pend) == 0u) {
+                        holding = 0;
+                        exhausted = 1;
+                    }
                 }
             }
         }
-        if (!__any(have) && __all(exhausted)) break;
+        if (!__any(have) && __all(exhausted)) {
+            if (dec && lane == 0) atomicSub(ws.mg_pending, (unsigned)dec);
+            break;
+        }
         if (have == 1) {
             const int s0 = __builtin_amdgcn_readfirstlane(slot);
             if (__all(slot == s0)) {
                 const JSX_CONST DParamsM *q = (const JSX_CONST DParamsM *)(reqs + s0);
                 JSX_OPAQUE_S(q);
-                mg_lane_step(L, q->m, q->im, q->c, q->S);
+                mg_lane_step(L, q->m, q->im, q->c, q->S, (double)ws.mg_rel, ws.mg_err);
             } else {
                 const DParamsM *q = reqs + slot;
                 JSX_OPAQUE_V(q);
-                mg_lane_step(L, q->m, q->im, q->c, q->S);
+                mg_lane_step(L, q->m, q->im, q->c, q->S, (double)ws.mg_rel, ws.mg_err);
             }
             if (L.done) have = 2;
+        } else if (part && !__any(have)) {
+            __builtin_amdgcn_s_sleep(16);    /* every lane waits on a ticket */
         }
         /* continuation pieces: one reservation per wavefront */
+        int spawned = 0, reserved = 0;
+        unsigned sbase = 0;
         {
-            const int m = (have == 1 && spawn_ok && L.n_rhs >= budget) ? mg_spawn_parts(L, budget) : 0;
+            /* a piece past its budget (rounds) or its part (single launch)
+               hands on the rest it estimates at more than two of them */
+            const unsigned lim = part ? part : budget;
+            int m = (have == 1 && spawn_ok && L.n_rhs >= lim) ? mg_spawn_parts(L, lim) : 0;
             const unsigned long long smask = __ballot(m > 0);
             if (smask) {
                 /* inclusive scan of the part counts over the wavefront */
@@ -1154,22 +1277,41 @@ __global__ __launch_bounds__(256, 2) void jsx_k_split_magnus(const DParamsM *__r
                 unsigned base = 0;
                 if (lane == 0) base = atomicAdd(ws.n_pieces, (unsigned)total);
                 base = __shfl(base, 0);
+                if (base < cap) reserved = (int)(base + (unsigned)total < cap ? (unsigned)total : cap - base);
                 const unsigned mine = base + (unsigned)(incl - m);
                 if (m > 0) {
-                    if (mine + (unsigned)m <= (unsigned)ws.piece_cap) {
-                        mg_spawn_at(ws, pid, L, mine, m);
-                        L.tb = L.t;
+                    if (mine + (unsigned)m <= cap) {
+                        mg_spawn_at(ws, (int)ticket, L, mine, m, part != 0u);
+                        L.U[JSX_MG_TB] = L.t;
                         L.done = 1;
                         have = 2;
                     } else {
-                        mg_spawn_void(ws, mine, m);
+                        mg_spawn_void(ws, mine, m, part != 0u);
                         spawn_ok = 0;   /* table full: this piece runs to its end */
                     }
                 }
+                spawned = part ? (int)m : 0;
+                sbase = mine;
+            }
+        }
+        dec += __popcll(__ballot(have == 2));
+        if (part) {
+            if (reserved || (it & 15u) == 0u) {
+                /* one atomic: the reserved slots on, the finished pieces off */
+                const int d = reserved - dec;
+                if (d != 0 && lane == 0) atomicAdd(ws.mg_pending, (unsigned)d);
+                dec = 0;
+            }
+            /* the new slots' marks once their pieces (stored at the coherence
+               point) and the pending count are complete: vmcnt(0) */
+            if (__any(spawned > 0)) {
+                __builtin_amdgcn_s_waitcnt(0x0F70);
+                for (unsigned i = sbase; i < sbase + (unsigned)spawned && i < cap; ++i)
+                    __hip_atomic_store(ws.mg_ready + i, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
         if (have == 2) {
-            PieceOut &o = ws.pout[pid];
+            PieceOut &o = ws.pout[ticket];
 #pragma unroll
             for (int i = 0; i < 4; ++i) o.P[i] = L.U[16 + i];
 #pragma unroll
@@ -1182,7 +1324,10 @@ __global__ __launch_bounds__(256, 2) void jsx_k_split_magnus(const DParamsM *__r
         }
     }
     unsigned long long tot = d_wave_sum_u64(rhs_acc);
-    if (lane == 0 && tot) atomicAdd(ws.rhs_mg, tot);
+    if (lane == 0) {
+        if (tot) atomicAdd(ws.rhs_mg, tot);
+        atomicAdd(ws.mg_stat, (unsigned long long)it);
+    }
 }
 
 /* ---- 6a. chain jumping: after round r every piece holds the composition of
@@ -1313,9 +1458,12 @@ __global__ void jsx_k_split_reset(SplitWS ws) {
         *ws.dp_end = 0u;
         *ws.mg_head = 0u;
         *ws.round_end = 0u;
+        *ws.mg_pending = 0u;
+        *ws.mg_epoch += 1u;
         *ws.n_mg = 0u;
         *ws.rhs_pieces = 0ull;
         *ws.rhs_mg = 0ull;
+        *ws.mg_stat = 0ull;
     }
 }
 

@@ -39,6 +39,8 @@ static inline float jsx_hb_exp2f(float x) { return exp2f(x); }
 #define __builtin_amdgcn_rcpf(x) (1.0f / (x))
 #define __builtin_amdgcn_rsqf(x) (1.0f / sqrtf(x))
 #define __builtin_amdgcn_readfirstlane(x) (x)
+#define __builtin_amdgcn_s_sleep(n) ((void)0)
+#define __builtin_amdgcn_s_waitcnt(n) ((void)0)
 
 static inline int __lane_id() { return 0; }
 static inline unsigned long long __ballot(int p) { return p ? 1ull : 0ull; }
@@ -75,6 +77,7 @@ struct double2 { double x, y; };
 static inline double2 make_double2(double a, double b) { return double2{a, b}; }
 
 static inline hipError_t hipMalloc(void *p, size_t n) { *(void **)p = calloc(1, n ? n : 1); return hipSuccess; }
+static inline hipError_t hipMemset(void *p, int v, size_t n) { memset(p, v, n); return hipSuccess; }
 /* file-scope device arrays: ordinary host arrays here, their own address */
 #define __align__(n) __attribute__((aligned(n)))
 #define HIP_SYMBOL(x) (&(x))

@@ -126,10 +126,10 @@ void run_piece(const DParamsM &q, const LongRay &R, const Piece &pc, PieceOut &p
 /* one piece by exponential propagators (the GPU's jsx_k_split_magnus lane) */
 void run_piece_magnus(const DParamsM &q, const LongRay &R, const Piece &pc, PieceOut &po) {
     MgLane L;
-    static double ustore[20 * JSX_MG_USTRIDE];
+    static double ustore[JSX_MG_NU * JSX_MG_USTRIDE];
     L.U.p = ustore;
-    mg_lane_begin(L, R, pc, JSX_SPLIT_REL_MIN);
-    while (!L.done) mg_lane_step(L, q.m, q.im, q.c, q.S);
+    mg_lane_begin(L, R, pc);
+    while (!L.done) mg_lane_step(L, q.m, q.im, q.c, q.S, JSX_SPLIT_REL_MIN, 0);
     for (int i = 0; i < 4; ++i) po.P[i] = L.U[16 + i];
     for (int i = 0; i < 16; ++i) po.Phi[i] = L.U[i];
     po.flags = L.flags;
@@ -139,26 +139,30 @@ void run_piece_magnus(const DParamsM &q, const LongRay &R, const Piece &pc, Piec
 
 /* exponential pieces in budgeted rounds, as the launch sequence of
  * jsx_k_split_magnus / jsx_k_split_round runs them (one lane after another) */
-void run_pieces_magnus_rounds(const DParamsM &q, const LongRay &R, SplitWS &ws, unsigned budget) {
+void run_pieces_magnus_rounds(const DParamsM &q, const LongRay &R, SplitWS &ws, unsigned budget,
+                              bool single = false) {
+    /* single: the single-launch mode's decomposition -- every piece, spawned
+       ones included, hands on its rest past `budget` right-hand sides, until
+       no piece spawns (the GPU runs them in one launch, in any order) */
     unsigned head = 0, end = *ws.n_pieces;
-    const int rounds = budget ? 4 : 1;
-    for (int r = 0; r < rounds; ++r) {
-        const unsigned b = r + 1 < rounds ? budget : 0u;
+    const int rounds = single ? 1 << 20 : budget ? 4 : 1;
+    for (int r = 0; r < rounds && head < end; ++r) {
+        const unsigned b = single || r + 1 < rounds ? budget : 0u;
         for (unsigned id = head; id < end; ++id) {
             if (ws.pieces[id].ray < 0) continue;
             MgLane L;
-            static double ustore[20 * JSX_MG_USTRIDE];
+            static double ustore[JSX_MG_NU * JSX_MG_USTRIDE];
             L.U.p = ustore;
-            mg_lane_begin(L, R, ws.pieces[id], JSX_SPLIT_REL_MIN);
+            mg_lane_begin(L, R, ws.pieces[id]);
             int spawn_ok = b != 0u;
             while (!L.done) {
-                mg_lane_step(L, q.m, q.im, q.c, q.S);
+                mg_lane_step(L, q.m, q.im, q.c, q.S, JSX_SPLIT_REL_MIN, 0);
                 const int m = (!L.done && spawn_ok && L.n_rhs >= b) ? mg_spawn_parts(L, b) : 0;
                 if (m > 0) {
                     const unsigned mine = atomicAdd(ws.n_pieces, (unsigned)m);
                     if (mine + (unsigned)m <= (unsigned)ws.piece_cap) {
                         mg_spawn_at(ws, (int)id, L, mine, m);
-                        L.tb = L.t;
+                        L.U[JSX_MG_TB] = L.t;
                         L.done = 1;
                     } else {
                         mg_spawn_void(ws, mine, m);
@@ -181,7 +185,8 @@ void run_pieces_magnus_rounds(const DParamsM &q, const LongRay &R, SplitWS &ws, 
 /* the split pipeline for one ray, as the GPU kernels do it (sampling, piece
  * counts for a given per-piece stiffness, cuts at quantiles, pieces, combine);
  * method 0: Dormand-Prince pieces (5 lanes), 2: exponential propagators,
- * 3: exponential propagators in budgeted rounds (64 right-hand sides) */
+ * 3: exponential propagators in budgeted rounds (64 right-hand sides),
+ * 4: exponential propagators split as the single launch does (48) */
 void run_mixed_split(const DParamsM &q, int64_t pix, unsigned defer_rhs, float w_piece, int method, Out1 &o,
                      int *n_pieces) {
     MLane<4> L;
@@ -246,7 +251,7 @@ void run_mixed_split(const DParamsM &q, int64_t pix, unsigned defer_rhs, float w
     if (R.seg) k = k < 2 ? 2 : k;
     R.piece0 = 0;
     R.npieces = k;
-    const int cap = method == 3 ? k + 65536 : k;
+    const int cap = method >= 3 ? k + 65536 : k;
     std::vector<Piece> pcs(cap);
     std::vector<PieceOut> pout(cap);
     unsigned n_pcs = (unsigned)k;
@@ -298,8 +303,9 @@ void run_mixed_split(const DParamsM &q, int64_t pix, unsigned defer_rhs, float w
             pc.pad = 0;
         }
     }
-    if (method == 3) {
-        run_pieces_magnus_rounds(q, R, ws, 64u);
+    if (method >= 3) {
+        if (method == 3) run_pieces_magnus_rounds(q, R, ws, 64u);
+        else run_pieces_magnus_rounds(q, R, ws, 48u, true);
     } else {
         for (int j = 0; j < k; ++j) {
             if (method == 2) run_piece_magnus(q, R, pcs[j], pout[j]);
@@ -313,7 +319,7 @@ void run_mixed_split(const DParamsM &q, int64_t pix, unsigned defer_rhs, float w
     ws.pout2 = pout2.data();
     ws.next_a = na.data();
     ws.next_b = nb.data();
-    const int jumped = method == 3;
+    const int jumped = method >= 3;
     if (jumped)
         for (int r = 0; r < 4; ++r) jsx_k_split_jump(ws, r);
     DOut ov = o.view();

@@ -170,6 +170,47 @@ def test_split_exponential_pieces_on_gpu(ctx):
     assert quv <= 5e-5
 
 
+def test_exponential_single_launch_and_rounds(ctx):
+    """The exponential pieces in one launch (mg_part: parts spawned and taken
+    while the launch runs) and in the budgeted rounds (mg_part 0): both within
+    the bar of FP64 on every hit pixel, with the same flags; the single launch
+    takes more pieces (its spawned parts split again) in one loop; a batch of
+    two models (pieces of both requests in one launch, wavefronts mixing
+    them) gives each model's image within the bar as well."""
+    j, im, p = models.paraboloid_helical_model(n_side=64, pixel_pc=0.08)
+    g64 = ctx.observe(j, im, p)
+    pm = models.with_precision(p, abi.PREC_MIXED, rel_tol=1e-7)
+    pm.flags |= OPT_PERSISTENT | OPT_APEX_FIRST
+    old = ctx.get_option("mg_part")
+    out = {}
+    try:
+        for part in (48, 0):
+            ctx.set_option("mg_part", part)
+            gm = ctx.observe(j, im, pm)
+            out[part] = (gm, ctx.get_option("stat_pieces"), ctx.get_option("stat_mg_trips"))
+            rel, quv, nhit = _mixed_errors(g64, gm)
+            print("mg_part %d: %d pieces, %d wavefront trips; max rel I %.3g, QUV/peak %.3g" % (
+                part, out[part][1], out[part][2], rel, quv), flush=True)
+            assert rel <= 1e-5
+            assert quv <= 5e-5
+        assert np.array_equal(out[48][0]["flags"], out[0][0]["flags"])
+        assert out[48][1] > out[0][1] > 0
+        # two models in one batched launch, single-launch mode
+        ctx.set_option("mg_part", 48)
+        j2, im2, p2 = models.paraboloid_helical_model(n_side=48, pixel_pc=0.1)
+        g64b = ctx.observe(j2, im2, p2)
+        pm2 = models.with_precision(p2, abi.PREC_MIXED, rel_tol=1e-7)
+        pm2.flags |= OPT_PERSISTENT | OPT_APEX_FIRST
+        ra, rb = ctx.observe_batch([(j, im, pm), (j2, im2, pm2)])
+        for ref, got in ((g64, ra), (g64b, rb)):
+            rel, quv, nhit = _mixed_errors(ref, got)
+            print("batch of two: max rel I %.3g, QUV/peak %.3g over %d hits" % (rel, quv, nhit), flush=True)
+            assert rel <= 1e-5
+            assert quv <= 5e-5
+    finally:
+        ctx.set_option("mg_part", old)
+
+
 def test_exponential_error_estimates_on_gpu(ctx):
     """Both step error estimates of the exponential pieces (mg_err 0: first
     order in the commutator terms; 1: a second, 2nd-order exponential per

@@ -174,6 +174,14 @@ def test_split_long_rays_exponential_pieces():
     print("rounds: pieces %s, max rel I %.3g, QUV/peak %.3g" % (hr["n_pieces"], rel_r, quv_r))
     assert rel_r <= 1e-6
     assert quv_r <= 5e-5
+    # the single launch's decomposition: every piece, spawned ones too, hands
+    # on its rest past 48 right-hand sides; same bars
+    hs = LH.observe(j, im, pm, idx, defer_rhs=4096, w_piece=256.0, method=4)
+    assert (hs["n_pieces"] > hr["n_pieces"]).all(), (hs["n_pieces"], hr["n_pieces"])
+    rel_s, quv_s = _errors(o, hs)
+    print("single launch: pieces %s, max rel I %.3g, QUV/peak %.3g" % (hs["n_pieces"], rel_s, quv_s))
+    assert rel_s <= 1e-6
+    assert quv_s <= 5e-5
 
 
 def test_mixed_thin_path_holds_callers_tolerance():
