@@ -49,7 +49,7 @@ PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 # defaults), from rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by a copy of
 # known size (profiles/c1_hbm_traffic_counters.md); reported as roofline
 # "traffic" when that kernel dominates the default headline run.
-TRAFFIC_BYTES_C1 = {"jsx_k_split_magnus": 143.8e6, "jsx_k_batch<PolMixed<4>>": 24.1e6}
+TRAFFIC_BYTES_C1 = {"jsx_k_split_magnus": 299.2e6, "jsx_k_batch<PolMixed<4>>": 13.7e6}
 # the same for the c2 workload's FP64 batch kernel, per launch of both images
 # (profiles/c2_grid_fields_fp64_hbm_traffic_counters.md)
 TRAFFIC_BYTES_C2_FP64 = 82.4e9

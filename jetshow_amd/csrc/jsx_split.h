@@ -980,12 +980,15 @@ struct MgLane {
     MgU U;                        /* Phi column-major (U[4 j + i]), then P (U[16 + i]),
                                      the pass origin [cm] (U[JSX_MG_O + i]), the
                                      piece's end (U[JSX_MG_TB]), p_floor (U[JSX_MG_PF]) */
-    double t, h;
+    double t, h, rel_min;
     uint32_t flags, n_rhs, n_steps, tries;
     int done;
+    int est;                      /* error estimate of mg_step (0 first order, 1 second exponential) */
 };
 
-__device__ inline void mg_lane_begin(MgLane &L, const LongRay &R, const Piece &pc) {
+__device__ inline void mg_lane_begin(MgLane &L, const LongRay &R, const Piece &pc, double rel_min, int est = 0) {
+    L.rel_min = rel_min;
+    L.est = est;
 #pragma unroll
     for (int i = 0; i < 20; ++i) L.U[i] = (i < 16 && (i >> 2) == (i & 3)) ? 1.0 : 0.0;
     for (int i = 0; i < 3; ++i) L.U[JSX_MG_O + i] = pc.interval ? R.s0out[i] : R.cout[i];
@@ -998,11 +1001,11 @@ __device__ inline void mg_lane_begin(MgLane &L, const LongRay &R, const Piece &p
     L.done = !(pc.tb > pc.ta);
 }
 
-/* one attempted step: accept (propagator updated, t advanced) or shrink h;
- * rel_min floors the relative step tolerance, est picks mg_step's error
- * estimate (0 first order, 1 second exponential) */
-__device__ inline void mg_lane_step(MgLane &L, const DModel &m, const DImage &im, const MConsts &c, const DSolver &S,
-                                    double rel_min, int est) {
+/* one attempted step: accept (propagator updated, t advanced) or shrink h.
+ * (The tolerance floor and the estimate's kind are kept per lane: passed as
+ * launch-uniform values they let the compiler duplicate the step code, which
+ * tripled the kernel's register spills.) */
+__device__ inline void mg_lane_step(MgLane &L, const DModel &m, const DImage &im, const MConsts &c, const DSolver &S) {
     JSX_MIXED_FMA
     double h = L.h;
     int last = 0;
@@ -1021,10 +1024,10 @@ __device__ inline void mg_lane_step(MgLane &L, const DModel &m, const DImage &im
         m_coefficients<1>(m, c, p1, A1);
     }
     L.n_rhs += 2;
-    const double rel = c.st_rel > rel_min ? c.st_rel : rel_min;
+    const double rel = c.st_rel > L.rel_min ? c.st_rel : L.rel_min;
     double E[16], w[4], err;
     const double P[4] = {L.U[16], L.U[17], L.U[18], L.U[19]};
-    const int ok = mg_step(A0, A1, h, rel, L.U[JSX_MG_PF], P, E, w, &err, est);
+    const int ok = mg_step(A0, A1, h, rel, L.U[JSX_MG_PF], P, E, w, &err, L.est);
     if (ok && err <= 1.0) {
 #pragma unroll
         for (int j = 0; j < 5; ++j) {
@@ -1215,7 +1218,7 @@ __global__ __launch_bounds__(256, 2) void jsx_k_split_magnus(const DParamsM *__r
                 if ((unsigned)pc.ray < nrays) {
                     const LongRay &R = ws.rays[pc.ray];
                     slot = R.slot;
-                    mg_lane_begin(L, R, pc);
+                    mg_lane_begin(L, R, pc, (double)ws.mg_rel, ws.mg_err);
                     have = L.done ? 2 : 1;
                     spawn_ok = budget != 0u || part != 0u;
                 } else {
@@ -1247,11 +1250,11 @@ __global__ __launch_bounds__(256, 2) void jsx_k_split_magnus(const DParamsM *__r
             if (__all(slot == s0)) {
                 const JSX_CONST DParamsM *q = (const JSX_CONST DParamsM *)(reqs + s0);
                 JSX_OPAQUE_S(q);
-                mg_lane_step(L, q->m, q->im, q->c, q->S, (double)ws.mg_rel, ws.mg_err);
+                mg_lane_step(L, q->m, q->im, q->c, q->S);
             } else {
                 const DParamsM *q = reqs + slot;
                 JSX_OPAQUE_V(q);
-                mg_lane_step(L, q->m, q->im, q->c, q->S, (double)ws.mg_rel, ws.mg_err);
+                mg_lane_step(L, q->m, q->im, q->c, q->S);
             }
             if (L.done) have = 2;
         } else if (part && !__any(have)) {

@@ -128,8 +128,8 @@ void run_piece_magnus(const DParamsM &q, const LongRay &R, const Piece &pc, Piec
     MgLane L;
     static double ustore[JSX_MG_NU * JSX_MG_USTRIDE];
     L.U.p = ustore;
-    mg_lane_begin(L, R, pc);
-    while (!L.done) mg_lane_step(L, q.m, q.im, q.c, q.S, JSX_SPLIT_REL_MIN, 0);
+    mg_lane_begin(L, R, pc, JSX_SPLIT_REL_MIN);
+    while (!L.done) mg_lane_step(L, q.m, q.im, q.c, q.S);
     for (int i = 0; i < 4; ++i) po.P[i] = L.U[16 + i];
     for (int i = 0; i < 16; ++i) po.Phi[i] = L.U[i];
     po.flags = L.flags;
@@ -153,10 +153,10 @@ void run_pieces_magnus_rounds(const DParamsM &q, const LongRay &R, SplitWS &ws, 
             MgLane L;
             static double ustore[JSX_MG_NU * JSX_MG_USTRIDE];
             L.U.p = ustore;
-            mg_lane_begin(L, R, ws.pieces[id]);
+            mg_lane_begin(L, R, ws.pieces[id], JSX_SPLIT_REL_MIN);
             int spawn_ok = b != 0u;
             while (!L.done) {
-                mg_lane_step(L, q.m, q.im, q.c, q.S, JSX_SPLIT_REL_MIN, 0);
+                mg_lane_step(L, q.m, q.im, q.c, q.S);
                 const int m = (!L.done && spawn_ok && L.n_rhs >= b) ? mg_spawn_parts(L, b) : 0;
                 if (m > 0) {
                     const unsigned mine = atomicAdd(ws.n_pieces, (unsigned)m);
