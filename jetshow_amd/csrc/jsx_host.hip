@@ -1069,7 +1069,7 @@ static size_t split_workspace_bytes() {
            2 * r(sizeof(PieceOut) * piece_cap) + 3 * r(sizeof(int) * piece_cap);
 }
 
-static int split_workspace(jsx_ctx *ctx) {
+static int split_workspace(jsx_ctx *ctx, hipStream_t st) {
     if (ctx->split_mem) return JSX_OK;
     const int cap = JSX_SPLIT_RAY_CAP, piece_cap = JSX_SPLIT_PIECE_CAP;
     size_t off = 0;
@@ -1086,8 +1086,13 @@ static int split_workspace(jsx_ctx *ctx) {
     size_t o_ready = take(sizeof(unsigned) * piece_cap);
     HIPCHECK(ctx, hipMalloc(&ctx->split_mem, off));
     /* the launch epoch and the slot marks start at zero (the first launch's
-       reset makes the epoch 1) */
-    HIPCHECK(ctx, hipMemset(ctx->split_mem, 0, off));
+       reset makes the epoch 1).  Cleared on the launch's own stream and waited
+       for: a launch on a non-blocking stream is not ordered after the null
+       stream, and a clear that landed while the split kernels ran wiped their
+       counters.  (The first split launch of a context is never captured: a
+       capture needs an earlier launch of the same shape.) */
+    HIPCHECK(ctx, hipMemsetAsync(ctx->split_mem, 0, off, st));
+    HIPCHECK(ctx, hipStreamSynchronize(st));
     char *b = (char *)ctx->split_mem;
     SplitWS &w = ctx->split;
     /* the three counters the exponential launch polls sit on lines of their own */
@@ -1341,7 +1346,7 @@ static int launch_batch(jsx_ctx *ctx, hipStream_t st, int precision, int stokes,
     }
     auto mark = [&](int k) { if (ev) (void)hipEventRecord(ev->e[k], st); };
     if (split) {
-        int rc = split_workspace(ctx);
+        int rc = split_workspace(ctx, st);
         if (rc) return rc;
         ws = ctx->split;
     }

@@ -1096,7 +1096,9 @@ __device__ inline void mg_piece_store(Piece *p, const Piece &c) {
 /* write the continuation pieces at slots base.. and chain them after pid */
 __device__ inline void mg_spawn_at(const SplitWS &ws, int pid, const MgLane &L, unsigned base, int m,
                                     bool coherent = false) {
-    const Piece self = ws.pieces[pid];
+    /* in the single launch the piece may have been written by another
+       wavefront in this launch: read at the coherence point, as taken */
+    const Piece self = coherent ? mg_piece_load(ws.pieces + pid) : ws.pieces[pid];
     const double len = (L.U[JSX_MG_TB] - L.t) / (double)m;
     for (int i = 0; i < m; ++i) {
         Piece c;

@@ -77,7 +77,7 @@ struct double2 { double x, y; };
 static inline double2 make_double2(double a, double b) { return double2{a, b}; }
 
 static inline hipError_t hipMalloc(void *p, size_t n) { *(void **)p = calloc(1, n ? n : 1); return hipSuccess; }
-static inline hipError_t hipMemset(void *p, int v, size_t n) { memset(p, v, n); return hipSuccess; }
+static inline hipError_t hipMemsetAsync(void *p, int v, size_t n, hipStream_t) { memset(p, v, n); return hipSuccess; }
 /* file-scope device arrays: ordinary host arrays here, their own address */
 #define __align__(n) __attribute__((aligned(n)))
 #define HIP_SYMBOL(x) (&(x))

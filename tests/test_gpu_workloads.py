@@ -192,13 +192,16 @@ def test_server_cone_and_grid_requests_vs_fp64(ctx):
         j, im, p = models.grid_jet_model((0, 0, 0), n_side=side, pixel_pc=2.0 / side, nu_ghz=nu,
                                          precision=abi.PREC_MIXED, stokes=abi.STOKES_FULL)
         reqs.append(("grid %d nu %.1f" % (side, nu), j, im, p, payload))
+    print("server: %d requests" % len(reqs), flush=True)
     with Server(ctx, abi.PREC_MIXED, abi.STOKES_FULL, max_active=4) as srv:
         ids = [srv.submit(j, im, p, grids=g) for _, j, im, p, g in reqs]
         outs = [srv.wait(i)[0] for i in ids]
         st = srv.stats()
+    print("server: done", flush=True)
     handles = [ctx.upload_grid(v, *geo, nan_value=d.nan_value) for d, v in zip(descs, (B, N, G))]
     try:
         for (label, j, im, p, g), gm in zip(reqs, outs):
+            print("  %s: FP64 references" % label, flush=True)
             jj = models.copy_struct(j)
             if g:
                 jj.grid_b, jj.grid_n, jj.grid_v = handles
